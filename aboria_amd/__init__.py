@@ -1,0 +1,390 @@
+"""aboria_amd -- MI355X-native particle hot path behind Aboria's API names.
+
+Python mirror of the reference's user-facing interface for the hot path:
+``Particles.init_neighbour_search`` / ``update_positions`` / ``get_query``
+(src/Particles.h:445-540), bulk ``euclidean_search`` (src/Search.h:839-845)
+and ``create_sparse_operator`` (src/Operators.h:472-516).  Device memory and
+streams are torch CUDA tensors; all compute goes through libaboria_hip.so.
+"""
+import ctypes
+
+import torch
+
+from . import _capi as C
+from ._capi import AbhError, check
+
+__all__ = [
+    "Context", "CellListOrdered", "Particles", "SearchResult", "euclidean_search",
+    "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
+    "FP64", "FP32_COMPENSATED",
+]
+
+FP64 = C.FP64
+FP32_COMPENSATED = C.FP32_COMPENSATED
+
+
+class _CAI:
+    """__cuda_array_interface__ wrapper of a raw device pointer."""
+
+    def __init__(self, ptr, shape, typestr):
+        self.__cuda_array_interface__ = {
+            "shape": tuple(shape), "typestr": typestr, "data": (int(ptr), False),
+            "version": 2, "strides": None,
+        }
+
+
+_TYPESTR = {torch.float64: "<f8", torch.int32: "<i4", torch.int64: "<i8", torch.uint8: "|u1",
+            torch.float32: "<f4"}
+_UINT32 = "<u4"
+
+
+def _view(ptr, shape, dtype, device):
+    """Zero-copy torch view of library-owned device memory."""
+    n = 1
+    for s in shape:
+        n *= s
+    if n == 0 or not ptr:
+        return torch.empty(shape, dtype=dtype, device=device)
+    ts = _UINT32 if dtype == "uint32" else _TYPESTR[dtype]
+    t = torch.as_tensor(_CAI(ptr, shape, ts), device=device)
+    return t
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _require_cuda(t, dtype, name):
+    if not (t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+        raise ValueError("%s must be a contiguous %s CUDA tensor" % (name, dtype))
+
+
+class Context:
+    """One HIP device + stream (torch's current stream by default)."""
+
+    def __init__(self, device=0, stream=None):
+        self.device = torch.device("cuda", device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        h = ctypes.c_void_p()
+        check(C.lib().abh_context_create(device, ctypes.c_void_p(stream.cuda_stream),
+                                         ctypes.byref(h)))
+        self.handle = h
+
+    def sync(self):
+        check(C.lib().abh_context_sync(self.handle), self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CellListOrdered:
+    """Sorted cell list (src/CellListOrdered.h).  Doubles as its own query
+    object: bucket arrays and sorted positions live on the device."""
+
+    def __init__(self, ctx, low, high, periodic, n_particles_in_leaf=10.0):
+        self.ctx = ctx
+        self.dim = len(low)
+        lo = (ctypes.c_double * self.dim)(*[float(v) for v in low])
+        hi = (ctypes.c_double * self.dim)(*[float(v) for v in high])
+        per = (ctypes.c_uint8 * self.dim)(*[1 if p else 0 for p in periodic])
+        h = ctypes.c_void_p()
+        check(C.lib().abh_cells_create(ctx.handle, self.dim, lo, hi, per,
+                                       float(n_particles_in_leaf), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        self.n_alive = 0
+
+    def update_positions(self, positions, alive=None):
+        """Wraps/kills in place, sorts by bucket.  Returns the alive count."""
+        _require_cuda(positions, torch.float64, "positions")
+        if positions.dim() != 2 or positions.shape[1] != self.dim:
+            raise ValueError("positions must be (n, %d)" % self.dim)
+        if alive is not None:
+            _require_cuda(alive, torch.uint8, "alive")
+        na = ctypes.c_int64()
+        check(C.lib().abh_cells_update(self.handle, positions.shape[0], _ptr(positions),
+                                       _ptr(alive), ctypes.byref(na)), self.ctx.handle)
+        self.n_alive = na.value
+        return na.value
+
+    @property
+    def info(self):
+        i = C.CellsInfo()
+        check(C.lib().abh_cells_get_info(self.handle, ctypes.byref(i)))
+        return i
+
+    @property
+    def grid(self):
+        i = self.info
+        return tuple(i.grid[d] for d in range(self.dim))
+
+    @property
+    def n_buckets(self):
+        return self.info.n_buckets
+
+    def order(self):
+        return _view(C.lib().abh_cells_order(self.handle), (self.n_alive,), torch.int32,
+                     self.ctx.device)
+
+    def bucket_index(self):
+        return _view(C.lib().abh_cells_bucket_index(self.handle), (self.n_alive,), "uint32",
+                     self.ctx.device)
+
+    def bucket_begin(self):
+        return _view(C.lib().abh_cells_bucket_begin(self.handle), (self.n_buckets,), "uint32",
+                     self.ctx.device)
+
+    def bucket_end(self):
+        return _view(C.lib().abh_cells_bucket_end(self.handle), (self.n_buckets,), "uint32",
+                     self.ctx.device)
+
+    def positions(self):
+        return _view(C.lib().abh_cells_positions(self.handle), (self.n_alive, self.dim),
+                     torch.float64, self.ctx.device)
+
+    def gather(self, src, out=None):
+        """out[k] = src[order[k]] for every alive particle."""
+        if not (src.is_cuda and src.is_contiguous()):
+            raise ValueError("src must be a contiguous CUDA tensor")
+        shape = (self.n_alive,) + tuple(src.shape[1:])
+        if out is None:
+            out = torch.empty(shape, dtype=src.dtype, device=src.device)
+        eb = src.element_size() * (src[0].numel() if src.dim() > 1 else 1)
+        check(C.lib().abh_gather(self.ctx.handle, ctypes.c_void_p(
+            C.lib().abh_cells_order(self.handle)), self.n_alive, _ptr(src), _ptr(out), eb),
+            self.ctx.handle)
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_cells_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Particles:
+    """SoA particle container (src/Particles.h) holding ``position``, ``id``,
+    ``alive`` and user columns as CUDA tensors."""
+
+    def __init__(self, ctx, positions, **columns):
+        self.ctx = ctx
+        self.position = positions.to(device=ctx.device, dtype=torch.float64).contiguous()
+        n = self.position.shape[0]
+        self.id = torch.arange(n, dtype=torch.int64, device=ctx.device)
+        self.alive = torch.ones(n, dtype=torch.uint8, device=ctx.device)
+        self.columns = {k: v.to(ctx.device).contiguous() for k, v in columns.items()}
+        self.search = None
+
+    @property
+    def dimension(self):
+        return self.position.shape[1]
+
+    def size(self):
+        return self.position.shape[0]
+
+    def init_neighbour_search(self, low, high, periodic, n_particles_in_leaf=10.0):
+        self.search = CellListOrdered(self.ctx, low, high, periodic, n_particles_in_leaf)
+        self.update_positions()
+
+    def update_positions(self):
+        """update_positions + reorder (src/Particles.h:526-540, 694-732)."""
+        s = self.search
+        s.update_positions(self.position, self.alive)
+        self.position = s.positions().clone()
+        self.id = s.gather(self.id)
+        self.alive = torch.ones(s.n_alive, dtype=torch.uint8, device=self.ctx.device)
+        self.columns = {k: s.gather(v) for k, v in self.columns.items()}
+
+    def get_query(self):
+        if self.search is None:
+            raise RuntimeError("init_neighbour_search not called on this particle set")
+        return self.search
+
+
+class SearchResult:
+    """Neighbour lists in CSR form: row i is col[row_ptr[i]:row_ptr[i+1]]."""
+
+    def __init__(self, row_ptr, col):
+        self.row_ptr = row_ptr
+        self.col = col
+
+    @property
+    def total(self):
+        return self.col.numel()
+
+
+class Search:
+    """Reusable bulk search handle: ``run`` counts, ``fill`` writes indices
+    into a caller buffer.  Device buffers persist across runs."""
+
+    def __init__(self, query, centres, radius):
+        _require_cuda(centres, torch.float64, "centres")
+        self.ctx = query.ctx
+        h = ctypes.c_void_p()
+        check(C.lib().abh_search_create(self.ctx.handle, query.handle, centres.shape[0],
+                                        _ptr(centres), float(radius), ctypes.byref(h)),
+              self.ctx.handle)
+        self.handle = h
+        self.nq = centres.shape[0]
+
+    def run(self, query, centres, radius):
+        check(C.lib().abh_search_update(self.handle, query.handle, centres.shape[0],
+                                        _ptr(centres), float(radius)), self.ctx.handle)
+        self.nq = centres.shape[0]
+        return self.total
+
+    @property
+    def total(self):
+        t = ctypes.c_int64()
+        check(C.lib().abh_search_total(self.handle, ctypes.byref(t)))
+        return t.value
+
+    def row_ptr(self):
+        return _view(C.lib().abh_search_row_ptr(self.handle), (self.nq + 1,), torch.int64,
+                     self.ctx.device)
+
+    def fill(self, col):
+        check(C.lib().abh_search_fill(self.handle, _ptr(col), col.numel()), self.ctx.handle)
+        return col
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_search_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def euclidean_search(query, centres, radius):
+    """Bulk euclidean_search(query, centre, radius) for every row of centres
+    (a (nq, dim) float64 CUDA tensor)."""
+    _require_cuda(centres, torch.float64, "centres")
+    ctx = query.ctx
+    h = ctypes.c_void_p()
+    check(C.lib().abh_search_create(ctx.handle, query.handle, centres.shape[0], _ptr(centres),
+                                    float(radius), ctypes.byref(h)), ctx.handle)
+    try:
+        total = ctypes.c_int64()
+        check(C.lib().abh_search_total(h, ctypes.byref(total)))
+        rp = _view(C.lib().abh_search_row_ptr(h), (centres.shape[0] + 1,), torch.int64,
+                   ctx.device).clone()
+        col = torch.empty(total.value, dtype=torch.int32, device=ctx.device)
+        check(C.lib().abh_search_fill(h, _ptr(col), total.value), ctx.handle)
+        ctx.sync()
+    finally:
+        C.lib().abh_search_destroy(h)
+    return SearchResult(rp, col)
+
+
+class Kernel:
+    """Kernel descriptor: phi(|dx|) * (w0 + w_row*a_i + w_col*b_j)."""
+
+    def __init__(self, kind, param=0.0, w0=0.0, w_row=0.0, w_col=0.0):
+        self.kind = kind
+        self.param = float(param)
+        self.w0, self.w_row, self.w_col = float(w0), float(w_row), float(w_col)
+
+    def to_c(self):
+        return C.Kernel(self.kind, 0, self.param, self.w0, self.w_row, self.w_col)
+
+    @staticmethod
+    def constant(**w):
+        return Kernel(C.KERNEL_CONSTANT, 0.0, **w)
+
+    @staticmethod
+    def gaussian(scale, **w):
+        return Kernel(C.KERNEL_GAUSSIAN, scale, **w)
+
+    @staticmethod
+    def wendland_c2(support, **w):
+        return Kernel(C.KERNEL_WENDLAND_C2, support, **w)
+
+    @staticmethod
+    def multiquadric(c, **w):
+        return Kernel(C.KERNEL_MULTIQUADRIC, c, **w)
+
+    @staticmethod
+    def smooth_inv_r(eps, **w):
+        return Kernel(C.KERNEL_SMOOTH_INV_R, eps, **w)
+
+
+class SparseOperator:
+    """Matrix-free sparse kernel operator (KernelSparseConst)."""
+
+    def __init__(self, ctx, cols, row_positions, radius, kernel, row_scalar=None,
+                 col_scalar=None):
+        _require_cuda(row_positions, torch.float64, "row_positions")
+        self.ctx = ctx
+        self.cols = cols
+        self.nrows = row_positions.shape[0]
+        self.ncols = cols.n_alive
+        self._keep = (row_positions, row_scalar, col_scalar)
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_sparse_op_create(ctx.handle, cols.handle, self.nrows,
+                                           _ptr(row_positions), float(radius), ctypes.byref(kd),
+                                           _ptr(row_scalar), _ptr(col_scalar), ctypes.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    def apply(self, x, y, precision=FP64):
+        """y += K x.  x: (ncols,) or (ncols, nrhs); y likewise with nrows."""
+        x2 = x if x.dim() == 2 else x[:, None]
+        y2 = y if y.dim() == 2 else y[:, None]
+        # column-major storage: transpose views of (nrhs, n) contiguous tensors
+        xt = x2.t().contiguous()
+        yt = y2.t().contiguous()
+        nrhs = xt.shape[0]
+        check(C.lib().abh_sparse_op_apply(self.handle, nrhs, _ptr(xt), xt.shape[1], _ptr(yt),
+                                          yt.shape[1], precision), self.ctx.handle)
+        y2.copy_(yt.t())
+        return y
+
+    def apply_colmajor(self, xt, yt, precision=FP64):
+        """y^T += (K x)^T on (nrhs, n) contiguous tensors, no copies."""
+        check(C.lib().abh_sparse_op_apply(self.handle, xt.shape[0], _ptr(xt), xt.shape[1],
+                                          _ptr(yt), yt.shape[1], precision), self.ctx.handle)
+        return yt
+
+    def __matmul__(self, x):
+        shape = (self.nrows,) + tuple(x.shape[1:])
+        y = torch.zeros(shape, dtype=torch.float64, device=x.device)
+        return self.apply(x, y)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_sparse_op_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def create_sparse_operator(row_particles, col_particles, radius, kernel, row_scalar=None,
+                           col_scalar=None):
+    """create_sparse_operator(rows, cols, radius, f) -> SparseOperator."""
+    rs = row_particles.columns[row_scalar] if isinstance(row_scalar, str) else row_scalar
+    cs = col_particles.columns[col_scalar] if isinstance(col_scalar, str) else col_scalar
+    return SparseOperator(col_particles.ctx, col_particles.get_query(), row_particles.position,
+                          radius, kernel, rs, cs)

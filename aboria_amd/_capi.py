@@ -1,0 +1,134 @@
+"""ctypes declarations of libaboria_hip.so (include/aboria_hip.h).
+
+The library is loaded from this package directory only; there is no
+fallback implementation -- a missing library raises at import of the
+first object that needs it.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaboria_hip.so")
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_size_t = ctypes.c_size_t
+P = ctypes.POINTER
+
+ABH_OK = 0
+STATUS_NAMES = {
+    0: "ABH_OK",
+    1: "ABH_ERR_INVALID_ARG",
+    2: "ABH_ERR_HIP",
+    3: "ABH_ERR_OUT_OF_MEMORY",
+    4: "ABH_ERR_CAPACITY",
+    5: "ABH_ERR_UNSUPPORTED",
+    6: "ABH_ERR_DOMAIN",
+    7: "ABH_ERR_NOT_CONVERGED",
+}
+
+KERNEL_CONSTANT = 0
+KERNEL_GAUSSIAN = 1
+KERNEL_WENDLAND_C2 = 2
+KERNEL_MULTIQUADRIC = 3
+KERNEL_SMOOTH_INV_R = 4
+
+FP64 = 0
+FP32_COMPENSATED = 1
+
+
+class CellsInfo(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("grid", ctypes.c_uint32 * 3),
+        ("side", ctypes.c_double * 3),
+        ("low", ctypes.c_double * 3),
+        ("high", ctypes.c_double * 3),
+        ("periodic", ctypes.c_uint8 * 3),
+        ("n_buckets", ctypes.c_int64),
+        ("n_alive", ctypes.c_int64),
+    ]
+
+
+class Kernel(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("param", ctypes.c_double),
+        ("w0", ctypes.c_double),
+        ("w_row", ctypes.c_double),
+        ("w_col", ctypes.c_double),
+    ]
+
+
+class AbhError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = status
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, str(status)), message))
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "abh_context_create": (c_int, [c_int, c_void_p, P(c_void_p)]),
+    "abh_context_destroy": (c_int, [c_void_p]),
+    "abh_context_sync": (c_int, [c_void_p]),
+    "abh_status_string": (ctypes.c_char_p, [c_int]),
+    "abh_last_error": (ctypes.c_char_p, [c_void_p]),
+    "abh_malloc": (c_int, [c_void_p, P(c_void_p), c_size_t]),
+    "abh_free": (c_int, [c_void_p, c_void_p]),
+    "abh_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "abh_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "abh_memset": (c_int, [c_void_p, c_void_p, c_int, c_size_t]),
+    "abh_cells_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), P(ctypes.c_uint8),
+                                 c_double, P(c_void_p)]),
+    "abh_cells_destroy": (c_int, [c_void_p]),
+    "abh_cells_update": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, P(c_int64)]),
+    "abh_cells_get_info": (c_int, [c_void_p, P(CellsInfo)]),
+    "abh_cells_order": (c_void_p, [c_void_p]),
+    "abh_cells_bucket_index": (c_void_p, [c_void_p]),
+    "abh_cells_bucket_begin": (c_void_p, [c_void_p]),
+    "abh_cells_bucket_end": (c_void_p, [c_void_p]),
+    "abh_cells_positions": (c_void_p, [c_void_p]),
+    "abh_gather": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_size_t]),
+    "abh_search_create": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double, P(c_void_p)]),
+    "abh_search_update": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double]),
+    "abh_search_destroy": (c_int, [c_void_p]),
+    "abh_search_total": (c_int, [c_void_p, P(c_int64)]),
+    "abh_search_row_ptr": (c_void_p, [c_void_p]),
+    "abh_search_fill": (c_int, [c_void_p, c_void_p, c_int64]),
+    "abh_sparse_op_create": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double, P(Kernel),
+                                     c_void_p, c_void_p, P(c_void_p)]),
+    "abh_sparse_op_destroy": (c_int, [c_void_p]),
+    "abh_sparse_op_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libaboria_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libaboria_hip.so not found at %s: run `python -m aboria_amd.build` "
+                "(or __graft_entry__.build()) first" % LIB_PATH)
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(status, ctx=None):
+    if status != ABH_OK:
+        msg = ""
+        if ctx:
+            m = lib().abh_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise AbhError(status, msg)

@@ -1,0 +1,905 @@
+// Radius search and sparse kernel operator over a CellListOrdered.
+//
+// Work decomposition (MI355X): queries are grouped by home cell; one
+// wavefront takes up to 64 queries of one home cell, one query per lane.  The
+// candidate buckets are the (2w+1)^D stencil around the home cell (w = r /
+// side rounded up), each mapped to a bucket and a periodic image.  Stencil
+// slots are visited in ascending bucket order, so candidates stream through
+// the wave in ascending container index with wave-uniform addresses (scalar
+// loads, one broadcast per candidate) and every lane tests its own query.
+//
+// Exactness: for every (query, slot) the lane evaluates the predicate "the
+// reference's search iterator visits this bucket under this image, and how
+// many times" -- the quadrant starts, clamping and box-distance pruning of
+// lattice_iterator_within_distance (src/NeighbourSearchBase.h:1860-2008)
+// evaluated with the same FP64 expressions -- and accepts a candidate iff
+// that multiplicity is non-zero and sum_d dx_d^2 <= r^2 (src/Search.h:438-465).
+// Each lane also checks that the buckets the reference would visit all lie in
+// the stencil; if any does not (possible only within rounding of a cell face)
+// the host widens the stencil and repeats.  The neighbour multiset of every
+// query therefore equals the reference's.
+#include <cmath>
+#include <vector>
+
+#include "cells.hpp"
+
+using abh::DevBuf;
+using abh::GridParams;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxSlots = 343;
+constexpr int kGroup = 64;  // queries per work item = lanes per wave
+
+struct WorkItem {
+  int32_t home[3];
+  int32_t start;  // first query in sorted query order
+  int32_t count;  // <= 64
+  int32_t pad;
+};
+
+struct Slot {
+  int32_t b;    // collapsed bucket index, -1 = no bucket
+  int32_t beg;  // bucket_begin
+  int32_t len;  // bucket_end - bucket_begin
+  int32_t img;  // image index: sum_d (I_d + 1) * 3^d
+};
+
+template <int D> __device__ __forceinline__ int home_of(const GridParams &g, const double *q, int d) {
+  double t = floor((q[d] - g.lo[d]) * g.inv[d]);
+  // clamp to the virtual ring [-1, size]; NaN maps to -1
+  if (!(t >= -1.0)) t = -1.0;
+  if (t > double(g.size[d])) t = double(g.size[d]);
+  return int(t);
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_home_keys(const double *__restrict__ q, int64_t n,
+                                                        GridParams g, uint32_t *__restrict__ keys,
+                                                        int32_t *__restrict__ vals) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  double p[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) p[d] = q[i * D + d];
+  uint32_t key = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) key = key * uint32_t(g.size[d] + 2) + uint32_t(home_of<D>(g, p, d) + 1);
+  keys[i] = key;
+  if (vals) vals[i] = int32_t(i);
+}
+
+__global__ __launch_bounds__(kThreads) void k_run_flags(const uint32_t *__restrict__ keys,
+                                                        int64_t n, uint32_t *__restrict__ flags) {
+  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= n) return;
+  flags[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void k_run_starts(const uint32_t *__restrict__ flags,
+                                                         const uint32_t *__restrict__ ridx,
+                                                         int64_t n,
+                                                         uint32_t *__restrict__ run_start) {
+  const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= n || !flags[k]) return;
+  run_start[ridx[k]] = uint32_t(k);
+}
+
+__global__ __launch_bounds__(kThreads) void k_run_chunks(const uint32_t *__restrict__ run_start,
+                                                         const uint32_t *__restrict__ nruns_dev,
+                                                         int64_t n, uint32_t *__restrict__ chunks) {
+  const int64_t r = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (r >= n) return;
+  const int64_t nruns = int64_t(*nruns_dev);
+  if (r >= nruns) {
+    chunks[r] = 0;
+    return;
+  }
+  const int64_t s = run_start[r];
+  const int64_t e = (r + 1 < nruns) ? int64_t(run_start[r + 1]) : n;
+  chunks[r] = uint32_t((e - s + kGroup - 1) / kGroup);
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_fill_work(
+    const uint32_t *__restrict__ run_start, const uint32_t *__restrict__ nruns_dev, int64_t n,
+    const uint32_t *__restrict__ choff, const uint32_t *__restrict__ chunks,
+    const double *__restrict__ q_sorted, GridParams g, WorkItem *__restrict__ work) {
+  const int64_t r = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t nruns = int64_t(*nruns_dev);
+  if (r >= nruns) return;
+  const int64_t s = run_start[r];
+  const int64_t e = (r + 1 < nruns) ? int64_t(run_start[r + 1]) : n;
+  double p[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) p[d] = q_sorted[s * D + d];
+  WorkItem w{};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) w.home[d] = d < D ? home_of<D>(g, p, d) : 0;
+  const uint32_t nc = chunks[r];
+  for (uint32_t c = 0; c < nc; ++c) {
+    w.start = int32_t(s + int64_t(c) * kGroup);
+    const int64_t left = e - int64_t(w.start);
+    w.count = int32_t(left < kGroup ? left : kGroup);
+    work[choff[r] + c] = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reference predicates
+// ---------------------------------------------------------------------------
+
+// (x + 0.5)*side + lo as find_bucket_centre, then max(|c - q| - half, 0)^2
+__device__ __forceinline__ double box_term(const GridParams &g, int d, int x, double q) {
+  const double centre = (double(x) + 0.5) * g.side[d] + g.lo[d];
+  double t = fabs(centre - q) - g.half[d];
+  t = t < 0.0 ? 0.0 : t;  // std::max(t, 0.0) keeps t unless t < 0
+  return t * t;
+}
+
+// Per-lane quadrant state for one image point (reset_min_and_index,
+// src/NeighbourSearchBase.h:1860-1932).
+template <int D> struct QuadState {
+  int start[1 << D][D];
+  uint32_t valid;  // bit Q set = quadrant Q non-empty
+};
+
+template <int D>
+__device__ __forceinline__ bool outside_domain(const GridParams &g, const double *q, double r2) {
+  double acc = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double mid = 0.5 * (g.lo[d] + g.hi[d]);
+    const double halfL = 0.5 * g.L[d];
+    double t = fabs(mid - q[d]) - halfL;
+    t = t < 0.0 ? 0.0 : t;
+    acc = acc + t * t;
+  }
+  return acc > r2;
+}
+
+template <int D>
+__device__ __forceinline__ void quad_state(const GridParams &g, const double *q, double r2,
+                                           QuadState<D> &qs) {
+  int mu[D], md[D];
+  double gu[D], gd[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    mu[d] = int(floor((q[d] + 0.5 * g.side[d] - g.lo[d]) * g.inv[d]));
+    md[d] = int(floor((q[d] + (-0.5) * g.side[d] - g.lo[d]) * g.inv[d]));
+    gu[d] = box_term(g, d, mu[d], q[d]);
+    gd[d] = box_term(g, d, md[d], q[d]);
+  }
+  qs.valid = 0;
+#pragma unroll
+  for (int Q = 0; Q < (1 << D); ++Q) {
+    double acc = 0.0;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const bool up = (Q >> d) & 1;
+      acc = acc + (up ? gu[d] : gd[d]);
+    }
+    ok = !(acc > r2);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const bool up = (Q >> d) & 1;
+      int s;
+      if (up) {
+        s = mu[d] < 0 ? 0 : mu[d];
+        if (mu[d] > g.end_bucket[d]) ok = false;
+      } else {
+        s = md[d] > g.end_bucket[d] ? g.end_bucket[d] : md[d];
+        if (md[d] < 0) ok = false;
+      }
+      qs.start[Q][d] = s;
+    }
+    if (ok) qs.valid |= 1u << Q;
+  }
+}
+
+// Number of times the reference visits bucket b for this image point.
+template <int D>
+__device__ __forceinline__ int visit_count(const GridParams &g, const double *q, double r2,
+                                           const QuadState<D> &qs, const int *b) {
+  if (!qs.valid) return 0;
+  double dist = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) dist = dist + box_term(g, d, b[d], q[d]);
+  const bool near = !(dist > r2);
+  int m = 0;
+#pragma unroll
+  for (int Q = 0; Q < (1 << D); ++Q) {
+    if (!((qs.valid >> Q) & 1u)) continue;
+    bool in = true, is_start = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const bool up = (Q >> d) & 1;
+      const int s = qs.start[Q][d];
+      in = in && (up ? b[d] >= s : b[d] <= s);
+      is_start = is_start && (b[d] == s);
+    }
+    if (in && (is_start || near)) ++m;
+  }
+  return m;
+}
+
+// True if the buckets visited for this image point leave the stencil
+// [c_d - w_d, c_d + w_d] (bucket coordinates of the image).
+template <int D>
+__device__ __forceinline__ bool escapes_stencil(const GridParams &g, const double *q, double r2,
+                                                const QuadState<D> &qs, const int *c,
+                                                const int *w) {
+#pragma unroll
+  for (int Q = 0; Q < (1 << D); ++Q) {
+    if (!((qs.valid >> Q) & 1u)) continue;
+    int s[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      s[d] = qs.start[Q][d];
+      if (s[d] < c[d] - w[d] || s[d] > c[d] + w[d]) return true;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const bool up = (Q >> d) & 1;
+      const int e = up ? c[d] + w[d] + 1 : c[d] - w[d] - 1;
+      if (e < 0 || e > g.end_bucket[d]) continue;
+      if (up ? e < s[d] : e > s[d]) continue;
+      double dist = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) dist = dist + box_term(g, k, k == d ? e : s[k], q[k]);
+      if (!(dist > r2)) return true;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// kernel profiles
+// ---------------------------------------------------------------------------
+
+struct KernelDesc {
+  int32_t type;
+  double param, param2;  // param2 = param^2
+  double w0, wr, wc;
+  const double *row_s;
+  const double *col_s;
+};
+
+__device__ __forceinline__ double phi64(const KernelDesc &k, double s) {
+  switch (k.type) {
+    case ABH_KERNEL_CONSTANT: return 1.0;
+    case ABH_KERNEL_GAUSSIAN: return exp(-(s / k.param2));
+    case ABH_KERNEL_WENDLAND_C2: {
+      const double q = sqrt(s) / k.param;
+      if (!(q < 1.0)) return 0.0;
+      const double t = 1.0 - q;
+      const double t2 = t * t;
+      return (t2 * t2) * (4.0 * q + 1.0);
+    }
+    case ABH_KERNEL_MULTIQUADRIC: return sqrt(s + k.param2);
+    case ABH_KERNEL_SMOOTH_INV_R: return 1.0 / (sqrt(s) + k.param);
+  }
+  return 0.0;
+}
+
+__device__ __forceinline__ float phi32(const KernelDesc &k, float s) {
+  const float p = float(k.param);
+  switch (k.type) {
+    case ABH_KERNEL_CONSTANT: return 1.0f;
+    case ABH_KERNEL_GAUSSIAN: return expf(-(s / float(k.param2)));
+    case ABH_KERNEL_WENDLAND_C2: {
+      const float q = sqrtf(s) / p;
+      if (!(q < 1.0f)) return 0.0f;
+      const float t = 1.0f - q;
+      const float t2 = t * t;
+      return (t2 * t2) * (4.0f * q + 1.0f);
+    }
+    case ABH_KERNEL_MULTIQUADRIC: return sqrtf(s + float(k.param2));
+    case ABH_KERNEL_SMOOTH_INV_R: return 1.0f / (sqrtf(s) + p);
+  }
+  return 0.0f;
+}
+
+// Neumaier's variant of Kahan summation: sum + c carries the FP32 sum.
+__device__ __forceinline__ void kb_add(float &sum, float &c, float v) {
+  const float t = sum + v;
+  if (fabsf(sum) >= fabsf(v)) c += (sum - t) + v;
+  else c += (v - t) + sum;
+  sum = t;
+}
+
+// ---------------------------------------------------------------------------
+// the search kernel
+// ---------------------------------------------------------------------------
+
+enum Mode { kCount = 0, kFill = 1, kMatvec64 = 2, kMatvec32 = 3 };
+constexpr int kMaxRhs = 8;
+
+struct SearchArgs {
+  const WorkItem *work;
+  const uint32_t *nwork;
+  const double *qpos;   // sorted query positions
+  const int32_t *qperm; // sorted query -> original row (null = identity)
+  const uint32_t *bbeg, *bend;
+  const double *pos;    // particle positions in container order
+  double r2;
+  int w[3];
+  int nslots;
+  // count / fill
+  int64_t *counts;
+  const int64_t *row_ptr;
+  int32_t *col;
+  uint32_t *escape_flag;
+  // matvec
+  KernelDesc k;
+  int nrhs;
+  const double *x;
+  int64_t ldx;
+  double *y;
+  int64_t ldy;
+};
+
+template <int D, int MODE>
+__global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  Slot *slots = reinterpret_cast<Slot *>(lds);
+  int32_t *sorder = reinterpret_cast<int32_t *>(slots + a.nslots);
+  uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + a.nslots);
+  const int lane = threadIdx.x;
+  const int S = a.nslots;
+  const int nwork = int(*a.nwork);
+  int wd[3] = {a.w[0], a.w[1], a.w[2]};
+  int span[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) span[d] = 2 * wd[d] + 1;
+
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const WorkItem it = a.work[wi];
+    const bool active = lane < it.count;
+    const int qk = it.start + (active ? lane : 0);
+    double q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = a.qpos[int64_t(qk) * D + d];
+    const int64_t row = a.qperm ? int64_t(a.qperm[qk]) : int64_t(qk);
+
+    // stencil slots: virtual cell v = home + o -> bucket b under image I
+    for (int s = lane; s < S; s += kGroup) {
+      int o[3] = {0, 0, 0};
+      int rem = s;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        o[d] = rem % span[d] - wd[d];
+        rem /= span[d];
+      }
+      bool ok = true;
+      int bv[3] = {0, 0, 0};
+      int imgc = 0, p3 = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int v = it.home[d] + o[d];
+        const int n = g.size[d];
+        int I = 0, b = v;
+        if (v < 0) { b = v + n; I = 1; }
+        else if (v >= n) { b = v - n; I = -1; }
+        if (I != 0 && !g.periodic[d]) ok = false;
+        if (b < 0 || b >= n) ok = false;  // would need an image beyond +-1
+        bv[d] = b;
+        imgc += (I + 1) * p3;
+        p3 *= 3;
+      }
+      if (ok) {
+        uint32_t index = 0, m = 1;
+#pragma unroll
+        for (int i = D - 1; i >= 0; --i) {
+          if (i != D - 1) m *= uint32_t(g.size[i + 1]);
+          index += m * uint32_t(bv[i]);
+        }
+        const int bidx = int(index);
+        const int beg = int(a.bbeg[bidx]);
+        const int end = int(a.bend[bidx]);
+        slots[s] = Slot{bidx, beg, end - beg, imgc};
+      } else {
+        slots[s] = Slot{-1, 0, 0, imgc};
+      }
+    }
+    __syncthreads();
+    // ascending bucket order (ties by slot index)
+    for (int s = lane; s < S; s += kGroup) {
+      const int bs = slots[s].b;
+      int rank = 0;
+      for (int t = 0; t < S; ++t) {
+        const int bt = slots[t].b;
+        rank += (bt < bs || (bt == bs && t < s)) ? 1 : 0;
+      }
+      sorder[rank] = s;
+    }
+    // per-lane visit multiplicities, one image at a time
+    bool escape = false;
+    for (int s = 0; s < S; ++s) mult[s * kGroup + lane] = 0;
+    {
+      int nimg = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) nimg *= 3;
+      for (int ii = 0; ii < nimg; ++ii) {
+        int I[D];
+        bool img_ok = true;
+        int t = ii;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          I[d] = t % 3 - 1;
+          t /= 3;
+          if (I[d] != 0 && !g.periodic[d]) img_ok = false;
+        }
+        if (!img_ok) continue;  // uniform
+        double qi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) qi[d] = q[d] + double(I[d]) * g.L[d];
+        const bool in_range = active && !outside_domain<D>(g, qi, a.r2);
+        if (__ballot(in_range) == 0) continue;
+        QuadState<D> qs;
+        if (in_range) {
+          quad_state<D>(g, qi, a.r2, qs);
+        } else {
+          qs.valid = 0;
+        }
+        if (MODE == kCount && in_range) {
+          int c[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[d] = it.home[d] + I[d] * g.size[d];
+          if (escapes_stencil<D>(g, qi, a.r2, qs, c, wd)) escape = true;
+        }
+        for (int s = 0; s < S; ++s) {
+          const Slot sl = slots[s];
+          if (sl.img != ii || sl.b < 0) continue;
+          int b[D];
+          int rem = sl.b;
+#pragma unroll
+          for (int d = D - 1; d >= 0; --d) {
+            b[d] = rem % g.size[d];
+            rem /= g.size[d];
+          }
+          const int m = in_range ? visit_count<D>(g, qi, a.r2, qs, b) : 0;
+          mult[s * kGroup + lane] = uint8_t(m);
+        }
+      }
+    }
+    if (MODE == kCount && escape) atomicOr(a.escape_flag, 1u);
+    __syncthreads();
+
+    int64_t cnt = 0;
+    int64_t out = 0;
+    if (MODE == kFill && active) out = a.row_ptr[row];
+    double acc64[kMaxRhs];
+    float acc32[kMaxRhs], cmp32[kMaxRhs];
+    if (MODE == kMatvec64 || MODE == kMatvec32) {
+#pragma unroll
+      for (int r = 0; r < kMaxRhs; ++r) {
+        acc64[r] = (active && r < a.nrhs) ? a.y[int64_t(r) * a.ldy + row] : 0.0;
+        acc32[r] = 0.0f;
+        cmp32[r] = 0.0f;
+      }
+    }
+    const double rs = (MODE >= kMatvec64 && a.k.row_s && active) ? a.k.row_s[row] : 0.0;
+
+    for (int k = 0; k < S; ++k) {
+      const int s = abh::uniform_i32(sorder[k]);
+      const int len = abh::uniform_i32(slots[s].len);
+      if (len <= 0) continue;
+      const int beg = abh::uniform_i32(slots[s].beg);
+      const int imgc = abh::uniform_i32(slots[s].img);
+      const int m = active ? int(mult[s * kGroup + lane]) : 0;
+      if (__ballot(m != 0) == 0) continue;
+      double qi[D];
+      {
+        int t = imgc;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const int I = t % 3 - 1;
+          t /= 3;
+          qi[d] = q[d] + double(I) * g.L[d];
+        }
+      }
+      for (int j = beg; j < beg + len; ++j) {
+        double acc = 0.0;
+        double dxv[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dxv[d] = a.pos[int64_t(j) * D + d] - qi[d];
+          acc = acc + dxv[d] * dxv[d];
+        }
+        const bool hit = m != 0 && !(acc > a.r2);
+        if (MODE == kCount) {
+          if (hit) cnt += m;
+        } else if (MODE == kFill) {
+          if (hit) {
+            for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j;
+            cnt += m;
+          }
+        } else if (MODE == kMatvec64) {
+          if (hit) {
+            double w = 1.0;
+            if (a.k.row_s || a.k.col_s) {
+              w = a.k.w0;
+              if (a.k.row_s) w = w + a.k.wr * rs;
+              if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j];
+            }
+            const double f = phi64(a.k, acc) * w;
+            for (int rep = 0; rep < m; ++rep) {
+#pragma unroll
+              for (int r = 0; r < kMaxRhs; ++r)
+                if (r < a.nrhs) acc64[r] += f * a.x[int64_t(r) * a.ldx + j];
+            }
+          }
+        } else {  // kMatvec32
+          if (hit) {
+            float w = 1.0f;
+            if (a.k.row_s || a.k.col_s) {
+              double wd64 = a.k.w0;
+              if (a.k.row_s) wd64 = wd64 + a.k.wr * rs;
+              if (a.k.col_s) wd64 = wd64 + a.k.wc * a.k.col_s[j];
+              w = float(wd64);
+            }
+            const float f = phi32(a.k, float(acc)) * w;
+            for (int rep = 0; rep < m; ++rep) {
+#pragma unroll
+              for (int r = 0; r < kMaxRhs; ++r)
+                if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(a.x[int64_t(r) * a.ldx + j]));
+            }
+          }
+        }
+      }
+    }
+    if (active) {
+      if (MODE == kCount) a.counts[row] = cnt;
+      if (MODE == kMatvec64) {
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r)
+          if (r < a.nrhs) a.y[int64_t(r) * a.ldy + row] = acc64[r];
+      }
+      if (MODE == kMatvec32) {
+#pragma unroll
+        for (int r = 0; r < kMaxRhs; ++r)
+          if (r < a.nrhs)
+            a.y[int64_t(r) * a.ldy + row] =
+                acc64[r] + (double(acc32[r]) + double(cmp32[r]));
+      }
+    }
+    __syncthreads();  // slots / mult reused by the next work item
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// plan (grouping of queries into work items), shared by search and operator
+// ---------------------------------------------------------------------------
+
+namespace abh {
+
+struct QueryPlan {
+  abh_context *ctx = nullptr;
+  const abh_cells *cells = nullptr;
+  int D = 3;
+  int64_t nq = 0;
+  bool self = false;
+  double radius = 0, r2 = 0;
+  int w[3] = {0, 0, 0};
+  int nslots = 1;
+  const double *q_sorted = nullptr;
+  DevBuf qbuf, perm, keys, flags, ridx, run_start, chunks, choff, work, counters, escape;
+
+  const int32_t *qperm() const { return self ? nullptr : perm.as<int32_t>(); }
+};
+
+static void set_stencil(QueryPlan &p, int extra) {
+  const abh_cells *c = p.cells;
+  p.nslots = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < p.D) {
+      // w = ceil(r / side), shaved by 1e-9 so r == side gives 1; rounding
+      // at the boundary is caught by the escape check.
+      int w = int(std::ceil(p.radius * c->inv[d] * (1.0 - 1e-9)));
+      if (w < 0) w = 0;
+      w += extra;
+      // virtual cells needed lie within [-size, 2*size): a wider stencil
+      // only adds slots that need images beyond +-1
+      if (w > 2 * int(c->size[d]) + 1) w = 2 * int(c->size[d]) + 1;
+      p.w[d] = w;
+      p.nslots *= 2 * w + 1;
+    } else {
+      p.w[d] = 0;
+    }
+  }
+  ABH_REQUIRE(p.nslots <= kMaxSlots, ABH_ERR_UNSUPPORTED,
+              "search radius spans too many buckets (stencil > 343 cells); "
+              "use fewer particles per bucket or a smaller radius");
+}
+
+template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
+  abh_context *ctx = p.ctx;
+  hipStream_t s = ctx->stream;
+  const GridParams g = p.cells->params();
+  const int64_t n = p.nq;
+  const unsigned blocks = unsigned(ceil_div(n, kThreads));
+  p.keys.ensure(size_t(n + 1) * 4);
+  uint32_t *keys = p.keys.as<uint32_t>();
+  if (p.self) {
+    p.q_sorted = q;
+    k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, keys, nullptr);
+    check_launch();
+  } else {
+    p.perm.ensure(size_t(n + 1) * 4);
+    k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, keys, p.perm.as<int32_t>());
+    check_launch();
+    uint64_t maxkey = 1;
+    for (int d = 0; d < D; ++d) maxkey *= uint64_t(g.size[d] + 2);
+    int bits = 0;
+    while (bits < 32 && ((maxkey - 1) >> bits) != 0) ++bits;
+    radix_sort_pairs(ctx, keys, p.perm.as<int32_t>(), n, bits);
+    p.qbuf.ensure(size_t(n + 1) * D * 8);
+    gather_bytes(ctx, p.perm.as<int32_t>(), n, q, p.qbuf.p, size_t(D) * 8);
+    p.q_sorted = p.qbuf.as<double>();
+  }
+  p.flags.ensure(size_t(n + 1) * 4);
+  p.ridx.ensure(size_t(n + 1) * 4);
+  p.run_start.ensure(size_t(n + 1) * 4);
+  p.chunks.ensure(size_t(n + 1) * 4);
+  p.choff.ensure(size_t(n + 1) * 4);
+  p.work.ensure(size_t(n + 1) * sizeof(WorkItem));
+  p.counters.ensure(16);
+  uint32_t *ctr = p.counters.as<uint32_t>();  // [0] = runs, [1] = work items
+  k_run_flags<<<blocks, kThreads, 0, s>>>(keys, n, p.flags.as<uint32_t>());
+  check_launch();
+  scan_exclusive(ctx, p.flags.as<uint32_t>(), p.ridx.as<uint32_t>(), n, ctr + 0);
+  k_run_starts<<<blocks, kThreads, 0, s>>>(p.flags.as<uint32_t>(), p.ridx.as<uint32_t>(), n,
+                                           p.run_start.as<uint32_t>());
+  check_launch();
+  k_run_chunks<<<blocks, kThreads, 0, s>>>(p.run_start.as<uint32_t>(), ctr + 0, n,
+                                           p.chunks.as<uint32_t>());
+  check_launch();
+  scan_exclusive(ctx, p.chunks.as<uint32_t>(), p.choff.as<uint32_t>(), n, ctr + 1);
+  k_fill_work<D><<<blocks, kThreads, 0, s>>>(p.run_start.as<uint32_t>(), ctr + 0, n,
+                                             p.choff.as<uint32_t>(), p.chunks.as<uint32_t>(),
+                                             p.q_sorted, g, p.work.as<WorkItem>());
+  check_launch();
+}
+
+static void build_plan(QueryPlan &p, abh_context *ctx, const abh_cells *cells, int64_t nq,
+                       const double *q, double radius) {
+  ABH_REQUIRE(cells && cells->ctx == ctx, ABH_ERR_INVALID_ARG, "cells belong to another context");
+  ABH_REQUIRE(nq >= 0 && nq < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "nq out of range");
+  ABH_REQUIRE(nq == 0 || q, ABH_ERR_INVALID_ARG, "query positions are null");
+  ABH_REQUIRE(radius >= 0 && std::isfinite(radius), ABH_ERR_INVALID_ARG, "radius must be finite");
+  p.ctx = ctx;
+  p.cells = cells;
+  p.D = cells->dim;
+  p.nq = nq;
+  p.radius = radius;
+  p.r2 = radius * radius;  // distance_helper<2>::get_value_to_accumulate
+  p.self = (q == cells->pos.as<double>() && nq == cells->n_alive);
+  set_stencil(p, 0);
+  if (nq == 0) return;
+  if (p.D == 1) build_plan_d<1>(p, q);
+  else if (p.D == 2) build_plan_d<2>(p, q);
+  else build_plan_d<3>(p, q);
+}
+
+static SearchArgs base_args(const QueryPlan &p) {
+  SearchArgs a{};
+  a.work = p.work.as<WorkItem>();
+  a.nwork = p.counters.as<uint32_t>() + 1;
+  a.qpos = p.q_sorted;
+  a.qperm = p.qperm();
+  a.bbeg = p.cells->begin.as<uint32_t>();
+  a.bend = p.cells->end.as<uint32_t>();
+  a.pos = p.cells->pos.as<double>();
+  a.r2 = p.r2;
+  for (int d = 0; d < 3; ++d) a.w[d] = p.w[d];
+  a.nslots = p.nslots;
+  return a;
+}
+
+static size_t lds_bytes(int nslots) {
+  return size_t(nslots) * (sizeof(Slot) + 4 + kGroup);
+}
+
+template <int MODE> static void launch_search(const QueryPlan &p, const SearchArgs &a) {
+  const GridParams g = p.cells->params();
+  const unsigned grid = unsigned(std::min<int64_t>(ceil_div(p.nq, kGroup), int64_t(p.ctx->n_cu) * 32));
+  const size_t lds = lds_bytes(a.nslots);
+  if (grid == 0) return;
+  if (p.D == 1) k_search<1, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
+  else if (p.D == 2) k_search<2, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
+  else k_search<3, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
+  check_launch();
+}
+
+// Counts neighbours into counts[nq] (device), widening the stencil until no
+// query's visited buckets escape it.
+static void count_with_escape(QueryPlan &p, int64_t *counts) {
+  p.escape.ensure(4);
+  for (int extra = 0;; ++extra) {
+    if (extra > 0) set_stencil(p, extra);
+    ABH_HIP(hipMemsetAsync(p.escape.p, 0, 4, p.ctx->stream));
+    SearchArgs a = base_args(p);
+    a.counts = counts;
+    a.escape_flag = p.escape.as<uint32_t>();
+    launch_search<kCount>(p, a);
+    uint32_t esc = 0;
+    ABH_HIP(hipMemcpyAsync(&esc, p.escape.p, 4, hipMemcpyDeviceToHost, p.ctx->stream));
+    ABH_HIP(hipStreamSynchronize(p.ctx->stream));
+    if (!esc) return;
+    ABH_REQUIRE(extra < 4, ABH_ERR_UNSUPPORTED, "search stencil failed to cover visited buckets");
+  }
+}
+
+}  // namespace abh
+
+struct abh_search {
+  abh::QueryPlan plan;
+  abh::DevBuf row_ptr, total_buf;
+  int64_t total = 0;
+};
+
+struct abh_sparse_op {
+  abh::QueryPlan plan;
+  KernelDesc k{};
+  int64_t nnz = 0;
+  int64_t ncols = 0;
+};
+
+extern "C" {
+
+abh_status abh_search_update(abh_search *s, const abh_cells *cells, int64_t nq,
+                             const double *q, double radius) {
+  if (!s) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = s->plan.ctx;
+  return abh::guard(ctx, [&] {
+    abh::build_plan(s->plan, ctx, cells, nq, q, radius);
+    s->row_ptr.ensure(size_t(nq + 1) * 8);
+    s->total_buf.ensure(8);
+    int64_t *rp = s->row_ptr.as<int64_t>();
+    ABH_HIP(hipMemsetAsync(rp + nq, 0, 8, ctx->stream));
+    if (nq > 0) abh::count_with_escape(s->plan, rp);
+    int64_t *total_dev = s->total_buf.as<int64_t>();
+    abh::scan_exclusive(ctx, rp, rp, nq + 1, total_dev);
+    int64_t total = 0;
+    ABH_HIP(hipMemcpyAsync(&total, total_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+    ABH_HIP(hipStreamSynchronize(ctx->stream));
+    s->total = total;
+  });
+}
+
+abh_status abh_search_create(abh_context *ctx, const abh_cells *cells, int64_t nq,
+                             const double *q, double radius, abh_search **out) {
+  if (!ctx || !out) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_search *s = new (std::nothrow) abh_search();
+  if (!s) return ABH_ERR_OUT_OF_MEMORY;
+  s->plan.ctx = ctx;
+  abh_status st = abh_search_update(s, cells, nq, q, radius);
+  if (st != ABH_OK) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  return ABH_OK;
+}
+
+abh_status abh_search_destroy(abh_search *s) {
+  if (!s) return ABH_OK;
+  if (s->plan.ctx) {
+    (void)hipSetDevice(s->plan.ctx->device);
+    (void)hipStreamSynchronize(s->plan.ctx->stream);
+  }
+  delete s;
+  return ABH_OK;
+}
+
+abh_status abh_search_total(const abh_search *s, int64_t *total) {
+  if (!s || !total) return ABH_ERR_INVALID_ARG;
+  *total = s->total;
+  return ABH_OK;
+}
+
+const int64_t *abh_search_row_ptr(const abh_search *s) {
+  return s ? s->row_ptr.as<int64_t>() : nullptr;
+}
+
+abh_status abh_search_fill(abh_search *s, int32_t *col, int64_t capacity) {
+  if (!s) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = s->plan.ctx;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(capacity >= s->total, ABH_ERR_CAPACITY, "neighbour index buffer too small");
+    ABH_REQUIRE(s->total == 0 || col, ABH_ERR_INVALID_ARG, "col is null");
+    if (s->plan.nq == 0 || s->total == 0) return;
+    SearchArgs a = abh::base_args(s->plan);
+    a.row_ptr = s->row_ptr.as<int64_t>();
+    a.col = col;
+    abh::launch_search<kFill>(s->plan, a);
+  });
+}
+
+abh_status abh_sparse_op_create(abh_context *ctx, const abh_cells *cols, int64_t nrows,
+                                const double *row_pos, double radius, const abh_kernel *kernel,
+                                const double *row_scalar, const double *col_scalar,
+                                abh_sparse_op **out) {
+  if (!ctx || !out || !kernel) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_sparse_op *op = new (std::nothrow) abh_sparse_op();
+  if (!op) return ABH_ERR_OUT_OF_MEMORY;
+  abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    abh::build_plan(op->plan, ctx, cols, nrows, row_pos, radius);
+    op->ncols = cols->n_alive;
+    op->k.type = kernel->type;
+    op->k.param = kernel->param;
+    op->k.param2 = kernel->param * kernel->param;
+    op->k.w0 = kernel->w0;
+    op->k.wr = kernel->w_row;
+    op->k.wc = kernel->w_col;
+    op->k.row_s = row_scalar;
+    op->k.col_s = col_scalar;
+    if (nrows > 0) {
+      // the count pass fixes the stencil (escape check) and gives nnz
+      abh::DevBuf counts;
+      counts.ensure(size_t(nrows + 1) * 8);
+      int64_t *c = counts.as<int64_t>();
+      ABH_HIP(hipMemsetAsync(c + nrows, 0, 8, ctx->stream));
+      abh::count_with_escape(op->plan, c);
+      abh::DevBuf total_buf;
+      total_buf.ensure(8);
+      int64_t *total_dev = total_buf.as<int64_t>();
+      abh::scan_exclusive(ctx, c, c, nrows + 1, total_dev);
+      ABH_HIP(hipMemcpyAsync(&op->nnz, total_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+      ABH_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  });
+  if (st != ABH_OK) {
+    delete op;
+    return st;
+  }
+  *out = op;
+  return ABH_OK;
+}
+
+abh_status abh_sparse_op_destroy(abh_sparse_op *op) {
+  if (!op) return ABH_OK;
+  if (op->plan.ctx) {
+    (void)hipSetDevice(op->plan.ctx->device);
+    (void)hipStreamSynchronize(op->plan.ctx->stream);
+  }
+  delete op;
+  return ABH_OK;
+}
+
+abh_status abh_sparse_op_apply(abh_sparse_op *op, int nrhs, const double *x, int64_t ldx,
+                               double *y, int64_t ldy, abh_precision precision) {
+  if (!op) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = op->plan.ctx;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(nrhs >= 0, ABH_ERR_INVALID_ARG, "nrhs must be >= 0");
+    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
+                "unknown precision");
+    if (nrhs == 0 || op->plan.nq == 0) return;
+    ABH_REQUIRE(x && y, ABH_ERR_INVALID_ARG, "x or y is null");
+    ABH_REQUIRE(ldx >= op->ncols && ldy >= op->plan.nq, ABH_ERR_INVALID_ARG,
+                "leading dimension smaller than the operator");
+    for (int r0 = 0; r0 < nrhs; r0 += kMaxRhs) {
+      SearchArgs a = abh::base_args(op->plan);
+      a.k = op->k;
+      a.nrhs = std::min(kMaxRhs, nrhs - r0);
+      a.x = x + int64_t(r0) * ldx;
+      a.ldx = ldx;
+      a.y = y + int64_t(r0) * ldy;
+      a.ldy = ldy;
+      if (precision == ABH_FP64) abh::launch_search<kMatvec64>(op->plan, a);
+      else abh::launch_search<kMatvec32>(op->plan, a);
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+/*
+ * aboria_hip.h -- C ABI of libaboria_hip.so, the MI355X (gfx950) particle
+ * hot path: CellListOrdered build, radius search, sparse / dense kernel
+ * operators, Chebyshev black-box FMM and the Schwarz-preconditioned GMRES.
+ *
+ * Conventions
+ *   - Every entry point returns an abh_status; nothing throws across the ABI.
+ *     abh_last_error() gives a message for the last failure on a context.
+ *   - Pointers documented as "device" are HIP device pointers (hipMalloc,
+ *     torch CUDA tensors, abh_malloc).  Host pointers are marked "host".
+ *   - Positions are double[n][dim], row-major (the layout of Aboria's
+ *     position column, a contiguous array of Vector<double,D>).
+ *   - All work is enqueued on the context's stream.  Calls that must return a
+ *     size to the host (number of alive particles, number of neighbour pairs)
+ *     synchronise that stream, as the reference reads the same values on the
+ *     host.
+ *   - Handles are not thread-safe; use one context per host thread.
+ *
+ * Reference interfaces each group replaces are cited per declaration
+ * (paths are relative to the Aboria source tree).
+ */
+#ifndef ABORIA_HIP_H_
+#define ABORIA_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status / context                                                    */
+/* ------------------------------------------------------------------ */
+
+typedef enum abh_status {
+  ABH_OK = 0,
+  ABH_ERR_INVALID_ARG = 1,   /* bad dimension, null pointer, wrong size      */
+  ABH_ERR_HIP = 2,           /* a HIP runtime call failed                    */
+  ABH_ERR_OUT_OF_MEMORY = 3, /* device allocation failed                     */
+  ABH_ERR_CAPACITY = 4,      /* caller-provided output buffer too small      */
+  ABH_ERR_UNSUPPORTED = 5,   /* kernel type / precision not implemented      */
+  ABH_ERR_DOMAIN = 6,        /* position too far outside a periodic domain   */
+  ABH_ERR_NOT_CONVERGED = 7  /* iterative solve hit its iteration limit      */
+} abh_status;
+
+typedef struct abh_context abh_context;
+
+/* Creates a context on HIP device `device`.  `stream` is a hipStream_t to
+ * enqueue on (NULL = the context creates and owns one). */
+abh_status abh_context_create(int device, void *stream, abh_context **out);
+abh_status abh_context_destroy(abh_context *ctx);
+abh_status abh_context_sync(abh_context *ctx);
+const char *abh_status_string(abh_status s);
+const char *abh_last_error(const abh_context *ctx);
+
+/* Device memory helpers, so a host without HIP headers (the C++ API layer,
+ * a cgo / ctypes binding) can own device buffers.  Copies are synchronous
+ * with respect to the context stream. */
+abh_status abh_malloc(abh_context *ctx, void **ptr, size_t bytes);
+abh_status abh_free(abh_context *ctx, void *ptr);
+abh_status abh_memcpy_h2d(abh_context *ctx, void *dst_dev, const void *src_host,
+                          size_t bytes);
+abh_status abh_memcpy_d2h(abh_context *ctx, void *dst_host, const void *src_dev,
+                          size_t bytes);
+abh_status abh_memset(abh_context *ctx, void *dst_dev, int value, size_t bytes);
+
+/* ------------------------------------------------------------------ */
+/* CellListOrdered                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_cells abh_cells;
+
+typedef struct abh_cells_info {
+  int32_t dim;
+  uint32_t grid[3];     /* buckets per dimension (m_size)                   */
+  double side[3];       /* bucket side length (m_bucket_side_length)        */
+  double low[3];        /* domain lower bound                               */
+  double high[3];       /* domain upper bound                               */
+  uint8_t periodic[3];  /* periodicity per dimension                        */
+  int64_t n_buckets;    /* prod(grid)                                       */
+  int64_t n_alive;      /* particles kept by the last update                */
+} abh_cells_info;
+
+/* Replaces neighbour_search_base::set_domain (src/NeighbourSearchBase.h:252-268)
+ * as called by Particles::init_neighbour_search (src/Particles.h:445-455).
+ * dim in {1,2,3}.  low/high/periodic are host arrays of length dim. */
+abh_status abh_cells_create(abh_context *ctx, int dim, const double *low,
+                            const double *high, const uint8_t *periodic,
+                            double n_particles_in_leaf, abh_cells **out);
+abh_status abh_cells_destroy(abh_cells *cells);
+
+/* Replaces neighbour_search_base::update_positions + CellListOrdered::
+ * update_positions_impl (src/NeighbourSearchBase.h:350-495,
+ * src/CellListOrdered.h:190-259).
+ *   pos   device, n x dim, read and written: periodic dimensions are wrapped
+ *         in place (enforce_domain_lambda, src/NeighbourSearchBase.h:185-238)
+ *   alive device, n bytes, may be NULL (= all alive); written in place:
+ *         non-finite positions and positions outside a non-periodic domain
+ *         are marked dead.
+ * On return the handle holds the gather order (m_alive_indices sorted by
+ * bucket, stable), the bucket index of every kept particle, bucket
+ * begin/end and the positions in gather order.  *n_alive_out (host) gets the
+ * number of kept particles; this synchronises the stream. */
+abh_status abh_cells_update(abh_cells *cells, int64_t n, double *pos,
+                            uint8_t *alive, int64_t *n_alive_out);
+
+abh_status abh_cells_get_info(const abh_cells *cells, abh_cells_info *info);
+
+/* Device arrays owned by the handle, valid until the next update/destroy. */
+const int32_t *abh_cells_order(const abh_cells *cells);        /* n_alive   */
+const uint32_t *abh_cells_bucket_index(const abh_cells *cells); /* n_alive  */
+const uint32_t *abh_cells_bucket_begin(const abh_cells *cells); /* n_buckets*/
+const uint32_t *abh_cells_bucket_end(const abh_cells *cells);   /* n_buckets*/
+const double *abh_cells_positions(const abh_cells *cells); /* n_alive x dim */
+
+/* dst[k] = src[order[k]] for k < n, elements of elem_bytes bytes (the
+ * per-column gather of Particles::reorder, src/Particles.h:694-732). */
+abh_status abh_gather(abh_context *ctx, const int32_t *order, int64_t n,
+                      const void *src, void *dst, size_t elem_bytes);
+
+/* ------------------------------------------------------------------ */
+/* radius search (euclidean_search, bulk form)                         */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_search abh_search;
+
+/* Bulk replacement of euclidean_search(query, centre, r)
+ * (src/Search.h:839-845) for every centre q[i], i < nq (device, nq x dim).
+ * The neighbour set of centre i is exactly the set the reference search
+ * iterator yields: every particle of the cell list, under every periodic
+ * image the reference visits, with sum_d dx_d^2 <= r^2 in FP64.
+ * Creating the handle counts the neighbours (synchronises). */
+abh_status abh_search_create(abh_context *ctx, const abh_cells *cells,
+                             int64_t nq, const double *q, double radius,
+                             abh_search **out);
+/* Re-runs the search with new centres / cells / radius, reusing the handle's
+ * device buffers (no allocation once they are large enough). */
+abh_status abh_search_update(abh_search *s, const abh_cells *cells, int64_t nq,
+                             const double *q, double radius);
+abh_status abh_search_destroy(abh_search *s);
+abh_status abh_search_total(const abh_search *s, int64_t *total_pairs);
+/* device, nq+1 exclusive offsets into the neighbour index array */
+const int64_t *abh_search_row_ptr(const abh_search *s);
+/* Writes the neighbour indices (positions in the cell list's gather order,
+ * i.e. container indices after reorder) of centre i to
+ * col[row_ptr[i] .. row_ptr[i+1]).  Within a row the indices are ascending
+ * except where one bucket is reached through two periodic images.
+ * col is device memory of at least `capacity` entries. */
+abh_status abh_search_fill(abh_search *s, int32_t *col, int64_t capacity);
+
+/* ------------------------------------------------------------------ */
+/* kernel descriptors and sparse operator                              */
+/* ------------------------------------------------------------------ */
+
+typedef enum abh_kernel_type {
+  ABH_KERNEL_CONSTANT = 0,     /* phi = 1                                   */
+  ABH_KERNEL_GAUSSIAN = 1,     /* phi = exp(-|dx|^2 / param^2)               */
+  ABH_KERNEL_WENDLAND_C2 = 2,  /* q=|dx|/param, phi=(1-q)^4(4q+1) for q<1    */
+  ABH_KERNEL_MULTIQUADRIC = 3, /* phi = sqrt(|dx|^2 + param^2)               */
+  ABH_KERNEL_SMOOTH_INV_R = 4  /* phi = 1 / (|dx| + param)                   */
+} abh_kernel_type;
+
+/* The reference takes an arbitrary C++ functor f(dx, a, b)
+ * (src/Operators.h:472-516).  Across the C ABI a kernel is a radial profile
+ * times an optional per-particle affine weight:
+ *   f(dx, a_i, b_j) = phi(|dx|) * (w0 + w_row * row_scalar[i]
+ *                                      + w_col * col_scalar[j])
+ * when scalars are supplied, and phi(|dx|) otherwise. */
+typedef struct abh_kernel {
+  int32_t type;
+  int32_t reserved;
+  double param;
+  double w0, w_row, w_col;
+} abh_kernel;
+
+typedef enum abh_precision {
+  ABH_FP64 = 0,               /* FP64 kernel and accumulation               */
+  ABH_FP32_COMPENSATED = 1    /* FP64 radius test, FP32 kernel, FP32 sum
+                                 with Kahan-Babuska compensation            */
+} abh_precision;
+
+typedef struct abh_sparse_op abh_sparse_op;
+
+/* Replaces create_sparse_operator(rows, cols, radius, f)
+ * (src/Operators.h:508-516, KernelSparseConst src/Kernels.h:754-776).
+ * row_pos: device, nrows x dim.  row_scalar (nrows) / col_scalar (n_alive of
+ * cols) are device arrays or NULL. */
+abh_status abh_sparse_op_create(abh_context *ctx, const abh_cells *cols,
+                                int64_t nrows, const double *row_pos,
+                                double radius, const abh_kernel *kernel,
+                                const double *row_scalar,
+                                const double *col_scalar, abh_sparse_op **out);
+abh_status abh_sparse_op_destroy(abh_sparse_op *op);
+/* y += K x for nrhs right-hand sides (KernelSparse::evaluate,
+ * src/Kernels.h:720-751).  x: device, column-major ncols x nrhs with leading
+ * dimension ldx; y: device, nrows x nrhs, leading dimension ldy. */
+abh_status abh_sparse_op_apply(abh_sparse_op *op, int nrhs, const double *x,
+                               int64_t ldx, double *y, int64_t ldy,
+                               abh_precision precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ABORIA_HIP_H_ */

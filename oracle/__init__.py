@@ -1,0 +1,158 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; never by the product package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+P = ctypes.POINTER
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class KernelC(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("reserved", ctypes.c_int32), ("param", ctypes.c_double),
+                ("w0", ctypes.c_double), ("w_row", ctypes.c_double), ("w_col", ctypes.c_double)]
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        l = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        l.orc_cells_create.restype = vp
+        l.orc_cells_create.argtypes = [ctypes.c_int, _dp, _dp, _u8p, ctypes.c_double]
+        l.orc_cells_destroy.argtypes = [vp]
+        l.orc_cells_update.restype = ctypes.c_int64
+        l.orc_cells_update.argtypes = [vp, ctypes.c_int64, _dp, _u8p]
+        l.orc_cells_grid.restype = ctypes.c_int64
+        l.orc_cells_grid.argtypes = [vp, _u32p, _dp]
+        l.orc_cells_get.argtypes = [vp, _i32p, _u32p, _u32p, _u32p, _dp]
+        l.orc_search_count.restype = ctypes.c_int64
+        l.orc_search_count.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p, ctypes.c_int]
+        l.orc_search_fill.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p, _i32p,
+                                      ctypes.c_int]
+        l.orc_sparse_matvec.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, P(KernelC),
+                                        vp, vp, ctypes.c_int, _dp, ctypes.c_int64, _dp,
+                                        ctypes.c_int64, ctypes.c_int]
+        l.orc_dense_matvec.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, ctypes.c_int64, _dp,
+                                       P(KernelC), ctypes.c_int, _dp, ctypes.c_int64, _dp,
+                                       ctypes.c_int64, ctypes.c_int]
+        l.orc_max_threads.restype = ctypes.c_int
+        _lib = l
+    return _lib
+
+
+def nthreads_default():
+    return lib().orc_max_threads()
+
+
+class Cells:
+    """CellListOrdered restated on the CPU."""
+
+    def __init__(self, low, high, periodic, n_leaf=10.0):
+        self.dim = len(low)
+        self.h = lib().orc_cells_create(self.dim, np.asarray(low, np.float64),
+                                        np.asarray(high, np.float64),
+                                        np.asarray(periodic, np.uint8), float(n_leaf))
+        self.n_alive = 0
+
+    def update(self, pos, alive=None):
+        """pos (n, dim) float64 and alive (n,) uint8 are updated in place."""
+        pos = np.ascontiguousarray(pos, np.float64)
+        n = pos.shape[0]
+        if alive is None:
+            alive = np.ones(n, np.uint8)
+        self.pos_in = pos
+        self.alive_in = alive
+        self.n_alive = lib().orc_cells_update(self.h, n, pos.reshape(-1), alive)
+        grid = np.zeros(3, np.uint32)
+        side = np.zeros(3, np.float64)
+        self.nb = lib().orc_cells_grid(self.h, grid, side)
+        self.grid = tuple(int(g) for g in grid[:self.dim])
+        self.side = side[:self.dim].copy()
+        na, nb = self.n_alive, self.nb
+        self.order = np.zeros(na, np.int32)
+        self.keys = np.zeros(na, np.uint32)
+        self.begin = np.zeros(nb, np.uint32)
+        self.end = np.zeros(nb, np.uint32)
+        self.positions = np.zeros((na, self.dim), np.float64)
+        lib().orc_cells_get(self.h, self.order, self.keys, self.begin, self.end,
+                            self.positions.reshape(-1))
+        return self.n_alive
+
+    def search(self, centres, radius, nthreads=None):
+        """Neighbour lists in the reference's iteration order (CSR)."""
+        c = np.ascontiguousarray(centres, np.float64)
+        nq = c.shape[0]
+        nt = nthreads or nthreads_default()
+        counts = np.zeros(nq, np.int64)
+        total = lib().orc_search_count(self.h, nq, c.reshape(-1), float(radius), counts, nt)
+        row_ptr = np.zeros(nq + 1, np.int64)
+        np.cumsum(counts, out=row_ptr[1:])
+        col = np.zeros(max(total, 1), np.int32)
+        lib().orc_search_fill(self.h, nq, c.reshape(-1), float(radius), row_ptr, col, nt)
+        return row_ptr, col[:total]
+
+    def sparse_matvec(self, centres, radius, kernel, x, y=None, row_s=None, col_s=None,
+                      nthreads=None):
+        """y += K x, x (ncols,) or (ncols, nrhs)."""
+        c = np.ascontiguousarray(centres, np.float64)
+        nq = c.shape[0]
+        x2 = np.asarray(x, np.float64)
+        if x2.ndim == 1:
+            x2 = x2[:, None]
+        nrhs = x2.shape[1]
+        xt = np.ascontiguousarray(x2.T)
+        yt = np.zeros((nrhs, nq)) if y is None else np.ascontiguousarray(np.asarray(y).reshape(nq, -1).T)
+        kd = KernelC(kernel[0], 0, kernel[1], *(kernel[2:] if len(kernel) > 2 else (0.0, 0.0, 0.0)))
+        rs = np.ascontiguousarray(row_s, np.float64) if row_s is not None else None
+        cs = np.ascontiguousarray(col_s, np.float64) if col_s is not None else None
+        lib().orc_sparse_matvec(self.h, nq, c.reshape(-1), float(radius), ctypes.byref(kd),
+                                rs.ctypes.data if rs is not None else None,
+                                cs.ctypes.data if cs is not None else None,
+                                nrhs, xt.reshape(-1), xt.shape[1], yt.reshape(-1), yt.shape[1],
+                                nthreads or nthreads_default())
+        out = yt.T.copy()
+        return out[:, 0] if np.asarray(x).ndim == 1 else out
+
+    def __del__(self):
+        try:
+            lib().orc_cells_destroy(self.h)
+        except Exception:
+            pass
+
+
+def dense_matvec(a, b, kernel, x, nthreads=None):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x2 = np.asarray(x, np.float64)
+    if x2.ndim == 1:
+        x2 = x2[:, None]
+    xt = np.ascontiguousarray(x2.T)
+    yt = np.zeros((x2.shape[1], a.shape[0]))
+    kd = KernelC(kernel[0], 0, kernel[1], 0.0, 0.0, 0.0)
+    lib().orc_dense_matvec(a.shape[1], a.shape[0], a.reshape(-1), b.shape[0], b.reshape(-1),
+                           ctypes.byref(kd), xt.shape[0], xt.reshape(-1), xt.shape[1],
+                           yt.reshape(-1), yt.shape[1], nthreads or nthreads_default())
+    out = yt.T.copy()
+    return out[:, 0] if np.asarray(x).ndim == 1 else out

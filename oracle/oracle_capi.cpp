@@ -1,0 +1,196 @@
+// C entry points of the CPU oracle (liboracle.so), loaded by tests/ and by
+// bench.py's cpu_baseline leg through ctypes.  TEST INFRASTRUCTURE ONLY.
+#include <omp.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "cells_search.hpp"
+#include "kernels.hpp"
+
+using namespace orc;
+
+namespace {
+
+struct AnyCells {
+  int dim = 0;
+  Cells<1> c1;
+  Cells<2> c2;
+  Cells<3> c3;
+};
+
+template <int D> Cells<D> &get(AnyCells *h);
+template <> Cells<1> &get<1>(AnyCells *h) { return h->c1; }
+template <> Cells<2> &get<2>(AnyCells *h) { return h->c2; }
+template <> Cells<3> &get<3>(AnyCells *h) { return h->c3; }
+
+template <int D>
+void init(AnyCells *h, const double *lo, const double *hi, const uint8_t *per, double n_leaf) {
+  Grid<D> &g = get<D>(h).g;
+  for (int d = 0; d < D; ++d) {
+    g.lo[d] = lo[d];
+    g.hi[d] = hi[d];
+    g.periodic[d] = per[d] != 0;
+  }
+  g.n_leaf = n_leaf;
+  g.sized_with_n = std::numeric_limits<size_t>::max();
+  // set_domain at init_neighbour_search time sizes with the (empty) alive set
+  size_grid<D>(g, 0);
+}
+
+template <class F> auto dispatch(int dim, F &&f) {
+  if (dim == 1) return f(std::integral_constant<int, 1>());
+  if (dim == 2) return f(std::integral_constant<int, 2>());
+  return f(std::integral_constant<int, 3>());
+}
+
+}  // namespace
+
+extern "C" {
+
+void *orc_cells_create(int dim, const double *lo, const double *hi, const uint8_t *periodic,
+                       double n_leaf) {
+  if (dim < 1 || dim > 3) return nullptr;
+  auto *h = new AnyCells();
+  h->dim = dim;
+  dispatch(dim, [&](auto DD) {
+    init<decltype(DD)::value>(h, lo, hi, periodic, n_leaf);
+    return 0;
+  });
+  return h;
+}
+
+void orc_cells_destroy(void *p) { delete static_cast<AnyCells *>(p); }
+
+// pos (n x dim) and alive (n) are updated in place, as the reference does.
+int64_t orc_cells_update(void *p, int64_t n, double *pos, uint8_t *alive) {
+  auto *h = static_cast<AnyCells *>(p);
+  return dispatch(h->dim, [&](auto DD) {
+    return int64_t(update<decltype(DD)::value>(get<decltype(DD)::value>(h), size_t(n), pos, alive));
+  });
+}
+
+// grid[dim], side[dim]; returns number of buckets
+int64_t orc_cells_grid(void *p, uint32_t *grid, double *side) {
+  auto *h = static_cast<AnyCells *>(p);
+  return dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Grid<D> &g = get<D>(h).g;
+    for (int d = 0; d < D; ++d) {
+      grid[d] = g.size[d];
+      side[d] = g.side[d];
+    }
+    return int64_t(g.nbuckets());
+  });
+}
+
+void orc_cells_get(void *p, int32_t *order, uint32_t *keys, uint32_t *begin, uint32_t *end,
+                   double *pos) {
+  auto *h = static_cast<AnyCells *>(p);
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Cells<D> &c = get<D>(h);
+    if (order) std::memcpy(order, c.order.data(), c.order.size() * 4);
+    if (keys) std::memcpy(keys, c.keys.data(), c.keys.size() * 4);
+    if (begin) std::memcpy(begin, c.begin.data(), c.begin.size() * 4);
+    if (end) std::memcpy(end, c.end.data(), c.end.size() * 4);
+    if (pos) std::memcpy(pos, c.pos.data(), c.pos.size() * 8);
+    return 0;
+  });
+}
+
+// counts[i] = number of neighbours of q[i]; returns the total.
+int64_t orc_search_count(void *p, int64_t nq, const double *q, double r, int64_t *counts,
+                         int nthreads) {
+  auto *h = static_cast<AnyCells *>(p);
+  return dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Cells<D> &c = get<D>(h);
+    int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : total) num_threads(nthreads)
+    for (int64_t i = 0; i < nq; ++i) {
+      int64_t cnt = 0;
+      euclidean_search<D>(c, q + i * D, r, [&](uint32_t, const double *, double) { ++cnt; });
+      counts[i] = cnt;
+      total += cnt;
+    }
+    return total;
+  });
+}
+
+// Fills col[row_ptr[i]..] with the neighbours of q[i] in the reference's
+// iteration order.
+void orc_search_fill(void *p, int64_t nq, const double *q, double r, const int64_t *row_ptr,
+                     int32_t *col, int nthreads) {
+  auto *h = static_cast<AnyCells *>(p);
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Cells<D> &c = get<D>(h);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int64_t i = 0; i < nq; ++i) {
+      int64_t k = row_ptr[i];
+      euclidean_search<D>(c, q + i * D, r,
+                          [&](uint32_t j, const double *, double) { col[k++] = int32_t(j); });
+    }
+    return 0;
+  });
+}
+
+// KernelSparse::evaluate (src/Kernels.h:720-751): y[i] += f(dx,a_i,b_j)*x[j]
+// over the euclidean_search neighbours of row position q[i], in iteration
+// order, one right-hand side at a time (the reference's GEMV-only product,
+// src/detail/Operators.h:206-231).  x: column-major ncols x nrhs (ldx),
+// y: nrows x nrhs (ldy).
+void orc_sparse_matvec(void *p, int64_t nq, const double *q, double r, const Kernel *k,
+                       const double *row_s, const double *col_s, int nrhs, const double *x,
+                       int64_t ldx, double *y, int64_t ldy, int nthreads) {
+  auto *h = static_cast<AnyCells *>(p);
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Cells<D> &c = get<D>(h);
+    for (int rhs = 0; rhs < nrhs; ++rhs) {
+      const double *xr = x + rhs * ldx;
+      double *yr = y + rhs * ldy;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+      for (int64_t i = 0; i < nq; ++i) {
+        double acc = yr[i];
+        euclidean_search<D>(c, q + i * D, r, [&](uint32_t j, const double *, double s) {
+          const double f = phi(*k, s) * weight(*k, row_s, col_s, size_t(i), j);
+          acc += f * xr[j];
+        });
+        yr[i] = acc;
+      }
+    }
+    return 0;
+  });
+}
+
+// KernelDense::evaluate (src/Kernels.h:212-267): y[i] += f(a_i,b_j)*x[j] over
+// all j in order.  Positions a (na x dim) and b (nb x dim).
+void orc_dense_matvec(int dim, int64_t na, const double *a, int64_t nb, const double *b,
+                      const Kernel *k, int nrhs, const double *x, int64_t ldx, double *y,
+                      int64_t ldy, int nthreads) {
+  for (int rhs = 0; rhs < nrhs; ++rhs) {
+    const double *xr = x + rhs * ldx;
+    double *yr = y + rhs * ldy;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < na; ++i) {
+      double acc = yr[i];
+      for (int64_t j = 0; j < nb; ++j) {
+        double s = 0;
+        for (int d = 0; d < dim; ++d) {
+          const double dx = b[j * dim + d] - a[i * dim + d];
+          s = s + dx * dx;
+        }
+        acc += phi(*k, s) * xr[j];
+      }
+      yr[i] = acc;
+    }
+  }
+}
+
+int orc_max_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"

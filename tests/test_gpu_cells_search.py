@@ -1,0 +1,172 @@
+"""GPU parity of the CellListOrdered build, the radius search and the sparse
+operator against the CPU oracle, called through the C ABI.
+
+Bar: build outputs (gather order, bucket indices, bucket begin/end, wrapped
+positions, alive flags) bit-exact; neighbour multisets per centre bit-exact;
+FP64 sparse matvec within 1e-13 * sum_j |f_ij x_j| of the oracle (different
+summation order only); FP32-compensated within 2e-6 * sum_j |f_ij x_j| of the
+FP64 sum.
+"""
+import numpy as np
+import pytest
+import torch
+
+import aboria_amd as ab
+import oracle
+from aboria_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_build(ctx, pos, alive, low, high, periodic, n_leaf):
+    p = torch.tensor(pos, device="cuda")
+    a = torch.tensor(alive, device="cuda")
+    cl = ab.CellListOrdered(ctx, low, high, periodic, n_leaf)
+    na = cl.update_positions(p, a)
+    return cl, p, a, na
+
+
+def orc_build(pos, alive, low, high, periodic, n_leaf):
+    c = oracle.Cells(low, high, periodic, n_leaf)
+    p = pos.copy()
+    a = alive.copy()
+    na = c.update(p, a)
+    return c, p, a, na
+
+
+CASES = [
+    # (name, D, N, periodic, n_leaf, radius, low, high)
+    ("d1_np", 1, 3000, False, 3.0, 0.01, -1.0, 1.0),
+    ("d1_p", 1, 3000, True, 3.0, 0.01, -1.0, 1.0),
+    ("d2_np", 2, 4000, False, 5.0, 0.06, -1.0, 1.0),
+    ("d2_p", 2, 4000, True, 5.0, 0.06, -1.0, 1.0),
+    ("d3_np", 3, 6000, False, 10.0, 0.15, -1.0, 1.0),
+    ("d3_p", 3, 6000, True, 10.0, 0.15, -1.0, 1.0),
+    ("d3_p_wide", 3, 2000, True, 2.0, 0.25, 0.0, 1.0),
+    ("d3_tiny_grid", 3, 50, True, 10.0, 0.6, 0.0, 1.0),
+    ("d2_outside", 2, 3000, False, 4.0, 0.1, 0.0, 1.0),
+]
+
+
+def make_case(D, N, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(lo - 0.1 * (hi - lo), hi + 0.1 * (hi - lo), (N, D))
+    alive = np.ones(N, np.uint8)
+    alive[rng.integers(0, N, 5)] = 0
+    pos[3, 0] = np.nan
+    return pos, alive
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_build_and_search_parity(ctx, case):
+    name, D, N, per, n_leaf, r, lo, hi = case
+    print("case", name, flush=True)
+    pos, alive = make_case(D, N, lo, hi, sum(name.encode()) * 7 + N)
+    low, high, periodic = [lo] * D, [hi] * D, [per] * D
+    cl, gp, ga, gna = gpu_build(ctx, pos, alive, low, high, periodic, n_leaf)
+    oc, op, oa, ona = orc_build(pos, alive, low, high, periodic, n_leaf)
+    assert gna == ona
+    assert cl.grid == oc.grid
+    assert np.array_equal(ga.cpu().numpy(), oa)
+    gpw = gp.cpu().numpy()
+    assert np.array_equal(np.isnan(gpw), np.isnan(op))
+    assert np.array_equal(np.nan_to_num(gpw), np.nan_to_num(op))
+    assert np.array_equal(cl.order().cpu().numpy(), oc.order)
+    assert np.array_equal(cl.bucket_index().cpu().numpy().view(np.uint32), oc.keys)
+    assert np.array_equal(cl.bucket_begin().cpu().numpy().view(np.uint32), oc.begin)
+    assert np.array_equal(cl.bucket_end().cpu().numpy().view(np.uint32), oc.end)
+    assert np.array_equal(cl.positions().cpu().numpy(), oc.positions)
+    # search: self centres and arbitrary centres (inside and outside the domain)
+    sp = cl.positions().clone()
+    rng = np.random.default_rng(5)
+    other = rng.uniform(lo - 0.2, hi + 0.2, (500, D))
+    for centres in (sp, torch.tensor(other, device="cuda")):
+        res = ab.euclidean_search(cl, centres, r)
+        rp_o, col_o = oc.search(centres.cpu().numpy(), r)
+        rp_g = res.row_ptr.cpu().numpy()
+        col_g = res.col.cpu().numpy()
+        assert np.array_equal(rp_g, rp_o)
+        for i in range(len(rp_o) - 1):
+            a = col_g[rp_g[i]:rp_g[i + 1]]
+            b = col_o[rp_o[i]:rp_o[i + 1]]
+            assert np.array_equal(np.sort(a), np.sort(b)), (name, i)
+
+
+def test_uniform_workload_full(ctx):
+    """The N=100,000 uniform workload: build bit-exact, every neighbour list
+    equal to the oracle's, sparse Gaussian matvec within tolerance."""
+    c = wl.UNIFORM
+    pos = wl.uniform_positions(c["n"], seed=c["seed"])
+    alive = np.ones(c["n"], np.uint8)
+    cl, gp, ga, gna = gpu_build(ctx, pos, alive, c["low"], c["high"], c["periodic"], c["n_leaf"])
+    oc, op, oa, ona = orc_build(pos, alive, c["low"], c["high"], c["periodic"], c["n_leaf"])
+    assert cl.grid == (25, 25, 25)
+    assert np.array_equal(cl.order().cpu().numpy(), oc.order)
+    assert np.array_equal(cl.bucket_begin().cpu().numpy().view(np.uint32), oc.begin)
+    sp = cl.positions().clone()
+    res = ab.euclidean_search(cl, sp, c["radius"])
+    rp_o, col_o = oc.search(oc.positions, c["radius"])
+    rp_g = res.row_ptr.cpu().numpy()
+    col_g = res.col.cpu().numpy()
+    assert np.array_equal(rp_g, rp_o)
+    # rows are ascending on the GPU; sort the oracle's rows to compare
+    key_o = np.repeat(np.arange(len(rp_o) - 1), np.diff(rp_o)) * (c["n"] + 1) + col_o
+    assert np.array_equal(np.sort(key_o) % (c["n"] + 1), col_g)
+    # sparse Gaussian operator on b ~ U[0,1) (keyed by particle id)
+    b_id = wl.uniform_rhs(c["n"], 1, seed=c["seed"] + 1)[:, 0]
+    ids = oc.order
+    b = b_id[ids]
+    K = ab.SparseOperator(ctx, cl, sp, c["radius"], ab.Kernel.gaussian(c["gauss_scale"]))
+    y = (K @ torch.tensor(b, device="cuda")).cpu().numpy()
+    kern = (1, c["gauss_scale"])
+    y_o = oc.sparse_matvec(oc.positions, c["radius"], kern, b)
+    scale = oc.sparse_matvec(oc.positions, c["radius"], kern, np.abs(b))
+    assert np.all(np.abs(y - y_o) <= 1e-13 * scale + 1e-300)
+
+
+def test_sparse_kat(ctx):
+    """tests/operators.h:810-964 on the GPU: C*(1,2,3) = (9,18,15) exactly."""
+    d = 0.1
+    pos = torch.tensor([[0, 0, 0], [d * 0.9, 0, 0], [d * 1.8, 0, 0]], dtype=torch.float64,
+                       device="cuda")
+    parts = ab.Particles(ctx, pos, s1=torch.full((3,), 1.0, dtype=torch.float64),
+                         s2=torch.full((3,), 2.0, dtype=torch.float64))
+    parts.init_neighbour_search([-1] * 3, [1] * 3, [False] * 3)
+    C = ab.create_sparse_operator(parts, parts, d, ab.Kernel.constant(w0=0.0, w_row=1.0, w_col=1.0),
+                                  row_scalar="s1", col_scalar="s2")
+    ids = parts.id.cpu().numpy()
+    v = torch.tensor(np.array([1.0, 2.0, 3.0])[ids], device="cuda")
+    ans = (C @ v).cpu().numpy()
+    assert np.array_equal(ans, np.array([9.0, 18.0, 15.0])[ids])
+    res = ab.euclidean_search(parts.get_query(), parts.position, d)
+    assert res.total == 7
+
+
+@pytest.mark.parametrize("precision", [ab.FP64, ab.FP32_COMPENSATED])
+def test_mixture_sparse_subset(ctx, precision):
+    """Periodic Gaussian-mixture workload (reduced N, same cell side and
+    radius): Wendland C2 operator on 8 right-hand sides."""
+    n = 400_000
+    pos = wl.mixture_positions(n, seed=7)
+    r = 0.01
+    # same 100^3 grid as the full workload: n_leaf scaled with N
+    n_leaf = 3.95 * n / 4_000_000
+    alive = np.ones(n, np.uint8)
+    cl, gp, ga, gna = gpu_build(ctx, pos, alive, [0] * 3, [1] * 3, [True] * 3, n_leaf)
+    oc, op, oa, ona = orc_build(pos, alive, [0] * 3, [1] * 3, [True] * 3, n_leaf)
+    assert cl.grid == oc.grid == (100, 100, 100)
+    assert np.array_equal(cl.order().cpu().numpy(), oc.order)
+    sp = cl.positions().clone()
+    x = wl.normal_rhs(n, 8, seed=11)
+    K = ab.SparseOperator(ctx, cl, sp, r, ab.Kernel.wendland_c2(r))
+    xt = torch.tensor(x.T.copy(), device="cuda")
+    yt = torch.zeros_like(xt)
+    K.apply_colmajor(xt, yt, precision)
+    y = yt.cpu().numpy().T
+    rows = np.random.default_rng(1).choice(n, 3000, replace=False)
+    kern = (2, r)
+    y_o = oc.sparse_matvec(oc.positions[rows], r, kern, x)
+    scale = oc.sparse_matvec(oc.positions[rows], r, kern, np.abs(x))
+    tol = 1e-13 if precision == ab.FP64 else 2e-6
+    err = np.abs(y[rows] - y_o)
+    assert np.all(err <= tol * scale + 1e-300), float((err / scale).max())
