@@ -21,6 +21,19 @@
 using abh::DevBuf;
 using abh::GridParams;
 
+namespace abh {
+void context_free(abh_context *ctx) {
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->scan_blocks.release();
+  ctx->sort_keys.release();
+  ctx->sort_vals.release();
+  ctx->sort_hist.release();
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+}  // namespace abh
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -172,12 +185,9 @@ abh_status abh_context_create(int device, void *stream, abh_context **out) {
   if (!ctx) return ABH_ERR_OUT_OF_MEMORY;
   ctx->device = device;
   abh_status st = abh::guard(ctx, [&] {
-    if (stream) {
-      ctx->stream = static_cast<hipStream_t>(stream);
-    } else {
-      ABH_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-      ctx->own_stream = true;
-    }
+    // NULL is HIP's null stream (torch's default stream), used as is
+    ctx->stream = static_cast<hipStream_t>(stream);
+    ctx->own_stream = false;
     hipDeviceProp_t prop;
     ABH_HIP(hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
@@ -191,15 +201,9 @@ abh_status abh_context_create(int device, void *stream, abh_context **out) {
 }
 
 abh_status abh_context_destroy(abh_context *ctx) {
-  if (!ctx) return ABH_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  ctx->scan_blocks.release();
-  ctx->sort_keys.release();
-  ctx->sort_vals.release();
-  ctx->sort_hist.release();
-  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  if (!ctx || ctx->closed) return ABH_OK;
+  ctx->closed = true;
+  if (ctx->refs == 0) abh::context_free(ctx);
   return ABH_OK;
 }
 
@@ -261,6 +265,7 @@ abh_status abh_cells_create(abh_context *ctx, int dim, const double *low, const 
   abh_cells *c = new (std::nothrow) abh_cells();
   if (!c) return ABH_ERR_OUT_OF_MEMORY;
   abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
     c->ctx = ctx;
     c->dim = dim;
     for (int d = 0; d < dim; ++d) {
@@ -279,15 +284,18 @@ abh_status abh_cells_create(abh_context *ctx, int dim, const double *low, const 
     delete c;
     return st;
   }
+  abh::ctx_retain(ctx);
   *out = c;
   return ABH_OK;
 }
 
 abh_status abh_cells_destroy(abh_cells *c) {
   if (!c) return ABH_OK;
-  (void)hipSetDevice(c->ctx->device);
-  (void)hipStreamSynchronize(c->ctx->stream);
+  abh_context *ctx = c->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete c;
+  abh::ctx_release(ctx);
   return ABH_OK;
 }
 

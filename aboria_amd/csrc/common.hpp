@@ -68,6 +68,10 @@ struct abh_context {
   abh::DevBuf scan_blocks;                     // per-tile sums of scan_exclusive
   abh::DevBuf sort_keys, sort_vals, sort_hist;  // radix sort ping-pong + histograms
   int n_cu = 256;
+  // Handles created on this context keep it alive: abh_context_destroy only
+  // marks it closed while any of them exists, the last one frees it.
+  int refs = 0;
+  bool closed = false;
 };
 
 namespace abh {
@@ -91,6 +95,12 @@ template <class F> abh_status guard(abh_context *ctx, F &&f) {
 }
 
 inline void check_launch() { ABH_HIP(hipGetLastError()); }
+
+void context_free(abh_context *ctx);  // cells.hip
+inline void ctx_retain(abh_context *ctx) { ++ctx->refs; }
+inline void ctx_release(abh_context *ctx) {
+  if (--ctx->refs == 0 && ctx->closed) context_free(ctx);
+}
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

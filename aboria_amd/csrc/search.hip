@@ -46,11 +46,22 @@ struct Slot {
   int32_t img;  // image index: sum_d (I_d + 1) * 3^d
 };
 
+// Virtual home cell of a centre in dimension d: floor((q - lo) * inv),
+// clamped to [-1, size] for a non-periodic dimension and to [-size, 2*size)
+// for a periodic one (centres up to one domain length outside still reach
+// the domain through the +-1 images).  NaN maps to the lowest value.
+__device__ __forceinline__ int home_lo(const GridParams &g, int d) {
+  return g.periodic[d] ? -g.size[d] : -1;
+}
+__device__ __forceinline__ int home_span(const GridParams &g, int d) {
+  return g.periodic[d] ? 3 * g.size[d] : g.size[d] + 2;
+}
 template <int D> __device__ __forceinline__ int home_of(const GridParams &g, const double *q, int d) {
   double t = floor((q[d] - g.lo[d]) * g.inv[d]);
-  // clamp to the virtual ring [-1, size]; NaN maps to -1
-  if (!(t >= -1.0)) t = -1.0;
-  if (t > double(g.size[d])) t = double(g.size[d]);
+  const double lo = double(home_lo(g, d));
+  const double hi = lo + double(home_span(g, d) - 1);
+  if (!(t >= lo)) t = lo;
+  if (t > hi) t = hi;
   return int(t);
 }
 
@@ -65,7 +76,8 @@ __global__ __launch_bounds__(kThreads) void k_home_keys(const double *__restrict
   for (int d = 0; d < D; ++d) p[d] = q[i * D + d];
   uint32_t key = 0;
 #pragma unroll
-  for (int d = 0; d < D; ++d) key = key * uint32_t(g.size[d] + 2) + uint32_t(home_of<D>(g, p, d) + 1);
+  for (int d = 0; d < D; ++d)
+    key = key * uint32_t(home_span(g, d)) + uint32_t(home_of<D>(g, p, d) - home_lo(g, d));
   keys[i] = key;
   if (vals) vals[i] = int32_t(i);
 }
@@ -634,7 +646,11 @@ template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
     k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, keys, p.perm.as<int32_t>());
     check_launch();
     uint64_t maxkey = 1;
-    for (int d = 0; d < D; ++d) maxkey *= uint64_t(g.size[d] + 2);
+    for (int d = 0; d < D; ++d)
+      maxkey *= uint64_t(g.periodic[d] ? 3 * g.size[d] : g.size[d] + 2);
+    ABH_REQUIRE(maxkey <= (uint64_t(1) << 32), ABH_ERR_UNSUPPORTED,
+                "too many buckets to group arbitrary centres; search from the particle "
+                "positions themselves");
     int bits = 0;
     while (bits < 32 && ((maxkey - 1) >> bits) != 0) ++bits;
     radix_sort_pairs(ctx, keys, p.perm.as<int32_t>(), n, bits);
@@ -778,23 +794,28 @@ abh_status abh_search_create(abh_context *ctx, const abh_cells *cells, int64_t n
   *out = nullptr;
   abh_search *s = new (std::nothrow) abh_search();
   if (!s) return ABH_ERR_OUT_OF_MEMORY;
+  if (ctx->closed) {
+    delete s;
+    return ABH_ERR_INVALID_ARG;
+  }
   s->plan.ctx = ctx;
   abh_status st = abh_search_update(s, cells, nq, q, radius);
   if (st != ABH_OK) {
     delete s;
     return st;
   }
+  abh::ctx_retain(ctx);
   *out = s;
   return ABH_OK;
 }
 
 abh_status abh_search_destroy(abh_search *s) {
   if (!s) return ABH_OK;
-  if (s->plan.ctx) {
-    (void)hipSetDevice(s->plan.ctx->device);
-    (void)hipStreamSynchronize(s->plan.ctx->stream);
-  }
+  abh_context *ctx = s->plan.ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete s;
+  abh::ctx_release(ctx);
   return ABH_OK;
 }
 
@@ -831,6 +852,7 @@ abh_status abh_sparse_op_create(abh_context *ctx, const abh_cells *cols, int64_t
   abh_sparse_op *op = new (std::nothrow) abh_sparse_op();
   if (!op) return ABH_ERR_OUT_OF_MEMORY;
   abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
     ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
                 ABH_ERR_UNSUPPORTED, "unknown kernel type");
     abh::build_plan(op->plan, ctx, cols, nrows, row_pos, radius);
@@ -862,17 +884,18 @@ abh_status abh_sparse_op_create(abh_context *ctx, const abh_cells *cols, int64_t
     delete op;
     return st;
   }
+  abh::ctx_retain(ctx);
   *out = op;
   return ABH_OK;
 }
 
 abh_status abh_sparse_op_destroy(abh_sparse_op *op) {
   if (!op) return ABH_OK;
-  if (op->plan.ctx) {
-    (void)hipSetDevice(op->plan.ctx->device);
-    (void)hipStreamSynchronize(op->plan.ctx->stream);
-  }
+  abh_context *ctx = op->plan.ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete op;
+  abh::ctx_release(ctx);
   return ABH_OK;
 }
 

@@ -46,8 +46,9 @@ typedef enum abh_status {
 
 typedef struct abh_context abh_context;
 
-/* Creates a context on HIP device `device`.  `stream` is a hipStream_t to
- * enqueue on (NULL = the context creates and owns one). */
+/* Creates a context on HIP device `device`.  `stream` is the hipStream_t all
+ * work is enqueued on; NULL is the null (default) stream.  The caller keeps
+ * ownership of the stream. */
 abh_status abh_context_create(int device, void *stream, abh_context **out);
 abh_status abh_context_destroy(abh_context *ctx);
 abh_status abh_context_sync(abh_context *ctx);
