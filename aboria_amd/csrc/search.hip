@@ -43,7 +43,7 @@ struct Slot {
   int32_t b;    // collapsed bucket index, -1 = no bucket
   int32_t beg;  // bucket_begin
   int32_t len;  // bucket_end - bucket_begin
-  int32_t img;  // image index: sum_d (I_d + 1) * 3^d
+  int32_t img;  // image digits (I_d + 1) packed two bits per dimension
 };
 
 // Virtual home cell of a centre in dimension d: floor((q - lo) * inv),
@@ -344,6 +344,7 @@ struct SearchArgs {
   const int64_t *row_ptr;
   int32_t *col;
   uint32_t *escape_flag;
+  uint32_t *next;  // dynamic work counter, zero at launch
   // matvec
   KernelDesc k;
   int nrhs;
@@ -353,28 +354,66 @@ struct SearchArgs {
   int64_t ldy;
 };
 
+// Staged candidates, structure of arrays: positions, stencil slot and
+// container index of up to kTile candidates.
+constexpr int kTile = 64;
+struct Tile {
+  double x[kTile], y[kTile], z[kTile];
+  int32_t slot[kTile];
+  int32_t img[kTile];
+  int32_t j[kTile];
+};
+
+typedef __attribute__((address_space(4))) const double CDouble;
+
+__device__ __forceinline__ double pick3(int digit, double m, double z, double p) {
+  return digit == 0 ? m : (digit == 1 ? z : p);
+}
+
 template <int D, int MODE>
 __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
   extern __shared__ __align__(16) unsigned char lds[];
-  Slot *slots = reinterpret_cast<Slot *>(lds);
-  int32_t *sorder = reinterpret_cast<int32_t *>(slots + a.nslots);
-  uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + a.nslots);
-  const int lane = threadIdx.x;
+  constexpr bool kMV = (MODE == kMatvec64 || MODE == kMatvec32);
   const int S = a.nslots;
+  Tile *tile = reinterpret_cast<Tile *>(lds);
+  double *xt = reinterpret_cast<double *>(tile + 1);  // [kMaxRhs][kTile], matvec only
+  Slot *slots = reinterpret_cast<Slot *>(xt + (kMV ? kTile * kMaxRhs : 0));
+  int32_t *sorder = reinterpret_cast<int32_t *>(slots + S);
+  uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + S);  // [S][64]
+  const int lane = threadIdx.x;
   const int nwork = int(*a.nwork);
   int wd[3] = {a.w[0], a.w[1], a.w[2]};
   int span[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) span[d] = 2 * wd[d] + 1;
 
-  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+  // work items are taken dynamically: their cost varies by orders of
+  // magnitude between sparse and dense cells
+  for (;;) {
+    int wi = 0;
+    if (lane == 0) wi = int(atomicAdd(a.next, 1u));
+    wi = __shfl(wi, 0, 64);
+    if (wi >= nwork) break;
     const WorkItem it = a.work[wi];
-    const bool active = lane < it.count;
-    const int qk = it.start + (active ? lane : 0);
+    const int qk = it.start + (lane < it.count ? lane : 0);
     double q[D];
+    bool finite = true;
 #pragma unroll
-    for (int d = 0; d < D; ++d) q[d] = a.qpos[int64_t(qk) * D + d];
+    for (int d = 0; d < D; ++d) {
+      q[d] = a.qpos[int64_t(qk) * D + d];
+      finite = finite && isfinite(q[d]);
+    }
+    // a centre with a non-finite coordinate has no neighbours
+    const bool active = lane < it.count && finite;
+    const bool has_row = lane < it.count;
     const int64_t row = a.qperm ? int64_t(a.qperm[qk]) : int64_t(qk);
+    // image points q + I*L for I = -1, 0, +1 (search_iterator, src/Search.h:188-190)
+    double qm[D], qp[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      qm[d] = q[d] + double(-1) * g.L[d];
+      qp[d] = q[d] + double(1) * g.L[d];
+    }
 
     // stencil slots: virtual cell v = home + o -> bucket b under image I
     for (int s = lane; s < S; s += kGroup) {
@@ -398,8 +437,8 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         if (I != 0 && !g.periodic[d]) ok = false;
         if (b < 0 || b >= n) ok = false;  // would need an image beyond +-1
         bv[d] = b;
-        imgc += (I + 1) * p3;
-        p3 *= 3;
+        imgc |= (I + 1) << (2 * d);
+        (void)p3;
       }
       if (ok) {
         uint32_t index = 0, m = 1;
@@ -445,9 +484,12 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
           if (I[d] != 0 && !g.periodic[d]) img_ok = false;
         }
         if (!img_ok) continue;  // uniform
+        int pk = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) pk |= (I[d] + 1) << (2 * d);
         double qi[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) qi[d] = q[d] + double(I[d]) * g.L[d];
+        for (int d = 0; d < D; ++d) qi[d] = pick3(I[d] + 1, qm[d], q[d], qp[d]);
         const bool in_range = active && !outside_domain<D>(g, qi, a.r2);
         if (__ballot(in_range) == 0) continue;
         QuadState<D> qs;
@@ -464,7 +506,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         }
         for (int s = 0; s < S; ++s) {
           const Slot sl = slots[s];
-          if (sl.img != ii || sl.b < 0) continue;
+          if (sl.img != pk || sl.b < 0) continue;
           int b[D];
           int rem = sl.b;
 #pragma unroll
@@ -479,91 +521,180 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     }
     if (MODE == kCount && escape) atomicOr(a.escape_flag, 1u);
     __syncthreads();
-
-    int64_t cnt = 0;
+    int cnt = 0;
     int64_t out = 0;
     if (MODE == kFill && active) out = a.row_ptr[row];
     double acc64[kMaxRhs];
     float acc32[kMaxRhs], cmp32[kMaxRhs];
-    if (MODE == kMatvec64 || MODE == kMatvec32) {
+    if (kMV) {
 #pragma unroll
       for (int r = 0; r < kMaxRhs; ++r) {
-        acc64[r] = (active && r < a.nrhs) ? a.y[int64_t(r) * a.ldy + row] : 0.0;
+        acc64[r] = (has_row && r < a.nrhs) ? a.y[int64_t(r) * a.ldy + row] : 0.0;
         acc32[r] = 0.0f;
         cmp32[r] = 0.0f;
       }
     }
-    const double rs = (MODE >= kMatvec64 && a.k.row_s && active) ? a.k.row_s[row] : 0.0;
+    double wrow = 1.0;
+    if (kMV && (a.k.row_s || a.k.col_s)) {
+      wrow = a.k.w0;
+      if (a.k.row_s && active) wrow = wrow + a.k.wr * a.k.row_s[row];
+    }
+    // candidate positions through the constant address space: wave-uniform
+    // indices become scalar loads whose values feed the VALU as SGPRs
+    const CDouble *cpos = (const CDouble *)(a.pos);
 
     for (int k = 0; k < S; ++k) {
       const int s = abh::uniform_i32(sorder[k]);
       const int len = abh::uniform_i32(slots[s].len);
       if (len <= 0) continue;
-      const int beg = abh::uniform_i32(slots[s].beg);
-      const int imgc = abh::uniform_i32(slots[s].img);
       const int m = active ? int(mult[s * kGroup + lane]) : 0;
-      if (__ballot(m != 0) == 0) continue;
-      double qi[D];
-      {
-        int t = imgc;
+      if (__ballot(m != 0) == 0) continue;  // no lane visits this bucket
+      const int beg = abh::uniform_i32(slots[s].beg);
+      const int img = abh::uniform_i32(slots[s].img);
+      double qi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const int I = t % 3 - 1;
-          t /= 3;
-          qi[d] = q[d] + double(I) * g.L[d];
-        }
-      }
-      for (int j = beg; j < beg + len; ++j) {
-        double acc = 0.0;
-        double dxv[D];
+      for (int d = 0; d < D; ++d) qi[d] = pick3((img >> (2 * d)) & 3, qm[d], q[d], qp[d]);
+      const int end = beg + len;
+
+      if (MODE == kCount && __ballot(m > 1) == 0) {
+        // every lane visits this bucket at most once: fold the multiplicity
+        // into the radius test.  acc <= r2 equals the reference's
+        // !(acc > r2) for every finite centre (acc is never NaN there);
+        // lanes with m == 0 compare against -1.
+        const double r2m = m != 0 ? a.r2 : -1.0;
+        const CDouble *pj = cpos + int64_t(beg) * D;
+        int j = beg;
+        for (; j + 4 <= end; j += 4, pj += 4 * D) {
+          double e[4][3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          dxv[d] = a.pos[int64_t(j) * D + d] - qi[d];
-          acc = acc + dxv[d] * dxv[d];
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int d = 0; d < D; ++d) e[u][d] = pj[u * D + d];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double dx = e[u][d] - qi[d];
+              acc = acc + dx * dx;
+            }
+            cnt += (acc <= r2m) ? 1 : 0;
+          }
         }
-        const bool hit = m != 0 && !(acc > a.r2);
-        if (MODE == kCount) {
-          if (hit) cnt += m;
-        } else if (MODE == kFill) {
-          if (hit) {
-            for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j;
+        for (; j < end; ++j, pj += D) {
+          double acc = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const double dx = pj[d] - qi[d];
+            acc = acc + dx * dx;
+          }
+          cnt += (acc <= r2m) ? 1 : 0;
+        }
+      } else if (MODE == kCount || MODE == kFill) {
+        const double r2m = m != 0 ? a.r2 : -1.0;
+        const CDouble *pj = cpos + int64_t(beg) * D;
+        int j = beg;
+        for (; j + 4 <= end; j += 4, pj += 4 * D) {
+          double e[4][3];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int d = 0; d < D; ++d) e[u][d] = pj[u * D + d];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double dx = e[u][d] - qi[d];
+              acc = acc + dx * dx;
+            }
+            if (acc <= r2m) {
+              if (MODE == kFill)
+                for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j + u;
+              cnt += m;
+            }
+          }
+        }
+        for (; j < end; ++j, pj += D) {
+          double acc = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const double dx = pj[d] - qi[d];
+            acc = acc + dx * dx;
+          }
+          if (acc <= r2m) {
+            if (MODE == kFill)
+              for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j;
             cnt += m;
           }
-        } else if (MODE == kMatvec64) {
-          if (hit) {
-            double w = 1.0;
-            if (a.k.row_s || a.k.col_s) {
-              w = a.k.w0;
-              if (a.k.row_s) w = w + a.k.wr * rs;
-              if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j];
-            }
-            const double f = phi64(a.k, acc) * w;
-            for (int rep = 0; rep < m; ++rep) {
+        }
+      } else {
+        // matvec: chunks of 64 candidates; the radius test streams the
+        // positions as scalars, the hits are then evaluated lane-compacted
+        // (each iteration every lane takes its next hit) from an LDS copy
+        const double r2m = m != 0 ? a.r2 : -1.0;
+        for (int c0 = beg; c0 < end; c0 += kTile) {
+          const int nt = end - c0 < kTile ? end - c0 : kTile;
+          if (lane < nt) {
+            const int j = c0 + lane;
+            tile->x[lane] = a.pos[int64_t(j) * D + 0];
+            if (D > 1) tile->y[lane] = a.pos[int64_t(j) * D + (D > 1 ? 1 : 0)];
+            if (D > 2) tile->z[lane] = a.pos[int64_t(j) * D + (D > 2 ? 2 : 0)];
 #pragma unroll
-              for (int r = 0; r < kMaxRhs; ++r)
-                if (r < a.nrhs) acc64[r] += f * a.x[int64_t(r) * a.ldx + j];
+            for (int r = 0; r < kMaxRhs; ++r)
+              if (r < a.nrhs) xt[r * kTile + lane] = a.x[int64_t(r) * a.ldx + j];
+          }
+          uint64_t hits = 0;
+          const CDouble *pc = cpos + int64_t(c0) * D;
+          for (int c = 0; c < nt; ++c, pc += D) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double dx = pc[d] - qi[d];
+              acc = acc + dx * dx;
+            }
+            if (acc <= r2m) hits |= uint64_t(1) << c;
+          }
+          __syncthreads();
+          while (hits) {
+            const int c = __ffsll((unsigned long long)hits) - 1;
+            hits &= hits - 1;
+            double acc = 0.0;
+            {
+              const double dx0 = tile->x[c] - qi[0];
+              acc = acc + dx0 * dx0;
+              if (D > 1) {
+                const double dx1 = tile->y[c] - qi[1];
+                acc = acc + dx1 * dx1;
+              }
+              if (D > 2) {
+                const double dx2 = tile->z[c] - qi[2];
+                acc = acc + dx2 * dx2;
+              }
+            }
+            double w = wrow;
+            if (a.k.col_s) w = w + a.k.wc * a.k.col_s[c0 + c];
+            if (MODE == kMatvec64) {
+              const double f = phi64(a.k, acc) * w;
+              for (int rep = 0; rep < m; ++rep) {
+#pragma unroll
+                for (int r = 0; r < kMaxRhs; ++r)
+                  if (r < a.nrhs) acc64[r] += f * xt[r * kTile + c];
+              }
+            } else {
+              const float f = phi32(a.k, float(acc)) * float(w);
+              for (int rep = 0; rep < m; ++rep) {
+#pragma unroll
+                for (int r = 0; r < kMaxRhs; ++r)
+                  if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(xt[r * kTile + c]));
+              }
             }
           }
-        } else {  // kMatvec32
-          if (hit) {
-            float w = 1.0f;
-            if (a.k.row_s || a.k.col_s) {
-              double wd64 = a.k.w0;
-              if (a.k.row_s) wd64 = wd64 + a.k.wr * rs;
-              if (a.k.col_s) wd64 = wd64 + a.k.wc * a.k.col_s[j];
-              w = float(wd64);
-            }
-            const float f = phi32(a.k, float(acc)) * w;
-            for (int rep = 0; rep < m; ++rep) {
-#pragma unroll
-              for (int r = 0; r < kMaxRhs; ++r)
-                if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(a.x[int64_t(r) * a.ldx + j]));
-            }
-          }
+          __syncthreads();  // tile reused by the next chunk
         }
       }
     }
-    if (active) {
+    if (has_row) {
       if (MODE == kCount) a.counts[row] = cnt;
       if (MODE == kMatvec64) {
 #pragma unroll
@@ -574,8 +705,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
 #pragma unroll
         for (int r = 0; r < kMaxRhs; ++r)
           if (r < a.nrhs)
-            a.y[int64_t(r) * a.ldy + row] =
-                acc64[r] + (double(acc32[r]) + double(cmp32[r]));
+            a.y[int64_t(r) * a.ldy + row] = acc64[r] + (double(acc32[r]) + double(cmp32[r]));
       }
     }
     __syncthreads();  // slots / mult reused by the next work item
@@ -600,7 +730,7 @@ struct QueryPlan {
   int w[3] = {0, 0, 0};
   int nslots = 1;
   const double *q_sorted = nullptr;
-  DevBuf qbuf, perm, keys, flags, ridx, run_start, chunks, choff, work, counters, escape;
+  DevBuf qbuf, perm, keys, flags, ridx, run_start, chunks, choff, work, counters, escape, sched;
 
   const int32_t *qperm() const { return self ? nullptr : perm.as<int32_t>(); }
 };
@@ -665,6 +795,7 @@ template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
   p.choff.ensure(size_t(n + 1) * 4);
   p.work.ensure(size_t(n + 1) * sizeof(WorkItem));
   p.counters.ensure(16);
+  p.sched.ensure(16);
   uint32_t *ctr = p.counters.as<uint32_t>();  // [0] = runs, [1] = work items
   k_run_flags<<<blocks, kThreads, 0, s>>>(keys, n, p.flags.as<uint32_t>());
   check_launch();
@@ -680,6 +811,7 @@ template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
                                              p.choff.as<uint32_t>(), p.chunks.as<uint32_t>(),
                                              p.q_sorted, g, p.work.as<WorkItem>());
   check_launch();
+}
 }
 
 static void build_plan(QueryPlan &p, abh_context *ctx, const abh_cells *cells, int64_t nq,
@@ -717,15 +849,23 @@ static SearchArgs base_args(const QueryPlan &p) {
   return a;
 }
 
-static size_t lds_bytes(int nslots) {
-  return size_t(nslots) * (sizeof(Slot) + 4 + kGroup);
+static size_t lds_bytes(int nslots, bool matvec) {
+  return sizeof(Tile) + (matvec ? sizeof(double) * kTile * kMaxRhs : 0) +
+         size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
 }
 
-template <int MODE> static void launch_search(const QueryPlan &p, const SearchArgs &a) {
-  const GridParams g = p.cells->params();
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
+
+template <int MODE> static void launch_search(const QueryPlan &p, const SearchArgs &a0) {
   const unsigned grid = unsigned(std::min<int64_t>(ceil_div(p.nq, kGroup), int64_t(p.ctx->n_cu) * 32));
-  const size_t lds = lds_bytes(a.nslots);
   if (grid == 0) return;
+  SearchArgs a = a0;
+  a.next = p.sched.as<uint32_t>();
+  ABH_REQUIRE(a.next, ABH_ERR_INVALID_ARG, "search plan not built");
+  k_zero_u32<<<1, 1, 0, p.ctx->stream>>>(a.next);
+  check_launch();
+  const GridParams g = p.cells->params();
+  const size_t lds = lds_bytes(a.nslots, MODE == kMatvec64 || MODE == kMatvec32);
   if (p.D == 1) k_search<1, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
   else if (p.D == 2) k_search<2, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
   else k_search<3, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);

@@ -234,6 +234,10 @@ template <int D, class Visit>
 void euclidean_search(const Cells<D> &c, const double *centre, double radius, Visit &&visit) {
   const Grid<D> &g = c.g;
   const double r2 = radius * radius;
+  // A non-finite centre has no neighbours (the reference converts
+  // floor(NaN) to int in get_min_index_by_quadrant, which is undefined).
+  for (int d = 0; d < D; ++d)
+    if (!std::isfinite(centre[d])) return;
   int lo_i[D], hi_i[D], img[D];
   for (int d = 0; d < D; ++d) {
     lo_i[d] = g.periodic[d] ? -1 : 0;

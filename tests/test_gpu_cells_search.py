@@ -80,6 +80,8 @@ def test_build_and_search_parity(ctx, case):
     sp = cl.positions().clone()
     rng = np.random.default_rng(5)
     other = rng.uniform(lo - 0.2, hi + 0.2, (500, D))
+    other[7, 0] = np.nan
+    other[11, -1] = np.inf
     for centres in (sp, torch.tensor(other, device="cuda")):
         res = ab.euclidean_search(cl, centres, r)
         rp_o, col_o = oc.search(centres.cpu().numpy(), r)
