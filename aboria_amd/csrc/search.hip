@@ -812,7 +812,6 @@ template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
                                              p.q_sorted, g, p.work.as<WorkItem>());
   check_launch();
 }
-}
 
 static void build_plan(QueryPlan &p, abh_context *ctx, const abh_cells *cells, int64_t nq,
                        const double *q, double radius) {
