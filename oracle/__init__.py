@@ -58,6 +58,17 @@ def lib():
                                        P(KernelC), ctypes.c_int, _dp, ctypes.c_int64, _dp,
                                        ctypes.c_int64, ctypes.c_int]
         l.orc_max_threads.restype = ctypes.c_int
+        l.orc_octree_create.restype = vp
+        l.orc_octree_create.argtypes = [_dp, _dp, ctypes.c_int64, _dp, ctypes.c_double]
+        l.orc_octree_destroy.argtypes = [vp]
+        l.orc_octree_info.argtypes = [vp, _i64p]
+        l.orc_octree_get.argtypes = [vp, _i32p, _i32p, _i32p, _i32p]
+        l.orc_fmm_matvec.restype = ctypes.c_int
+        l.orc_fmm_matvec.argtypes = [vp, ctypes.c_int, P(KernelC), _dp, _dp]
+        l.orc_cheb_node.restype = ctypes.c_double
+        l.orc_cheb_node.argtypes = [ctypes.c_uint, ctypes.c_uint]
+        l.orc_cheb_Sn.restype = ctypes.c_double
+        l.orc_cheb_Sn.argtypes = [ctypes.c_double, ctypes.c_uint, ctypes.c_uint]
         _lib = l
     return _lib
 
@@ -156,3 +167,46 @@ def dense_matvec(a, b, kernel, x, nthreads=None):
                            yt.reshape(-1), yt.shape[1], nthreads or nthreads_default())
     out = yt.T.copy()
     return out[:, 0] if np.asarray(x).ndim == 1 else out
+
+
+class Octree:
+    """HyperOctree restated on the CPU (3D)."""
+
+    def __init__(self, low, high, pos, n_leaf):
+        pos = np.ascontiguousarray(pos, np.float64)
+        self.n = pos.shape[0]
+        self.h = lib().orc_octree_create(np.asarray(low, np.float64), np.asarray(high, np.float64),
+                                         self.n, pos.reshape(-1), float(n_leaf))
+        info = np.zeros(3, np.int64)
+        lib().orc_octree_info(self.h, info)
+        self.n_nodes, self.n_leaves, self.levels = (int(v) for v in info)
+        self.order = np.zeros(self.n, np.int32)
+        self.tags = np.zeros(self.n, np.int32)
+        self.nodes = np.zeros(self.n_nodes, np.int32)
+        self.leaves = np.zeros((self.n_leaves, 2), np.int32)
+        lib().orc_octree_get(self.h, self.order, self.tags, self.nodes, self.leaves.reshape(-1))
+        self.positions = pos[self.order]
+
+    def fmm_matvec(self, order, kernel, q):
+        """y = K q (q in tree order), one right-hand side."""
+        q = np.ascontiguousarray(q, np.float64)
+        y = np.zeros_like(q)
+        kd = KernelC(kernel[0], 0, kernel[1], 0.0, 0.0, 0.0)
+        rc = lib().orc_fmm_matvec(self.h, int(order), ctypes.byref(kd), q, y)
+        if rc != 0:
+            raise ValueError("unsupported Chebyshev order %d" % order)
+        return y
+
+    def __del__(self):
+        try:
+            lib().orc_octree_destroy(self.h)
+        except Exception:
+            pass
+
+
+def cheb_node(i, n):
+    return lib().orc_cheb_node(i, n)
+
+
+def cheb_Sn(x, i, n):
+    return lib().orc_cheb_Sn(float(x), i, n)

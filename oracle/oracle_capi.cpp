@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "cells_search.hpp"
+#include "fmm.hpp"
 #include "kernels.hpp"
 
 using namespace orc;
@@ -38,6 +39,11 @@ void init(AnyCells *h, const double *lo, const double *hi, const uint8_t *per, d
   g.sized_with_n = std::numeric_limits<size_t>::max();
   // set_domain at init_neighbour_search time sizes with the (empty) alive set
   size_grid<D>(g, 0);
+}
+
+template <int N> void fmm_n(Octree<3> *t, const Kernel *k, const double *q, double *y) {
+  FMM<3, N> f(*t, *k);
+  f.matvec(q, y);
 }
 
 template <class F> auto dispatch(int dim, F &&f) {
@@ -192,5 +198,57 @@ void orc_dense_matvec(int dim, int64_t na, const double *a, int64_t nb, const do
 }
 
 int orc_max_threads() { return omp_get_max_threads(); }
+
+// ---------------------------------------------------------------------------
+// HyperOctree + black-box Chebyshev FMM (3D, one RHS per call)
+// ---------------------------------------------------------------------------
+
+void *orc_octree_create(const double *lo, const double *hi, int64_t n, const double *pos,
+                        double n_leaf) {
+  auto *t = new Octree<3>();
+  for (int d = 0; d < 3; ++d) {
+    t->bounds.lo[d] = lo[d];
+    t->bounds.hi[d] = hi[d];
+  }
+  t->build(pos, size_t(n), n_leaf);
+  return t;
+}
+void orc_octree_destroy(void *p) { delete static_cast<Octree<3> *>(p); }
+// counts[0] = nodes, [1] = leaves, [2] = levels
+void orc_octree_info(void *p, int64_t *counts) {
+  auto *t = static_cast<Octree<3> *>(p);
+  counts[0] = int64_t(t->nodes.size());
+  counts[1] = int64_t(t->leaves.size());
+  counts[2] = t->levels;
+}
+void orc_octree_get(void *p, int32_t *order, int32_t *tags, int32_t *nodes, int32_t *leaves) {
+  auto *t = static_cast<Octree<3> *>(p);
+  if (order) std::memcpy(order, t->order.data(), t->order.size() * 4);
+  if (tags) std::memcpy(tags, t->tags.data(), t->tags.size() * 4);
+  if (nodes) std::memcpy(nodes, t->nodes.data(), t->nodes.size() * 4);
+  if (leaves)
+    for (size_t i = 0; i < t->leaves.size(); ++i) {
+      leaves[2 * i] = t->leaves[i].first;
+      leaves[2 * i + 1] = t->leaves[i].second;
+    }
+}
+
+// y += K q with Chebyshev order n; q, y in tree (sorted) order.
+int orc_fmm_matvec(void *p, int order, const Kernel *k, const double *q, double *y) {
+  auto *t = static_cast<Octree<3> *>(p);
+  switch (order) {
+    case 2: fmm_n<2>(t, k, q, y); return 0;
+    case 3: fmm_n<3>(t, k, q, y); return 0;
+    case 4: fmm_n<4>(t, k, q, y); return 0;
+    case 5: fmm_n<5>(t, k, q, y); return 0;
+    case 6: fmm_n<6>(t, k, q, y); return 0;
+    case 8: fmm_n<8>(t, k, q, y); return 0;
+    default: return -1;
+  }
+}
+
+// Chebyshev helpers for the closed-form checks
+double orc_cheb_node(unsigned i, unsigned n) { return cheb_node(i, n); }
+double orc_cheb_Sn(double x, unsigned i, unsigned n) { return cheb_Sn(x, i, n); }
 
 }  // extern "C"
