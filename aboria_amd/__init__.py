@@ -16,11 +16,12 @@ from ._capi import AbhError, check
 __all__ = [
     "Context", "CellListOrdered", "Particles", "SearchResult", "euclidean_search",
     "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
-    "FP64", "FP32_COMPENSATED",
+    "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
 ]
 
 FP64 = C.FP64
 FP32_COMPENSATED = C.FP32_COMPENSATED
+MIXED = C.MIXED
 
 
 class _CAI:
@@ -388,3 +389,61 @@ def create_sparse_operator(row_particles, col_particles, radius, kernel, row_sca
     cs = col_particles.columns[col_scalar] if isinstance(col_scalar, str) else col_scalar
     return SparseOperator(col_particles.ctx, col_particles.get_query(), row_particles.position,
                           radius, kernel, rs, cs)
+
+
+class FmmOperator:
+    """Black-box Chebyshev FMM operator over a HyperOctree (3D)."""
+
+    def __init__(self, ctx, positions, low, high, n_particles_in_leaf, order, kernel):
+        _require_cuda(positions, torch.float64, "positions")
+        self.ctx = ctx
+        self.n = positions.shape[0]
+        self._keep = positions
+        lo = (ctypes.c_double * 3)(*[float(v) for v in low])
+        hi = (ctypes.c_double * 3)(*[float(v) for v in high])
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_fmm_create(ctx.handle, 3, lo, hi, self.n, _ptr(positions),
+                                     float(n_particles_in_leaf), int(order), ctypes.byref(kd),
+                                     ctypes.byref(h)), ctx.handle)
+        self.handle = h
+
+    @property
+    def info(self):
+        i = C.FmmInfo()
+        check(C.lib().abh_fmm_get_info(self.handle, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in C.FmmInfo._fields_}
+
+    def order(self):
+        return _view(C.lib().abh_fmm_order(self.handle), (self.n,), torch.int32, self.ctx.device)
+
+    def apply_colmajor(self, xt, yt, precision=FP64):
+        """yt += (K x)^T for (nrhs, n) contiguous xt, yt."""
+        check(C.lib().abh_fmm_apply(self.handle, xt.shape[0], _ptr(xt), xt.shape[1], _ptr(yt),
+                                    yt.shape[1], precision), self.ctx.handle)
+        return yt
+
+    def __matmul__(self, x):
+        x2 = x if x.dim() == 2 else x[:, None]
+        xt = x2.t().contiguous()
+        yt = torch.zeros_like(xt)
+        self.apply_colmajor(xt, yt)
+        y = yt.t()
+        return y if x.dim() == 2 else y[:, 0].contiguous()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_fmm_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def create_fmm_operator(particles, order, kernel, low, high, n_particles_in_leaf=10.0):
+    """create_fmm_operator<N>(particles, particles, f, f) on a HyperOctree."""
+    return FmmOperator(particles.ctx, particles.position, low, high, n_particles_in_leaf, order,
+                       kernel)

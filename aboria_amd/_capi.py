@@ -64,6 +64,21 @@ class Kernel(ctypes.Structure):
     ]
 
 
+class FmmInfo(ctypes.Structure):
+    _fields_ = [
+        ("order", ctypes.c_int32),
+        ("levels", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int64),
+        ("n_leaves", ctypes.c_int64),
+        ("m2l_pairs", ctypes.c_int64),
+        ("p2p_pairs", ctypes.c_int64),
+        ("p2p_interactions", ctypes.c_int64),
+    ]
+
+
+MIXED = 2
+
+
 class AbhError(RuntimeError):
     def __init__(self, status, message):
         self.status = status
@@ -103,6 +118,12 @@ SIGNATURES = {
                                      c_void_p, c_void_p, P(c_void_p)]),
     "abh_sparse_op_destroy": (c_int, [c_void_p]),
     "abh_sparse_op_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+    "abh_fmm_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), c_int64, c_void_p,
+                               c_double, c_int, P(Kernel), P(c_void_p)]),
+    "abh_fmm_destroy": (c_int, [c_void_p]),
+    "abh_fmm_get_info": (c_int, [c_void_p, P(FmmInfo)]),
+    "abh_fmm_order": (c_void_p, [c_void_p]),
+    "abh_fmm_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
 }
 
 _lib = None

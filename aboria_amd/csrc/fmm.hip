@@ -1,0 +1,1035 @@
+// Black-box Chebyshev FMM over a HyperOctree (3D).
+//
+// Set-up (abh_fmm_create):
+//   - octree tags on the GPU (point_to_tag, src/detail/SpatialUtil.h:255-286),
+//     stable radix sort, tree order gather;
+//   - the node / leaf structure on the host from the sorted tags, level by
+//     level exactly as HyperOctree::build_tree (src/OctTree.h:328-475);
+//   - the reference's dual traversal (src/FastMultipoleMethod.h:442-548 with
+//     theta_condition, src/detail/FastMultipoleMethod.h:1249-1279) on the
+//     host to list the P2P (target leaf, source leaf) pairs and count the
+//     M2L pairs;
+//   - per level a dense map from box coordinates to node, which the M2L
+//     kernel uses to enumerate each target's interaction list (same-level
+//     boxes whose parents touch the target's parent, minus the boxes that
+//     touch the target -- exactly the M2L pairs of the traversal);
+//   - per level and box offset o in [-3,3]^3 the M2L matrix K(t_i, s_j)
+//     between the Chebyshev nodes of the two boxes (boxes of one level are
+//     congruent, so the operator depends on the offset only).
+//
+// Apply (y += K x, 16 right-hand sides per pass):
+//   P2M per leaf, M2M per level (separable: three 1-D contractions with the
+//   6x6 child-interpolation matrices), M2L per level with targets grouped in
+//   sibling tiles of 8 and the offsets looped in a fixed order (each tile
+//   owns its targets' local expansions: no atomics, deterministic), L2L per
+//   level, then L2P + P2P per target leaf.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernel_fn.hpp"
+
+using abh::DevBuf;
+using abh::KernelDesc;
+
+namespace {
+
+constexpr int kMaxLevel = 8;  // 32 / D - 2 for D = 3 (src/OctTree.h:131)
+constexpr int kRhs = 16;      // right-hand sides per pass
+constexpr int kOffsets = 343;  // [-3,3]^3
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kPi2 = kPi / 2;
+
+// ---------------------------------------------------------------------------
+// host helpers (chebyshev_node / chebyshev_Sn, src/detail/Chebyshev.h:101-153)
+// ---------------------------------------------------------------------------
+
+double cheb_node(unsigned i, unsigned n) { return std::cos((2.0 * i + 1.0) * kPi / (2.0 * n)); }
+
+double cheb_Sn(double x, unsigned i, unsigned n) {
+  double b1 = 0.0, b2 = 0.0;
+  const double invn = 1.0 / n;
+  for (unsigned k = n - 1; k >= 1; --k) {
+    const double bk = 2 * invn * std::cos(k * (2.0 * i + 1.0) * kPi2 * invn) + 2.0 * x * b1 - b2;
+    b2 = b1;
+    b1 = bk;
+  }
+  return invn + x * b1 - b2;
+}
+
+struct Box {
+  double lo[3], hi[3];
+};
+
+Box child_box(const Box &b, int c) {
+  Box r = b;
+  for (int i = 0; i < 3; ++i) {
+    const bool high = c & (1 << (2 - i));
+    if (high) r.lo[i] = 0.5 * (r.hi[i] + r.lo[i]);
+    else r.hi[i] = 0.5 * (r.hi[i] + r.lo[i]);
+  }
+  return r;
+}
+
+// theta_condition::check: true = too close for M2L
+bool theta_close(const Box &t, const Box &s) {
+  double md = std::numeric_limits<double>::min();
+  for (int i = 0; i < 3; ++i)
+    if (t.hi[i] - t.lo[i] > md) md = t.hi[i] - t.lo[i];
+  double sq = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double dist = 0.0;
+    if (t.hi[i] < s.lo[i]) dist = s.lo[i] - t.hi[i];
+    else if (s.hi[i] < t.lo[i]) dist = t.lo[i] - s.hi[i];
+    if (s.hi[i] - s.lo[i] > md) md = s.hi[i] - s.lo[i];
+    sq += dist * dist;
+  }
+  return md * md > 1.0 * sq;
+}
+
+// ---------------------------------------------------------------------------
+// device: tags
+// ---------------------------------------------------------------------------
+
+struct Bounds {
+  double lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(256) void k_tags(const double *__restrict__ pos, int64_t n, Bounds b,
+                                              uint32_t *__restrict__ keys,
+                                              int32_t *__restrict__ vals) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  double p[3], lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    p[d] = pos[i * 3 + d];
+    lo[d] = b.lo[d];
+    hi[d] = b.hi[d];
+  }
+  int result = 0;
+  for (int level = 1; level <= kMaxLevel; ++level) {
+    int hh[3];
+    double mid[3];
+    for (int d = 0; d < 3; ++d) {
+      mid[d] = 0.5 * (lo[d] + hi[d]);
+      hh[d] = (p[d] < mid[d]) ? 0 : 1;
+      result |= hh[d];
+      result <<= 1;
+    }
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = hh[d] ? mid[d] : lo[d];
+      hi[d] = hh[d] ? hi[d] : mid[d];
+    }
+  }
+  result >>= 1;
+  keys[i] = uint32_t(result);
+  vals[i] = int32_t(i);
+}
+
+// ---------------------------------------------------------------------------
+// device: operators
+// ---------------------------------------------------------------------------
+
+template <int N> struct Cheb {
+  static constexpr int NC = N * N * N;
+};
+
+struct FmmConsts {
+  double node[8];        // Chebyshev nodes on [-1,1]
+  double clen[8][8];     // Clenshaw coefficients a[i][k] = 2/n cos(k(2i+1)pi/(2n))
+  double s1[2][8][8];    // child half h: s1[h][i][j] = S_n(u'_j, i), u' = node j of the
+                         // child mapped into the parent's [-1,1]
+};
+
+// chebyshev_Sn(x, i, n) with precomputed Clenshaw coefficients
+template <int N> __device__ __forceinline__ double sn_dev(const FmmConsts &c, double x, int i) {
+  double b1 = 0.0, b2 = 0.0;
+  const double invn = 1.0 / N;
+  for (int k = N - 1; k >= 1; --k) {
+    const double bk = c.clen[i][k] + 2.0 * x * b1 - b2;
+    b2 = b1;
+    b1 = bk;
+  }
+  return invn + x * b1 - b2;
+}
+
+// M2L matrices: M[l][o][i][j] = K(target node i, source node j) for boxes of
+// level l at offset o (in box units).
+template <int N>
+__global__ __launch_bounds__(256) void k_m2l_ops(double *__restrict__ M, int level0, int nlev,
+                                                 Bounds b, KernelDesc k, FmmConsts c) {
+  constexpr int NC = Cheb<N>::NC;
+  const int64_t total = int64_t(nlev) * kOffsets * NC * NC;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    int64_t r = t;
+    const int j = int(r % NC);
+    r /= NC;
+    const int i = int(r % NC);
+    r /= NC;
+    const int o = int(r % kOffsets);
+    const int l = level0 + int(r / kOffsets);
+    const int off[3] = {o / 49 - 3, (o / 7) % 7 - 3, o % 7 - 3};
+    const int ii[3] = {i / (N * N), (i / N) % N, i % N};
+    const int jj[3] = {j / (N * N), (j / N) % N, j % N};
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double h = (b.hi[d] - b.lo[d]) / double(1 << l);
+      const double ti = 0.5 * (c.node[ii[d]] + 1) * h;
+      const double sj = 0.5 * (c.node[jj[d]] + 1) * h + double(off[d]) * h;
+      const double dx = sj - ti;
+      s = s + dx * dx;
+    }
+    M[t] = abh::phi64(k, s);
+  }
+}
+
+// x (column-major, input order) -> xs[p][r] in tree order
+__global__ __launch_bounds__(256) void k_gather_rhs(const int32_t *__restrict__ order, int64_t n,
+                                                    int nrhs, const double *__restrict__ x,
+                                                    int64_t ldx, double *__restrict__ xs) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= n * kRhs) return;
+  const int64_t p = t / kRhs;
+  const int r = int(t % kRhs);
+  xs[t] = r < nrhs ? x[int64_t(r) * ldx + order[p]] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_rhs(const int32_t *__restrict__ order, int64_t n,
+                                                     int nrhs, const double *__restrict__ ys,
+                                                     double *__restrict__ y, int64_t ldy) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= n * int64_t(nrhs)) return;
+  const int r = int(t / n);
+  const int64_t p = t % n;
+  y[int64_t(r) * ldy + order[p]] += ys[p * kRhs + r];
+}
+
+// P2M: W[node][j][r] = sum_p R_j(x_p; leaf box) q_p[r]
+template <int N>
+__global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaf_nodes, int nleaf,
+                                             const int2 *__restrict__ leaf_range,
+                                             const double *__restrict__ box,  // 6 per node
+                                             const double *__restrict__ pos,
+                                             const double *__restrict__ xs,
+                                             double *__restrict__ W, FmmConsts c) {
+  constexpr int NC = Cheb<N>::NC;
+  __shared__ double S[64][3][N];
+  __shared__ double Q[64][kRhs];
+  const int li = blockIdx.x;
+  if (li >= nleaf) return;
+  const int node = leaf_nodes[li];
+  const int2 rg = leaf_range[li];
+  double acc[(NC * kRhs + 255) / 256];
+#pragma unroll
+  for (int u = 0; u < (NC * kRhs + 255) / 256; ++u) acc[u] = 0.0;
+  double blo[3], bhi[3];
+  for (int d = 0; d < 3; ++d) {
+    blo[d] = box[int64_t(node) * 6 + d];
+    bhi[d] = box[int64_t(node) * 6 + 3 + d];
+  }
+  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
+    const int np = min(64, rg.y - p0);
+    __syncthreads();
+    if (threadIdx.x < np) {
+      const int p = p0 + threadIdx.x;
+      for (int d = 0; d < 3; ++d) {
+        const double span = bhi[d] - blo[d];
+        const double x = span > 0.0 ? (2 * pos[int64_t(p) * 3 + d] - blo[d] - bhi[d]) / span : 0.0;
+        for (int m = 0; m < N; ++m) S[threadIdx.x][d][m] = sn_dev<N>(c, x, m);
+      }
+    }
+    for (int t = threadIdx.x; t < np * kRhs; t += 256)
+      Q[t / kRhs][t % kRhs] = xs[int64_t(p0 + t / kRhs) * kRhs + t % kRhs];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < (NC * kRhs + 255) / 256; ++u) {
+      const int o = u * 256 + threadIdx.x;
+      if (o < NC * kRhs) {
+        const int j = o / kRhs, r = o % kRhs;
+        const int jx = j / (N * N), jy = (j / N) % N, jz = j % N;
+        double a = acc[u];
+        for (int p = 0; p < np; ++p) a += ((S[p][0][jx] * S[p][1][jy]) * S[p][2][jz]) * Q[p][r];
+        acc[u] = a;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < (NC * kRhs + 255) / 256; ++u) {
+    const int o = u * 256 + threadIdx.x;
+    if (o < NC * kRhs) W[int64_t(node) * NC * kRhs + o] = acc[u];
+  }
+}
+
+// Separable child <-> parent transfer.  M2M: W_p[i] += sum_j prod_d
+// s1[h_d][i_d][j_d] W_c[j]; L2L (transpose): G_c[i] += sum_j prod_d
+// s1[h_d][j_d][i_d] G_p[j].
+template <int N, bool UP>
+__global__ __launch_bounds__(256) void k_transfer(const int32_t *__restrict__ parents, int npar,
+                                                  const int32_t *__restrict__ child0,
+                                                  double *__restrict__ E, FmmConsts c) {
+  constexpr int NC = Cheb<N>::NC;
+  constexpr int R = 8;  // rhs per LDS pass
+  __shared__ double A[NC * R], B[NC * R];
+  const int pi = blockIdx.x;
+  if (pi >= npar) return;
+  const int parent = parents[pi];
+  const int first = child0[parent];
+  for (int r0 = 0; r0 < kRhs; r0 += R) {
+    double accp[(NC * R + 255) / 256];
+#pragma unroll
+    for (int u = 0; u < (NC * R + 255) / 256; ++u) accp[u] = 0.0;
+    if (!UP) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < NC * R; t += 256)
+        A[t] = E[int64_t(parent) * NC * kRhs + (t / R) * kRhs + r0 + t % R];
+    }
+    for (int ch = 0; ch < 8; ++ch) {
+      const int child = first + ch;
+      const int h[3] = {(ch >> 2) & 1, (ch >> 1) & 1, ch & 1};
+      __syncthreads();
+      if (UP) {
+        for (int t = threadIdx.x; t < NC * R; t += 256)
+          B[t] = E[int64_t(child) * NC * kRhs + (t / R) * kRhs + r0 + t % R];
+      } else {
+        for (int t = threadIdx.x; t < NC * R; t += 256) B[t] = A[t];
+      }
+      __syncthreads();
+      // three 1-D passes over z, y, x: src -> dst alternating B -> A' ...
+      // pass z: T1[jx][jy][iz] = sum_jz m_z(iz, jz) src[jx][jy][jz]
+      double tmp[(NC * R + 255) / 256];
+#pragma unroll
+      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
+        const int o = u * 256 + threadIdx.x;
+        tmp[u] = 0.0;
+        if (o < NC * R) {
+          const int e = o / R, r = o % R;
+          const int x = e / (N * N), y = (e / N) % N, z = e % N;
+          double a = 0.0;
+          for (int k = 0; k < N; ++k) {
+            const double m = UP ? c.s1[h[2]][z][k] : c.s1[h[2]][k][z];
+            a += m * B[((x * N + y) * N + k) * R + r];
+          }
+          tmp[u] = a;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
+        const int o = u * 256 + threadIdx.x;
+        if (o < NC * R) B[o] = tmp[u];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
+        const int o = u * 256 + threadIdx.x;
+        tmp[u] = 0.0;
+        if (o < NC * R) {
+          const int e = o / R, r = o % R;
+          const int x = e / (N * N), y = (e / N) % N, z = e % N;
+          double a = 0.0;
+          for (int k = 0; k < N; ++k) {
+            const double m = UP ? c.s1[h[1]][y][k] : c.s1[h[1]][k][y];
+            a += m * B[((x * N + k) * N + z) * R + r];
+          }
+          tmp[u] = a;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
+        const int o = u * 256 + threadIdx.x;
+        if (o < NC * R) B[o] = tmp[u];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
+        const int o = u * 256 + threadIdx.x;
+        if (o < NC * R) {
+          const int e = o / R, r = o % R;
+          const int x = e / (N * N), y = (e / N) % N, z = e % N;
+          double a = 0.0;
+          for (int k = 0; k < N; ++k) {
+            const double m = UP ? c.s1[h[0]][x][k] : c.s1[h[0]][k][x];
+            a += m * B[((k * N + y) * N + z) * R + r];
+          }
+          if (UP) {
+            accp[u] += a;
+          } else {
+            double *g = &E[int64_t(child) * NC * kRhs + e * kRhs + r0 + r];
+            *g += a;
+          }
+        }
+      }
+    }
+    if (UP) {
+#pragma unroll
+      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
+        const int o = u * 256 + threadIdx.x;
+        if (o < NC * R) E[int64_t(parent) * NC * kRhs + (o / R) * kRhs + r0 + o % R] = accp[u];
+      }
+    }
+  }
+}
+
+// M2L, FP64: one block per sibling tile (the 8 children of a level-(l-1)
+// node); each thread accumulates a 14 x 8 block of the tile's 224 x 128
+// output (216 node rows padded, 8 targets x 16 right-hand sides).
+template <int N>
+__global__ __launch_bounds__(256) void k_m2l_f64(const int32_t *__restrict__ tile_parent,
+                                                 int ntiles, const int32_t *__restrict__ child0,
+                                                 const int32_t *__restrict__ coord,  // 3 per node
+                                                 const int32_t *__restrict__ map, int side,
+                                                 const double *__restrict__ M,  // [343][NC][NC]
+                                                 const double *__restrict__ W,
+                                                 double *__restrict__ G) {
+  constexpr int NC = Cheb<N>::NC;
+  constexpr int RP = (NC + 15) / 16;  // rows per thread
+  constexpr int KC = 8;               // k chunk
+  __shared__ double As[16 * RP][KC + 1];
+  __shared__ double Bs[KC][8 * kRhs];
+  __shared__ int src[8];
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int parent = tile_parent[tile];
+  const int first = child0[parent];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
+  double acc[RP][8];
+#pragma unroll
+  for (int a = 0; a < RP; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+  for (int o = 0; o < kOffsets; ++o) {
+    const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
+    if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+      const int ch = threadIdx.x;
+      const int cx = (ch >> 2) & 1, cy = (ch >> 1) & 1, cz = ch & 1;
+      // parent of the source must touch the target's parent
+      const int qx = cx + ox, qy = cy + oy, qz = cz + oz;  // in [-3, 4]
+      int s = -1;
+      if (qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
+        const int sx = 2 * pc[0] + qx, sy = 2 * pc[1] + qy, sz = 2 * pc[2] + qz;
+        if (sx >= 0 && sx < side && sy >= 0 && sy < side && sz >= 0 && sz < side)
+          s = map[(int64_t(sx) * side + sy) * side + sz];
+      }
+      src[ch] = s;
+    }
+    __syncthreads();
+    bool any = false;
+    for (int ch = 0; ch < 8; ++ch) any |= src[ch] >= 0;
+    if (!any) continue;
+    const double *Mo = M + int64_t(o) * NC * NC;
+    for (int k0 = 0; k0 < NC; k0 += KC) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < 16 * RP * KC; t += 256) {
+        const int i = t / KC, k = t % KC;
+        As[i][k] = (i < NC && k0 + k < NC) ? Mo[int64_t(i) * NC + k0 + k] : 0.0;
+      }
+      for (int t = threadIdx.x; t < KC * 8 * kRhs; t += 256) {
+        const int k = t / (8 * kRhs), col = t % (8 * kRhs);
+        const int ch = col / kRhs, r = col % kRhs;
+        const int s = src[ch];
+        Bs[k][col] = (s >= 0 && k0 + k < NC) ? W[int64_t(s) * NC * kRhs + (k0 + k) * kRhs + r] : 0.0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        double av[RP], bv[8];
+#pragma unroll
+        for (int a = 0; a < RP; ++a) av[a] = As[ty + 16 * a][k];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) bv[b] = Bs[k][tx + 16 * b];
+#pragma unroll
+        for (int a = 0; a < RP; ++a)
+#pragma unroll
+          for (int b = 0; b < 8; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < RP; ++a) {
+    const int i = ty + 16 * a;
+    if (i >= NC) continue;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int col = tx + 16 * b;
+      const int ch = col / kRhs, r = col % kRhs;
+      const int t = first + ch;
+      G[int64_t(t) * NC * kRhs + i * kRhs + r] += acc[a][b];
+    }
+  }
+}
+
+// L2P + P2P for one target leaf, 256 threads: thread t serves target particle
+// t % 64 of the current round; for L2P its quarter t / 64 of the 16
+// right-hand sides, for P2P every fourth source (partial sums of the four
+// quarters are combined through LDS).
+template <int N, bool FP32>
+__global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_nodes, int nleaf,
+                                              const int2 *__restrict__ leaf_range,
+                                              const int32_t *__restrict__ p2p_ptr,
+                                              const int32_t *__restrict__ p2p_src,
+                                              const double *__restrict__ box,
+                                              const double *__restrict__ pos,
+                                              const double *__restrict__ xs,
+                                              const double *__restrict__ G,
+                                              double *__restrict__ ys, KernelDesc k, FmmConsts c) {
+  constexpr int NC = Cheb<N>::NC;
+  constexpr int SRC = 256;  // sources staged per round
+  // leaf local expansion, then the [4][64][kRhs] reduction of the P2P parts
+  __shared__ double Gs[NC * kRhs > 4 * 64 * kRhs ? NC * kRhs : 4 * 64 * kRhs];
+  __shared__ double Ps[SRC][3];
+  __shared__ double Qs[SRC][kRhs];
+  const int li = blockIdx.x;
+  if (li >= nleaf) return;
+  const int node = leaf_nodes[li];
+  const int2 rg = leaf_range[li];
+  const int pl = threadIdx.x & 63, part = threadIdx.x >> 6;
+  double blo[3], bhi[3];
+  for (int d = 0; d < 3; ++d) {
+    blo[d] = box[int64_t(node) * 6 + d];
+    bhi[d] = box[int64_t(node) * 6 + 3 + d];
+  }
+  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
+    const int p = p0 + pl;
+    const bool act = p < rg.y;
+    double x[3] = {0.0, 0.0, 0.0};
+    if (act)
+      for (int d = 0; d < 3; ++d) x[d] = pos[int64_t(p) * 3 + d];
+    __syncthreads();
+    for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
+    __syncthreads();
+    // L2P for right-hand sides 4*part .. 4*part+3
+    double l2p[4] = {0.0, 0.0, 0.0, 0.0};
+    if (act) {
+      double S[3][N];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double span = bhi[d] - blo[d];
+        const double u = span > 0.0 ? (2 * x[d] - blo[d] - bhi[d]) / span : 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) S[d][m] = sn_dev<N>(c, u, m);
+      }
+#pragma unroll
+      for (int jx = 0; jx < N; ++jx)
+#pragma unroll
+        for (int jy = 0; jy < N; ++jy) {
+          const double sxy = S[0][jx] * S[1][jy];
+#pragma unroll
+          for (int jz = 0; jz < N; ++jz) {
+            const double R = sxy * S[2][jz];
+            const int j = (jx * N + jy) * N + jz;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) l2p[r] += R * Gs[j * kRhs + 4 * part + r];
+          }
+        }
+    }
+    // P2P: this thread takes sources part, part+4, ...
+    double acc[kRhs];
+    float a32[kRhs], c32[kRhs];
+#pragma unroll
+    for (int r = 0; r < kRhs; ++r) {
+      acc[r] = 0.0;
+      a32[r] = 0.0f;
+      c32[r] = 0.0f;
+    }
+    for (int e = p2p_ptr[li]; e < p2p_ptr[li + 1]; ++e) {
+      const int2 sr = leaf_range[p2p_src[e]];
+      for (int s0 = sr.x; s0 < sr.y; s0 += SRC) {
+        const int ns = min(SRC, sr.y - s0);
+        __syncthreads();
+        if (threadIdx.x < ns) {
+          const int q = s0 + threadIdx.x;
+          for (int d = 0; d < 3; ++d) Ps[threadIdx.x][d] = pos[int64_t(q) * 3 + d];
+        }
+        for (int t = threadIdx.x; t < ns * kRhs; t += 256)
+          Qs[t / kRhs][t % kRhs] = xs[int64_t(s0 + t / kRhs) * kRhs + t % kRhs];
+        __syncthreads();
+        if (act) {
+          for (int j = part; j < ns; j += 4) {
+            double s2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+              const double dx = Ps[j][d] - x[d];
+              s2 = s2 + dx * dx;
+            }
+            if (FP32) {
+              const float f = abh::phi32(k, float(s2));
+#pragma unroll
+              for (int r = 0; r < kRhs; ++r) abh::kb_add(a32[r], c32[r], f * float(Qs[j][r]));
+            } else {
+              const double f = abh::phi64(k, s2);
+#pragma unroll
+              for (int r = 0; r < kRhs; ++r) acc[r] += f * Qs[j][r];
+            }
+          }
+        }
+      }
+    }
+    // combine the four partial P2P sums (Gs is free now)
+    __syncthreads();
+    double *red = Gs;  // [4][64][kRhs]
+#pragma unroll
+    for (int r = 0; r < kRhs; ++r)
+      red[(part * 64 + pl) * kRhs + r] = acc[r] + (FP32 ? double(a32[r]) + double(c32[r]) : 0.0);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = 4 * part + r;
+        double v = l2p[r];
+        for (int q = 0; q < 4; ++q) v += red[(q * 64 + pl) * kRhs + rr];
+        ys[int64_t(p) * kRhs + rr] = v;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the FMM object
+// ---------------------------------------------------------------------------
+
+struct abh_fmm {
+  abh_context *ctx = nullptr;
+  int order = 6;
+  int64_t n = 0;
+  Box bounds{};
+  KernelDesc kern{};
+  FmmConsts consts{};
+  int nlevels = 0;
+  abh_fmm_info info{};
+  // host tree
+  std::vector<int32_t> nodes;
+  std::vector<std::pair<int, int>> leaves;
+  // device
+  DevBuf order_buf, pos, keys, box, child0, coord, leaf_nodes, leaf_range, p2p_ptr, p2p_src,
+      M, W, G, xs, ys, maps;
+  std::vector<int64_t> map_off;          // per level offset into maps
+  std::vector<int64_t> par_off;          // per level offset into parents
+  DevBuf parents;                        // non-leaf nodes, level by level
+  std::vector<std::vector<int32_t>> parents_at;  // non-leaf nodes per level
+  int nleaf = 0;
+  int64_t n_nodes = 0;
+};
+
+namespace {
+
+struct HostBuild {
+  std::vector<int32_t> nodes;
+  std::vector<std::pair<int, int>> leaves;
+  std::vector<int8_t> level;
+  std::vector<int32_t> coord;  // 3 per node
+  std::vector<Box> boxes;
+  std::vector<int32_t> leaf_of;  // node -> leaf index (or -1)
+  int nlevels = 0;
+};
+
+// HyperOctree::build_tree (src/OctTree.h:328-475) from sorted tags.
+void build_tree(HostBuild &hb, const std::vector<uint32_t> &tags, double n_leaf, const Box &root) {
+  constexpr int D = 3, nchild = 8;
+  struct Active {
+    int tag;
+    int node;  // -1 for the root
+  };
+  std::vector<Active> active{{0, -1}};
+  for (int level = 1; !active.empty(); ++level) {
+    ABH_REQUIRE(level <= kMaxLevel, ABH_ERR_INVALID_ARG, "octree exceeded its maximum depth");
+    const int children_begin = int(hb.nodes.size());
+    const int nch = int(active.size()) * nchild;
+    hb.nodes.resize(hb.nodes.size() + nch);
+    hb.level.resize(hb.nodes.size());
+    hb.coord.resize(hb.nodes.size() * 3);
+    hb.boxes.resize(hb.nodes.size());
+    hb.leaf_of.resize(hb.nodes.size(), -1);
+    const int length = (1 << (kMaxLevel - level) * D) - 1;
+    std::vector<Active> next;
+    int node_idx = 0;
+    for (int c = 0; c < nch; ++c) {
+      const Active &a = active[c / nchild];
+      const int which = c & (nchild - 1);
+      const int child_tag = a.tag | (which << ((kMaxLevel - level) * D));
+      const int k = children_begin + c;
+      const int lb = int(std::lower_bound(tags.begin(), tags.end(), uint32_t(child_tag)) - tags.begin());
+      const int ub = int(std::upper_bound(tags.begin(), tags.end(), uint32_t(child_tag + length)) -
+                         tags.begin());
+      const int count = ub - lb;
+      hb.level[k] = int8_t(level);
+      const Box pb = a.node < 0 ? root : hb.boxes[a.node];
+      hb.boxes[k] = child_box(pb, which);
+      for (int d = 0; d < 3; ++d) {
+        const int pcrd = a.node < 0 ? 0 : hb.coord[a.node * 3 + d];
+        hb.coord[k * 3 + d] = 2 * pcrd + ((which >> (2 - d)) & 1);
+      }
+      if (count == 0) {
+        hb.nodes[k] = int32_t(0xffffffff);
+      } else if (level == kMaxLevel || count <= n_leaf) {
+        hb.leaf_of[k] = int32_t(hb.leaves.size());
+        hb.nodes[k] = int32_t(0x80000000u | uint32_t(hb.leaves.size()));
+        hb.leaves.push_back({lb, ub});
+      } else {
+        hb.nodes[k] = int32_t(hb.nodes.size()) + nchild * node_idx;
+        ++node_idx;
+        next.push_back({child_tag, k});
+      }
+    }
+    active.swap(next);
+    hb.nlevels = level;
+  }
+}
+
+bool is_leaf_code(int32_t id) { return id < 0; }
+
+// The reference's downward traversal (calculate_dive_M2L_and_L2L) reduced to
+// its interaction lists: P2P pairs (target leaf, source leaf) and the M2L
+// pair count.
+struct Lists {
+  std::vector<std::vector<int32_t>> p2p;  // per leaf index: source leaf indices
+  int64_t m2l = 0;
+};
+
+void collect_leaves(const HostBuild &hb, int k, std::vector<int32_t> &out) {
+  const int32_t id = hb.nodes[k];
+  if (is_leaf_code(id)) {
+    if (hb.leaf_of[k] >= 0) out.push_back(hb.leaf_of[k]);
+    return;
+  }
+  for (int c = 0; c < 8; ++c) collect_leaves(hb, id + c, out);
+}
+
+void traverse(const HostBuild &hb, int t, const std::vector<int32_t> &conn_parent, Lists &L) {
+  std::vector<int32_t> conn;
+  const Box &tb = hb.boxes[t];
+  if (conn_parent.empty()) {
+    for (int c = 0; c < 8; ++c) {
+      if (theta_close(tb, hb.boxes[c])) conn.push_back(c);
+      else ++L.m2l;
+    }
+  } else {
+    for (int32_t src : conn_parent) {
+      const int32_t sid = hb.nodes[src];
+      if (is_leaf_code(sid)) {
+        conn.push_back(src);
+      } else {
+        for (int c = 0; c < 8; ++c) {
+          if (theta_close(tb, hb.boxes[sid + c])) conn.push_back(sid + c);
+          else ++L.m2l;
+        }
+      }
+    }
+  }
+  const int32_t tid = hb.nodes[t];
+  if (!is_leaf_code(tid)) {
+    for (int c = 0; c < 8; ++c) traverse(hb, tid + c, conn, L);
+    return;
+  }
+  const int tl = hb.leaf_of[t];
+  if (tl < 0) return;  // empty target
+  std::vector<int32_t> &dst = L.p2p[tl];
+  for (int32_t s : conn) collect_leaves(hb, s, dst);
+}
+
+template <int N> void make_consts(FmmConsts &c) {
+  std::memset(&c, 0, sizeof(c));
+  for (int i = 0; i < N; ++i) c.node[i] = cheb_node(unsigned(i), unsigned(N));
+  const double invn = 1.0 / N;
+  for (int i = 0; i < N; ++i)
+    for (int k = 1; k < N; ++k)
+      c.clen[i][k] = 2 * invn * std::cos(k * (2.0 * i + 1.0) * kPi2 * invn);
+  for (int h = 0; h < 2; ++h)
+    for (int j = 0; j < N; ++j) {
+      // node j of the child half h mapped into the parent's [-1, 1]
+      const double u = h ? 0.5 * (c.node[j] + 1) : 0.5 * (c.node[j] + 1) - 1.0;
+      for (int i = 0; i < N; ++i) c.s1[h][i][j] = cheb_Sn(u, unsigned(i), unsigned(N));
+    }
+}
+
+template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_leaf) {
+  abh_context *ctx = f->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t n = f->n;
+  make_consts<N>(f->consts);
+  // tags, sort, gather
+  f->keys.ensure(size_t(n + 1) * 4);
+  f->order_buf.ensure(size_t(n + 1) * 4);
+  f->pos.ensure(size_t(n + 1) * 24);
+  Bounds b;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = f->bounds.lo[d];
+    b.hi[d] = f->bounds.hi[d];
+  }
+  if (n > 0) {
+    k_tags<<<unsigned(abh::ceil_div(n, 256)), 256, 0, s>>>(pos_in, n, b, f->keys.as<uint32_t>(),
+                                                          f->order_buf.as<int32_t>());
+    abh::check_launch();
+    abh::radix_sort_pairs(ctx, f->keys.as<uint32_t>(), f->order_buf.as<int32_t>(), n,
+                          3 * kMaxLevel);
+    abh::gather_bytes(ctx, f->order_buf.as<int32_t>(), n, pos_in, f->pos.p, 24);
+  }
+  std::vector<uint32_t> tags(static_cast<size_t>(n));
+  if (n > 0)
+    ABH_HIP(hipMemcpyAsync(tags.data(), f->keys.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+  ABH_HIP(hipStreamSynchronize(s));
+  HostBuild hb;
+  build_tree(hb, tags, n_leaf, f->bounds);
+  f->nlevels = hb.nlevels;
+  const int64_t nn = int64_t(hb.nodes.size());
+  f->n_nodes = nn;
+  // traversal lists
+  Lists L;
+  L.p2p.resize(hb.leaves.size());
+  {
+    std::vector<int32_t> none;
+    for (int c = 0; c < 8; ++c) traverse(hb, c, none, L);
+  }
+  // device tree arrays
+  std::vector<double> boxv(size_t(nn) * 6);
+  std::vector<int32_t> child0(static_cast<size_t>(nn)), coordv(static_cast<size_t>(nn) * 3);
+  for (int64_t k = 0; k < nn; ++k) {
+    for (int d = 0; d < 3; ++d) {
+      boxv[k * 6 + d] = hb.boxes[k].lo[d];
+      boxv[k * 6 + 3 + d] = hb.boxes[k].hi[d];
+      coordv[k * 3 + d] = hb.coord[k * 3 + d];
+    }
+    child0[k] = is_leaf_code(hb.nodes[k]) ? -1 : hb.nodes[k];
+  }
+  // non-empty leaves, their node and particle range; P2P CSR
+  std::vector<int32_t> leaf_nodes, leaf_index_of(hb.leaves.size(), -1);
+  std::vector<int2> ranges;
+  for (int64_t k = 0; k < nn; ++k) {
+    const int l = hb.leaf_of[k];
+    if (l < 0) continue;
+    leaf_index_of[l] = int32_t(leaf_nodes.size());
+    leaf_nodes.push_back(int32_t(k));
+    ranges.push_back(make_int2(hb.leaves[l].first, hb.leaves[l].second));
+  }
+  const int nleaf = int(leaf_nodes.size());
+  f->nleaf = nleaf;
+  std::vector<int32_t> pptr(size_t(nleaf) + 1, 0), psrc;
+  int64_t inter = 0;
+  for (int li = 0; li < nleaf; ++li) {
+    const int l = hb.leaf_of[leaf_nodes[li]];
+    for (int32_t sl : L.p2p[l]) {
+      psrc.push_back(leaf_index_of[sl]);
+      inter += int64_t(ranges[li].y - ranges[li].x) *
+               (hb.leaves[sl].second - hb.leaves[sl].first);
+    }
+    pptr[li + 1] = int32_t(psrc.size());
+  }
+  // parents per level (non-leaf nodes) and level maps
+  f->parents_at.assign(size_t(hb.nlevels) + 1, {});
+  f->map_off.assign(size_t(hb.nlevels) + 2, 0);
+  for (int l = 1; l <= hb.nlevels; ++l) {
+    const int64_t side = int64_t(1) << l;
+    f->map_off[l + 1] = f->map_off[l] + side * side * side;
+  }
+  std::vector<int32_t> maps(size_t(f->map_off[hb.nlevels + 1]), -1);
+  for (int64_t k = 0; k < nn; ++k) {
+    const int l = hb.level[k];
+    if (!is_leaf_code(hb.nodes[k])) f->parents_at[l].push_back(int32_t(k));
+    const int64_t side = int64_t(1) << l;
+    if (hb.nodes[k] != int32_t(0xffffffff))  // empty boxes carry no expansion
+      maps[size_t(f->map_off[l] + (hb.coord[k * 3] * side + hb.coord[k * 3 + 1]) * side +
+                  hb.coord[k * 3 + 2])] = int32_t(k);
+  }
+  auto up = [&](DevBuf &buf, const void *src, size_t bytes) {
+    buf.ensure(bytes + 16);
+    if (bytes) ABH_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  up(f->box, boxv.data(), boxv.size() * 8);
+  up(f->child0, child0.data(), child0.size() * 4);
+  up(f->coord, coordv.data(), coordv.size() * 4);
+  up(f->leaf_nodes, leaf_nodes.data(), leaf_nodes.size() * 4);
+  up(f->leaf_range, ranges.data(), ranges.size() * sizeof(int2));
+  up(f->p2p_ptr, pptr.data(), pptr.size() * 4);
+  up(f->p2p_src, psrc.data(), psrc.size() * 4);
+  up(f->maps, maps.data(), maps.size() * 4);
+  {
+    std::vector<int32_t> flat;
+    f->par_off.assign(size_t(hb.nlevels) + 2, 0);
+    for (int l = 1; l <= hb.nlevels; ++l) {
+      f->par_off[l] = int64_t(flat.size());
+      flat.insert(flat.end(), f->parents_at[l].begin(), f->parents_at[l].end());
+    }
+    f->par_off[hb.nlevels + 1] = int64_t(flat.size());
+    up(f->parents, flat.data(), flat.size() * 4);
+  }
+  // M2L operators for levels 2..nlevels
+  constexpr int NC = Cheb<N>::NC;
+  const int nlev = std::max(0, hb.nlevels - 1);
+  if (nlev > 0) {
+    f->M.ensure(size_t(nlev) * kOffsets * NC * NC * 8);
+    k_m2l_ops<N><<<4096, 256, 0, s>>>(f->M.as<double>(), 2, nlev, b, f->kern, f->consts);
+    abh::check_launch();
+  }
+  f->W.ensure(size_t(nn) * NC * kRhs * 8);
+  f->G.ensure(size_t(nn) * NC * kRhs * 8);
+  f->xs.ensure(size_t(n + 1) * kRhs * 8);
+  f->ys.ensure(size_t(n + 1) * kRhs * 8);
+  ABH_HIP(hipStreamSynchronize(s));  // host vectors above go out of scope
+  f->info.order = N;
+  f->info.levels = hb.nlevels;
+  f->info.n_nodes = nn;
+  f->info.n_leaves = nleaf;
+  f->info.m2l_pairs = L.m2l;
+  f->info.p2p_pairs = int64_t(psrc.size());
+  f->info.p2p_interactions = inter;
+}
+
+template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
+                                 int64_t ldy, abh_precision prec) {
+  constexpr int NC = Cheb<N>::NC;
+  abh_context *ctx = f->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t n = f->n;
+  const int64_t nn = f->n_nodes;
+  const int32_t *par = f->parents.as<int32_t>();
+  auto npar = [&](int l) { return int(f->par_off[l + 1] - f->par_off[l]); };
+  for (int r0 = 0; r0 < nrhs; r0 += kRhs) {
+    const int nr = std::min(kRhs, nrhs - r0);
+    k_gather_rhs<<<unsigned(abh::ceil_div(n * kRhs, 256)), 256, 0, s>>>(
+        f->order_buf.as<int32_t>(), n, nr, x + int64_t(r0) * ldx, ldx, f->xs.as<double>());
+    abh::check_launch();
+    ABH_HIP(hipMemsetAsync(f->W.p, 0, size_t(nn) * NC * kRhs * 8, s));
+    ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
+    if (f->nleaf > 0) {
+      k_p2m<N><<<f->nleaf, 256, 0, s>>>(f->leaf_nodes.as<int32_t>(), f->nleaf,
+                                        f->leaf_range.as<int2>(), f->box.as<double>(),
+                                        f->pos.as<double>(), f->xs.as<double>(),
+                                        f->W.as<double>(), f->consts);
+      abh::check_launch();
+    }
+    // M2M: deepest parents first
+    for (int l = f->nlevels - 1; l >= 1; --l) {
+      if (npar(l) == 0) continue;
+      k_transfer<N, true><<<npar(l), 256, 0, s>>>(par + f->par_off[l], npar(l),
+                                                  f->child0.as<int32_t>(), f->W.as<double>(),
+                                                  f->consts);
+      abh::check_launch();
+    }
+    // downward: level l receives L2L from level l-1 and its M2L
+    for (int l = 2; l <= f->nlevels; ++l) {
+      if (npar(l - 1) == 0) continue;
+      k_transfer<N, false><<<npar(l - 1), 256, 0, s>>>(par + f->par_off[l - 1], npar(l - 1),
+                                                       f->child0.as<int32_t>(),
+                                                       f->G.as<double>(), f->consts);
+      abh::check_launch();
+      k_m2l_f64<N><<<npar(l - 1), 256, 0, s>>>(
+          par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
+          f->maps.as<int32_t>() + f->map_off[l], 1 << l,
+          f->M.as<double>() + int64_t(l - 2) * kOffsets * NC * NC, f->W.as<double>(),
+          f->G.as<double>());
+      abh::check_launch();
+    }
+    if (f->nleaf > 0) {
+      if (prec == ABH_FP64)
+        k_leaf<N, false><<<f->nleaf, 256, 0, s>>>(
+            f->leaf_nodes.as<int32_t>(), f->nleaf, f->leaf_range.as<int2>(),
+            f->p2p_ptr.as<int32_t>(), f->p2p_src.as<int32_t>(), f->box.as<double>(),
+            f->pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
+            f->kern, f->consts);
+      else
+        k_leaf<N, true><<<f->nleaf, 256, 0, s>>>(
+            f->leaf_nodes.as<int32_t>(), f->nleaf, f->leaf_range.as<int2>(),
+            f->p2p_ptr.as<int32_t>(), f->p2p_src.as<int32_t>(), f->box.as<double>(),
+            f->pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
+            f->kern, f->consts);
+      abh::check_launch();
+      k_scatter_rhs<<<unsigned(abh::ceil_div(n * nr, 256)), 256, 0, s>>>(
+          f->order_buf.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
+      abh::check_launch();
+    }
+  }
+}
+
+template <class F> void dispatch_order(int order, F &&fn) {
+  switch (order) {
+    case 3: fn(std::integral_constant<int, 3>()); break;
+    case 4: fn(std::integral_constant<int, 4>()); break;
+    case 5: fn(std::integral_constant<int, 5>()); break;
+    case 6: fn(std::integral_constant<int, 6>()); break;
+    default: throw abh::Error{ABH_ERR_UNSUPPORTED, "Chebyshev order must be 3..6"};
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low, const double *high,
+                          int64_t n, const double *pos, double n_leaf, int order,
+                          const abh_kernel *kernel, abh_fmm **out) {
+  if (!ctx || !out || !low || !high || !kernel) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_fmm *f = new (std::nothrow) abh_fmm();
+  if (!f) return ABH_ERR_OUT_OF_MEMORY;
+  abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
+    ABH_REQUIRE(dim == 3, ABH_ERR_UNSUPPORTED, "the FMM is implemented for dim == 3");
+    ABH_REQUIRE(n >= 0 && n < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "n out of range");
+    ABH_REQUIRE(n == 0 || pos, ABH_ERR_INVALID_ARG, "pos is null");
+    ABH_REQUIRE(n_leaf > 0, ABH_ERR_INVALID_ARG, "n_particles_in_leaf must be positive");
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    f->ctx = ctx;
+    f->n = n;
+    f->order = order;
+    for (int d = 0; d < 3; ++d) {
+      ABH_REQUIRE(high[d] > low[d], ABH_ERR_INVALID_ARG, "domain high must exceed low");
+      f->bounds.lo[d] = low[d];
+      f->bounds.hi[d] = high[d];
+    }
+    f->kern.type = kernel->type;
+    f->kern.param = kernel->param;
+    f->kern.param2 = kernel->param * kernel->param;
+    dispatch_order(order, [&](auto NN) { create_impl<decltype(NN)::value>(f, pos, n_leaf); });
+  });
+  if (st != ABH_OK) {
+    delete f;
+    return st;
+  }
+  abh::ctx_retain(ctx);
+  *out = f;
+  return ABH_OK;
+}
+
+abh_status abh_fmm_destroy(abh_fmm *f) {
+  if (!f) return ABH_OK;
+  abh_context *ctx = f->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete f;
+  abh::ctx_release(ctx);
+  return ABH_OK;
+}
+
+abh_status abh_fmm_get_info(const abh_fmm *f, abh_fmm_info *info) {
+  if (!f || !info) return ABH_ERR_INVALID_ARG;
+  *info = f->info;
+  return ABH_OK;
+}
+
+const int32_t *abh_fmm_order(const abh_fmm *f) { return f ? f->order_buf.as<int32_t>() : nullptr; }
+
+abh_status abh_fmm_apply(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
+                         int64_t ldy, abh_precision precision) {
+  if (!f) return ABH_ERR_INVALID_ARG;
+  return abh::guard(f->ctx, [&] {
+    ABH_REQUIRE(nrhs >= 0, ABH_ERR_INVALID_ARG, "nrhs must be >= 0");
+    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_MIXED, ABH_ERR_UNSUPPORTED,
+                "FMM precision must be ABH_FP64 or ABH_MIXED");
+    if (nrhs == 0 || f->n == 0) return;
+    ABH_REQUIRE(x && y, ABH_ERR_INVALID_ARG, "x or y is null");
+    ABH_REQUIRE(ldx >= f->n && ldy >= f->n, ABH_ERR_INVALID_ARG, "leading dimension too small");
+    dispatch_order(f->order, [&](auto NN) {
+      apply_impl<decltype(NN)::value>(f, nrhs, x, ldx, y, ldy, precision);
+    });
+  });
+}
+
+}  // extern "C"

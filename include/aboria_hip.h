@@ -177,8 +177,11 @@ typedef struct abh_kernel {
 
 typedef enum abh_precision {
   ABH_FP64 = 0,               /* FP64 kernel and accumulation               */
-  ABH_FP32_COMPENSATED = 1    /* FP64 radius test, FP32 kernel, FP32 sum
+  ABH_FP32_COMPENSATED = 1,   /* FP64 radius test, FP32 kernel, FP32 sum
                                  with Kahan-Babuska compensation            */
+  ABH_MIXED = 2               /* FMM: M2L on BF16 MFMA with a three-term
+                                 split and FP32 accumulation, P2P as
+                                 ABH_FP32_COMPENSATED, other stages FP64    */
 } abh_precision;
 
 typedef struct abh_sparse_op abh_sparse_op;
@@ -199,6 +202,43 @@ abh_status abh_sparse_op_destroy(abh_sparse_op *op);
 abh_status abh_sparse_op_apply(abh_sparse_op *op, int nrhs, const double *x,
                                int64_t ldx, double *y, int64_t ldy,
                                abh_precision precision);
+
+/* ------------------------------------------------------------------ */
+/* black-box Chebyshev FMM over a HyperOctree                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_fmm abh_fmm;
+
+typedef struct abh_fmm_info {
+  int32_t order;        /* Chebyshev nodes per dimension                  */
+  int32_t levels;       /* deepest octree level                           */
+  int64_t n_nodes;      /* octree boxes below the root (incl. empty)      */
+  int64_t n_leaves;     /* non-empty leaves                               */
+  int64_t m2l_pairs;    /* M2L (target, source) box pairs                 */
+  int64_t p2p_pairs;    /* P2P (target leaf, source leaf) pairs           */
+  int64_t p2p_interactions; /* particle pairs evaluated by P2P            */
+} abh_fmm_info;
+
+/* Replaces create_fmm_operator<N>(particles, particles, f_pos, f)
+ * (src/Operators.h:396-407, KernelFMM src/Kernels.h:533-576) for a particle
+ * set searched by a HyperOctree (src/OctTree.h:102-475) built here with
+ * n_particles_in_leaf.  dim must be 3; order (Chebyshev nodes per dimension)
+ * in 3..6.  pos: device, n x 3.  Builds the tree, the interaction lists and
+ * the M2L operators (synchronises). */
+abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low,
+                          const double *high, int64_t n, const double *pos,
+                          double n_particles_in_leaf, int order,
+                          const abh_kernel *kernel, abh_fmm **out);
+abh_status abh_fmm_destroy(abh_fmm *fmm);
+abh_status abh_fmm_get_info(const abh_fmm *fmm, abh_fmm_info *info);
+/* device, n: tree order (sorted position -> index of the input position) */
+const int32_t *abh_fmm_order(const abh_fmm *fmm);
+/* y += K x (FastMultipoleMethod::matrix_vector_multiply,
+ * src/FastMultipoleMethod.h:110-167) for nrhs right-hand sides; x, y are
+ * indexed like the positions given at creation (column-major, leading
+ * dimensions ldx / ldy).  precision: ABH_FP64 or ABH_MIXED. */
+abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
+                         double *y, int64_t ldy, abh_precision precision);
 
 #ifdef __cplusplus
 }
