@@ -17,6 +17,7 @@ __all__ = [
     "Context", "CellListOrdered", "Particles", "SearchResult", "euclidean_search",
     "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
     "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
+    "DenseBatchOperator",
 ]
 
 FP64 = C.FP64
@@ -447,3 +448,55 @@ def create_fmm_operator(particles, order, kernel, low, high, n_particles_in_leaf
     """create_fmm_operator<N>(particles, particles, f, f) on a HyperOctree."""
     return FmmOperator(particles.ctx, particles.position, low, high, n_particles_in_leaf, order,
                        kernel)
+
+
+class DenseBatchOperator:
+    """Batch of independent dense kernel operators (create_dense_operator per
+    particle set), evaluated and solved together."""
+
+    def __init__(self, ctx, offsets, positions, kernel):
+        _require_cuda(positions, torch.float64, "positions")
+        self.ctx = ctx
+        self.offsets = [int(v) for v in offsets]
+        self.nprob = len(self.offsets) - 1
+        self.dim = positions.shape[1]
+        self._keep = positions
+        off = (ctypes.c_int64 * len(self.offsets))(*self.offsets)
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_dense_batch_create(ctx.handle, self.dim, self.nprob, off,
+                                             _ptr(positions), ctypes.byref(kd), ctypes.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    def apply(self, x, y, precision=FP64):
+        """y += K x (concatenated vectors)."""
+        check(C.lib().abh_dense_batch_apply(self.handle, _ptr(x), _ptr(y), precision),
+              self.ctx.handle)
+        return y
+
+    def __matmul__(self, x):
+        y = torch.zeros_like(x)
+        return self.apply(x, y)
+
+    def gmres(self, b, x=None, restart=50, max_iters=1000, tol=1e-8, precision=FP64):
+        """Solves K x = b for every problem; returns (x, stats)."""
+        if x is None:
+            x = torch.zeros_like(b)
+        opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision), 0)
+        stats = (C.GmresStats * self.nprob)()
+        check(C.lib().abh_dense_batch_gmres(self.handle, _ptr(b), _ptr(x), ctypes.byref(opts),
+                                            stats), self.ctx.handle)
+        return x, [{"iterations": s.iterations, "converged": bool(s.converged),
+                    "residual": s.residual} for s in stats]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_dense_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -79,6 +79,16 @@ class FmmInfo(ctypes.Structure):
 MIXED = 2
 
 
+class GmresOptions(ctypes.Structure):
+    _fields_ = [("restart", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("tol", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class GmresStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("residual", ctypes.c_double)]
+
+
 class AbhError(RuntimeError):
     def __init__(self, status, message):
         self.status = status
@@ -124,6 +134,12 @@ SIGNATURES = {
     "abh_fmm_get_info": (c_int, [c_void_p, P(FmmInfo)]),
     "abh_fmm_order": (c_void_p, [c_void_p]),
     "abh_fmm_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+    "abh_dense_batch_create": (c_int, [c_void_p, c_int, c_int, P(c_int64), c_void_p, P(Kernel),
+                                       P(c_void_p)]),
+    "abh_dense_batch_destroy": (c_int, [c_void_p]),
+    "abh_dense_batch_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    "abh_dense_batch_gmres": (c_int, [c_void_p, c_void_p, c_void_p, P(GmresOptions),
+                                      P(GmresStats)]),
 }
 
 _lib = None

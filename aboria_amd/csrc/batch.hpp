@@ -1,0 +1,59 @@
+// Batched vectors (many independent problems stored back to back) and the
+// restarted GMRES(m) engine that runs every problem of a batch in lockstep.
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "common.hpp"
+
+namespace abh {
+
+// A run of at most kChunk consecutive entries of one problem.
+struct Chunk {
+  int32_t prob;
+  int32_t pad;
+  int64_t start;
+  int64_t len;
+};
+
+struct BatchLayout {
+  int nprob = 0;
+  std::vector<int64_t> off;  // nprob + 1
+  std::vector<int32_t> chunk_ptr_h;
+  int nchunks = 0;
+  DevBuf chunks, chunk_ptr;
+  int64_t total() const { return off.empty() ? 0 : off.back(); }
+  void build(abh_context *ctx, const int64_t *offsets, int np);
+};
+
+// Vector kernels over a batch (batch.hip).
+// partial[c * ld + i] = <V_i, w> restricted to chunk c, then h[p * ld + i]
+// sums the chunks of problem p in order (deterministic).
+void batch_dots(abh_context *ctx, const BatchLayout &L, const double *V, int64_t vstride, int nv,
+                const double *w, double *partial, double *h, int ld);
+// out[j] += sign * sum_i coef[p(j) * ld + i] * V_i[j]
+void batch_combine(abh_context *ctx, const BatchLayout &L, const double *V, int64_t vstride,
+                   int nv, const double *coef, int ld, double sign, double *out);
+// out[j] = in[j] * s[p(j)]
+void batch_scale(abh_context *ctx, const BatchLayout &L, const double *in, const double *s,
+                 double *out);
+
+// Restarted GMRES(m), right preconditioned, every problem of the batch in
+// lockstep.  op(in, out): out = A in; prec(in, out): out = M^-1 in (or copy).
+// The Arnoldi basis is orthogonalised by classical Gram-Schmidt applied twice
+// (CGS2) in FP64; the small Hessenberg systems are reduced with Givens
+// rotations on the host.  Each restart recomputes the true residual with
+// op_true (FP64 operator).
+struct GmresResult {
+  std::vector<abh_gmres_stats> stats;
+};
+
+void batch_gmres(abh_context *ctx, const BatchLayout &L,
+                 const std::function<void(const double *, double *)> &op,
+                 const std::function<void(const double *, double *)> &op_true,
+                 const std::function<void(const double *, double *)> *prec,
+                 const std::function<void(const uint8_t *)> &set_active, const double *b,
+                 double *x, const abh_gmres_options &opt, GmresResult &res);
+
+}  // namespace abh

@@ -1,0 +1,222 @@
+// Batched dense kernel operators: many independent particle sets (problems
+// of uneven size) evaluated in one launch, and their GMRES solve.
+//
+// Work decomposition: a tile is 256 consecutive rows of one problem; one
+// 256-thread workgroup per tile streams the problem's sources through LDS
+// in chunks of 256 (positions + x), each thread accumulating one row
+// (the P2P tile of the dense operator).  Tiles are ordered largest problem
+// first so the long rows of the big problems start early.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "batch.hpp"
+#include "kernel_fn.hpp"
+
+using abh::DevBuf;
+using abh::KernelDesc;
+
+namespace {
+
+constexpr int kRows = 256;
+
+template <int D, bool FP32>
+__global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles, int ntiles,
+                                                 const int64_t *__restrict__ off,
+                                                 const double *__restrict__ pos,
+                                                 const double *__restrict__ x,
+                                                 double *__restrict__ y,
+                                                 const uint8_t *__restrict__ active,
+                                                 KernelDesc k, double alpha, int overwrite) {
+  __shared__ double sp[kRows][D];
+  __shared__ double sx[kRows];
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int p = tiles[t].x;
+  if (active && !active[p]) return;
+  const int64_t base = off[p];
+  const int64_t n = off[p + 1] - base;
+  const int64_t i = int64_t(tiles[t].y) + threadIdx.x;
+  const bool act = i < n;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = act ? pos[(base + i) * D + d] : 0.0;
+  double acc = 0.0;
+  float a32 = 0.0f, c32 = 0.0f;
+  for (int64_t c0 = 0; c0 < n; c0 += kRows) {
+    const int m = int(n - c0 < kRows ? n - c0 : kRows);
+    __syncthreads();
+    if (threadIdx.x < m) {
+      const int64_t j = base + c0 + threadIdx.x;
+#pragma unroll
+      for (int d = 0; d < D; ++d) sp[threadIdx.x][d] = pos[j * D + d];
+      sx[threadIdx.x] = x[j];
+    }
+    __syncthreads();
+    if (act) {
+      for (int j = 0; j < m; ++j) {
+        if (FP32) {
+          float s = 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const float dx = float(sp[j][d] - xi[d]);
+            s = s + dx * dx;
+          }
+          abh::kb_add(a32, c32, abh::phi32(k, s) * float(sx[j]));
+        } else {
+          double s = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const double dx = sp[j][d] - xi[d];
+            s = s + dx * dx;
+          }
+          acc += abh::phi64(k, s) * sx[j];
+        }
+      }
+    }
+  }
+  if (act) {
+    const double v = FP32 ? double(a32) + double(c32) : acc;
+    double *yp = &y[base + i];
+    *yp = (overwrite ? 0.0 : *yp) + alpha * v;
+  }
+}
+
+}  // namespace
+
+struct abh_dense_batch {
+  abh_context *ctx = nullptr;
+  int dim = 2;
+  abh::BatchLayout L;
+  KernelDesc k{};
+  const double *pos = nullptr;
+  DevBuf tiles, off, active;
+  int ntiles = 0;
+
+  void apply(const double *x, double *y, bool fp32, double alpha, bool overwrite,
+             const uint8_t *act) {
+    if (ntiles == 0) return;
+    hipStream_t s = ctx->stream;
+    const int2 *tl = tiles.as<int2>();
+    const int64_t *o = off.as<int64_t>();
+#define ABH_DENSE_LAUNCH(DD)                                                                   \
+  if (fp32)                                                                                    \
+    k_dense<DD, true><<<ntiles, kRows, 0, s>>>(tl, ntiles, o, pos, x, y, act, k, alpha,       \
+                                               overwrite ? 1 : 0);                             \
+  else                                                                                         \
+    k_dense<DD, false><<<ntiles, kRows, 0, s>>>(tl, ntiles, o, pos, x, y, act, k, alpha,      \
+                                                overwrite ? 1 : 0);
+    if (dim == 1) { ABH_DENSE_LAUNCH(1) }
+    else if (dim == 2) { ABH_DENSE_LAUNCH(2) }
+    else { ABH_DENSE_LAUNCH(3) }
+#undef ABH_DENSE_LAUNCH
+    abh::check_launch();
+  }
+};
+
+extern "C" {
+
+abh_status abh_dense_batch_create(abh_context *ctx, int dim, int nprob, const int64_t *offsets,
+                                  const double *pos, const abh_kernel *kernel,
+                                  abh_dense_batch **out) {
+  if (!ctx || !out || !offsets || !kernel) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_dense_batch *op = new (std::nothrow) abh_dense_batch();
+  if (!op) return ABH_ERR_OUT_OF_MEMORY;
+  abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
+    ABH_REQUIRE(dim >= 1 && dim <= 3, ABH_ERR_INVALID_ARG, "dim must be 1..3");
+    ABH_REQUIRE(nprob >= 1, ABH_ERR_INVALID_ARG, "nprob must be >= 1");
+    ABH_REQUIRE(offsets[0] == 0, ABH_ERR_INVALID_ARG, "offsets[0] must be 0");
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    op->ctx = ctx;
+    op->dim = dim;
+    op->pos = pos;
+    op->k.type = kernel->type;
+    op->k.param = kernel->param;
+    op->k.param2 = kernel->param * kernel->param;
+    op->L.build(ctx, offsets, nprob);
+    ABH_REQUIRE(op->L.total() == 0 || pos, ABH_ERR_INVALID_ARG, "pos is null");
+    // tiles, largest problem first
+    std::vector<int> order(nprob);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
+    });
+    std::vector<int2> tl;
+    for (int p : order)
+      for (int64_t r = 0; r < offsets[p + 1] - offsets[p]; r += kRows)
+        tl.push_back(make_int2(p, int(r)));
+    op->ntiles = int(tl.size());
+    op->tiles.ensure(tl.size() * sizeof(int2) + 16);
+    op->off.ensure(size_t(nprob + 1) * 8);
+    op->active.ensure(size_t(nprob) + 16);
+    hipStream_t s = ctx->stream;
+    if (!tl.empty())
+      ABH_HIP(hipMemcpyAsync(op->tiles.p, tl.data(), tl.size() * sizeof(int2),
+                             hipMemcpyHostToDevice, s));
+    ABH_HIP(hipMemcpyAsync(op->off.p, offsets, size_t(nprob + 1) * 8, hipMemcpyHostToDevice, s));
+    ABH_HIP(hipStreamSynchronize(s));
+  });
+  if (st != ABH_OK) {
+    delete op;
+    return st;
+  }
+  abh::ctx_retain(ctx);
+  *out = op;
+  return ABH_OK;
+}
+
+abh_status abh_dense_batch_destroy(abh_dense_batch *op) {
+  if (!op) return ABH_OK;
+  abh_context *ctx = op->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete op;
+  abh::ctx_release(ctx);
+  return ABH_OK;
+}
+
+abh_status abh_dense_batch_apply(abh_dense_batch *op, const double *x, double *y,
+                                 abh_precision precision) {
+  if (!op) return ABH_ERR_INVALID_ARG;
+  return abh::guard(op->ctx, [&] {
+    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
+                "dense precision must be ABH_FP64 or ABH_FP32_COMPENSATED");
+    ABH_REQUIRE(op->L.total() == 0 || (x && y), ABH_ERR_INVALID_ARG, "x or y is null");
+    op->apply(x, y, precision == ABH_FP32_COMPENSATED, 1.0, false, nullptr);
+  });
+}
+
+abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x,
+                                 const abh_gmres_options *opts, abh_gmres_stats *stats) {
+  if (!op || !opts) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = op->ctx;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(opts->restart >= 1 && opts->max_iters >= 1 && opts->tol > 0, ABH_ERR_INVALID_ARG,
+                "invalid GMRES options");
+    ABH_REQUIRE(opts->precision == ABH_FP64 || opts->precision == ABH_FP32_COMPENSATED,
+                ABH_ERR_UNSUPPORTED, "GMRES operator precision must be FP64 or FP32_COMPENSATED");
+    ABH_REQUIRE(op->L.total() == 0 || (b && x), ABH_ERR_INVALID_ARG, "b or x is null");
+    const bool fp32 = opts->precision == ABH_FP32_COMPENSATED;
+    const uint8_t *act = op->active.as<uint8_t>();
+    std::function<void(const double *, double *)> f_op = [&](const double *in, double *out) {
+      op->apply(in, out, fp32, 1.0, true, act);
+    };
+    std::function<void(const double *, double *)> f_true = [&](const double *in, double *out) {
+      op->apply(in, out, false, 1.0, true, act);
+    };
+    std::function<void(const uint8_t *)> f_act = [&](const uint8_t *a) {
+      ABH_HIP(hipMemcpyAsync(op->active.p, a, size_t(op->L.nprob), hipMemcpyHostToDevice,
+                             ctx->stream));
+      ABH_HIP(hipStreamSynchronize(ctx->stream));
+    };
+    abh::GmresResult res;
+    abh::batch_gmres(ctx, op->L, f_op, f_true, nullptr, f_act, b, x, *opts, res);
+    if (stats)
+      for (int p = 0; p < op->L.nprob; ++p) stats[p] = res.stats[p];
+  });
+}
+
+}  // extern "C"

@@ -240,6 +240,55 @@ const int32_t *abh_fmm_order(const abh_fmm *fmm);
 abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
                          double *y, int64_t ldy, abh_precision precision);
 
+/* ------------------------------------------------------------------ */
+/* batched dense kernel operators and GMRES                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_dense_batch abh_dense_batch;
+
+/* Replaces create_dense_operator(particles, particles, f)
+ * (src/Operators.h:309-317, KernelDense src/Kernels.h:138-268) for a batch
+ * of independent particle sets evaluated together: problem p owns points
+ * offsets[p] .. offsets[p+1]-1 of pos (device, total x dim) and the same
+ * rows of every vector.  offsets: host, nprob+1 entries, offsets[0] == 0.
+ * The kernel argument is the position difference (f(a,b) = phi(|b - a|)). */
+abh_status abh_dense_batch_create(abh_context *ctx, int dim, int nprob,
+                                  const int64_t *offsets, const double *pos,
+                                  const abh_kernel *kernel,
+                                  abh_dense_batch **out);
+abh_status abh_dense_batch_destroy(abh_dense_batch *op);
+/* y += K x for every problem (KernelDense::evaluate, src/Kernels.h:212-267);
+ * x, y device vectors of length total.  precision: ABH_FP64 or
+ * ABH_FP32_COMPENSATED (FP64 differences, FP32 kernel, compensated sum). */
+abh_status abh_dense_batch_apply(abh_dense_batch *op, const double *x, double *y,
+                                 abh_precision precision);
+
+typedef struct abh_gmres_options {
+  int32_t restart;       /* Krylov dimension m of GMRES(m)                   */
+  int32_t max_iters;     /* Arnoldi steps per problem (all restarts)         */
+  double tol;            /* stop when ||b - A x|| <= tol ||b||               */
+  int32_t precision;     /* operator precision inside the Arnoldi process;
+                            residuals, orthogonalisation and updates are
+                            always FP64.  With ABH_FP32_COMPENSATED every
+                            restart recomputes the true residual in FP64
+                            (iterative refinement).                          */
+  int32_t reserved;
+} abh_gmres_options;
+
+typedef struct abh_gmres_stats {
+  int32_t iterations;    /* Arnoldi steps taken                              */
+  int32_t converged;     /* 1 if the tolerance was met                       */
+  double residual;       /* final ||b - A x|| / ||b||, operator in FP64      */
+} abh_gmres_stats;
+
+/* Solves K x = b for every problem with restarted GMRES (the Krylov solve
+ * the reference delegates to Eigen::GMRES, e.g. tests/rbf_interpolation.h:
+ * 360-369).  b, x: device, length total; x holds the initial guess on entry.
+ * stats: host array of nprob entries (may be NULL). */
+abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x,
+                                 const abh_gmres_options *opts,
+                                 abh_gmres_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
