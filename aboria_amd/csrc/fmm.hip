@@ -465,6 +465,178 @@ __global__ __launch_bounds__(256) void k_m2l_f64(const int32_t *__restrict__ til
   }
 }
 
+// ---------------------------------------------------------------------------
+// M2L on the matrix cores: BF16 MFMA with a three-term split
+// (a = a_hi + a_lo, ab ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation)
+// ---------------------------------------------------------------------------
+
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int kPad = 224;  // node rows / columns padded to 7 x 32
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ void split_bf16(double v, uint16_t &hi, uint16_t &lo) {
+  const float f = float(v);
+  hi = bf16_rne(f);
+  lo = bf16_rne(f - bf16_to_f(hi));
+}
+
+// M (FP64 [lev][343][NC][NC]) -> Mh, Ml (BF16 [lev][343][kPad][kPad], zero padded)
+template <int N>
+__global__ __launch_bounds__(256) void k_split_m(const double *__restrict__ M, int nlev,
+                                                 uint16_t *__restrict__ Mh,
+                                                 uint16_t *__restrict__ Ml) {
+  constexpr int NC = Cheb<N>::NC;
+  const int64_t total = int64_t(nlev) * kOffsets * kPad * kPad;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    const int k = int(t % kPad);
+    const int i = int((t / kPad) % kPad);
+    const int64_t lo = t / (int64_t(kPad) * kPad);  // level * 343 + offset
+    uint16_t h = 0, l = 0;
+    if (i < NC && k < NC) split_bf16(M[(lo * NC + i) * NC + k], h, l);
+    Mh[t] = h;
+    Ml[t] = l;
+  }
+}
+
+// W (FP64 [node][NC][16]) -> Wh, Wl (BF16 [node][kPad][16], zero padded)
+template <int N>
+__global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, int64_t nn,
+                                                 uint16_t *__restrict__ Wh,
+                                                 uint16_t *__restrict__ Wl) {
+  constexpr int NC = Cheb<N>::NC;
+  const int64_t total = nn * kPad * kRhs;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    const int r = int(t % kRhs);
+    const int k = int((t / kRhs) % kPad);
+    const int64_t node = t / (int64_t(kPad) * kRhs);
+    uint16_t h = 0, l = 0;
+    if (k < NC) split_bf16(W[(node * NC + k) * kRhs + r], h, l);
+    Wh[t] = h;
+    Wl[t] = l;
+  }
+}
+
+// One block per sibling tile (8 targets x 16 right-hand sides = 128
+// columns); wave w owns columns 32w..32w+31 (targets 2w, 2w+1) and all
+// seven 32-row tiles of the 224-row output.  For each box offset the M2L
+// matrix streams through LDS in K chunks of 32 together with the sources'
+// expansions (stored column-major in LDS so every lane reads 16 contiguous
+// bytes per operand).
+template <int N>
+__global__ __launch_bounds__(256) void k_m2l_mfma(const int32_t *__restrict__ tile_parent,
+                                                  int ntiles, const int32_t *__restrict__ child0,
+                                                  const int32_t *__restrict__ coord,
+                                                  const int32_t *__restrict__ map, int side,
+                                                  const uint16_t *__restrict__ Mh,
+                                                  const uint16_t *__restrict__ Ml,  // [343][224][224]
+                                                  const uint16_t *__restrict__ Wh,
+                                                  const uint16_t *__restrict__ Wl,  // [node][224][16]
+                                                  double *__restrict__ G) {
+  constexpr int NC = Cheb<N>::NC;
+  constexpr int KC = 32;
+  constexpr int LDA = KC + 8;  // padded LDS row (bf16 elements)
+  __shared__ __align__(16) uint16_t Ah[kPad * LDA], Al[kPad * LDA];
+  __shared__ __align__(16) uint16_t Bh[8 * kRhs * LDA], Bl[8 * kRhs * LDA];
+  __shared__ int src[8];
+  const int tile = blockIdx.x;
+  if (tile >= ntiles) return;
+  const int parent = tile_parent[tile];
+  const int first = child0[parent];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
+  f32x16 acc[7];
+#pragma unroll
+  for (int rt = 0; rt < 7; ++rt)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[rt][i] = 0.0f;
+  for (int o = 0; o < kOffsets; ++o) {
+    const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
+    if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+      const int ch = threadIdx.x;
+      const int qx = ((ch >> 2) & 1) + ox, qy = ((ch >> 1) & 1) + oy, qz = (ch & 1) + oz;
+      int s = -1;
+      if (qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
+        const int sx = 2 * pc[0] + qx, sy = 2 * pc[1] + qy, sz = 2 * pc[2] + qz;
+        if (sx >= 0 && sx < side && sy >= 0 && sy < side && sz >= 0 && sz < side)
+          s = map[(int64_t(sx) * side + sy) * side + sz];
+      }
+      src[ch] = s;
+    }
+    __syncthreads();
+    int any = 0;
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) any |= src[ch] >= 0 ? 1 : 0;
+    if (!any) continue;
+    const uint16_t *mh = Mh + int64_t(o) * kPad * kPad;
+    const uint16_t *ml = Ml + int64_t(o) * kPad * kPad;
+    for (int k0 = 0; k0 < kPad; k0 += KC) {
+      __syncthreads();
+      // A chunk: 224 rows x 32 k, 8 bf16 (16 B) per load
+      for (int t = threadIdx.x; t < kPad * (KC / 8); t += 256) {
+        const int i = t / (KC / 8), q = t % (KC / 8);
+        *reinterpret_cast<uint4 *>(&Ah[i * LDA + q * 8]) =
+            *reinterpret_cast<const uint4 *>(&mh[int64_t(i) * kPad + k0 + q * 8]);
+        *reinterpret_cast<uint4 *>(&Al[i * LDA + q * 8]) =
+            *reinterpret_cast<const uint4 *>(&ml[int64_t(i) * kPad + k0 + q * 8]);
+      }
+      // B chunk, transposed to [column = target*16 + r][k]
+      for (int t = threadIdx.x; t < 8 * KC * kRhs; t += 256) {
+        const int ch = t / (KC * kRhs), k = (t / kRhs) % KC, r = t % kRhs;
+        const int s = src[ch];
+        uint16_t h = 0, l = 0;
+        if (s >= 0) {
+          const int64_t e = (int64_t(s) * kPad + k0 + k) * kRhs + r;
+          h = Wh[e];
+          l = Wl[e];
+        }
+        Bh[(ch * kRhs + r) * LDA + k] = h;
+        Bl[(ch * kRhs + r) * LDA + k] = l;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < KC / 16; ++ks) {
+        const int kk = ks * 16 + 8 * (lane >> 5);
+        const int col = wave * 32 + (lane & 31);
+        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bh[col * LDA + kk]);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Bl[col * LDA + kk]);
+#pragma unroll
+        for (int rt = 0; rt < 7; ++rt) {
+          const int row = rt * 32 + (lane & 31);
+          const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&Ah[row * LDA + kk]);
+          const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&Al[row * LDA + kk]);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[rt], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // D layout: register i of lane l holds row 8*(i/4) + 4*(l/32) + (i%4),
+  // column l%32 of the 32 x 32 tile
+  const int col = wave * 32 + (lane & 31);
+  const int ch = col / kRhs, r = col % kRhs;
+  const int t = first + ch;
+#pragma unroll
+  for (int rt = 0; rt < 7; ++rt)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int row = rt * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
+      if (row < NC) G[(int64_t(t) * NC + row) * kRhs + r] += double(acc[rt][i]);
+    }
+}
+
 // L2P + P2P for one target leaf, 256 threads: thread t serves target particle
 // t % 64 of the current round; for L2P its quarter t / 64 of the 16
 // right-hand sides, for P2P every fourth source (partial sums of the four
@@ -616,6 +788,8 @@ struct abh_fmm {
   std::vector<std::vector<int32_t>> parents_at;  // non-leaf nodes per level
   int nleaf = 0;
   int64_t n_nodes = 0;
+  DevBuf Mh, Ml, Wh, Wl;  // BF16 hi / lo parts for the MFMA M2L
+  bool split_ready = false;
 };
 
 namespace {
@@ -912,6 +1086,23 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                                   f->consts);
       abh::check_launch();
     }
+    const bool mixed = prec == ABH_MIXED;
+    if (mixed && NC <= kPad && f->nlevels >= 2) {
+      if (!f->split_ready) {
+        const int nlev = f->nlevels - 1;
+        f->Mh.ensure(size_t(nlev) * kOffsets * kPad * kPad * 2);
+        f->Ml.ensure(size_t(nlev) * kOffsets * kPad * kPad * 2);
+        k_split_m<N><<<8192, 256, 0, s>>>(f->M.as<double>(), nlev, f->Mh.as<uint16_t>(),
+                                          f->Ml.as<uint16_t>());
+        abh::check_launch();
+        f->Wh.ensure(size_t(nn) * kPad * kRhs * 2);
+        f->Wl.ensure(size_t(nn) * kPad * kRhs * 2);
+        f->split_ready = true;
+      }
+      k_split_w<N><<<8192, 256, 0, s>>>(f->W.as<double>(), nn, f->Wh.as<uint16_t>(),
+                                        f->Wl.as<uint16_t>());
+      abh::check_launch();
+    }
     // downward: level l receives L2L from level l-1 and its M2L
     for (int l = 2; l <= f->nlevels; ++l) {
       if (npar(l - 1) == 0) continue;
@@ -919,11 +1110,20 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                                        f->child0.as<int32_t>(),
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
-      k_m2l_f64<N><<<npar(l - 1), 256, 0, s>>>(
-          par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
-          f->maps.as<int32_t>() + f->map_off[l], 1 << l,
-          f->M.as<double>() + int64_t(l - 2) * kOffsets * NC * NC, f->W.as<double>(),
-          f->G.as<double>());
+      if (mixed && NC <= kPad) {
+        k_m2l_mfma<N><<<npar(l - 1), 256, 0, s>>>(
+            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
+            f->maps.as<int32_t>() + f->map_off[l], 1 << l,
+            f->Mh.as<uint16_t>() + int64_t(l - 2) * kOffsets * kPad * kPad,
+            f->Ml.as<uint16_t>() + int64_t(l - 2) * kOffsets * kPad * kPad, f->Wh.as<uint16_t>(),
+            f->Wl.as<uint16_t>(), f->G.as<double>());
+      } else {
+        k_m2l_f64<N><<<npar(l - 1), 256, 0, s>>>(
+            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
+            f->maps.as<int32_t>() + f->map_off[l], 1 << l,
+            f->M.as<double>() + int64_t(l - 2) * kOffsets * NC * NC, f->W.as<double>(),
+            f->G.as<double>());
+      }
       abh::check_launch();
     }
     if (f->nleaf > 0) {

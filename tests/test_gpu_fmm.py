@@ -3,8 +3,8 @@ traversal) and against an FP64 direct sum.
 
 Bar: the FP64 GPU FMM equals the oracle's FMM to 1e-11 relative (2-norm;
 summation order and the translated M2L operators differ only by rounding);
-the mixed form (FP32-compensated P2P) is within 1e-6 relative of the FP64
-form; the order-3 multiquadric case meets the reference's own accuracy test
+the mixed form (BF16x3 MFMA M2L with FP32 accumulation, FP32-compensated
+P2P) is within 5e-5 relative of the FP64 form; the order-3 multiquadric case meets the reference's own accuracy test
 (tests/fmm.h:107-109: sum (y - y_direct)^2 / sum y_direct^2 < 1e-2).
 """
 import numpy as np
@@ -71,7 +71,9 @@ def test_fmm_multi_rhs_and_mixed(ctx):
         y1 = (f @ xt[r].contiguous()).cpu().numpy()
         assert rel(y64[r], y1) < 1e-13
     ymx = f.apply_colmajor(xt, torch.zeros_like(xt), ab.MIXED).cpu().numpy()
-    assert rel(ymx, y64) < 1e-6
+    err_mixed = rel(ymx, y64)
+    print("mixed vs fp64 relative difference %.3e" % err_mixed, flush=True)
+    assert err_mixed < 5e-5
     # accuracy against the direct sum on 500 targets
     idx = rng.choice(n, 500, replace=False)
     ref = oracle.dense_matvec(pos[idx], pos, (KI, 1e-3), x[0])
