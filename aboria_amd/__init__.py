@@ -17,7 +17,8 @@ __all__ = [
     "Context", "CellListOrdered", "Particles", "SearchResult", "euclidean_search",
     "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
     "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
-    "DenseBatchOperator",
+    "DenseBatchOperator", "SchwartzPreconditioner", "knn", "Mlp", "ApproxInverse",
+    "gmres_with_fallback",
 ]
 
 FP64 = C.FP64
@@ -371,6 +372,25 @@ class SparseOperator:
         y = torch.zeros(shape, dtype=torch.float64, device=x.device)
         return self.apply(x, y)
 
+    def gmres(self, b, x=None, diag_shift=0.0, preconditioner=None, restart=50, max_iters=1000,
+              tol=1e-8, precision=FP64):
+        """Solves (K + diag_shift I) x = b with restarted GMRES, right
+        preconditioned by a SchwartzPreconditioner or an ApproxInverse (the
+        reference's Eigen::GMRES<MatrixReplacement, SchwartzPreconditioner>).
+        x (if given) is the initial guess.  Returns (x, stats)."""
+        if x is None:
+            x = torch.zeros_like(b)
+        sw = preconditioner.handle if isinstance(preconditioner, SchwartzPreconditioner) else None
+        ai = preconditioner.handle if isinstance(preconditioner, ApproxInverse) else None
+        if preconditioner is not None and sw is None and ai is None:
+            raise TypeError("preconditioner must be a SchwartzPreconditioner or an ApproxInverse")
+        opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision), 0)
+        st = C.GmresStats()
+        check(C.lib().abh_sparse_op_gmres(self.handle, float(diag_shift), sw, ai, _ptr(b), _ptr(x),
+                                          ctypes.byref(opts), ctypes.byref(st)), self.ctx.handle)
+        return x, {"iterations": st.iterations, "converged": bool(st.converged),
+                   "residual": st.residual}
+
     def close(self):
         if getattr(self, "handle", None):
             C.lib().abh_sparse_op_destroy(self.handle)
@@ -500,3 +520,222 @@ class DenseBatchOperator:
             self.close()
         except Exception:
             pass
+
+
+class SchwartzPreconditioner:
+    """Restricted additive Schwarz preconditioner of A = K + diag_shift I over
+    a cell list (SchwartzPreconditioner<Eigen::LLT<MatrixXd>>,
+    src/Preconditioners.h:1121-1537): analyzePattern + factorize happen at
+    construction, solve(b) is _solve_impl.  The kernel is restricted to
+    |dx| <= radius (radius <= 0: every pair)."""
+
+    def __init__(self, ctx, cells, kernel, radius, diag_shift=0.0):
+        self.ctx = ctx
+        self.cells = cells
+        self.n = cells.n_alive
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_schwarz_create(ctx.handle, cells.handle, ctypes.byref(kd),
+                                         float(radius), float(diag_shift), ctypes.byref(h)),
+              ctx.handle)
+        self.handle = h
+
+    @property
+    def info(self):
+        i = C.SchwarzInfo()
+        check(C.lib().abh_schwarz_get_info(self.handle, ctypes.byref(i)), self.ctx.handle)
+        return {f: getattr(i, f) for f, _ in C.SchwarzInfo._fields_}
+
+    def domains(self):
+        """(offsets, indices, n_interior) as host int arrays."""
+        import numpy as np
+        inf = self.info
+        nd = inf["n_domains"]
+        off = np.zeros(nd + 1, np.int64)
+        check(C.lib().abh_schwarz_get_domains(self.handle, off.ctypes.data, None, None),
+              self.ctx.handle)
+        idx = np.zeros(max(int(off[-1]), 1), np.int32)
+        nint = np.zeros(max(nd, 1), np.int32)
+        check(C.lib().abh_schwarz_get_domains(self.handle, None, idx.ctypes.data,
+                                              nint.ctypes.data), self.ctx.handle)
+        return off, idx[:int(off[-1])], nint[:nd]
+
+    def apply(self, b, x):
+        check(C.lib().abh_schwarz_apply(self.handle, _ptr(b), _ptr(x)), self.ctx.handle)
+        return x
+
+    def solve(self, b):
+        return self.apply(b, torch.empty_like(b))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_schwarz_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def knn(cells, k, with_dist2=False):
+    """k nearest particles of every particle of the cell list (itself first),
+    as an (n, k) int32 tensor (and the squared distances)."""
+    ctx = cells.ctx
+    n = cells.n_alive
+    nbr = torch.empty((n, k), dtype=torch.int32, device=ctx.device)
+    d2 = torch.empty((n, k), dtype=torch.float64, device=ctx.device) if with_dist2 else None
+    check(C.lib().abh_knn(ctx.handle, cells.handle, int(k), _ptr(nbr), _ptr(d2)), ctx.handle)
+    return (nbr, d2) if with_dist2 else nbr
+
+
+class Mlp:
+    """n_in -> hidden x n_hidden (SiLU) -> n_out network with hand-written
+    FP32 training kernels and a BF16 MFMA inference kernel."""
+
+    def __init__(self, ctx, n_in, hidden=128, n_hidden=3, n_out=32, seed=0):
+        self.ctx = ctx
+        self.dims = [n_in] + [hidden] * n_hidden + [n_out]
+        h = ctypes.c_void_p()
+        check(C.lib().abh_mlp_create(ctx.handle, int(n_in), int(hidden), int(n_hidden), int(n_out),
+                                     int(seed), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        n = ctypes.c_int64()
+        check(C.lib().abh_mlp_num_params(h, ctypes.byref(n)), ctx.handle)
+        self.num_params = n.value
+
+    def get_params(self):
+        import numpy as np
+        out = np.zeros(self.num_params, np.float32)
+        check(C.lib().abh_mlp_get_params(self.handle, out.ctypes.data), self.ctx.handle)
+        return out
+
+    def set_params(self, values):
+        import numpy as np
+        v = np.ascontiguousarray(values, np.float32)
+        if v.size != self.num_params:
+            raise ValueError("expected %d parameters" % self.num_params)
+        check(C.lib().abh_mlp_set_params(self.handle, v.ctypes.data), self.ctx.handle)
+
+    def forward(self, x, bf16=False):
+        _require_cuda(x, torch.float32, "x")
+        y = torch.empty((x.shape[0], self.dims[-1]), dtype=torch.float32, device=x.device)
+        check(C.lib().abh_mlp_forward(self.handle, x.shape[0], _ptr(x), _ptr(y), int(bool(bf16))),
+              self.ctx.handle)
+        return y
+
+    def loss_grad(self, x, t):
+        """(mean squared error, gradient tensor of num_params FP32)."""
+        _require_cuda(x, torch.float32, "x")
+        _require_cuda(t, torch.float32, "t")
+        g = torch.empty(self.num_params, dtype=torch.float32, device=x.device)
+        loss = ctypes.c_double()
+        check(C.lib().abh_mlp_loss_grad(self.handle, x.shape[0], _ptr(x), _ptr(t),
+                                        ctypes.byref(loss), _ptr(g)), self.ctx.handle)
+        return loss.value, g
+
+    def train(self, x, t, epochs=20, batch=1024, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+              lr_decay=1.0, seed=0):
+        """Adam on the mean squared error; returns the per-epoch mean loss."""
+        import numpy as np
+        _require_cuda(x, torch.float32, "x")
+        _require_cuda(t, torch.float32, "t")
+        o = C.MlpTrainOptions(int(epochs), int(batch), float(lr), float(beta1), float(beta2),
+                              float(eps), float(lr_decay), 0, int(seed))
+        losses = np.zeros(max(int(epochs), 1), np.float32)
+        check(C.lib().abh_mlp_train(self.handle, x.shape[0], _ptr(x), _ptr(t), ctypes.byref(o),
+                                    losses.ctypes.data), self.ctx.handle)
+        return losses[:int(epochs)]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ApproxInverse:
+    """Approximate-inverse rows on the k-nearest-neighbour pattern of a cell
+    list: x_i = sum_k w_ik b_nbr_ik.  Built either from least squares on the
+    operator (lsq) or by the network (learned)."""
+
+    def __init__(self, ctx, handle, n, k, dim):
+        self.ctx = ctx
+        self.handle = handle
+        self.n, self.k, self.dim = n, k, dim
+
+    @classmethod
+    def lsq(cls, ctx, cells, kernel, radius, diag_shift, k=32):
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_ainv_create_lsq(ctx.handle, cells.handle, ctypes.byref(kd),
+                                          float(radius), float(diag_shift), int(k),
+                                          ctypes.byref(h)), ctx.handle)
+        return cls(ctx, h, cells.n_alive, k, cells.info.dim)
+
+    @classmethod
+    def learned(cls, ctx, cells, mlp, k, feature_scale, out_scale, bf16=True):
+        h = ctypes.c_void_p()
+        check(C.lib().abh_ainv_create_learned(ctx.handle, cells.handle, mlp.handle, int(k),
+                                              float(feature_scale), float(out_scale),
+                                              int(bool(bf16)), ctypes.byref(h)), ctx.handle)
+        return cls(ctx, h, cells.n_alive, k, cells.info.dim)
+
+    def pattern(self):
+        out = torch.empty((self.n, self.k), dtype=torch.int32, device=self.ctx.device)
+        check(C.lib().abh_ainv_get(self.handle, _ptr(out), None), self.ctx.handle)
+        return out
+
+    def weights(self):
+        out = torch.empty((self.n, self.k), dtype=torch.float64, device=self.ctx.device)
+        check(C.lib().abh_ainv_get(self.handle, None, _ptr(out)), self.ctx.handle)
+        return out
+
+    def features(self, feature_scale):
+        out = torch.empty((self.n, self.k * self.dim), dtype=torch.float32,
+                          device=self.ctx.device)
+        check(C.lib().abh_ainv_features(self.handle, float(feature_scale), _ptr(out)),
+              self.ctx.handle)
+        return out
+
+    def apply(self, b, x):
+        check(C.lib().abh_ainv_apply(self.handle, _ptr(b), _ptr(x)), self.ctx.handle)
+        return x
+
+    def solve(self, b):
+        return self.apply(b, torch.empty_like(b))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_ainv_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gmres_with_fallback(op, b, diag_shift, learned, schwarz, restart=50, max_iters=2000,
+                        learned_iters=100, tol=1e-8):
+    """GMRES preconditioned by the learned approximate inverse for at most
+    learned_iters iterations; if that has not converged, continues from its
+    iterate with the plain Schwarz preconditioner (the fallback).  Returns
+    (x, report)."""
+    x = torch.zeros_like(b)
+    rep = {"learned": None, "fallback": None}
+    if learned is not None:
+        x, st = op.gmres(b, x, diag_shift, learned, restart, learned_iters, tol)
+        rep["learned"] = st
+        if st["converged"]:
+            return x, rep
+    x, st = op.gmres(b, x, diag_shift, schwarz, restart, max_iters, tol)
+    rep["fallback"] = st
+    return x, rep

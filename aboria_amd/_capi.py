@@ -28,6 +28,7 @@ STATUS_NAMES = {
     5: "ABH_ERR_UNSUPPORTED",
     6: "ABH_ERR_DOMAIN",
     7: "ABH_ERR_NOT_CONVERGED",
+    8: "ABH_ERR_NUMERICAL",
 }
 
 KERNEL_CONSTANT = 0
@@ -89,6 +90,20 @@ class GmresStats(ctypes.Structure):
                 ("residual", ctypes.c_double)]
 
 
+class SchwarzInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("n_domains", ctypes.c_int64), ("min_size", ctypes.c_int64),
+                ("max_size", ctypes.c_int64), ("min_interior", ctypes.c_int64),
+                ("max_interior", ctypes.c_int64), ("factor_bytes", ctypes.c_int64),
+                ("failed_domains", ctypes.c_int64)]
+
+
+class MlpTrainOptions(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+                ("lr_decay", ctypes.c_float), ("reserved", ctypes.c_int32),
+                ("seed", ctypes.c_uint64)]
+
+
 class AbhError(RuntimeError):
     def __init__(self, status, message):
         self.status = status
@@ -140,6 +155,33 @@ SIGNATURES = {
     "abh_dense_batch_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
     "abh_dense_batch_gmres": (c_int, [c_void_p, c_void_p, c_void_p, P(GmresOptions),
                                       P(GmresStats)]),
+    "abh_schwarz_create": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double,
+                                   P(c_void_p)]),
+    "abh_schwarz_destroy": (c_int, [c_void_p]),
+    "abh_schwarz_get_info": (c_int, [c_void_p, P(SchwarzInfo)]),
+    "abh_schwarz_get_domains": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "abh_schwarz_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "abh_knn": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "abh_mlp_create": (c_int, [c_void_p, c_int, c_int, c_int, c_int, ctypes.c_uint64,
+                               P(c_void_p)]),
+    "abh_mlp_destroy": (c_int, [c_void_p]),
+    "abh_mlp_num_params": (c_int, [c_void_p, P(c_int64)]),
+    "abh_mlp_get_params": (c_int, [c_void_p, c_void_p]),
+    "abh_mlp_set_params": (c_int, [c_void_p, c_void_p]),
+    "abh_mlp_forward": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_int]),
+    "abh_mlp_loss_grad": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, P(c_double), c_void_p]),
+    "abh_mlp_train": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, P(MlpTrainOptions),
+                              c_void_p]),
+    "abh_ainv_create_lsq": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double, c_int,
+                                    P(c_void_p)]),
+    "abh_ainv_create_learned": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_double,
+                                        c_int, P(c_void_p)]),
+    "abh_ainv_destroy": (c_int, [c_void_p]),
+    "abh_ainv_get": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "abh_ainv_features": (c_int, [c_void_p, c_double, c_void_p]),
+    "abh_ainv_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "abh_sparse_op_gmres": (c_int, [c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    P(GmresOptions), P(GmresStats)]),
 }
 
 _lib = None

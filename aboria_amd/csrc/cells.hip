@@ -222,6 +222,7 @@ const char *abh_status_string(abh_status s) {
     case ABH_ERR_UNSUPPORTED: return "unsupported";
     case ABH_ERR_DOMAIN: return "position outside domain";
     case ABH_ERR_NOT_CONVERGED: return "not converged";
+    case ABH_ERR_NUMERICAL: return "numerical failure";
   }
   return "unknown status";
 }

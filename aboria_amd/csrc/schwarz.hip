@@ -1,0 +1,593 @@
+// Restricted additive Schwarz preconditioner over a cell list.
+//
+// Analyse (host, O(3^dim N)): one domain per non-empty bucket; particles of
+// the 3^dim block of buckets around it inside the bucket bounds are the
+// interior, the rest the buffer (src/Preconditioners.h:1169-1230).
+//
+// Factorise (device): domains are sorted by size and processed in groups of
+// equal padded size n (a multiple of 64).  Per group every local matrix
+// A_g = K + shift I (identity in the padding) is assembled in HBM,
+// factorised by a right-looking blocked Cholesky (64-wide panels: the
+// diagonal block in LDS, panel and trailing updates as strided-batched
+// DGEMMs), and the interior columns of A_g^-1 (= the interior rows, A_g is
+// symmetric) are obtained by a blocked forward and backward substitution on
+// the unit columns.  They stay resident: R_g is n x ni column-major, so row a
+// of the local inverse is the contiguous column a.
+//
+// Apply (device, HBM-bound): x = b, then for every domain and interior row a
+// x[idx[a]] = <R_g[:, a], b[idx]>, one wavefront per row, the gathered
+// b[idx] staged once per workgroup in LDS.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "cells.hpp"
+#include "kernel_fn.hpp"
+#include "solvers.hpp"
+
+using abh::DevBuf;
+using abh::KernelDesc;
+
+namespace {
+
+constexpr int kNb = 64;            // Cholesky panel width
+constexpr int kApplyRows = 64;     // interior rows per apply workgroup
+constexpr int kApplyLds = 7936;    // max domain size staged in LDS by apply (62 KiB)
+
+#define ABH_BLAS(expr)                                                                   \
+  do {                                                                                   \
+    rocblas_status s_ = (expr);                                                          \
+    if (s_ != rocblas_status_success)                                                    \
+      throw ::abh::Error{ABH_ERR_HIP, std::string(#expr) + " failed: " +                 \
+                                          rocblas_status_to_string(s_)};                 \
+  } while (0)
+
+struct DomDev {
+  int64_t idx_off;  // first entry in the index array
+  int64_t r_off;    // first double of the domain's inverse rows
+  int32_t m;        // interior + buffer
+  int32_t ni;       // interior
+  int32_t ld;       // padded size n of its group (leading dimension of R)
+  int32_t pad;
+};
+
+// A[g] (n x n, column-major) = K + shift I over the domain's particles,
+// identity on the padding.
+template <int D>
+__global__ __launch_bounds__(256) void k_assemble(const DomDev *__restrict__ doms,
+                                                  const int32_t *__restrict__ idx,
+                                                  const double *__restrict__ pos, KernelDesc k,
+                                                  double r2, double shift, int n,
+                                                  double *__restrict__ A) {
+  __shared__ double pi[64][D], pj[64][D];
+  const int g = blockIdx.z;
+  const DomDev dm = doms[g];
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+  if (t < 64) {
+    const int i = i0 + t;
+#pragma unroll
+    for (int d = 0; d < D; ++d) pi[t][d] = i < dm.m ? pos[int64_t(idx[dm.idx_off + i]) * D + d] : 0.0;
+  } else if (t < 128) {
+    const int j = j0 + t - 64;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      pj[t - 64][d] = j < dm.m ? pos[int64_t(idx[dm.idx_off + j]) * D + d] : 0.0;
+  }
+  __syncthreads();
+  double *Ag = A + int64_t(g) * n * n;
+  const int li = t & 63;
+  const int i = i0 + li;
+  for (int lj = t >> 6; lj < 64; lj += 4) {
+    const int j = j0 + lj;
+    double v;
+    if (i < dm.m && j < dm.m) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double dx = pj[lj][d] - pi[li][d];
+        s = s + dx * dx;
+      }
+      v = (r2 < 0.0 || s <= r2) ? abh::phi64(k, s) : 0.0;
+      if (i == j) v = v + shift;
+    } else {
+      v = i == j ? 1.0 : 0.0;
+    }
+    Ag[int64_t(j) * n + i] = v;
+  }
+}
+
+// Cholesky of the 64 x 64 diagonal block at (o, o) of every matrix of the
+// group, in LDS; writes L back and L^-1 to dinv (64 x 64 column-major).
+__global__ __launch_bounds__(256) void k_potrf_diag(double *__restrict__ A, int n, int o,
+                                                    double *__restrict__ dinv,
+                                                    int64_t dinv_stride,
+                                                    uint8_t *__restrict__ failed) {
+  __shared__ double L[kNb][kNb + 1];
+  __shared__ double X[kNb][kNb + 1];
+  __shared__ int bad;
+  const int g = blockIdx.x;
+  const int t = threadIdx.x;
+  double *Ag = A + int64_t(g) * n * n + int64_t(o) * n + o;
+  for (int e = t; e < kNb * kNb; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    L[r][c] = Ag[int64_t(c) * n + r];
+  }
+  if (t == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < kNb; ++j) {
+    if (t == 0) {
+      double d = L[j][j];
+      if (!(d > 0.0)) {
+        bad = 1;
+        d = 1.0;
+      }
+      L[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    const double djj = L[j][j];
+    if (t > j && t < kNb) L[t][j] = L[t][j] / djj;
+    __syncthreads();
+    // trailing update of the lower triangle, columns j+1..63
+    for (int e = t; e < kNb * kNb; e += 256) {
+      const int r = e & 63, c = e >> 6;
+      if (c > j && r >= c) L[r][c] = L[r][c] - L[r][j] * L[c][j];
+    }
+    __syncthreads();
+  }
+  // X = L^-1, one column per thread (forward substitution)
+  if (t < kNb) {
+    const int c = t;
+    for (int r = 0; r < c; ++r) X[r][c] = 0.0;
+    X[c][c] = 1.0 / L[c][c];
+    for (int r = c + 1; r < kNb; ++r) {
+      double s = 0.0;
+      for (int q = c; q < r; ++q) s = s + L[r][q] * X[q][c];
+      X[r][c] = -s / L[r][r];
+    }
+  }
+  __syncthreads();
+  double *Dg = dinv + int64_t(g) * dinv_stride;
+  for (int e = t; e < kNb * kNb; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    Ag[int64_t(c) * n + r] = r >= c ? L[r][c] : 0.0;
+    Dg[c * kNb + r] = X[r][c];
+  }
+  if (t == 0 && bad) failed[g] = 1;
+}
+
+// dst[g] (rows x cols, ld ldd) = src[g] (ld lds), batched strided copy
+__global__ void k_copy_block(const double *__restrict__ src, int64_t src_stride, int lds,
+                             double *__restrict__ dst, int64_t dst_stride, int ldd, int rows,
+                             int cols) {
+  const int g = blockIdx.y;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(rows) * cols) return;
+  const int r = int(e % rows), c = int(e / rows);
+  dst[g * dst_stride + int64_t(c) * ldd + r] = src[g * src_stride + int64_t(c) * lds + r];
+}
+
+// Y[g] (n x ni) = the first ni columns of the identity
+__global__ void k_unit_cols(double *__restrict__ Y, int64_t stride, int n, int ni) {
+  const int g = blockIdx.y;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(n) * ni) return;
+  const int r = int(e % n), c = int(e / n);
+  Y[g * stride + e] = r == c ? 1.0 : 0.0;
+}
+
+struct ApplyChunk {
+  int32_t dom;
+  int32_t row0;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_ras_apply(const ApplyChunk *__restrict__ chunks,
+                                                   const DomDev *__restrict__ doms,
+                                                   const int32_t *__restrict__ idx,
+                                                   const double *__restrict__ R,
+                                                   const double *__restrict__ b,
+                                                   double *__restrict__ x) {
+  extern __shared__ double bl[];
+  const ApplyChunk ch = chunks[blockIdx.x];
+  const DomDev dm = doms[ch.dom];
+  const int32_t *id = idx + dm.idx_off;
+  if (LDS) {
+    for (int j = threadIdx.x; j < dm.m; j += 256) bl[j] = b[id[j]];
+    __syncthreads();
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rows = min(kApplyRows, dm.ni - ch.row0);
+  for (int a = wave; a < rows; a += 4) {
+    const double *Ra = R + dm.r_off + int64_t(ch.row0 + a) * dm.ld;
+    double s = 0.0;
+    for (int j = lane; j < dm.m; j += 64) s = s + Ra[j] * (LDS ? bl[j] : b[id[j]]);
+    s = abh::wave_sum(s);
+    if (lane == 0) x[id[ch.row0 + a]] = s;
+  }
+}
+
+}  // namespace
+
+struct abh_schwarz {
+  abh_context *ctx = nullptr;
+  int dim = 2;
+  int64_t n = 0;
+  std::vector<int64_t> off;   // host domain offsets (n_domains + 1)
+  std::vector<int32_t> idx;   // host domain lists
+  std::vector<int32_t> nint;  // host interior counts
+  int64_t failed = 0;
+  int max_m = 0;
+  DevBuf d_idx, d_doms, d_chunks, R;
+  int nchunks = 0;
+  size_t factor_bytes = 0;
+};
+
+namespace {
+
+void analyse(abh_schwarz *s, const abh_cells *c, const std::vector<double> &pos,
+             const std::vector<uint32_t> &beg, const std::vector<uint32_t> &end) {
+  const int D = c->dim;
+  int64_t sz[3] = {1, 1, 1};
+  for (int d = 0; d < D; ++d) sz[d] = c->size[d];
+  s->off.assign(1, 0);
+  s->idx.clear();
+  s->nint.clear();
+  std::vector<int32_t> buf;
+  for (int64_t b = 0; b < c->nb; ++b) {
+    if (end[b] <= beg[b]) continue;  // empty bucket: no domain
+    // bucket index vector, last dimension fastest
+    int64_t iv[3] = {0, 0, 0}, r = b;
+    for (int d = D - 1; d >= 0; --d) {
+      iv[d] = r % sz[d];
+      r /= sz[d];
+    }
+    double bmin[3], bmax[3];
+    for (int d = 0; d < D; ++d) {
+      bmin[d] = double(iv[d]) * c->side[d] + c->lo[d];
+      bmax[d] = double(iv[d] + 1) * c->side[d] + c->lo[d];
+    }
+    buf.clear();
+    const size_t first = s->idx.size();
+    int64_t lo3[3] = {0, 0, 0}, hi3[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+      lo3[d] = std::max<int64_t>(iv[d] - 1, 0);
+      hi3[d] = std::min<int64_t>(iv[d] + 1, sz[d] - 1);
+    }
+    for (int64_t x0 = lo3[0]; x0 <= hi3[0]; ++x0)
+      for (int64_t x1 = lo3[1]; x1 <= hi3[1]; ++x1)
+        for (int64_t x2 = lo3[2]; x2 <= hi3[2]; ++x2) {
+          const int64_t nbk = (x0 * sz[1] + x1) * sz[2] + x2;
+          for (uint32_t p = beg[nbk]; p < end[nbk]; ++p) {
+            bool inside = true;
+            for (int d = 0; d < D; ++d) {
+              const double v = pos[size_t(p) * D + d];
+              if (v < bmin[d] || v >= bmax[d]) inside = false;
+            }
+            if (inside) s->idx.push_back(int32_t(p));
+            else buf.push_back(int32_t(p));
+          }
+        }
+    const int32_t ni = int32_t(s->idx.size() - first);
+    if (ni == 0) {  // no interior: the domain writes nothing
+      s->idx.resize(first);
+      continue;
+    }
+    s->idx.insert(s->idx.end(), buf.begin(), buf.end());
+    s->nint.push_back(ni);
+    s->off.push_back(int64_t(s->idx.size()));
+  }
+}
+
+void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r2,
+               double shift) {
+  abh_context *ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int nd = int(s->nint.size());
+  if (nd == 0) return;
+  // domains by size, largest first; groups share the padded size of their
+  // first member
+  std::vector<int> order(nd);
+  std::iota(order.begin(), order.end(), 0);
+  auto msize = [&](int d) { return int(s->off[d + 1] - s->off[d]); };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return msize(a) > msize(b); });
+  size_t free_b = 0, total_b = 0;
+  ABH_HIP(hipMemGetInfo(&free_b, &total_b));
+  // inverse rows: n_pad x ni_pad per domain
+  std::vector<DomDev> dd(nd);
+  struct Group {
+    int first, count, n, ni;
+  };
+  std::vector<Group> groups;
+  int64_t r_total = 0;
+  {
+    int i = 0;
+    while (i < nd) {
+      const int n = int(abh::ceil_div(msize(order[i]), kNb) * kNb);
+      // working set per member: A (n^2) + panel copy (64 n) + dinv (n 64)
+      const double per = double(n) * n * 8.0 + double(n) * kNb * 16.0;
+      const double budget = std::min(8.0e9, 0.35 * double(free_b));
+      int gmax = int(std::max(1.0, std::min(256.0, budget / per)));
+      Group gr{i, 0, n, 0};
+      while (i < nd && gr.count < gmax && int(abh::ceil_div(msize(order[i]), kNb) * kNb) == n) {
+        gr.ni = std::max(gr.ni, s->nint[order[i]]);
+        ++gr.count;
+        ++i;
+      }
+      gr.ni = int(abh::ceil_div(gr.ni, kNb) * kNb);
+      for (int q = 0; q < gr.count; ++q) {
+        const int d = order[gr.first + q];
+        dd[d].idx_off = s->off[d];
+        dd[d].r_off = r_total + int64_t(q) * n * gr.ni;
+        dd[d].m = msize(d);
+        dd[d].ni = s->nint[d];
+        dd[d].ld = n;
+        dd[d].pad = 0;
+      }
+      r_total += int64_t(gr.count) * n * gr.ni;
+      groups.push_back(gr);
+    }
+  }
+  s->factor_bytes = size_t(r_total) * 8;
+  s->R.ensure(s->factor_bytes + 64);
+  s->d_idx.ensure(s->idx.size() * 4 + 64);
+  ABH_HIP(hipMemcpyAsync(s->d_idx.p, s->idx.data(), s->idx.size() * 4, hipMemcpyHostToDevice, st));
+  // domain table in group order (the group's members are contiguous)
+  std::vector<DomDev> dg(nd);
+  for (int i = 0; i < nd; ++i) dg[i] = dd[order[i]];
+  DevBuf d_dg;
+  d_dg.ensure(size_t(nd) * sizeof(DomDev) + 64);
+  ABH_HIP(hipMemcpyAsync(d_dg.p, dg.data(), size_t(nd) * sizeof(DomDev), hipMemcpyHostToDevice,
+                         st));
+  // apply table in domain order
+  s->d_doms.ensure(size_t(nd) * sizeof(DomDev) + 64);
+  ABH_HIP(hipMemcpyAsync(s->d_doms.p, dd.data(), size_t(nd) * sizeof(DomDev),
+                         hipMemcpyHostToDevice, st));
+  std::vector<ApplyChunk> ch;
+  for (int d = 0; d < nd; ++d)
+    for (int r = 0; r < s->nint[d]; r += kApplyRows) ch.push_back(ApplyChunk{d, r});
+  s->nchunks = int(ch.size());
+  s->d_chunks.ensure(ch.size() * sizeof(ApplyChunk) + 64);
+  ABH_HIP(hipMemcpyAsync(s->d_chunks.p, ch.data(), ch.size() * sizeof(ApplyChunk),
+                         hipMemcpyHostToDevice, st));
+
+  rocblas_handle h = nullptr;
+  ABH_BLAS(rocblas_create_handle(&h));
+  struct HandleGuard {
+    rocblas_handle h;
+    ~HandleGuard() { rocblas_destroy_handle(h); }
+  } hg{h};
+  ABH_BLAS(rocblas_set_stream(h, st));
+  ABH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
+
+  int gmax = 0, nmax = 0, nimax = 0;
+  for (const Group &gr : groups) {
+    gmax = std::max(gmax, gr.count);
+    nmax = std::max(nmax, gr.n);
+    nimax = std::max(nimax, gr.ni);
+  }
+  DevBuf A, W, Tm, Dinv, fail;
+  A.ensure(size_t(gmax) * nmax * nmax * 8);
+  W.ensure(size_t(gmax) * nmax * kNb * 8);
+  Tm.ensure(size_t(gmax) * kNb * nimax * 8);
+  Dinv.ensure(size_t(gmax) * (nmax / kNb) * kNb * kNb * 8);
+  fail.ensure(size_t(nd) + 64);
+  ABH_HIP(hipMemsetAsync(fail.p, 0, size_t(nd), st));
+  const double *pos = c->pos.as<double>();
+  const int32_t *didx = s->d_idx.as<int32_t>();
+
+  for (const Group &gr : groups) {
+    const int G = gr.count, n = gr.n, ni = gr.ni, K = n / kNb;
+    const rocblas_stride sA = rocblas_stride(n) * n;
+    const rocblas_stride sD = rocblas_stride(K) * kNb * kNb;
+    const rocblas_stride sW = rocblas_stride(n) * kNb;
+    const rocblas_stride sT = rocblas_stride(kNb) * ni;
+    const rocblas_stride sY = rocblas_stride(n) * ni;
+    double *Ap = A.as<double>(), *Wp = W.as<double>(), *Tp = Tm.as<double>();
+    double *Dp = Dinv.as<double>();
+    double *Y = s->R.as<double>() + dd[order[gr.first]].r_off;
+    uint8_t *fl = fail.as<uint8_t>() + gr.first;
+    const DomDev *gd = d_dg.as<DomDev>() + gr.first;
+    {
+      dim3 grid(n / 64, n / 64, G);
+      if (s->dim == 1) k_assemble<1><<<grid, 256, 0, st>>>(gd, didx, pos, k, r2, shift, n, Ap);
+      else if (s->dim == 2) k_assemble<2><<<grid, 256, 0, st>>>(gd, didx, pos, k, r2, shift, n, Ap);
+      else k_assemble<3><<<grid, 256, 0, st>>>(gd, didx, pos, k, r2, shift, n, Ap);
+      abh::check_launch();
+    }
+    // blocked Cholesky, lower
+    for (int kb = 0; kb < K; ++kb) {
+      const int o = kb * kNb;
+      k_potrf_diag<<<G, 256, 0, st>>>(Ap, n, o, Dp + int64_t(kb) * kNb * kNb, sD, fl);
+      abh::check_launch();
+      const int r = n - o - kNb;
+      if (r == 0) break;
+      // panel P = A[o+64:, o:o+64] * L_kk^-T
+      {
+        dim3 grid(unsigned(abh::ceil_div(int64_t(r) * kNb, 256)), G);
+        k_copy_block<<<grid, 256, 0, st>>>(Ap + o + kNb + int64_t(o) * n, sA, n, Wp, sW, n, r,
+                                           kNb);
+        abh::check_launch();
+      }
+      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, kNb, kNb, &one, Wp, n, sW,
+                                             Dp + int64_t(kb) * kNb * kNb, kNb, sD, &zero,
+                                             Ap + o + kNb + int64_t(o) * n, n, sA, G));
+      // trailing A[o+64:, o+64:] -= P P^T
+      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, r, kNb, &mone,
+                                             Ap + o + kNb + int64_t(o) * n, n, sA,
+                                             Ap + o + kNb + int64_t(o) * n, n, sA, &one,
+                                             Ap + o + kNb + int64_t(o + kNb) * n, n, sA, G));
+    }
+    // Y = L^-1 E (E = first ni unit columns); Y is lower trapezoidal
+    {
+      dim3 grid(unsigned(abh::ceil_div(int64_t(n) * ni, 256)), G);
+      k_unit_cols<<<grid, 256, 0, st>>>(Y, sY, n, ni);
+      abh::check_launch();
+    }
+    for (int kb = 0; kb < K; ++kb) {
+      const int o = kb * kNb;
+      const int nc = std::min(ni, o + kNb);  // columns >= o+64 are still zero
+      {
+        dim3 grid(unsigned(abh::ceil_div(int64_t(kNb) * nc, 256)), G);
+        k_copy_block<<<grid, 256, 0, st>>>(Y + o, sY, n, Tp, sT, kNb, kNb, nc);
+        abh::check_launch();
+      }
+      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, N, kNb, nc, kNb, &one,
+                                             Dp + int64_t(kb) * kNb * kNb, kNb, sD, Tp, kNb, sT,
+                                             &zero, Y + o, n, sY, G));
+      const int r = n - o - kNb;
+      if (r > 0)
+        ABH_BLAS(rocblas_dgemm_strided_batched(h, N, N, r, nc, kNb, &mone,
+                                               Ap + o + kNb + int64_t(o) * n, n, sA, Y + o, n,
+                                               sY, &one, Y + o + kNb, n, sY, G));
+    }
+    // Z = L^-T Y, in place
+    for (int kb = K - 1; kb >= 0; --kb) {
+      const int o = kb * kNb;
+      const int r = n - o - kNb;
+      if (r > 0)
+        ABH_BLAS(rocblas_dgemm_strided_batched(h, T, N, kNb, ni, r, &mone,
+                                               Ap + o + kNb + int64_t(o) * n, n, sA,
+                                               Y + o + kNb, n, sY, &one, Y + o, n, sY, G));
+      {
+        dim3 grid(unsigned(abh::ceil_div(int64_t(kNb) * ni, 256)), G);
+        k_copy_block<<<grid, 256, 0, st>>>(Y + o, sY, n, Tp, sT, kNb, kNb, ni);
+        abh::check_launch();
+      }
+      ABH_BLAS(rocblas_dgemm_strided_batched(h, T, N, kNb, ni, kNb, &one,
+                                             Dp + int64_t(kb) * kNb * kNb, kNb, sD, Tp, kNb, sT,
+                                             &zero, Y + o, n, sY, G));
+    }
+  }
+  std::vector<uint8_t> fh(nd);
+  ABH_HIP(hipMemcpyAsync(fh.data(), fail.p, size_t(nd), hipMemcpyDeviceToHost, st));
+  ABH_HIP(hipStreamSynchronize(st));
+  s->failed = std::accumulate(fh.begin(), fh.end(), int64_t(0));
+}
+
+}  // namespace
+
+namespace abh {
+
+void schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
+  hipStream_t st = s->ctx->stream;
+  if (s->n == 0) return;
+  ABH_HIP(hipMemcpyAsync(x, b, size_t(s->n) * 8, hipMemcpyDeviceToDevice, st));
+  if (s->nchunks == 0) return;
+  const ApplyChunk *ch = s->d_chunks.as<ApplyChunk>();
+  const DomDev *dm = s->d_doms.as<DomDev>();
+  const int32_t *id = s->d_idx.as<int32_t>();
+  const double *R = s->R.as<double>();
+  if (s->max_m <= kApplyLds)
+    k_ras_apply<true><<<s->nchunks, 256, size_t(s->max_m) * 8, st>>>(ch, dm, id, R, b, x);
+  else
+    k_ras_apply<false><<<s->nchunks, 256, 0, st>>>(ch, dm, id, R, b, x);
+  check_launch();
+}
+
+int64_t schwarz_size(const abh_schwarz *s) { return s->n; }
+
+}  // namespace abh
+
+extern "C" {
+
+abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const abh_kernel *kernel,
+                              double radius, double diag_shift, abh_schwarz **out) {
+  if (!ctx || !cells || !kernel || !out) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_schwarz *s = new (std::nothrow) abh_schwarz();
+  if (!s) return ABH_ERR_OUT_OF_MEMORY;
+  abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    for (int d = 0; d < cells->dim; ++d)
+      ABH_REQUIRE(!cells->periodic[d], ABH_ERR_UNSUPPORTED,
+                  "Schwarz preconditioner needs a non-periodic cell list");
+    s->ctx = ctx;
+    s->dim = cells->dim;
+    s->n = cells->n_alive;
+    const int D = cells->dim;
+    std::vector<double> pos(size_t(s->n) * D);
+    std::vector<uint32_t> beg(size_t(cells->nb)), end(size_t(cells->nb));
+    hipStream_t hs = ctx->stream;
+    if (s->n > 0) {
+      ABH_HIP(hipMemcpyAsync(pos.data(), cells->pos.p, pos.size() * 8, hipMemcpyDeviceToHost, hs));
+      ABH_HIP(hipMemcpyAsync(beg.data(), cells->begin.p, beg.size() * 4, hipMemcpyDeviceToHost, hs));
+      ABH_HIP(hipMemcpyAsync(end.data(), cells->end.p, end.size() * 4, hipMemcpyDeviceToHost, hs));
+    }
+    ABH_HIP(hipStreamSynchronize(hs));
+    if (s->n > 0) analyse(s, cells, pos, beg, end);
+    for (size_t d = 0; d + 1 < s->off.size(); ++d)
+      s->max_m = std::max(s->max_m, int(s->off[d + 1] - s->off[d]));
+    KernelDesc k{};
+    k.type = kernel->type;
+    k.param = kernel->param;
+    k.param2 = kernel->param * kernel->param;
+    k.w0 = 1.0;
+    const double r2 = radius > 0.0 ? radius * radius : -1.0;
+    factorise(s, cells, k, r2, diag_shift);
+  });
+  if (st != ABH_OK) {
+    delete s;
+    return st;
+  }
+  abh::ctx_retain(ctx);
+  *out = s;
+  return ABH_OK;
+}
+
+abh_status abh_schwarz_destroy(abh_schwarz *s) {
+  if (!s) return ABH_OK;
+  abh_context *ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete s;
+  abh::ctx_release(ctx);
+  return ABH_OK;
+}
+
+abh_status abh_schwarz_get_info(const abh_schwarz *s, abh_schwarz_info *info) {
+  if (!s || !info) return ABH_ERR_INVALID_ARG;
+  abh_schwarz_info r{};
+  r.n = s->n;
+  r.n_domains = int64_t(s->nint.size());
+  r.min_size = r.n_domains ? INT64_MAX : 0;
+  r.min_interior = r.n_domains ? INT64_MAX : 0;
+  for (int64_t d = 0; d < r.n_domains; ++d) {
+    const int64_t m = s->off[d + 1] - s->off[d];
+    r.min_size = std::min(r.min_size, m);
+    r.max_size = std::max(r.max_size, m);
+    r.min_interior = std::min<int64_t>(r.min_interior, s->nint[d]);
+    r.max_interior = std::max<int64_t>(r.max_interior, s->nint[d]);
+  }
+  r.factor_bytes = int64_t(s->factor_bytes);
+  r.failed_domains = s->failed;
+  *info = r;
+  return ABH_OK;
+}
+
+abh_status abh_schwarz_get_domains(const abh_schwarz *s, int64_t *offsets, int32_t *indices,
+                                   int32_t *n_interior) {
+  if (!s) return ABH_ERR_INVALID_ARG;
+  if (offsets) std::copy(s->off.begin(), s->off.end(), offsets);
+  if (indices) std::copy(s->idx.begin(), s->idx.end(), indices);
+  if (n_interior) std::copy(s->nint.begin(), s->nint.end(), n_interior);
+  return ABH_OK;
+}
+
+abh_status abh_schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
+  if (!s) return ABH_ERR_INVALID_ARG;
+  return abh::guard(s->ctx, [&] {
+    ABH_REQUIRE(s->n == 0 || (b && x), ABH_ERR_INVALID_ARG, "b or x is null");
+    ABH_REQUIRE(b != x || s->n == 0, ABH_ERR_INVALID_ARG, "b and x must be distinct");
+    abh::schwarz_apply(s, b, x);
+  });
+}
+
+}  // extern "C"

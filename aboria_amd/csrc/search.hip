@@ -23,6 +23,7 @@
 
 #include "cells.hpp"
 #include "kernel_fn.hpp"
+#include "solvers.hpp"
 
 using abh::DevBuf;
 using abh::GridParams;
@@ -858,6 +859,27 @@ struct abh_sparse_op {
   int64_t nnz = 0;
   int64_t ncols = 0;
 };
+
+namespace abh {
+
+void sparse_apply(abh_sparse_op *op, const double *x, double *y, bool fp32) {
+  if (op->plan.nq == 0) return;
+  SearchArgs a = base_args(op->plan);
+  a.k = op->k;
+  a.nrhs = 1;
+  a.x = x;
+  a.ldx = op->ncols;
+  a.y = y;
+  a.ldy = op->plan.nq;
+  if (fp32) launch_search<kMatvec32>(op->plan, a);
+  else launch_search<kMatvec64>(op->plan, a);
+}
+
+int64_t sparse_rows(const abh_sparse_op *op) { return op->plan.nq; }
+int64_t sparse_cols(const abh_sparse_op *op) { return op->ncols; }
+abh_context *sparse_context(const abh_sparse_op *op) { return op->plan.ctx; }
+
+}  // namespace abh
 
 extern "C" {
 

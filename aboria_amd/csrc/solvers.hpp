@@ -1,0 +1,23 @@
+// Internal entry points shared by the operator, preconditioner and Krylov
+// translation units (search.hip, schwarz.hip, learned.hip, krylov.hip).
+#pragma once
+
+#include "common.hpp"
+
+namespace abh {
+
+// y += K x for one right-hand side (search.hip).
+void sparse_apply(abh_sparse_op *op, const double *x, double *y, bool fp32);
+int64_t sparse_rows(const abh_sparse_op *op);
+int64_t sparse_cols(const abh_sparse_op *op);
+abh_context *sparse_context(const abh_sparse_op *op);
+
+// x = M^-1 b, restricted additive Schwarz (schwarz.hip).
+void schwarz_apply(const abh_schwarz *s, const double *b, double *x);
+int64_t schwarz_size(const abh_schwarz *s);
+
+// x = W b with W the approximate-inverse rows (learned.hip).
+void ainv_apply(const abh_ainv *a, const double *b, double *x);
+int64_t ainv_size(const abh_ainv *a);
+
+}  // namespace abh

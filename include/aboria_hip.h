@@ -41,7 +41,8 @@ typedef enum abh_status {
   ABH_ERR_CAPACITY = 4,      /* caller-provided output buffer too small      */
   ABH_ERR_UNSUPPORTED = 5,   /* kernel type / precision not implemented      */
   ABH_ERR_DOMAIN = 6,        /* position too far outside a periodic domain   */
-  ABH_ERR_NOT_CONVERGED = 7  /* iterative solve hit its iteration limit      */
+  ABH_ERR_NOT_CONVERGED = 7, /* iterative solve hit its iteration limit      */
+  ABH_ERR_NUMERICAL = 8      /* a dense factorisation met a bad pivot        */
 } abh_status;
 
 typedef struct abh_context abh_context;
@@ -288,6 +289,141 @@ typedef struct abh_gmres_stats {
 abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x,
                                  const abh_gmres_options *opts,
                                  abh_gmres_stats *stats);
+
+/* ------------------------------------------------------------------ */
+/* restricted additive Schwarz preconditioner                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_schwarz abh_schwarz;
+
+typedef struct abh_schwarz_info {
+  int64_t n;               /* rows: particles of the cell list             */
+  int64_t n_domains;       /* domains with a non-empty interior            */
+  int64_t min_size, max_size;         /* interior + buffer particles       */
+  int64_t min_interior, max_interior; /* interior particles                */
+  int64_t factor_bytes;    /* device bytes of the stored inverse rows      */
+  int64_t failed_domains;  /* local Cholesky factorisations that met a
+                              non-positive pivot                           */
+} abh_schwarz_info;
+
+/* Replaces SchwartzPreconditioner<Eigen::LLT<MatrixXd>>::analyzePattern +
+ * factorize (src/Preconditioners.h:1169-1316 and :1402-1467) for the
+ * operator A = K + diag_shift * I over the cell list's particles (gather
+ * order), with K_ij = phi(|x_i - x_j|) when |x_i - x_j|^2 <= radius^2 (every
+ * pair when radius <= 0) — the matrix an abh_sparse_op of the same kernel
+ * and radius applies, plus the diagonal shift of the reference's
+ * "self kernel" (tests/rbf_interpolation.h:673-680).
+ * One domain per non-empty bucket: its particles inside the bucket bounds
+ * are the interior, every other particle of the 3^dim block of buckets
+ * around it is the buffer (get_buckets_near_point with the default
+ * neighbourhood buffer).  The interior rows of each local inverse are
+ * computed once, in FP64 (blocked Cholesky), and kept on the device.
+ * Non-periodic cell lists only; the kernel takes no per-particle scalars. */
+abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells,
+                              const abh_kernel *kernel, double radius,
+                              double diag_shift, abh_schwarz **out);
+abh_status abh_schwarz_destroy(abh_schwarz *s);
+abh_status abh_schwarz_get_info(const abh_schwarz *s, abh_schwarz_info *info);
+/* Domain lists as host arrays (any pointer may be NULL): offsets has
+ * n_domains+1 entries; domain d is indices[offsets[d] .. offsets[d+1]), its
+ * first n_interior[d] entries the interior.  Domains are in bucket order and
+ * particles in bucket order, the reference's visiting order being a
+ * permutation of the same sets. */
+abh_status abh_schwarz_get_domains(const abh_schwarz *s, int64_t *offsets,
+                                   int32_t *indices, int32_t *n_interior);
+/* x = M^-1 b (SchwartzPreconditioner::_solve_impl,
+ * src/Preconditioners.h:1476-1522): x = b, then every domain overwrites its
+ * interior rows with its local solution.  b, x: device, length n, distinct. */
+abh_status abh_schwarz_apply(const abh_schwarz *s, const double *b, double *x);
+
+/* ------------------------------------------------------------------ */
+/* learned approximate-inverse preconditioner                          */
+/* ------------------------------------------------------------------ */
+
+/* No counterpart in the reference (the input of the learned
+ * preconditioner): for every particle i of the cell list (gather order) its
+ * k nearest particles, itself first, by FP64 squared distance with ties
+ * broken by index.  nbr: device, n x k int32; dist2: device n x k or NULL.
+ * Non-periodic cell lists only; k <= 64. */
+abh_status abh_knn(abh_context *ctx, const abh_cells *cells, int k, int32_t *nbr,
+                   double *dist2);
+
+typedef struct abh_mlp abh_mlp;
+
+/* No counterpart in the reference: the network that stands in for the
+ * Schwarz local solves.  Fully connected n_in -> hidden (n_hidden layers,
+ * SiLU) -> n_out (linear), FP32 parameters, initialised from seed.
+ * Parameter layout (host and gradient arrays): for each layer its weight
+ * matrix (out x in, row-major) then its bias. */
+abh_status abh_mlp_create(abh_context *ctx, int n_in, int hidden, int n_hidden,
+                          int n_out, uint64_t seed, abh_mlp **out);
+abh_status abh_mlp_destroy(abh_mlp *m);
+abh_status abh_mlp_num_params(const abh_mlp *m, int64_t *n);
+abh_status abh_mlp_get_params(const abh_mlp *m, float *host);
+abh_status abh_mlp_set_params(abh_mlp *m, const float *host);
+/* y = net(x).  x: device, n x n_in FP32 row-major; y: device n x n_out.
+ * bf16 = 0: FP32 arithmetic; bf16 = 1: weights and activations rounded to
+ * BF16, products on MFMA with FP32 accumulation. */
+abh_status abh_mlp_forward(abh_mlp *m, int64_t n, const float *x, float *y, int bf16);
+/* loss = mean over rows and outputs of (net(x) - t)^2, and its gradient
+ * with respect to the parameters (grad: device, num_params FP32, FP32
+ * forward and backward kernels). */
+abh_status abh_mlp_loss_grad(abh_mlp *m, int64_t n, const float *x, const float *t,
+                             double *loss, float *grad);
+
+typedef struct abh_mlp_train_options {
+  int32_t epochs;
+  int32_t batch;         /* rows per Adam step                               */
+  float lr, beta1, beta2, eps;  /* Adam                                      */
+  float lr_decay;        /* learning rate multiplied by this every epoch    */
+  int32_t reserved;
+  uint64_t seed;         /* shuffling                                        */
+} abh_mlp_train_options;
+
+/* Adam on the mean squared error over (x, t) rows; loss_per_epoch: host
+ * array of epochs entries (mean training loss of each epoch) or NULL. */
+abh_status abh_mlp_train(abh_mlp *m, int64_t n, const float *x, const float *t,
+                         const abh_mlp_train_options *opts, float *loss_per_epoch);
+
+typedef struct abh_ainv abh_ainv;
+
+/* Approximate-inverse rows on the k-nearest-neighbour pattern: row i has
+ * weights w_i on nbr_i (abh_knn) and applies as x_i = sum_k w_ik b_nbr_ik.
+ * _lsq: w_i minimises || A[:, nbr_i] w - e_i ||_2 for the Schwarz operator
+ * A = K + diag_shift I (same kernel/radius convention as abh_schwarz_create),
+ * solved from the local dense normal equations in FP64: the training target
+ * of the network and its exact counterpart.
+ * _learned: w_i = out_scale * net(features_i), features_i = the relative
+ * positions (x_nbr_ik - x_i) / feature_scale, k x dim FP32 (abh_ainv_features). */
+abh_status abh_ainv_create_lsq(abh_context *ctx, const abh_cells *cells,
+                               const abh_kernel *kernel, double radius,
+                               double diag_shift, int k, abh_ainv **out);
+abh_status abh_ainv_create_learned(abh_context *ctx, const abh_cells *cells,
+                                   abh_mlp *net, int k, double feature_scale,
+                                   double out_scale, int bf16, abh_ainv **out);
+abh_status abh_ainv_destroy(abh_ainv *a);
+/* device copies of the pattern (n x k int32) and weights (n x k FP64);
+ * either may be NULL */
+abh_status abh_ainv_get(const abh_ainv *a, int32_t *nbr, double *w);
+/* features of every particle (device, n x (k*dim) FP32) */
+abh_status abh_ainv_features(const abh_ainv *a, double feature_scale, float *features);
+abh_status abh_ainv_apply(const abh_ainv *a, const double *b, double *x);
+
+/* ------------------------------------------------------------------ */
+/* GMRES on a sparse operator                                          */
+/* ------------------------------------------------------------------ */
+
+/* Solves (K + diag_shift I) x = b for a square sparse operator (rows = the
+ * cell list's particles) with restarted GMRES, right preconditioned by
+ * schwarz or ainv (at most one non-NULL; both NULL = unpreconditioned) —
+ * Eigen::GMRES<MatrixReplacement, SchwartzPreconditioner> in the reference
+ * (tests/rbf_interpolation.h:351-369).  b, x: device, length n; x holds the
+ * initial guess.  stats: host, one entry (may be NULL). */
+abh_status abh_sparse_op_gmres(abh_sparse_op *op, double diag_shift,
+                               const abh_schwarz *schwarz, const abh_ainv *ainv,
+                               const double *b, double *x,
+                               const abh_gmres_options *opts,
+                               abh_gmres_stats *stats);
 
 #ifdef __cplusplus
 }

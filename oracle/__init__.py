@@ -69,6 +69,16 @@ def lib():
         l.orc_cheb_node.argtypes = [ctypes.c_uint, ctypes.c_uint]
         l.orc_cheb_Sn.restype = ctypes.c_double
         l.orc_cheb_Sn.argtypes = [ctypes.c_double, ctypes.c_uint, ctypes.c_uint]
+        l.orc_schwarz_create.restype = vp
+        l.orc_schwarz_create.argtypes = [vp, P(KernelC), ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double]
+        l.orc_schwarz_destroy.argtypes = [vp]
+        l.orc_schwarz_info.argtypes = [vp, _i64p]
+        l.orc_schwarz_domains.argtypes = [vp, _i64p, _i32p, _i32p]
+        l.orc_schwarz_apply.argtypes = [vp, _dp, _dp]
+        l.orc_knn.argtypes = [vp, ctypes.c_int, _i32p, _dp]
+        l.orc_lsq_rows.argtypes = [vp, P(KernelC), ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   _i32p, _dp]
         _lib = l
     return _lib
 
@@ -146,9 +156,60 @@ class Cells:
         out = yt.T.copy()
         return out[:, 0] if np.asarray(x).ndim == 1 else out
 
+    def schwarz(self, kernel, radius, shift, buffer=1e5 * np.finfo(float).eps):
+        """Restricted additive Schwarz preconditioner of A = K + shift I."""
+        return Schwarz(self, kernel, radius, shift, buffer)
+
+    def knn(self, k):
+        """(n, k) neighbour indices (self first) and squared distances."""
+        nbr = np.zeros((self.n_alive, k), np.int32)
+        d2 = np.zeros((self.n_alive, k), np.float64)
+        lib().orc_knn(self.h, int(k), nbr.reshape(-1), d2.reshape(-1))
+        return nbr, d2
+
+    def lsq_rows(self, kernel, radius, shift, nbr):
+        """Least-squares approximate-inverse rows on the pattern nbr (n, k)."""
+        nbr = np.ascontiguousarray(nbr, np.int32)
+        w = np.zeros(nbr.shape, np.float64)
+        kd = KernelC(kernel[0], 0, kernel[1], 0.0, 0.0, 0.0)
+        lib().orc_lsq_rows(self.h, ctypes.byref(kd), float(radius), float(shift), nbr.shape[1],
+                           nbr.reshape(-1), w.reshape(-1))
+        return w
+
     def __del__(self):
         try:
             lib().orc_cells_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Schwarz:
+    """SchwartzPreconditioner<LLT> restated on the CPU over a Cells object."""
+
+    def __init__(self, cells, kernel, radius, shift, buffer):
+        self.cells = cells
+        kd = KernelC(kernel[0], 0, kernel[1], 0.0, 0.0, 0.0)
+        self.h = lib().orc_schwarz_create(cells.h, ctypes.byref(kd), float(radius), float(shift),
+                                          float(buffer))
+        info = np.zeros(3, np.int64)
+        lib().orc_schwarz_info(self.h, info)
+        self.n_domains, total, self.failed = (int(v) for v in info)
+        self.offsets = np.zeros(self.n_domains + 1, np.int64)
+        self.indices = np.zeros(max(total, 1), np.int32)
+        self.n_interior = np.zeros(max(self.n_domains, 1), np.int32)
+        lib().orc_schwarz_domains(self.h, self.offsets, self.indices, self.n_interior)
+        self.indices = self.indices[:total]
+        self.n_interior = self.n_interior[:self.n_domains]
+
+    def apply(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b)
+        lib().orc_schwarz_apply(self.h, b, x)
+        return x
+
+    def __del__(self):
+        try:
+            lib().orc_schwarz_destroy(self.h)
         except Exception:
             pass
 

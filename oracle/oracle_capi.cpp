@@ -10,6 +10,7 @@
 #include "cells_search.hpp"
 #include "fmm.hpp"
 #include "kernels.hpp"
+#include "schwarz.hpp"
 
 using namespace orc;
 
@@ -250,5 +251,66 @@ int orc_fmm_matvec(void *p, int order, const Kernel *k, const double *q, double 
 // Chebyshev helpers for the closed-form checks
 double orc_cheb_node(unsigned i, unsigned n) { return cheb_node(i, n); }
 double orc_cheb_Sn(double x, unsigned i, unsigned n) { return cheb_Sn(x, i, n); }
+
+// ---- Schwarz preconditioner, k nearest neighbours, least-squares rows ----
+
+struct SchwarzH {
+  Schwarz s;
+};
+
+// buffer: the reference's m_neighbourhood_buffer (default 1e5 * DBL_EPSILON)
+void *orc_schwarz_create(void *p, const Kernel *k, double radius, double shift, double buffer) {
+  auto *h = static_cast<AnyCells *>(p);
+  auto *sh = new SchwarzH();
+  const double r2 = radius > 0.0 ? radius * radius : -1.0;
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    schwarz_analyse<D>(get<D>(h), buffer, sh->s);
+    schwarz_factorise<D>(get<D>(h), *k, r2, shift, sh->s);
+    return 0;
+  });
+  return sh;
+}
+
+void orc_schwarz_destroy(void *p) { delete static_cast<SchwarzH *>(p); }
+
+// counts = {n_domains, total indices, failed}
+void orc_schwarz_info(void *p, int64_t *counts) {
+  const Schwarz &s = static_cast<SchwarzH *>(p)->s;
+  counts[0] = int64_t(s.nint.size());
+  counts[1] = int64_t(s.idx.size());
+  counts[2] = s.failed;
+}
+
+void orc_schwarz_domains(void *p, int64_t *off, int32_t *idx, int32_t *nint) {
+  const Schwarz &s = static_cast<SchwarzH *>(p)->s;
+  std::copy(s.off.begin(), s.off.end(), off);
+  std::copy(s.idx.begin(), s.idx.end(), idx);
+  std::copy(s.nint.begin(), s.nint.end(), nint);
+}
+
+void orc_schwarz_apply(void *p, const double *b, double *x) {
+  schwarz_apply(static_cast<SchwarzH *>(p)->s, b, x);
+}
+
+void orc_knn(void *p, int k, int32_t *nbr, double *dist2) {
+  auto *h = static_cast<AnyCells *>(p);
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    knn<D>(get<D>(h), k, nbr, dist2);
+    return 0;
+  });
+}
+
+void orc_lsq_rows(void *p, const Kernel *kern, double radius, double shift, int k,
+                  const int32_t *nbr, double *w) {
+  auto *h = static_cast<AnyCells *>(p);
+  const double r2 = radius > 0.0 ? radius * radius : -1.0;
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    lsq_rows<D>(get<D>(h), *kern, r2, shift, k, nbr, w);
+    return 0;
+  });
+}
 
 }  // extern "C"

@@ -1,0 +1,205 @@
+"""GPU Schwarz preconditioner, sparse GMRES, k-nearest-neighbour patterns,
+least-squares approximate-inverse rows and the learned preconditioner's
+network, against the CPU oracle, called through the C ABI.
+
+Bars: domain lists and kNN patterns (indices and FP64 squared distances)
+bit-exact; Schwarz application within 1e-9 relative of the oracle (FP64,
+different factorisation order); least-squares rows within 1e-7 relative;
+network loss within 1e-5 and gradient within 1e-4 (max-norm relative) of the
+FP64 oracle (FP32 kernels); BF16 MFMA inference within 3e-2 of FP32; GMRES
+residuals recomputed by the oracle's sparse matvec.
+"""
+import numpy as np
+import pytest
+import torch
+
+import aboria_amd as ab
+import oracle
+from aboria_amd import learned, workloads as wl
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+GAUSS = 1
+
+
+def build(ctx, pos, n_leaf, low=None, high=None):
+    dim = pos.shape[1]
+    low = low if low is not None else [0.0] * dim
+    high = high if high is not None else [1.0] * dim
+    cl = ab.CellListOrdered(ctx, low, high, [False] * dim, n_leaf)
+    p = torch.tensor(pos, device="cuda")
+    cl.update_positions(p)
+    oc = oracle.Cells(low, high, [0] * dim, n_leaf)
+    oc.update(pos.copy())
+    assert np.array_equal(cl.positions().cpu().numpy(), oc.positions)
+    return cl, oc
+
+
+def op_matvec(oc, kernel, radius, shift, x):
+    """(K + shift I) x on the CPU oracle."""
+    return oc.sparse_matvec(oc.positions, radius, kernel, x) + shift * x
+
+
+@pytest.mark.parametrize("dim,n,leaf", [(2, 3000, 60.0), (3, 4000, 80.0), (1, 500, 20.0)])
+def test_schwarz_domains_match_oracle(ctx, dim, n, leaf):
+    print("schwarz domains", dim, n, leaf, flush=True)
+    pos = np.random.default_rng(dim).random((n, dim))
+    cl, oc = build(ctx, pos, leaf)
+    k = ab.Kernel.gaussian(0.02)
+    sw = ab.SchwartzPreconditioner(ctx, cl, k, 0.1, 1e-3)
+    off, idx, nint = sw.domains()
+    os_ = oc.schwarz((GAUSS, 0.02), 0.1, 1e-3)
+    assert np.array_equal(off, os_.offsets)
+    assert np.array_equal(idx, os_.indices)
+    assert np.array_equal(nint, os_.n_interior)
+    info = sw.info
+    assert info["n_domains"] == os_.n_domains and info["failed_domains"] == 0
+
+
+@pytest.mark.parametrize("sigma_h,leaf", [(1.5, 36.0), (2.5, 100.0)])
+def test_schwarz_apply_matches_oracle(ctx, sigma_h, leaf):
+    print("schwarz apply", sigma_h, leaf, flush=True)
+    pos = wl.jittered_grid(40, 40, seed=7)
+    h = 1.0 / 40
+    sigma, radius, shift = sigma_h * h, 6 * sigma_h * h, 1e-4
+    cl, oc = build(ctx, pos, leaf)
+    sw = ab.SchwartzPreconditioner(ctx, cl, ab.Kernel.gaussian(sigma), radius, shift)
+    b = np.random.default_rng(3).normal(size=cl.n_alive)
+    x = sw.solve(torch.tensor(b, device="cuda")).cpu().numpy()
+    ref = oc.schwarz((GAUSS, sigma), radius, shift).apply(b)
+    err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+    print("schwarz apply rel err %.2e" % err, flush=True)
+    assert err < 1e-9
+
+
+def test_schwarz_single_domain_is_the_inverse(ctx):
+    pos = np.random.default_rng(4).random((700, 2))
+    cl, oc = build(ctx, pos, 5000.0)
+    k = ab.Kernel.gaussian(0.05)
+    sw = ab.SchwartzPreconditioner(ctx, cl, k, 0.3, 1e-3)
+    assert sw.info["n_domains"] == 1
+    b = np.random.default_rng(5).normal(size=700)
+    x = sw.solve(torch.tensor(b, device="cuda")).cpu().numpy()
+    r = op_matvec(oc, (GAUSS, 0.05), 0.3, 1e-3, x) - b
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-9
+
+
+def test_sparse_gmres_with_schwarz(ctx):
+    pos = wl.jittered_grid(48, 48, seed=8)
+    h = 1.0 / 48
+    sigma, radius, shift = 1.5 * h, 9 * h, 1e-5
+    cl, oc = build(ctx, pos, 64.0)
+    k = ab.Kernel.gaussian(sigma)
+    p = cl.positions()
+    op = ab.SparseOperator(ctx, cl, p, radius, k)
+    b = np.random.default_rng(9).normal(size=cl.n_alive)
+    bt = torch.tensor(b, device="cuda")
+    sw = ab.SchwartzPreconditioner(ctx, cl, k, radius, shift)
+    x, st = op.gmres(bt, diag_shift=shift, preconditioner=sw, restart=50, max_iters=500)
+    _, st0 = op.gmres(bt, diag_shift=shift, restart=50, max_iters=500)
+    print("gmres schwarz", st, "none", st0, flush=True)
+    assert st["converged"]
+    r = op_matvec(oc, (GAUSS, sigma), radius, shift, x.cpu().numpy()) - b
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
+    assert st["iterations"] < st0["iterations"]
+
+
+@pytest.mark.parametrize("dim,k,leaf", [(2, 32, 8.0), (3, 32, 20.0), (2, 1, 8.0), (2, 64, 200.0)])
+def test_knn_matches_oracle(ctx, dim, k, leaf):
+    pos = np.random.default_rng(20 + dim + k).random((5000, dim))
+    cl, oc = build(ctx, pos, leaf)
+    nbr, d2 = ab.knn(cl, k, with_dist2=True)
+    onbr, od2 = oc.knn(k)
+    assert np.array_equal(nbr.cpu().numpy(), onbr)
+    assert np.array_equal(d2.cpu().numpy(), od2)
+
+
+def test_lsq_rows_match_oracle(ctx):
+    pos = wl.jittered_grid(30, 30, seed=12)
+    h = 1.0 / 30
+    sigma, radius, shift, k = 1.26 * h, 6 * 1.26 * h, 1e-5, 32
+    cl, oc = build(ctx, pos, 16.0)
+    ai = ab.ApproxInverse.lsq(ctx, cl, ab.Kernel.gaussian(sigma), radius, shift, k)
+    nbr = ai.pattern().cpu().numpy()
+    assert np.array_equal(nbr, oc.knn(k)[0])
+    w = ai.weights().cpu().numpy()
+    wo = oc.lsq_rows((GAUSS, sigma), radius, shift, nbr)
+    err = np.linalg.norm(w - wo) / np.linalg.norm(wo)
+    print("lsq rows rel err %.2e" % err, flush=True)
+    assert err < 1e-7
+    # apply = the row gather
+    b = np.random.default_rng(1).normal(size=cl.n_alive)
+    x = ai.solve(torch.tensor(b, device="cuda")).cpu().numpy()
+    assert np.allclose(x, (wo * b[nbr]).sum(1), rtol=1e-6, atol=1e-9 * np.abs(wo).max())
+
+
+def test_mlp_loss_and_gradient_match_oracle(ctx):
+    dims = [64, 128, 128, 128, 32]
+    net = ab.Mlp(ctx, 64, 128, 3, 32, seed=3)
+    rng = np.random.default_rng(4)
+    _, npar = omlp.layout(dims)
+    assert net.num_params == npar
+    p = net.get_params().astype(np.float64)
+    x = rng.normal(0, 1, (1000, 64)).astype(np.float32)
+    t = rng.normal(0, 1, (1000, 32)).astype(np.float32)
+    loss, g = net.loss_grad(torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda"))
+    lo, go = omlp.loss_grad(dims, p, x.astype(np.float64), t.astype(np.float64))
+    gg = g.cpu().numpy()
+    print("loss %.6e oracle %.6e grad max err %.2e" % (loss, lo,
+          np.max(np.abs(gg - go)) / np.max(np.abs(go))), flush=True)
+    assert abs(loss - lo) / lo < 1e-5
+    assert np.max(np.abs(gg - go)) / np.max(np.abs(go)) < 1e-4
+    # forward, FP32 and BF16 MFMA
+    xt = torch.tensor(x, device="cuda")
+    y32 = net.forward(xt).cpu().numpy()
+    yo = omlp.forward(dims, p, x.astype(np.float64))
+    assert np.max(np.abs(y32 - yo)) / np.max(np.abs(yo)) < 1e-5
+    y16 = net.forward(xt, bf16=True).cpu().numpy()
+    e16 = np.linalg.norm(y16 - yo) / np.linalg.norm(yo)
+    print("bf16 forward rel err %.2e" % e16, flush=True)
+    assert e16 < 3e-2
+
+
+def test_mlp_training_reduces_loss(ctx):
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, (8192, 64)).astype(np.float32)
+    W = rng.normal(0, 0.2, (64, 32))
+    t = np.tanh(x @ W).astype(np.float32)
+    net = ab.Mlp(ctx, 64, 128, 3, 32, seed=1)
+    xt, tt = torch.tensor(x, device="cuda"), torch.tensor(t, device="cuda")
+    l0, _ = net.loss_grad(xt, tt)
+    losses = net.train(xt, tt, epochs=15, batch=256, lr=2e-3)
+    l1, _ = net.loss_grad(xt, tt)
+    print("train loss %.3e -> %.3e" % (l0, l1), losses, flush=True)
+    assert l1 < 0.1 * l0
+
+
+def test_learned_preconditioner_gain(ctx):
+    """Trained on least-squares rows of seeded point sets, applied to a new
+    one: the learned preconditioner cuts GMRES iterations well below the
+    unpreconditioned count (the plain Schwarz count is printed beside)."""
+    n1 = 64
+    h = 1.0 / n1
+    sigma, radius, shift, k = 1.26 * h, 6 * 1.26 * h, 1e-5, 32
+    kern = ab.Kernel.gaussian(sigma)
+    sets = [wl.jittered_grid(n1, n1, seed=100 + s) for s in range(4)]
+    X, T = learned.training_set(ctx, sets, kern, radius, shift, 16.0, k, feature_scale=sigma)
+    net, scale, losses = learned.train(ctx, X, T, epochs=25, batch=512)
+    print("training losses", losses[0], losses[-1], flush=True)
+    pos = wl.jittered_grid(n1, n1, seed=999)
+    cl, oc = build(ctx, pos, 16.0)
+    op = ab.SparseOperator(ctx, cl, cl.positions(), radius, kern)
+    b = torch.tensor(np.random.default_rng(3).normal(size=cl.n_alive), device="cuda")
+    res = {}
+    for name, pre in [("none", None),
+                      ("lsq", ab.ApproxInverse.lsq(ctx, cl, kern, radius, shift, k)),
+                      ("learned", ab.ApproxInverse.learned(ctx, cl, net, k, sigma, scale)),
+                      ("schwarz", ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift))]:
+        x, st = op.gmres(b, diag_shift=shift, preconditioner=pre, restart=50, max_iters=2000)
+        r = op_matvec(oc, (GAUSS, sigma), radius, shift, x.cpu().numpy()) - b.cpu().numpy()
+        st["true_residual"] = float(np.linalg.norm(r) / torch.linalg.norm(b).item())
+        res[name] = st
+    print("gmres iterations", {k_: (v["iterations"], v["converged"]) for k_, v in res.items()},
+          flush=True)
+    assert res["learned"]["converged"] and res["learned"]["true_residual"] <= 1.01e-8
+    assert res["learned"]["iterations"] < 0.5 * res["none"]["iterations"]
