@@ -18,7 +18,7 @@ __all__ = [
     "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
     "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
     "DenseBatchOperator", "SchwartzPreconditioner", "knn", "Mlp", "ApproxInverse",
-    "gmres_with_fallback",
+    "gmres_with_fallback", "dense_apply",
 ]
 
 FP64 = C.FP64
@@ -468,6 +468,25 @@ def create_fmm_operator(particles, order, kernel, low, high, n_particles_in_leaf
     """create_fmm_operator<N>(particles, particles, f, f) on a HyperOctree."""
     return FmmOperator(particles.ctx, particles.position, low, high, n_particles_in_leaf, order,
                        kernel)
+
+
+def dense_apply(ctx, row_positions, col_positions, kernel, x, y=None, precision=FP64):
+    """y += K x for the rectangular dense operator create_dense_operator(rows,
+    cols, f); x (ncols,) or (nrhs, ncols) (rows of x are right-hand sides)."""
+    _require_cuda(row_positions, torch.float64, "row_positions")
+    _require_cuda(col_positions, torch.float64, "col_positions")
+    xt = x if x.dim() == 2 else x[None, :]
+    xt = xt.contiguous()
+    nrows = row_positions.shape[0]
+    if y is None:
+        y = torch.zeros((xt.shape[0], nrows), dtype=torch.float64, device=x.device)
+    yt = y if y.dim() == 2 else y[None, :]
+    kd = kernel.to_c()
+    check(C.lib().abh_dense_apply(ctx.handle, row_positions.shape[1], nrows, _ptr(row_positions),
+                                  col_positions.shape[0], _ptr(col_positions), ctypes.byref(kd),
+                                  xt.shape[0], _ptr(xt), xt.shape[1], _ptr(yt), yt.shape[1],
+                                  precision), ctx.handle)
+    return y
 
 
 class DenseBatchOperator:

@@ -82,6 +82,77 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
   }
 }
 
+// Rectangular operator y += K x over separate row / column point sets,
+// up to kRectRhs right-hand sides per pass (column-major x, y).
+constexpr int kRectRhs = 16;
+
+template <int D, bool FP32>
+__global__ __launch_bounds__(kRows) void k_dense_rect(int64_t nrows, const double *__restrict__ rp,
+                                                      int64_t ncols, const double *__restrict__ cp,
+                                                      KernelDesc k, int nrhs,
+                                                      const double *__restrict__ x, int64_t ldx,
+                                                      double *__restrict__ y, int64_t ldy) {
+  __shared__ double sp[kRows][D];
+  __shared__ double sx[kRectRhs][kRows];
+  const int64_t i = int64_t(blockIdx.x) * kRows + threadIdx.x;
+  const bool act = i < nrows;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = act ? rp[i * D + d] : 0.0;
+  double acc[kRectRhs];
+  float a32[kRectRhs], c32[kRectRhs];
+#pragma unroll
+  for (int r = 0; r < kRectRhs; ++r) {
+    acc[r] = 0.0;
+    a32[r] = 0.0f;
+    c32[r] = 0.0f;
+  }
+  for (int64_t c0 = 0; c0 < ncols; c0 += kRows) {
+    const int m = int(ncols - c0 < kRows ? ncols - c0 : kRows);
+    __syncthreads();
+    if (threadIdx.x < m) {
+      const int64_t j = c0 + threadIdx.x;
+#pragma unroll
+      for (int d = 0; d < D; ++d) sp[threadIdx.x][d] = cp[j * D + d];
+      for (int r = 0; r < nrhs; ++r) sx[r][threadIdx.x] = x[int64_t(r) * ldx + j];
+    }
+    __syncthreads();
+    if (!act) continue;
+    for (int j = 0; j < m; ++j) {
+      if (FP32) {
+        float s = 0.0f;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const float dx = float(sp[j][d] - xi[d]);
+          s = s + dx * dx;
+        }
+        const float f = abh::phi32(k, s);
+#pragma unroll
+        for (int r = 0; r < kRectRhs; ++r)
+          if (r < nrhs) abh::kb_add(a32[r], c32[r], f * float(sx[r][j]));
+      } else {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const double dx = sp[j][d] - xi[d];
+          s = s + dx * dx;
+        }
+        const double f = abh::phi64(k, s);
+#pragma unroll
+        for (int r = 0; r < kRectRhs; ++r)
+          if (r < nrhs) acc[r] += f * sx[r][j];
+      }
+    }
+  }
+  if (act)
+#pragma unroll
+    for (int r = 0; r < kRectRhs; ++r)
+      if (r < nrhs) {
+        const double v = FP32 ? double(a32[r]) + double(c32[r]) : acc[r];
+        y[int64_t(r) * ldy + i] += v;
+      }
+}
+
 }  // namespace
 
 struct abh_dense_batch {
@@ -216,6 +287,50 @@ abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x
     abh::batch_gmres(ctx, op->L, f_op, f_true, nullptr, f_act, b, x, *opts, res);
     if (stats)
       for (int p = 0; p < op->L.nprob; ++p) stats[p] = res.stats[p];
+  });
+}
+
+abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const double *row_pos,
+                           int64_t ncols, const double *col_pos, const abh_kernel *kernel,
+                           int nrhs, const double *x, int64_t ldx, double *y, int64_t ldy,
+                           abh_precision precision) {
+  if (!ctx || !kernel) return ABH_ERR_INVALID_ARG;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(dim >= 1 && dim <= 3, ABH_ERR_INVALID_ARG, "dim must be 1..3");
+    ABH_REQUIRE(nrows >= 0 && ncols >= 0 && nrhs >= 0, ABH_ERR_INVALID_ARG, "negative size");
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
+                "dense precision must be ABH_FP64 or ABH_FP32_COMPENSATED");
+    if (nrows == 0 || nrhs == 0) return;
+    ABH_REQUIRE(row_pos && y && (ncols == 0 || (col_pos && x)), ABH_ERR_INVALID_ARG,
+                "null pointer");
+    ABH_REQUIRE(ldx >= ncols && ldy >= nrows, ABH_ERR_INVALID_ARG,
+                "leading dimension smaller than the operator");
+    KernelDesc k{};
+    k.type = kernel->type;
+    k.param = kernel->param;
+    k.param2 = kernel->param * kernel->param;
+    const bool f32 = precision == ABH_FP32_COMPENSATED;
+    hipStream_t s = ctx->stream;
+    const unsigned grid = unsigned(abh::ceil_div(nrows, kRows));
+    for (int r0 = 0; r0 < nrhs; r0 += kRectRhs) {
+      const int nr = std::min(kRectRhs, nrhs - r0);
+      const double *xr = x + int64_t(r0) * ldx;
+      double *yr = y + int64_t(r0) * ldy;
+#define ABH_RECT(DD)                                                                            \
+  if (f32)                                                                                      \
+    k_dense_rect<DD, true><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr, ldx, \
+                                                  yr, ldy);                                     \
+  else                                                                                          \
+    k_dense_rect<DD, false><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr,   \
+                                                   ldx, yr, ldy);
+      if (dim == 1) { ABH_RECT(1) }
+      else if (dim == 2) { ABH_RECT(2) }
+      else { ABH_RECT(3) }
+#undef ABH_RECT
+      abh::check_launch();
+    }
   });
 }
 

@@ -264,6 +264,18 @@ abh_status abh_dense_batch_destroy(abh_dense_batch *op);
 abh_status abh_dense_batch_apply(abh_dense_batch *op, const double *x, double *y,
                                  abh_precision precision);
 
+/* Replaces create_dense_operator(rows, cols, f) with distinct row and column
+ * particle sets (src/Operators.h:309-317, KernelDense::evaluate
+ * src/Kernels.h:212-267): y += K x, K_ij = phi(|x_col_j - x_row_i|), for nrhs
+ * right-hand sides.  row_pos (nrows x dim), col_pos (ncols x dim): device;
+ * x: column-major ncols x nrhs (leading dimension ldx); y: nrows x nrhs
+ * (ldy).  precision: ABH_FP64 or ABH_FP32_COMPENSATED. */
+abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows,
+                           const double *row_pos, int64_t ncols,
+                           const double *col_pos, const abh_kernel *kernel,
+                           int nrhs, const double *x, int64_t ldx, double *y,
+                           int64_t ldy, abh_precision precision);
+
 typedef struct abh_gmres_options {
   int32_t restart;       /* Krylov dimension m of GMRES(m)                   */
   int32_t max_iters;     /* Arnoldi steps per problem (all restarts)         */

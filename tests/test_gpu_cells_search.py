@@ -172,3 +172,27 @@ def test_mixture_sparse_subset(ctx, precision):
     tol = 1e-13 if precision == ab.FP64 else 2e-6
     err = np.abs(y[rows] - y_o)
     assert np.all(err <= tol * scale + 1e-300), float((err / scale).max())
+
+
+def test_golden_c1_mini(ctx):
+    """The committed C1-mini fixture: build bit-exact, neighbour lists equal
+    as multisets per row, Gaussian matvec within 1e-13 relative."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_mini.npz"))
+    pos = torch.tensor(g["positions"], device="cuda")
+    cl = ab.CellListOrdered(ctx, [0, 0, 0], [1, 1, 1], [False] * 3, 6.0)
+    cl.update_positions(pos)
+    assert np.array_equal(cl.order().cpu().numpy(), g["order"])
+    assert np.array_equal(cl.bucket_index().cpu().numpy().view(np.uint32), g["keys"])
+    assert np.array_equal(cl.bucket_begin().cpu().numpy().view(np.uint32), g["begin"])
+    assert np.array_equal(cl.bucket_end().cpu().numpy().view(np.uint32), g["end"])
+    r = float(g["radius"])
+    res = ab.euclidean_search(cl, cl.positions(), r)
+    rp = res.row_ptr.cpu().numpy()
+    col = res.col.cpu().numpy()
+    assert np.array_equal(rp, g["row_ptr"])
+    for i in range(len(rp) - 1):
+        assert np.array_equal(np.sort(col[rp[i]:rp[i + 1]]), np.sort(g["col"][rp[i]:rp[i + 1]]))
+    op = ab.SparseOperator(ctx, cl, cl.positions(), r, ab.Kernel.gaussian(r / 2))
+    y = (op @ torch.tensor(g["b"], device="cuda")).cpu().numpy()
+    assert np.linalg.norm(y - g["y"]) / np.linalg.norm(g["y"]) < 1e-13

@@ -62,3 +62,22 @@ def test_batched_gmres_residual(ctx, precision):
         res = np.linalg.norm(r) / np.linalg.norm(b[a:e])
         assert st["converged"], st
         assert res <= 1.01e-8, (p, res, st)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_rectangular_dense_apply_matches_oracle(ctx, dim):
+    rng = np.random.default_rng(50 + dim)
+    a = rng.random((777, dim))
+    b = rng.random((1500, dim))
+    x = rng.normal(size=(20, 1500))  # 20 right-hand sides: two passes
+    kern = (KM, 0.05)
+    ref = oracle.dense_matvec(a, b, kern, x.T).T
+    y0 = rng.normal(size=(20, 777))
+    at, bt = torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda")
+    y = ab.dense_apply(ctx, at, bt, ab.Kernel.multiquadric(0.05), torch.tensor(x, device="cuda"),
+                       torch.tensor(y0, device="cuda")).cpu().numpy()
+    scale = np.abs(oracle.dense_matvec(a, b, kern, np.abs(x.T)).T)
+    assert np.max(np.abs(y - y0 - ref) / scale) < 1e-13
+    y32 = ab.dense_apply(ctx, at, bt, ab.Kernel.multiquadric(0.05), torch.tensor(x, device="cuda"),
+                         precision=ab.FP32_COMPENSATED).cpu().numpy()
+    assert np.max(np.abs(y32 - ref) / scale) < 2e-6
