@@ -42,3 +42,11 @@ def test_status_strings_without_gpu():
     lib = _capi.lib()
     assert lib.abh_status_string(0) == b"ok"
     assert lib.abh_status_string(4) == b"output buffer too small"
+
+
+def test_cpp_layer_compiles_and_links():
+    """include/aboria_hip.hpp compiles with plain g++ (no HIP headers) and
+    the example links against libaboria_hip.so."""
+    import __graft_entry__ as ge
+    out = ge.build_cpp_example()
+    assert os.path.exists(out)

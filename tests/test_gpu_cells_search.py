@@ -196,3 +196,14 @@ def test_golden_c1_mini(ctx):
     op = ab.SparseOperator(ctx, cl, cl.positions(), r, ab.Kernel.gaussian(r / 2))
     y = (op @ torch.tensor(g["b"], device="cuda")).cpu().numpy()
     assert np.linalg.norm(y - g["y"]) / np.linalg.norm(g["y"]) < 1e-13
+
+
+def test_cpp_layer_example_runs():
+    """tests/cpp/shim_example (built by build()): cell list, search and
+    sparse operator through include/aboria_hip.hpp vs a host brute force."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "shim_example")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
