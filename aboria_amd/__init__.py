@@ -518,6 +518,15 @@ class DenseBatchOperator:
         y = torch.zeros_like(x)
         return self.apply(x, y)
 
+    def tune(self, precision=FP64, force=0):
+        """Runs the tile-shape autotuner (or sets rows-per-thread `force`);
+        returns (rows_per_thread, {rpt: ms})."""
+        rpt = ctypes.c_int(int(force))
+        ms = (ctypes.c_float * 3)()
+        check(C.lib().abh_dense_batch_tune(self.handle, precision, ctypes.byref(rpt), ms),
+              self.ctx.handle)
+        return rpt.value, {1: ms[0], 2: ms[1], 4: ms[2]}
+
     def gmres(self, b, x=None, restart=50, max_iters=1000, tol=1e-8, precision=FP64):
         """Solves K x = b for every problem; returns (x, stats)."""
         if x is None:

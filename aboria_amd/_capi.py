@@ -157,6 +157,7 @@ SIGNATURES = {
                                       P(GmresStats)]),
     "abh_dense_apply": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int64, c_void_p, P(Kernel),
                                 c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+    "abh_dense_batch_tune": (c_int, [c_void_p, c_int, P(c_int), P(ctypes.c_float)]),
     "abh_schwarz_create": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double,
                                    P(c_void_p)]),
     "abh_schwarz_destroy": (c_int, [c_void_p]),

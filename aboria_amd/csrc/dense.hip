@@ -20,7 +20,12 @@ namespace {
 
 constexpr int kRows = 256;
 
-template <int D, bool FP32>
+// Tunable variants: RPT rows per thread (tile = 256 RPT rows of one
+// problem), so every source staged in LDS serves RPT rows.
+constexpr int kVariants = 3;
+constexpr int kRptOf[kVariants] = {1, 2, 4};
+
+template <int D, bool FP32, int RPT>
 __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles, int ntiles,
                                                  const int64_t *__restrict__ off,
                                                  const double *__restrict__ pos,
@@ -36,13 +41,23 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
   if (active && !active[p]) return;
   const int64_t base = off[p];
   const int64_t n = off[p + 1] - base;
-  const int64_t i = int64_t(tiles[t].y) + threadIdx.x;
-  const bool act = i < n;
-  double xi[D];
+  double xi[RPT][D];
+  bool act[RPT];
+  double acc[RPT];
+  float a32[RPT], c32[RPT];
 #pragma unroll
-  for (int d = 0; d < D; ++d) xi[d] = act ? pos[(base + i) * D + d] : 0.0;
-  double acc = 0.0;
-  float a32 = 0.0f, c32 = 0.0f;
+  for (int q = 0; q < RPT; ++q) {
+    const int64_t i = int64_t(tiles[t].y) + threadIdx.x + q * kRows;
+    act[q] = i < n;
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[q][d] = act[q] ? pos[(base + i) * D + d] : 0.0;
+    acc[q] = 0.0;
+    a32[q] = 0.0f;
+    c32[q] = 0.0f;
+  }
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) any |= act[q];
   for (int64_t c0 = 0; c0 < n; c0 += kRows) {
     const int m = int(n - c0 < kRows ? n - c0 : kRows);
     __syncthreads();
@@ -53,30 +68,39 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
       sx[threadIdx.x] = x[j];
     }
     __syncthreads();
-    if (act) {
-      for (int j = 0; j < m; ++j) {
+    if (!any) continue;
+    for (int j = 0; j < m; ++j) {
+      double pj[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) pj[d] = sp[j][d];
+      const double xj = sx[j];
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
         if (FP32) {
           float s = 0.0f;
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            const float dx = float(sp[j][d] - xi[d]);
+            const float dx = float(pj[d] - xi[q][d]);
             s = s + dx * dx;
           }
-          abh::kb_add(a32, c32, abh::phi32(k, s) * float(sx[j]));
+          abh::kb_add(a32[q], c32[q], abh::phi32(k, s) * float(xj));
         } else {
           double s = 0.0;
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            const double dx = sp[j][d] - xi[d];
+            const double dx = pj[d] - xi[q][d];
             s = s + dx * dx;
           }
-          acc += abh::phi64(k, s) * sx[j];
+          acc[q] += abh::phi64(k, s) * xj;
         }
       }
     }
   }
-  if (act) {
-    const double v = FP32 ? double(a32) + double(c32) : acc;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    if (!act[q]) continue;
+    const int64_t i = int64_t(tiles[t].y) + threadIdx.x + q * kRows;
+    const double v = FP32 ? double(a32[q]) + double(c32[q]) : acc[q];
     double *yp = &y[base + i];
     *yp = (overwrite ? 0.0 : *yp) + alpha * v;
   }
@@ -161,27 +185,74 @@ struct abh_dense_batch {
   abh::BatchLayout L;
   KernelDesc k{};
   const double *pos = nullptr;
-  DevBuf tiles, off, active;
-  int ntiles = 0;
+  DevBuf tiles[kVariants], off, active;
+  int ntiles[kVariants] = {0, 0, 0};
+  int choice[2] = {-1, -1};        // tuned variant per precision (FP64, FP32c)
+  float tune_ms[2][kVariants] = {};
 
-  void apply(const double *x, double *y, bool fp32, double alpha, bool overwrite,
-             const uint8_t *act) {
-    if (ntiles == 0) return;
+  template <int RPT>
+  void launch(int v, const double *x, double *y, bool fp32, double alpha, bool overwrite,
+              const uint8_t *act) {
     hipStream_t s = ctx->stream;
-    const int2 *tl = tiles.as<int2>();
+    const int2 *tl = tiles[v].as<int2>();
     const int64_t *o = off.as<int64_t>();
-#define ABH_DENSE_LAUNCH(DD)                                                                   \
-  if (fp32)                                                                                    \
-    k_dense<DD, true><<<ntiles, kRows, 0, s>>>(tl, ntiles, o, pos, x, y, act, k, alpha,       \
-                                               overwrite ? 1 : 0);                             \
-  else                                                                                         \
-    k_dense<DD, false><<<ntiles, kRows, 0, s>>>(tl, ntiles, o, pos, x, y, act, k, alpha,      \
-                                                overwrite ? 1 : 0);
+    const int nt = ntiles[v];
+    const int ow = overwrite ? 1 : 0;
+#define ABH_DENSE_LAUNCH(DD)                                                                    \
+  if (fp32)                                                                                     \
+    k_dense<DD, true, RPT><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);       \
+  else                                                                                          \
+    k_dense<DD, false, RPT><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);
     if (dim == 1) { ABH_DENSE_LAUNCH(1) }
     else if (dim == 2) { ABH_DENSE_LAUNCH(2) }
     else { ABH_DENSE_LAUNCH(3) }
 #undef ABH_DENSE_LAUNCH
     abh::check_launch();
+  }
+
+  void run(int v, const double *x, double *y, bool fp32, double alpha, bool overwrite,
+           const uint8_t *act) {
+    if (ntiles[v] == 0) return;
+    if (v == 0) launch<1>(v, x, y, fp32, alpha, overwrite, act);
+    else if (v == 1) launch<2>(v, x, y, fp32, alpha, overwrite, act);
+    else launch<4>(v, x, y, fp32, alpha, overwrite, act);
+  }
+
+  // Autotuner: times every variant once (after a warm-up) on the batch
+  // itself and keeps the fastest for this precision.
+  int tune(bool fp32) {
+    const int pi = fp32 ? 1 : 0;
+    if (choice[pi] >= 0) return choice[pi];
+    const int64_t n = L.total();
+    DevBuf xs, ys;
+    xs.ensure(size_t(n) * 8 + 64);
+    ys.ensure(size_t(n) * 8 + 64);
+    hipStream_t s = ctx->stream;
+    ABH_HIP(hipMemsetAsync(xs.p, 0, size_t(n) * 8, s));
+    hipEvent_t e0, e1;
+    ABH_HIP(hipEventCreate(&e0));
+    ABH_HIP(hipEventCreate(&e1));
+    int best = 0;
+    for (int v = 0; v < kVariants; ++v) {
+      run(v, xs.as<double>(), ys.as<double>(), fp32, 1.0, true, nullptr);
+      ABH_HIP(hipEventRecord(e0, s));
+      run(v, xs.as<double>(), ys.as<double>(), fp32, 1.0, true, nullptr);
+      ABH_HIP(hipEventRecord(e1, s));
+      ABH_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      ABH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      tune_ms[pi][v] = ms;
+      if (ms < tune_ms[pi][best]) best = v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    choice[pi] = best;
+    return best;
+  }
+
+  void apply(const double *x, double *y, bool fp32, double alpha, bool overwrite,
+             const uint8_t *act) {
+    run(tune(fp32), x, y, fp32, alpha, overwrite, act);
   }
 };
 
@@ -215,18 +286,21 @@ abh_status abh_dense_batch_create(abh_context *ctx, int dim, int nprob, const in
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
       return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
     });
-    std::vector<int2> tl;
-    for (int p : order)
-      for (int64_t r = 0; r < offsets[p + 1] - offsets[p]; r += kRows)
-        tl.push_back(make_int2(p, int(r)));
-    op->ntiles = int(tl.size());
-    op->tiles.ensure(tl.size() * sizeof(int2) + 16);
+    hipStream_t s = ctx->stream;
+    std::vector<int2> tl[kVariants];
+    for (int v = 0; v < kVariants; ++v) {
+      const int64_t rows = int64_t(kRows) * kRptOf[v];
+      for (int p : order)
+        for (int64_t r = 0; r < offsets[p + 1] - offsets[p]; r += rows)
+          tl[v].push_back(make_int2(p, int(r)));
+      op->ntiles[v] = int(tl[v].size());
+      op->tiles[v].ensure(tl[v].size() * sizeof(int2) + 16);
+      if (!tl[v].empty())
+        ABH_HIP(hipMemcpyAsync(op->tiles[v].p, tl[v].data(), tl[v].size() * sizeof(int2),
+                               hipMemcpyHostToDevice, s));
+    }
     op->off.ensure(size_t(nprob + 1) * 8);
     op->active.ensure(size_t(nprob) + 16);
-    hipStream_t s = ctx->stream;
-    if (!tl.empty())
-      ABH_HIP(hipMemcpyAsync(op->tiles.p, tl.data(), tl.size() * sizeof(int2),
-                             hipMemcpyHostToDevice, s));
     ABH_HIP(hipMemcpyAsync(op->off.p, offsets, size_t(nprob + 1) * 8, hipMemcpyHostToDevice, s));
     ABH_HIP(hipStreamSynchronize(s));
   });
@@ -331,6 +405,28 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const doubl
 #undef ABH_RECT
       abh::check_launch();
     }
+  });
+}
+
+abh_status abh_dense_batch_tune(abh_dense_batch *op, abh_precision precision, int *rows_per_thread,
+                                float *variant_ms) {
+  if (!op) return ABH_ERR_INVALID_ARG;
+  return abh::guard(op->ctx, [&] {
+    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
+                "dense precision must be ABH_FP64 or ABH_FP32_COMPENSATED");
+    const bool fp32 = precision == ABH_FP32_COMPENSATED;
+    if (rows_per_thread && *rows_per_thread > 0) {  // forced choice
+      int v = -1;
+      for (int q = 0; q < kVariants; ++q)
+        if (kRptOf[q] == *rows_per_thread) v = q;
+      ABH_REQUIRE(v >= 0, ABH_ERR_INVALID_ARG, "rows_per_thread must be 1, 2 or 4");
+      op->choice[fp32 ? 1 : 0] = v;
+      return;
+    }
+    const int v = op->tune(fp32);
+    if (rows_per_thread) *rows_per_thread = kRptOf[v];
+    if (variant_ms)
+      for (int q = 0; q < kVariants; ++q) variant_ms[q] = op->tune_ms[fp32 ? 1 : 0][q];
   });
 }
 

@@ -374,43 +374,80 @@ __global__ __launch_bounds__(256) void k_transfer(const int32_t *__restrict__ pa
   }
 }
 
-// M2L, FP64: one block per sibling tile (the 8 children of a level-(l-1)
-// node); each thread accumulates a 14 x 8 block of the tile's 224 x 128
-// output (216 node rows padded, 8 targets x 16 right-hand sides).
+// ---------------------------------------------------------------------------
+// M2L, FP64 on the matrix cores (v_mfma_f64_16x16x4f64): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register v of lane l
+// holds C[l / 16 + 4 v][l % 16] (measured with one-hot operands).  16 columns = the 16 right-hand sides of
+// one target, so a 16 x 16 output block is (16 node rows) x (one target).
+// ---------------------------------------------------------------------------
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+
+template <int N> struct Mfma64 {
+  static constexpr int NC = Cheb<N>::NC;
+  static constexpr int RB = (NC + 15) / 16;  // 16-row blocks
+  static constexpr int KS = (NC + 3) / 4;    // k steps of 4
+  static constexpr int64_t PER_OFFSET = int64_t(KS) * RB * 64;
+};
+
+// M (FP64 [lev][343][NC][NC]) -> Mf [lev][343][KS][RB][64]: the A fragment
+// of (k step, row block) for every lane, zero padded
 template <int N>
-__global__ __launch_bounds__(256) void k_m2l_f64(const int32_t *__restrict__ tile_parent,
-                                                 int ntiles, const int32_t *__restrict__ child0,
-                                                 const int32_t *__restrict__ coord,  // 3 per node
-                                                 const int32_t *__restrict__ map, int side,
-                                                 const double *__restrict__ M,  // [343][NC][NC]
-                                                 const double *__restrict__ W,
-                                                 double *__restrict__ G) {
-  constexpr int NC = Cheb<N>::NC;
-  constexpr int RP = (NC + 15) / 16;  // rows per thread
-  constexpr int KC = 8;               // k chunk
-  __shared__ double As[16 * RP][KC + 1];
-  __shared__ double Bs[KC][8 * kRhs];
+__global__ __launch_bounds__(256) void k_pack_m64(const double *__restrict__ M, int nlev,
+                                                  double *__restrict__ Mf) {
+  using F = Mfma64<N>;
+  constexpr int NC = F::NC;
+  const int64_t total = int64_t(nlev) * kOffsets * F::PER_OFFSET;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    const int lane = int(t % 64);
+    const int rb = int((t / 64) % F::RB);
+    const int ks = int((t / (64 * F::RB)) % F::KS);
+    const int64_t lo = t / F::PER_OFFSET;  // level * 343 + offset
+    const int i = rb * 16 + lane % 16, k = ks * 4 + lane / 16;
+    Mf[t] = (i < NC && k < NC) ? M[(lo * NC + i) * NC + k] : 0.0;
+  }
+}
+
+// One 512-thread block per sibling tile (the 8 children of a level-(l-1)
+// node).  Wave w owns row blocks {w % 4, w % 4 + 4, ...} of the 8 x 16
+// output and the four children 4 (w / 4) .. 4 (w / 4) + 3; per box offset
+// each active child's source expansion is the B operand and the offset's
+// packed M2L matrix the A operand, both read straight from L2 in the
+// fragment layout (coalesced 512-byte rows).
+template <int N>
+__global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ tile_parent,
+                                                    int ntiles, const int32_t *__restrict__ child0,
+                                                    const int32_t *__restrict__ coord,
+                                                    const int32_t *__restrict__ map, int side,
+                                                    const double *__restrict__ Mf,
+                                                    const double *__restrict__ W,
+                                                    double *__restrict__ G) {
+  using F = Mfma64<N>;
+  constexpr int NC = F::NC, RB = F::RB, KS = F::KS;
+  constexpr int RPW = (RB + 3) / 4;  // row blocks per wave
   __shared__ int src[8];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
   const int parent = tile_parent[tile];
   const int first = child0[parent];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rq = wave & 3, c0 = 4 * (wave >> 2);
   const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
-  double acc[RP][8];
+  f64x4 acc[RPW][4];
 #pragma unroll
-  for (int a = 0; a < RP; ++a)
+  for (int i = 0; i < RPW; ++i)
 #pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+    for (int c = 0; c < 4; ++c) acc[i][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+  const int kl = lane >> 4, rl = lane & 15;
   for (int o = 0; o < kOffsets; ++o) {
     const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
     if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
     __syncthreads();
     if (threadIdx.x < 8) {
       const int ch = threadIdx.x;
-      const int cx = (ch >> 2) & 1, cy = (ch >> 1) & 1, cz = ch & 1;
-      // parent of the source must touch the target's parent
-      const int qx = cx + ox, qy = cy + oy, qz = cz + oz;  // in [-3, 4]
+      const int qx = ((ch >> 2) & 1) + ox, qy = ((ch >> 1) & 1) + oy, qz = (ch & 1) + oz;
       int s = -1;
       if (qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
         const int sx = 2 * pc[0] + qx, sy = 2 * pc[1] + qy, sz = 2 * pc[2] + qz;
@@ -420,47 +457,48 @@ __global__ __launch_bounds__(256) void k_m2l_f64(const int32_t *__restrict__ til
       src[ch] = s;
     }
     __syncthreads();
+    int sc[4];
     bool any = false;
-    for (int ch = 0; ch < 8; ++ch) any |= src[ch] >= 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      sc[c] = src[c0 + c];
+      any |= sc[c] >= 0;
+    }
     if (!any) continue;
-    const double *Mo = M + int64_t(o) * NC * NC;
-    for (int k0 = 0; k0 < NC; k0 += KC) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < 16 * RP * KC; t += 256) {
-        const int i = t / KC, k = t % KC;
-        As[i][k] = (i < NC && k0 + k < NC) ? Mo[int64_t(i) * NC + k0 + k] : 0.0;
+    const double *Mo = Mf + int64_t(o) * F::PER_OFFSET;
+    for (int ks = 0; ks < KS; ++ks) {
+      double a[RPW], b[4];
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const int rb = rq + 4 * i;
+        a[i] = rb < RB ? Mo[(int64_t(ks) * RB + rb) * 64 + lane] : 0.0;
       }
-      for (int t = threadIdx.x; t < KC * 8 * kRhs; t += 256) {
-        const int k = t / (8 * kRhs), col = t % (8 * kRhs);
-        const int ch = col / kRhs, r = col % kRhs;
-        const int s = src[ch];
-        Bs[k][col] = (s >= 0 && k0 + k < NC) ? W[int64_t(s) * NC * kRhs + (k0 + k) * kRhs + r] : 0.0;
-      }
-      __syncthreads();
+      const int k = ks * 4 + kl;
 #pragma unroll
-      for (int k = 0; k < KC; ++k) {
-        double av[RP], bv[8];
+      for (int c = 0; c < 4; ++c)
+        b[c] = (sc[c] >= 0 && k < NC) ? W[(int64_t(sc[c]) * NC + k) * kRhs + rl] : 0.0;
 #pragma unroll
-        for (int a = 0; a < RP; ++a) av[a] = As[ty + 16 * a][k];
+      for (int c = 0; c < 4; ++c) {
+        if (sc[c] < 0) continue;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) bv[b] = Bs[k][tx + 16 * b];
-#pragma unroll
-        for (int a = 0; a < RP; ++a)
-#pragma unroll
-          for (int b = 0; b < 8; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        for (int i = 0; i < RPW; ++i)
+          if (rq + 4 * i < RB)
+            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[c], acc[i][c], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int a = 0; a < RP; ++a) {
-    const int i = ty + 16 * a;
-    if (i >= NC) continue;
+  for (int i = 0; i < RPW; ++i) {
+    const int rb = rq + 4 * i;
+    if (rb >= RB) continue;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const int col = tx + 16 * b;
-      const int ch = col / kRhs, r = col % kRhs;
-      const int t = first + ch;
-      G[int64_t(t) * NC * kRhs + i * kRhs + r] += acc[a][b];
+    for (int c = 0; c < 4; ++c) {
+      const int t = first + c0 + c;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rb * 16 + kl + 4 * v;
+        if (row < NC) G[(int64_t(t) * NC + row) * kRhs + rl] += acc[i][c][v];
+      }
     }
   }
 }
@@ -488,18 +526,26 @@ __device__ __forceinline__ void split_bf16(double v, uint16_t &hi, uint16_t &lo)
   lo = bf16_rne(f - bf16_to_f(hi));
 }
 
-// M (FP64 [lev][343][NC][NC]) -> Mh, Ml (BF16 [lev][343][kPad][kPad], zero padded)
+constexpr int kKs16 = kPad / 16;  // k steps of 16
+constexpr int kRt = kPad / 32;    // 32-row tiles
+constexpr int64_t kBfPerOffset = int64_t(kKs16) * kRt * 64 * 8;  // bf16 per offset
+
+// M (FP64 [lev][343][NC][NC]) -> Mh, Ml (BF16 [lev][343][ks][rt][lane][8]):
+// the A fragment of every (k step, row tile) for every lane, zero padded
 template <int N>
 __global__ __launch_bounds__(256) void k_split_m(const double *__restrict__ M, int nlev,
                                                  uint16_t *__restrict__ Mh,
                                                  uint16_t *__restrict__ Ml) {
   constexpr int NC = Cheb<N>::NC;
-  const int64_t total = int64_t(nlev) * kOffsets * kPad * kPad;
+  const int64_t total = int64_t(nlev) * kOffsets * kBfPerOffset;
   for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * 256) {
-    const int k = int(t % kPad);
-    const int i = int((t / kPad) % kPad);
-    const int64_t lo = t / (int64_t(kPad) * kPad);  // level * 343 + offset
+    const int j = int(t % 8);
+    const int lane = int((t / 8) % 64);
+    const int rt = int((t / 512) % kRt);
+    const int ks = int((t / (512 * kRt)) % kKs16);
+    const int64_t lo = t / kBfPerOffset;  // level * 343 + offset
+    const int i = rt * 32 + lane % 32, k = ks * 16 + 8 * (lane / 32) + j;
     uint16_t h = 0, l = 0;
     if (i < NC && k < NC) split_bf16(M[(lo * NC + i) * NC + k], h, l);
     Mh[t] = h;
@@ -507,7 +553,8 @@ __global__ __launch_bounds__(256) void k_split_m(const double *__restrict__ M, i
   }
 }
 
-// W (FP64 [node][NC][16]) -> Wh, Wl (BF16 [node][kPad][16], zero padded)
+// W (FP64 [node][NC][16]) -> Wh, Wl (BF16 [node][16][kPad], zero padded):
+// per right-hand side the expansion is contiguous in k
 template <int N>
 __global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, int64_t nn,
                                                  uint16_t *__restrict__ Wh,
@@ -516,8 +563,8 @@ __global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, i
   const int64_t total = nn * kPad * kRhs;
   for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * 256) {
-    const int r = int(t % kRhs);
-    const int k = int((t / kRhs) % kPad);
+    const int k = int(t % kPad);
+    const int r = int((t / kPad) % kRhs);
     const int64_t node = t / (int64_t(kPad) * kRhs);
     uint16_t h = 0, l = 0;
     if (k < NC) split_bf16(W[(node * NC + k) * kRhs + r], h, l);
@@ -526,39 +573,40 @@ __global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, i
   }
 }
 
-// One block per sibling tile (8 targets x 16 right-hand sides = 128
-// columns); wave w owns columns 32w..32w+31 (targets 2w, 2w+1) and all
-// seven 32-row tiles of the 224-row output.  For each box offset the M2L
-// matrix streams through LDS in K chunks of 32 together with the sources'
-// expansions (stored column-major in LDS so every lane reads 16 contiguous
-// bytes per operand).
+// One 512-thread block per sibling tile (8 targets x 16 right-hand sides =
+// 128 columns = 4 pairs of targets).  Wave w owns row tiles {w % 4,
+// w % 4 + 4} of the 224-row output and target pairs 2 (w / 4), 2 (w / 4) + 1.
+// Per box offset both operands are read straight from L2 in the fragment
+// layout (16 contiguous bytes per lane); pairs with no source at the offset
+// are skipped.
 template <int N>
-__global__ __launch_bounds__(256) void k_m2l_mfma(const int32_t *__restrict__ tile_parent,
+__global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ tile_parent,
                                                   int ntiles, const int32_t *__restrict__ child0,
                                                   const int32_t *__restrict__ coord,
                                                   const int32_t *__restrict__ map, int side,
                                                   const uint16_t *__restrict__ Mh,
-                                                  const uint16_t *__restrict__ Ml,  // [343][224][224]
+                                                  const uint16_t *__restrict__ Ml,
                                                   const uint16_t *__restrict__ Wh,
-                                                  const uint16_t *__restrict__ Wl,  // [node][224][16]
+                                                  const uint16_t *__restrict__ Wl,
                                                   double *__restrict__ G) {
   constexpr int NC = Cheb<N>::NC;
-  constexpr int KC = 32;
-  constexpr int LDA = KC + 8;  // padded LDS row (bf16 elements)
-  __shared__ __align__(16) uint16_t Ah[kPad * LDA], Al[kPad * LDA];
-  __shared__ __align__(16) uint16_t Bh[8 * kRhs * LDA], Bl[8 * kRhs * LDA];
   __shared__ int src[8];
   const int tile = blockIdx.x;
   if (tile >= ntiles) return;
   const int parent = tile_parent[tile];
   const int first = child0[parent];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rq = wave & 3, p0 = 2 * (wave >> 2);
+  const int half = lane >> 5, col = lane & 31;
+  const int cq = col >> 4, r = col & 15;  // target within the pair, rhs
   const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
-  f32x16 acc[7];
+  f32x16 acc[2][2];
 #pragma unroll
-  for (int rt = 0; rt < 7; ++rt)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[rt][i] = 0.0f;
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][q][v] = 0.0f;
   for (int o = 0; o < kOffsets; ++o) {
     const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
     if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
@@ -575,66 +623,68 @@ __global__ __launch_bounds__(256) void k_m2l_mfma(const int32_t *__restrict__ ti
       src[ch] = s;
     }
     __syncthreads();
-    int any = 0;
+    bool act[2];
+    int sl[2];
 #pragma unroll
-    for (int ch = 0; ch < 8; ++ch) any |= src[ch] >= 0 ? 1 : 0;
-    if (!any) continue;
-    const uint16_t *mh = Mh + int64_t(o) * kPad * kPad;
-    const uint16_t *ml = Ml + int64_t(o) * kPad * kPad;
-    for (int k0 = 0; k0 < kPad; k0 += KC) {
-      __syncthreads();
-      // A chunk: 224 rows x 32 k, 8 bf16 (16 B) per load
-      for (int t = threadIdx.x; t < kPad * (KC / 8); t += 256) {
-        const int i = t / (KC / 8), q = t % (KC / 8);
-        *reinterpret_cast<uint4 *>(&Ah[i * LDA + q * 8]) =
-            *reinterpret_cast<const uint4 *>(&mh[int64_t(i) * kPad + k0 + q * 8]);
-        *reinterpret_cast<uint4 *>(&Al[i * LDA + q * 8]) =
-            *reinterpret_cast<const uint4 *>(&ml[int64_t(i) * kPad + k0 + q * 8]);
-      }
-      // B chunk, transposed to [column = target*16 + r][k]
-      for (int t = threadIdx.x; t < 8 * KC * kRhs; t += 256) {
-        const int ch = t / (KC * kRhs), k = (t / kRhs) % KC, r = t % kRhs;
-        const int s = src[ch];
-        uint16_t h = 0, l = 0;
-        if (s >= 0) {
-          const int64_t e = (int64_t(s) * kPad + k0 + k) * kRhs + r;
-          h = Wh[e];
-          l = Wl[e];
+    for (int q = 0; q < 2; ++q) {
+      const int p = p0 + q;
+      act[q] = src[2 * p] >= 0 || src[2 * p + 1] >= 0;
+      sl[q] = src[2 * p + cq];
+    }
+    if (!act[0] && !act[1]) continue;
+    const bf16x8 *mh = reinterpret_cast<const bf16x8 *>(Mh) + int64_t(o) * (kBfPerOffset / 8);
+    const bf16x8 *ml = reinterpret_cast<const bf16x8 *>(Ml) + int64_t(o) * (kBfPerOffset / 8);
+    for (int ks = 0; ks < kKs16; ++ks) {
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int rt = rq + 4 * i;
+        if (rt < kRt) {
+          ah[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
+          al[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
         }
-        Bh[(ch * kRhs + r) * LDA + k] = h;
-        Bl[(ch * kRhs + r) * LDA + k] = l;
       }
-      __syncthreads();
 #pragma unroll
-      for (int ks = 0; ks < KC / 16; ++ks) {
-        const int kk = ks * 16 + 8 * (lane >> 5);
-        const int col = wave * 32 + (lane & 31);
-        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bh[col * LDA + kk]);
-        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Bl[col * LDA + kk]);
+      for (int q = 0; q < 2; ++q) {
+        if (!act[q]) continue;
+        if (sl[q] >= 0) {
+          const int64_t e = (int64_t(sl[q]) * kRhs + r) * kPad + ks * 16 + 8 * half;
+          bh[q] = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
+          bl[q] = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
+        } else {
 #pragma unroll
-        for (int rt = 0; rt < 7; ++rt) {
-          const int row = rt * 32 + (lane & 31);
-          const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&Ah[row * LDA + kk]);
-          const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&Al[row * LDA + kk]);
-          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[rt], 0, 0, 0);
-          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[rt], 0, 0, 0);
-          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[rt], 0, 0, 0);
+          for (int j = 0; j < 8; ++j) bh[q][j] = bl[q][j] = 0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (!act[q]) continue;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          if (rq + 4 * i >= kRt) continue;
+          acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[q], acc[i][q], 0, 0, 0);
+          acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[q], acc[i][q], 0, 0, 0);
+          acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[q], acc[i][q], 0, 0, 0);
         }
       }
     }
   }
-  // D layout: register i of lane l holds row 8*(i/4) + 4*(l/32) + (i%4),
-  // column l%32 of the 32 x 32 tile
-  const int col = wave * 32 + (lane & 31);
-  const int ch = col / kRhs, r = col % kRhs;
-  const int t = first + ch;
+  // D layout: register v of lane l holds row 8 (v / 4) + 4 (l / 32) + v % 4,
+  // column l % 32 of the 32 x 32 tile
 #pragma unroll
-  for (int rt = 0; rt < 7; ++rt)
+  for (int i = 0; i < 2; ++i) {
+    const int rt = rq + 4 * i;
+    if (rt >= kRt) continue;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int row = rt * 32 + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
-      if (row < NC) G[(int64_t(t) * NC + row) * kRhs + r] += double(acc[rt][i]);
+    for (int q = 0; q < 2; ++q) {
+      const int t = first + 2 * (p0 + q) + cq;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = rt * 32 + 8 * (v / 4) + 4 * half + (v % 4);
+        if (row < NC) G[(int64_t(t) * NC + row) * kRhs + r] += double(acc[i][q][v]);
+      }
     }
+  }
 }
 
 // L2P + P2P for one target leaf, 256 threads: thread t serves target particle
@@ -790,6 +840,8 @@ struct abh_fmm {
   int64_t n_nodes = 0;
   DevBuf Mh, Ml, Wh, Wl;  // BF16 hi / lo parts for the MFMA M2L
   bool split_ready = false;
+  DevBuf Mf;              // FP64 M2L operators in the MFMA fragment layout
+  bool pack_ready = false;
 };
 
 namespace {
@@ -1090,8 +1142,8 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     if (mixed && NC <= kPad && f->nlevels >= 2) {
       if (!f->split_ready) {
         const int nlev = f->nlevels - 1;
-        f->Mh.ensure(size_t(nlev) * kOffsets * kPad * kPad * 2);
-        f->Ml.ensure(size_t(nlev) * kOffsets * kPad * kPad * 2);
+        f->Mh.ensure(size_t(nlev) * kOffsets * kBfPerOffset * 2);
+        f->Ml.ensure(size_t(nlev) * kOffsets * kBfPerOffset * 2);
         k_split_m<N><<<8192, 256, 0, s>>>(f->M.as<double>(), nlev, f->Mh.as<uint16_t>(),
                                           f->Ml.as<uint16_t>());
         abh::check_launch();
@@ -1111,17 +1163,25 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
       if (mixed && NC <= kPad) {
-        k_m2l_mfma<N><<<npar(l - 1), 256, 0, s>>>(
+        k_m2l_mfma<N><<<npar(l - 1), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
-            f->Mh.as<uint16_t>() + int64_t(l - 2) * kOffsets * kPad * kPad,
-            f->Ml.as<uint16_t>() + int64_t(l - 2) * kOffsets * kPad * kPad, f->Wh.as<uint16_t>(),
+            f->Mh.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset,
+            f->Ml.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset, f->Wh.as<uint16_t>(),
             f->Wl.as<uint16_t>(), f->G.as<double>());
       } else {
-        k_m2l_f64<N><<<npar(l - 1), 256, 0, s>>>(
+        using F = Mfma64<N>;
+        if (!f->pack_ready) {
+          const int nlev = f->nlevels - 1;
+          f->Mf.ensure(size_t(nlev) * kOffsets * F::PER_OFFSET * 8);
+          k_pack_m64<N><<<8192, 256, 0, s>>>(f->M.as<double>(), nlev, f->Mf.as<double>());
+          abh::check_launch();
+          f->pack_ready = true;
+        }
+        k_m2l_mfma64<N><<<npar(l - 1), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
-            f->M.as<double>() + int64_t(l - 2) * kOffsets * NC * NC, f->W.as<double>(),
+            f->Mf.as<double>() + int64_t(l - 2) * kOffsets * F::PER_OFFSET, f->W.as<double>(),
             f->G.as<double>());
       }
       abh::check_launch();

@@ -38,7 +38,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SECTIONS = ("search", "sparse", "fmm", "rbf", "schwarz")
+SECTIONS = ("search", "sparse", "fmm", "rbf", "schwarz", "copy")
+DEFAULT_SECTIONS = ("search", "sparse", "fmm", "rbf", "schwarz")
 
 
 def parse():
@@ -48,8 +49,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--n", type=int, default=4_000_000)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--sections", default=",".join(SECTIONS),
-                   help="comma-separated subset of %s" % (SECTIONS,))
+    p.add_argument("--sections", default=",".join(DEFAULT_SECTIONS),
+                   help="comma-separated subset of %s ('copy' is the calibration copy "
+                        "of the HBM traffic counters, not run by default)" % (SECTIONS,))
     p.add_argument("--rbf-iters", type=int, default=100,
                    help="GMRES iteration budget of the batched RBF systems")
     return p.parse_args()
@@ -265,6 +267,19 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
     extra["schwarz"] = out
 
 
+def bench_copy(torch, ctx, timer, extra):
+    """A plain device copy of known size (2 GiB read + 2 GiB written), the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    n = (2 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device=ctx.device)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    ms = timer(lambda: y.copy_(x), reps=3)
+    extra["copy"] = {"bytes_read": n * 8, "bytes_written": n * 8, "ms": ms,
+                     "gbs": 2 * n * 8 / (ms * 1e-3) / 1e9}
+    log("copy 2 GiB: %.3f ms, %.0f GB/s (read + write)" % (ms, extra["copy"]["gbs"]))
+
+
 def cpu_baseline(st, value_unit="pairs/s"):
     import oracle
     from aboria_amd import workloads as wl
@@ -336,6 +351,8 @@ def main():
         torch.cuda.empty_cache()
     if "schwarz" in sections:
         bench_schwarz(torch, ab, wl, ctx, extra)
+    if "copy" in sections:
+        bench_copy(torch, ctx, timer, extra)
 
     out = {
         "metric": "pairs/s (cell-list query)",

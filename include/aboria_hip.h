@@ -276,6 +276,16 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows,
                            int nrhs, const double *x, int64_t ldx, double *y,
                            int64_t ldy, abh_precision precision);
 
+/* Autotuner of the batched dense operator (no counterpart in the
+ * reference): times every P2P tile shape (256 x {1, 2, 4} rows per
+ * workgroup, each source staged in LDS serving 1, 2 or 4 rows per thread) on
+ * this batch once and keeps the fastest for the precision; apply and gmres
+ * tune on first use.  rows_per_thread (host, may be NULL) gets the choice,
+ * variant_ms (host, 3 floats, may be NULL) the measured times.  A positive
+ * *rows_per_thread on entry (1, 2 or 4) sets that shape instead of tuning. */
+abh_status abh_dense_batch_tune(abh_dense_batch *op, abh_precision precision,
+                                int *rows_per_thread, float *variant_ms);
+
 typedef struct abh_gmres_options {
   int32_t restart;       /* Krylov dimension m of GMRES(m)                   */
   int32_t max_iters;     /* Arnoldi steps per problem (all restarts)         */

@@ -81,3 +81,34 @@ def test_rectangular_dense_apply_matches_oracle(ctx, dim):
     y32 = ab.dense_apply(ctx, at, bt, ab.Kernel.multiquadric(0.05), torch.tensor(x, device="cuda"),
                          precision=ab.FP32_COMPENSATED).cpu().numpy()
     assert np.max(np.abs(y32 - ref) / scale) < 2e-6
+
+
+def test_dense_tile_variants_agree_and_tuner_picks_one(ctx):
+    """Every P2P tile shape gives the same FP64 rows (same summation order
+    per row) and the FP32c bar; the autotuner returns one of them."""
+    rng = np.random.default_rng(77)
+    sizes = [700, 1500, 2600, 90]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    pos = rng.random((off[-1], 2))
+    x = rng.normal(size=off[-1])
+    op = ab.DenseBatchOperator(ctx, off, torch.tensor(pos, device="cuda"),
+                               ab.Kernel.multiquadric(0.05))
+    xt = torch.tensor(x, device="cuda")
+    outs = {}
+    for rpt in (1, 2, 4):
+        op.tune(ab.FP64, force=rpt)
+        outs[rpt] = (op @ xt).cpu().numpy()
+    assert np.array_equal(outs[1], outs[2]) and np.array_equal(outs[1], outs[4])
+    ref = np.concatenate([oracle.dense_matvec(pos[a:b], pos[a:b], (KM, 0.05), x[a:b])
+                          for a, b in zip(off[:-1], off[1:])])
+    assert np.linalg.norm(outs[1] - ref) / np.linalg.norm(ref) < 1e-13
+    for rpt in (1, 2, 4):
+        op.tune(ab.FP32_COMPENSATED, force=rpt)
+        y32 = op.apply(xt, torch.zeros_like(xt), ab.FP32_COMPENSATED).cpu().numpy()
+        assert np.linalg.norm(y32 - ref) / np.linalg.norm(ref) < 2e-6
+    # a fresh operator tunes itself
+    op2 = ab.DenseBatchOperator(ctx, off, torch.tensor(pos, device="cuda"),
+                                ab.Kernel.multiquadric(0.05))
+    rpt, ms = op2.tune(ab.FP64)
+    print("tuned rows per thread", rpt, ms, flush=True)
+    assert rpt in (1, 2, 4) and all(v > 0 for v in ms.values())

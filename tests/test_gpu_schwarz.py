@@ -177,16 +177,19 @@ def test_mlp_training_reduces_loss(ctx):
 def test_learned_preconditioner_gain(ctx):
     """Trained on least-squares rows of seeded point sets, applied to a new
     one: the learned preconditioner cuts GMRES iterations well below the
-    unpreconditioned count (the plain Schwarz count is printed beside)."""
+    unpreconditioned count (the plain Schwarz count is printed beside).  The
+    grid jitter is 0.05 h: the distance order of the 32 neighbours, which
+    fixes which input slot each neighbour occupies, is then stable up to
+    swaps inside a shell of (nearly) equal weights."""
     n1 = 64
     h = 1.0 / n1
     sigma, radius, shift, k = 1.26 * h, 6 * 1.26 * h, 1e-5, 32
     kern = ab.Kernel.gaussian(sigma)
-    sets = [wl.jittered_grid(n1, n1, seed=100 + s) for s in range(4)]
+    sets = [wl.jittered_grid(n1, n1, jitter=0.05, seed=100 + s) for s in range(6)]
     X, T = learned.training_set(ctx, sets, kern, radius, shift, 16.0, k, feature_scale=sigma)
-    net, scale, losses = learned.train(ctx, X, T, epochs=25, batch=512)
+    net, scale, losses = learned.train(ctx, X, T, epochs=40, batch=512)
     print("training losses", losses[0], losses[-1], flush=True)
-    pos = wl.jittered_grid(n1, n1, seed=999)
+    pos = wl.jittered_grid(n1, n1, jitter=0.05, seed=999)
     cl, oc = build(ctx, pos, 16.0)
     op = ab.SparseOperator(ctx, cl, cl.positions(), radius, kern)
     b = torch.tensor(np.random.default_rng(3).normal(size=cl.n_alive), device="cuda")
