@@ -20,9 +20,11 @@ workload):
   rbf_batch   512 uneven 2D multiquadric (c=0.05) RBF systems, batched
               GMRES(50) on a bounded iteration budget, FP64 and FP32
               compensated operator (FP64 residual / orthogonalisation)
-  schwarz     2D jittered grid N=400,056, Gaussian sigma=0.01 (+1e-5 I),
-              GMRES to 1e-8: plain Schwarz vs the learned approximate inverse
-              (with the Schwarz fallback), setup / solve times and iterations
+  schwarz     2D jittered grid N=400,056, Gaussian sigma=0.01 (+1e-5 I) and a
+              narrow variant sigma=0.002, GMRES to 1e-8 on a 300-iteration
+              budget: plain Schwarz vs the learned approximate inverse (with
+              the Schwarz fallback), the exact least-squares approximate
+              inverse and none; setup / solve times and iterations
 and a CPU baseline of the headline -- the oracle (C++/OpenMP restatement of
 the reference's CPU search) timed on the host on a bounded sample of centres.
 """
@@ -130,9 +132,16 @@ def bench_search(args, torch, ab, wl, ctx, stream, extra):
     # algorithmic bytes of the fill launch: positions (24 B) + bucket
     # begin/end (8 B per bucket) + row offsets (8 B) read, 4 B per pair written
     fill_bytes = 24 * n + 8 * nb + 8 * n + 4 * total
+    # measured HBM bytes per fill launch (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+    # scaled by a plain copy of known size), recorded beside the profiles
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_search_fill_traffic.json")
+    if os.path.exists(tpath) and n == 4_000_000:
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {"kernel": "k_search<3,fill>", "bound": "hbm",
                 "achieved": fill_bytes / (fill_med * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "traffic": None,
+                "unit": "GB/s", "traffic": traffic,
                 "bytes_per_launch": int(fill_bytes), "launch_ms": fill_med}
     roofline["frac"] = roofline["achieved"] / roofline["peak"]
     extra["search"] = {"build_ms": float(np.median(build_ms)),
@@ -222,19 +231,17 @@ def bench_rbf(args, torch, ab, wl, ctx, timer, extra):
 
 
 def bench_schwarz(torch, ab, wl, ctx, extra):
+    """Schwarz-preconditioned GMRES on the jittered grid (N=400,056): the
+    configured kernel width sigma=0.01 and a narrow variant sigma=0.002 (same
+    points), each with the plain Schwarz preconditioner, the learned
+    approximate inverse (Schwarz fallback), the exact least-squares
+    approximate inverse and no preconditioner, on bounded iteration budgets."""
     from aboria_amd import learned
     c = wl.SCHWARZ
-    sigma, shift, k = c["sigma"], c["shift"], c["k"]
-    radius = c["cutoff"] * sigma
+    shift, k = c["shift"], c["k"]
     pos = wl.jittered_grid(c["nx"], c["ny"], c["jitter"], c["seed"])
+    train_pos = wl.jittered_grid(c["nx"], c["ny"], c["jitter"], c["seed"] + 100)
     sync = torch.cuda.synchronize
-    cl = ab.CellListOrdered(ctx, [0, 0], [1, 1], [False, False], c["n_leaf"])
-    cl.update_positions(torch.tensor(pos, device=ctx.device))
-    kern = ab.Kernel.gaussian(sigma)
-    op = ab.SparseOperator(ctx, cl, cl.positions(), radius, kern)
-    b = torch.tensor(np.random.default_rng(c["seed"] + 1).normal(size=cl.n_alive),
-                     device=ctx.device)
-    out = {"n": int(cl.n_alive), "grid": list(cl.grid)}
 
     def timed(fn):
         sync()
@@ -243,27 +250,49 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
         sync()
         return r, time.perf_counter() - t0
 
-    sw, t_sw = timed(lambda: ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift))
-    (x, st), t_solve = timed(lambda: op.gmres(b, diag_shift=shift, preconditioner=sw, restart=50,
-                                              max_iters=2000))
-    out["schwarz"] = {"setup_s": t_sw, "solve_s": t_solve, "info": sw.info, **st}
-    log("schwarz: setup %.2f s, gmres %d its %.2f s, converged %s" % (
-        t_sw, st["iterations"], t_solve, st["converged"]))
-    # learned: 200k least-squares rows of a seeded point set -> network
-    (X, T), t_data = timed(lambda: learned.training_set(
-        ctx, [wl.jittered_grid(c["nx"], c["ny"], c["jitter"], c["seed"] + 100)], kern, radius,
-        shift, 16.0, k, feature_scale=sigma, max_rows=200_000, seed=c["seed"]))
-    (net, oscale, losses), t_train = timed(lambda: learned.train(ctx, X, T, epochs=30,
-                                                                 batch=1024))
-    ai, t_ai = timed(lambda: ab.ApproxInverse.learned(ctx, cl, net, k, sigma, oscale, bf16=True))
-    (x2, rep), t_l = timed(lambda: ab.gmres_with_fallback(op, b, shift, ai, sw, restart=50,
-                                                          max_iters=2000, learned_iters=100))
-    out["learned"] = {"training_rows": int(X.shape[0]), "training_set_s": t_data,
-                      "train_s": t_train, "loss_first": float(losses[0]),
-                      "loss_last": float(losses[-1]), "setup_s": t_ai, "solve_s": t_l,
-                      "learned": rep["learned"], "fallback": rep["fallback"]}
-    log("learned: data %.1f s, train %.1f s, setup %.3f s, solve %.2f s, %s" % (
-        t_data, t_train, t_ai, t_l, rep))
+    out = {}
+    for name, sigma, leaf in (("sigma_0.01", c["sigma"], c["n_leaf"]), ("sigma_0.002", 0.002, 64.0)):
+        radius = c["cutoff"] * sigma
+        kern = ab.Kernel.gaussian(sigma)
+        cl = ab.CellListOrdered(ctx, [0, 0], [1, 1], [False, False], leaf)
+        cl.update_positions(torch.tensor(pos, device=ctx.device))
+        op = ab.SparseOperator(ctx, cl, cl.positions(), radius, kern)
+        b = torch.tensor(np.random.default_rng(c["seed"] + 1).normal(size=cl.n_alive),
+                         device=ctx.device)
+        r = {"n": int(cl.n_alive), "sigma": sigma, "n_leaf": leaf, "grid": list(cl.grid)}
+        sw, t_sw = timed(lambda: ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift))
+        (_, st), t = timed(lambda: op.gmres(b, diag_shift=shift, preconditioner=sw, restart=50,
+                                            max_iters=300))
+        r["schwarz"] = {"setup_s": t_sw, "solve_s": t, "info": sw.info, **st}
+        (X, T), t_data = timed(lambda: learned.training_set(
+            ctx, [train_pos], kern, radius, shift, 16.0, k, feature_scale=sigma,
+            max_rows=200_000, seed=c["seed"]))
+        (net, oscale, losses), t_train = timed(lambda: learned.train(ctx, X, T, epochs=30,
+                                                                     batch=1024))
+        ai, t_ai = timed(lambda: ab.ApproxInverse.learned(ctx, cl, net, k, sigma, oscale,
+                                                          bf16=True))
+        (_, rep), t = timed(lambda: ab.gmres_with_fallback(op, b, shift, ai, sw, restart=50,
+                                                           max_iters=300, learned_iters=100))
+        r["learned"] = {"training_rows": int(X.shape[0]), "training_set_s": t_data,
+                        "train_s": t_train, "loss_first": float(losses[0]),
+                        "loss_last": float(losses[-1]), "setup_s": t_ai, "solve_s": t,
+                        "learned": rep["learned"], "fallback": rep["fallback"]}
+        ls, t_ls = timed(lambda: ab.ApproxInverse.lsq(ctx, cl, kern, radius, shift, k))
+        (_, st), t = timed(lambda: op.gmres(b, diag_shift=shift, preconditioner=ls, restart=50,
+                                            max_iters=300))
+        r["lsq"] = {"setup_s": t_ls, "solve_s": t, **st}
+        (_, st), t = timed(lambda: op.gmres(b, diag_shift=shift, restart=50, max_iters=300))
+        r["none"] = {"solve_s": t, **st}
+        log("schwarz %s: plain %d its (%s) setup %.2f s solve %.2f s | learned %s fallback %s | "
+            "lsq %d its (%s) | none %d its (%s)" % (
+                name, r["schwarz"]["iterations"], r["schwarz"]["converged"], t_sw,
+                r["schwarz"]["solve_s"], rep["learned"], rep["fallback"],
+                r["lsq"]["iterations"], r["lsq"]["converged"], r["none"]["iterations"],
+                r["none"]["converged"]))
+        out[name] = r
+        for h in (ai, ls, sw, net, op, cl):
+            h.close()
+        torch.cuda.empty_cache()
     extra["schwarz"] = out
 
 
