@@ -466,17 +466,23 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
     }
     if (!any) continue;
     const double *Mo = Mf + int64_t(o) * F::PER_OFFSET;
-    for (int ks = 0; ks < KS; ++ks) {
-      double a[RPW], b[4];
+    // operand fragments of k step ks+1 are loaded while step ks runs on
+    // the matrix cores (register double buffer)
+    double a[RPW], b[4], an[RPW], bn[4];
+    auto load = [&](int ks, double *av, double *bv) {
 #pragma unroll
       for (int i = 0; i < RPW; ++i) {
         const int rb = rq + 4 * i;
-        a[i] = rb < RB ? Mo[(int64_t(ks) * RB + rb) * 64 + lane] : 0.0;
+        av[i] = rb < RB ? Mo[(int64_t(ks) * RB + rb) * 64 + lane] : 0.0;
       }
       const int k = ks * 4 + kl;
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        b[c] = (sc[c] >= 0 && k < NC) ? W[(int64_t(sc[c]) * NC + k) * kRhs + rl] : 0.0;
+        bv[c] = (sc[c] >= 0 && k < NC) ? W[(int64_t(sc[c]) * NC + k) * kRhs + rl] : 0.0;
+    };
+    load(0, a, b);
+    for (int ks = 0; ks < KS; ++ks) {
+      if (ks + 1 < KS) load(ks + 1, an, bn);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (sc[c] < 0) continue;
@@ -485,6 +491,10 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
           if (rq + 4 * i < RB)
             acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[c], acc[i][c], 0, 0, 0);
       }
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) a[i] = an[i];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = bn[c];
     }
   }
 #pragma unroll
@@ -634,14 +644,16 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
     if (!act[0] && !act[1]) continue;
     const bf16x8 *mh = reinterpret_cast<const bf16x8 *>(Mh) + int64_t(o) * (kBfPerOffset / 8);
     const bf16x8 *ml = reinterpret_cast<const bf16x8 *>(Ml) + int64_t(o) * (kBfPerOffset / 8);
-    for (int ks = 0; ks < kKs16; ++ks) {
-      bf16x8 ah[2], al[2], bh[2], bl[2];
+    // fragments of k step ks+1 are loaded while step ks runs on the
+    // matrix cores (register double buffer)
+    bf16x8 ah[2], al[2], bh[2], bl[2], ahn[2], aln[2], bhn[2], bln[2];
+    auto load = [&](int ks, bf16x8 *a_h, bf16x8 *a_l, bf16x8 *b_h, bf16x8 *b_l) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int rt = rq + 4 * i;
         if (rt < kRt) {
-          ah[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
-          al[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
+          a_h[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
+          a_l[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
         }
       }
 #pragma unroll
@@ -649,13 +661,17 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
         if (!act[q]) continue;
         if (sl[q] >= 0) {
           const int64_t e = (int64_t(sl[q]) * kRhs + r) * kPad + ks * 16 + 8 * half;
-          bh[q] = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
-          bl[q] = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
+          b_h[q] = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
+          b_l[q] = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
         } else {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) bh[q][j] = bl[q][j] = 0;
+          for (int j = 0; j < 8; ++j) b_h[q][j] = b_l[q][j] = 0;
         }
       }
+    };
+    load(0, ah, al, bh, bl);
+    for (int ks = 0; ks < kKs16; ++ks) {
+      if (ks + 1 < kKs16) load(ks + 1, ahn, aln, bhn, bln);
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         if (!act[q]) continue;
@@ -666,6 +682,16 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
           acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[q], acc[i][q], 0, 0, 0);
           acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[q], acc[i][q], 0, 0, 0);
         }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ah[i] = ahn[i];
+        al[i] = aln[i];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        bh[q] = bhn[q];
+        bl[q] = bln[q];
       }
     }
   }
