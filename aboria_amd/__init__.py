@@ -361,6 +361,24 @@ class SparseOperator:
         y2.copy_(yt.t())
         return y
 
+    @property
+    def nnz(self):
+        n = ctypes.c_int64()
+        check(C.lib().abh_sparse_op_nnz(self.handle, ctypes.byref(n)), self.ctx.handle)
+        return n.value
+
+    def assemble(self):
+        """The operator as CSR (row_ptr int64, col int32, val float64), the
+        triplets of KernelSparse::assemble."""
+        dev = self.ctx.device
+        nnz = self.nnz
+        rp = torch.empty(self.nrows + 1, dtype=torch.int64, device=dev)
+        col = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+        val = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+        check(C.lib().abh_sparse_op_assemble(self.handle, _ptr(rp), _ptr(col), _ptr(val), nnz),
+              self.ctx.handle)
+        return rp, col[:nnz], val[:nnz]
+
     def apply_colmajor(self, xt, yt, precision=FP64):
         """y^T += (K x)^T on (nrhs, n) contiguous tensors, no copies."""
         check(C.lib().abh_sparse_op_apply(self.handle, xt.shape[0], _ptr(xt), xt.shape[1],

@@ -281,7 +281,7 @@ __device__ __forceinline__ bool escapes_stencil(const GridParams &g, const doubl
 // the search kernel
 // ---------------------------------------------------------------------------
 
-enum Mode { kCount = 0, kFill = 1, kMatvec64 = 2, kMatvec32 = 3 };
+enum Mode { kCount = 0, kFill = 1, kMatvec64 = 2, kMatvec32 = 3, kAssemble = 4 };
 constexpr int kMaxRhs = 8;
 
 struct SearchArgs {
@@ -298,6 +298,7 @@ struct SearchArgs {
   int64_t *counts;
   const int64_t *row_ptr;
   int32_t *col;
+  double *val;  // assemble: kernel value of every stored pair
   uint32_t *escape_flag;
   uint32_t *next;  // dynamic work counter, zero at launch
   // matvec
@@ -478,7 +479,8 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     __syncthreads();
     int cnt = 0;
     int64_t out = 0;
-    if (MODE == kFill && active) out = a.row_ptr[row];
+    constexpr bool kList = (MODE == kFill || MODE == kAssemble);
+    if (kList && active) out = a.row_ptr[row];
     double acc64[kMaxRhs];
     float acc32[kMaxRhs], cmp32[kMaxRhs];
     if (kMV) {
@@ -490,7 +492,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       }
     }
     double wrow = 1.0;
-    if (kMV && (a.k.row_s || a.k.col_s)) {
+    if ((kMV || MODE == kAssemble) && (a.k.row_s || a.k.col_s)) {
       wrow = a.k.w0;
       if (a.k.row_s && active) wrow = wrow + a.k.wr * a.k.row_s[row];
     }
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
           }
           cnt += (acc <= r2m) ? 1 : 0;
         }
-      } else if (MODE == kCount || MODE == kFill) {
+      } else if (MODE == kCount || kList) {
         const double r2m = m != 0 ? a.r2 : -1.0;
         const CDouble *pj = cpos + int64_t(beg) * D;
         int j = beg;
@@ -564,8 +566,14 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
               acc = acc + dx * dx;
             }
             if (acc <= r2m) {
-              if (MODE == kFill)
+              if (kList)
                 for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j + u;
+              if (MODE == kAssemble) {
+                double w = wrow;
+                if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j + u];
+                const double f = phi64(a.k, acc) * w;
+                for (int rep = 0; rep < m; ++rep) a.val[out + cnt + rep] = f;
+              }
               cnt += m;
             }
           }
@@ -578,8 +586,14 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
             acc = acc + dx * dx;
           }
           if (acc <= r2m) {
-            if (MODE == kFill)
+            if (kList)
               for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j;
+            if (MODE == kAssemble) {
+              double w = wrow;
+              if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j];
+              const double f = phi64(a.k, acc) * w;
+              for (int rep = 0; rep < m; ++rep) a.val[out + cnt + rep] = f;
+            }
             cnt += m;
           }
         }
@@ -1011,6 +1025,36 @@ abh_status abh_sparse_op_destroy(abh_sparse_op *op) {
   (void)hipStreamSynchronize(ctx->stream);
   delete op;
   abh::ctx_release(ctx);
+  return ABH_OK;
+}
+
+abh_status abh_sparse_op_assemble(abh_sparse_op *op, int64_t *row_ptr, int32_t *col,
+                                  double *val, int64_t capacity) {
+  if (!op) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = op->plan.ctx;
+  return abh::guard(ctx, [&] {
+    const int64_t nq = op->plan.nq;
+    ABH_REQUIRE(row_ptr, ABH_ERR_INVALID_ARG, "row_ptr is null");
+    ABH_REQUIRE(capacity >= op->nnz, ABH_ERR_CAPACITY, "col / val capacity below nnz");
+    ABH_REQUIRE(op->nnz == 0 || (col && val), ABH_ERR_INVALID_ARG, "col or val is null");
+    ABH_HIP(hipMemsetAsync(row_ptr + nq, 0, 8, ctx->stream));
+    if (nq == 0) return;
+    abh::count_with_escape(op->plan, row_ptr);
+    abh::DevBuf total_buf;
+    total_buf.ensure(8);
+    abh::scan_exclusive(ctx, row_ptr, row_ptr, nq + 1, total_buf.as<int64_t>());
+    SearchArgs a = abh::base_args(op->plan);
+    a.k = op->k;
+    a.row_ptr = row_ptr;
+    a.col = col;
+    a.val = val;
+    abh::launch_search<kAssemble>(op->plan, a);
+  });
+}
+
+abh_status abh_sparse_op_nnz(const abh_sparse_op *op, int64_t *nnz) {
+  if (!op || !nnz) return ABH_ERR_INVALID_ARG;
+  *nnz = op->nnz;
   return ABH_OK;
 }
 

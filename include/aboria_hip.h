@@ -197,6 +197,15 @@ abh_status abh_sparse_op_create(abh_context *ctx, const abh_cells *cols,
                                 const double *row_scalar,
                                 const double *col_scalar, abh_sparse_op **out);
 abh_status abh_sparse_op_destroy(abh_sparse_op *op);
+/* number of stored pairs (neighbours within the radius, with multiplicity) */
+abh_status abh_sparse_op_nnz(const abh_sparse_op *op, int64_t *nnz);
+/* Replaces KernelSparse::assemble into triplets (src/Kernels.h:650-682):
+ * the operator as CSR, rows in row order, every row's entries in the
+ * search's order, val = f(dx) of each pair (a column reached through two
+ * periodic images appears twice, as the reference's triplets do).
+ * row_ptr: device, nrows+1; col / val: device, capacity >= nnz entries. */
+abh_status abh_sparse_op_assemble(abh_sparse_op *op, int64_t *row_ptr,
+                                  int32_t *col, double *val, int64_t capacity);
 /* y += K x for nrhs right-hand sides (KernelSparse::evaluate,
  * src/Kernels.h:720-751).  x: device, column-major ncols x nrhs with leading
  * dimension ldx; y: device, nrows x nrhs, leading dimension ldy. */

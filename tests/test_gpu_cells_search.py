@@ -207,3 +207,30 @@ def test_cpp_layer_example_runs():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_sparse_assemble(ctx, periodic):
+    """KernelSparse::assemble: the CSR columns are the search's lists, the
+    values f(dx) of each pair; sum(val * x) per row equals the matvec."""
+    rng = np.random.default_rng(31)
+    n, r = 3000, 0.12
+    pos = rng.random((n, 3))
+    cl = ab.CellListOrdered(ctx, [0] * 3, [1] * 3, [periodic] * 3, 8.0)
+    cl.update_positions(torch.tensor(pos, device="cuda"))
+    sp = cl.positions()
+    op = ab.SparseOperator(ctx, cl, sp, r, ab.Kernel.gaussian(0.05))
+    rp, col, val = op.assemble()
+    res = ab.euclidean_search(cl, sp, r)
+    assert np.array_equal(rp.cpu().numpy(), res.row_ptr.cpu().numpy())
+    assert np.array_equal(col.cpu().numpy(), res.col.cpu().numpy())
+    x = rng.normal(size=n)
+    y = (op @ torch.tensor(x, device="cuda")).cpu().numpy()
+    rpn, coln, valn = rp.cpu().numpy(), col.cpu().numpy(), val.cpu().numpy()
+    ys = np.add.reduceat(valn * x[coln], rpn[:-1]) if len(valn) else np.zeros(n)
+    assert np.max(np.abs(ys - y)) <= 1e-13 * np.max(np.abs(y))
+    if not periodic:
+        P = sp.cpu().numpy()
+        rows = np.repeat(np.arange(n), np.diff(rpn))
+        d2 = ((P[coln] - P[rows]) ** 2).sum(1)
+        assert np.allclose(valn, np.exp(-d2 / 0.05 ** 2), rtol=1e-14, atol=0)
