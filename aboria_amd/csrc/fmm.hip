@@ -563,8 +563,9 @@ __global__ __launch_bounds__(256) void k_split_m(const double *__restrict__ M, i
   }
 }
 
-// W (FP64 [node][NC][16]) -> Wh, Wl (BF16 [node][16][kPad], zero padded):
-// per right-hand side the expansion is contiguous in k
+// W (FP64 [node][NC][16]) -> Wh, Wl (BF16 [node][ks][16 rhs][2 halves][8],
+// zero padded): the B fragment of one source and k step is 512 contiguous
+// bytes (lane (rhs, half) reads the 8 k values it supplies)
 template <int N>
 __global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, int64_t nn,
                                                  uint16_t *__restrict__ Wh,
@@ -573,9 +574,12 @@ __global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, i
   const int64_t total = nn * kPad * kRhs;
   for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * 256) {
-    const int k = int(t % kPad);
-    const int r = int((t / kPad) % kRhs);
+    const int j = int(t % 8);
+    const int hf = int((t / 8) % 2);
+    const int r = int((t / 16) % kRhs);
+    const int ks = int((t / (16 * kRhs)) % kKs16);
     const int64_t node = t / (int64_t(kPad) * kRhs);
+    const int k = ks * 16 + 8 * hf + j;
     uint16_t h = 0, l = 0;
     if (k < NC) split_bf16(W[(node * NC + k) * kRhs + r], h, l);
     Wh[t] = h;
@@ -583,12 +587,13 @@ __global__ __launch_bounds__(256) void k_split_w(const double *__restrict__ W, i
   }
 }
 
-// One 512-thread block per sibling tile (8 targets x 16 right-hand sides =
-// 128 columns = 4 pairs of targets).  Wave w owns row tiles {w % 4,
-// w % 4 + 4} of the 224-row output and target pairs 2 (w / 4), 2 (w / 4) + 1.
-// Per box offset both operands are read straight from L2 in the fragment
-// layout (16 contiguous bytes per lane); pairs with no source at the offset
-// are skipped.
+// One 512-thread block per two sibling tiles (2 x 8 targets x 16
+// right-hand sides; a pair of targets = one 32-column MFMA tile).  Wave w
+// owns row tiles {w % 4, w % 4 + 4} of the 224-row output and target pairs
+// 2 (w / 4) and 2 (w / 4) + 1 of both tiles, so every operator fragment read
+// serves four pairs.  Per box offset both operands are read straight from
+// L2 in the fragment layout (16 contiguous bytes per lane); pairs with no
+// source at the offset are skipped.
 template <int N>
 __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ tile_parent,
                                                   int ntiles, const int32_t *__restrict__ child0,
@@ -600,99 +605,94 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
                                                   const uint16_t *__restrict__ Wl,
                                                   double *__restrict__ G) {
   constexpr int NC = Cheb<N>::NC;
-  __shared__ int src[8];
-  const int tile = blockIdx.x;
-  if (tile >= ntiles) return;
-  const int parent = tile_parent[tile];
-  const int first = child0[parent];
+  __shared__ int src[2][8];
+  const int tile0 = 2 * blockIdx.x;
+  if (tile0 >= ntiles) return;
+  const int ntl = ntiles - tile0 < 2 ? ntiles - tile0 : 2;  // tiles in this block
+  int parent[2], first[2], pc[2][3];
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt) {
+    parent[tt] = tile_parent[tile0 + (tt < ntl ? tt : 0)];
+    first[tt] = child0[parent[tt]];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pc[tt][d] = coord[parent[tt] * 3 + d];
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rq = wave & 3, p0 = 2 * (wave >> 2);
   const int half = lane >> 5, col = lane & 31;
   const int cq = col >> 4, r = col & 15;  // target within the pair, rhs
-  const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
-  f32x16 acc[2][2];
+  f32x16 acc[2][2][2];                     // [row tile][tile][pair]
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
+    for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) acc[i][q][v] = 0.0f;
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[i][tt][q][v] = 0.0f;
   for (int o = 0; o < kOffsets; ++o) {
     const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
     if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
     __syncthreads();
-    if (threadIdx.x < 8) {
-      const int ch = threadIdx.x;
+    if (threadIdx.x < 16) {
+      const int tt = threadIdx.x >> 3, ch = threadIdx.x & 7;
       const int qx = ((ch >> 2) & 1) + ox, qy = ((ch >> 1) & 1) + oy, qz = (ch & 1) + oz;
       int s = -1;
-      if (qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
-        const int sx = 2 * pc[0] + qx, sy = 2 * pc[1] + qy, sz = 2 * pc[2] + qz;
+      if (tt < ntl && qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
+        const int sx = 2 * pc[tt][0] + qx, sy = 2 * pc[tt][1] + qy, sz = 2 * pc[tt][2] + qz;
         if (sx >= 0 && sx < side && sy >= 0 && sy < side && sz >= 0 && sz < side)
           s = map[(int64_t(sx) * side + sy) * side + sz];
       }
-      src[ch] = s;
+      src[tt][ch] = s;
     }
     __syncthreads();
-    bool act[2];
-    int sl[2];
+    bool act[2][2];
+    int sl[2][2];
+    bool any = false;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int p = p0 + q;
-      act[q] = src[2 * p] >= 0 || src[2 * p + 1] >= 0;
-      sl[q] = src[2 * p + cq];
-    }
-    if (!act[0] && !act[1]) continue;
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int p = p0 + q;
+        act[tt][q] = src[tt][2 * p] >= 0 || src[tt][2 * p + 1] >= 0;
+        sl[tt][q] = src[tt][2 * p + cq];
+        any |= act[tt][q];
+      }
+    if (!any) continue;
     const bf16x8 *mh = reinterpret_cast<const bf16x8 *>(Mh) + int64_t(o) * (kBfPerOffset / 8);
     const bf16x8 *ml = reinterpret_cast<const bf16x8 *>(Ml) + int64_t(o) * (kBfPerOffset / 8);
-    // fragments of k step ks+1 are loaded while step ks runs on the
-    // matrix cores (register double buffer)
-    bf16x8 ah[2], al[2], bh[2], bl[2], ahn[2], aln[2], bhn[2], bln[2];
-    auto load = [&](int ks, bf16x8 *a_h, bf16x8 *a_l, bf16x8 *b_h, bf16x8 *b_l) {
+    for (int ks = 0; ks < kKs16; ++ks) {
+      bf16x8 ah[2], al[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int rt = rq + 4 * i;
         if (rt < kRt) {
-          a_h[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
-          a_l[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
+          ah[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
+          al[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
         }
       }
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        if (!act[q]) continue;
-        if (sl[q] >= 0) {
-          const int64_t e = (int64_t(sl[q]) * kRhs + r) * kPad + ks * 16 + 8 * half;
-          b_h[q] = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
-          b_l[q] = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
-        } else {
+      for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-          for (int j = 0; j < 8; ++j) b_h[q][j] = b_l[q][j] = 0;
+        for (int q = 0; q < 2; ++q) {
+          if (!act[tt][q]) continue;
+          bf16x8 bh, bl;
+          if (sl[tt][q] >= 0) {
+            const int64_t e = ((int64_t(sl[tt][q]) * kKs16 + ks) * kRhs + r) * 16 + 8 * half;
+            bh = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
+            bl = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bh[j] = bl[j] = 0;
+          }
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            if (rq + 4 * i >= kRt) continue;
+            acc[i][tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh, acc[i][tt][q], 0, 0, 0);
+            acc[i][tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl, acc[i][tt][q], 0, 0, 0);
+            acc[i][tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh, acc[i][tt][q], 0, 0, 0);
+          }
         }
-      }
-    };
-    load(0, ah, al, bh, bl);
-    for (int ks = 0; ks < kKs16; ++ks) {
-      if (ks + 1 < kKs16) load(ks + 1, ahn, aln, bhn, bln);
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        if (!act[q]) continue;
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          if (rq + 4 * i >= kRt) continue;
-          acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[q], acc[i][q], 0, 0, 0);
-          acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[q], acc[i][q], 0, 0, 0);
-          acc[i][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[q], acc[i][q], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ah[i] = ahn[i];
-        al[i] = aln[i];
-      }
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        bh[q] = bhn[q];
-        bl[q] = bln[q];
-      }
     }
   }
   // D layout: register v of lane l holds row 8 (v / 4) + 4 (l / 32) + v % 4,
@@ -702,12 +702,16 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
     const int rt = rq + 4 * i;
     if (rt >= kRt) continue;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int t = first + 2 * (p0 + q) + cq;
+    for (int tt = 0; tt < 2; ++tt) {
+      if (tt >= ntl) continue;
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int row = rt * 32 + 8 * (v / 4) + 4 * half + (v % 4);
-        if (row < NC) G[(int64_t(t) * NC + row) * kRhs + r] += double(acc[i][q][v]);
+      for (int q = 0; q < 2; ++q) {
+        const int t = first[tt] + 2 * (p0 + q) + cq;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int row = rt * 32 + 8 * (v / 4) + 4 * half + (v % 4);
+          if (row < NC) G[(int64_t(t) * NC + row) * kRhs + r] += double(acc[i][tt][q][v]);
+        }
       }
     }
   }
@@ -1189,7 +1193,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
       if (mixed && NC <= kPad) {
-        k_m2l_mfma<N><<<npar(l - 1), 512, 0, s>>>(
+        k_m2l_mfma<N><<<unsigned(abh::ceil_div(npar(l - 1), 2)), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mh.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset,
