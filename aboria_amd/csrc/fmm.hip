@@ -381,6 +381,16 @@ __global__ __launch_bounds__(256) void k_transfer(const int32_t *__restrict__ pa
 // one target, so a 16 x 16 output block is (16 node rows) x (one target).
 // ---------------------------------------------------------------------------
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2);
+// renumbering them so that XCD x runs a contiguous range of tiles keeps
+// neighbouring tiles -- which share source expansions -- on one L2.  The
+// grid is padded to a multiple of 8; indices past `count` do nothing.
+__device__ __forceinline__ int xcd_block(int count) {
+  const int per = (count + 7) / 8;
+  return int(blockIdx.x % 8) * per + int(blockIdx.x / 8);
+}
+inline unsigned xcd_grid(int count) { return unsigned((count + 7) / 8 * 8); }
+
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 
@@ -428,7 +438,7 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
   constexpr int NC = F::NC, RB = F::RB, KS = F::KS;
   constexpr int RPW = (RB + 3) / 4;  // row blocks per wave
   __shared__ int src[8];
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(ntiles);
   if (tile >= ntiles) return;
   const int parent = tile_parent[tile];
   const int first = child0[parent];
@@ -606,7 +616,7 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
                                                   double *__restrict__ G) {
   constexpr int NC = Cheb<N>::NC;
   __shared__ int src[2][8];
-  const int tile0 = 2 * blockIdx.x;
+  const int tile0 = 2 * xcd_block((ntiles + 1) / 2);
   if (tile0 >= ntiles) return;
   const int ntl = ntiles - tile0 < 2 ? ntiles - tile0 : 2;  // tiles in this block
   int parent[2], first[2], pc[2][3];
@@ -1193,7 +1203,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
       if (mixed && NC <= kPad) {
-        k_m2l_mfma<N><<<unsigned(abh::ceil_div(npar(l - 1), 2)), 512, 0, s>>>(
+        k_m2l_mfma<N><<<xcd_grid((npar(l - 1) + 1) / 2), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mh.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset,
@@ -1208,7 +1218,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
           abh::check_launch();
           f->pack_ready = true;
         }
-        k_m2l_mfma64<N><<<npar(l - 1), 512, 0, s>>>(
+        k_m2l_mfma64<N><<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mf.as<double>() + int64_t(l - 2) * kOffsets * F::PER_OFFSET, f->W.as<double>(),
