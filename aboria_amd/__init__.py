@@ -264,6 +264,17 @@ class Search:
         check(C.lib().abh_search_fill(self.handle, _ptr(col), col.numel()), self.ctx.handle)
         return col
 
+    def run_fused(self, query, centres, radius, col):
+        """Counts and fills in one traversal of the candidates (rows staged at
+        upper-bound offsets in a handle-owned buffer, then packed into col).
+        Returns the number of pairs; row_ptr() is valid afterwards."""
+        t = ctypes.c_int64()
+        check(C.lib().abh_search_run_fused(self.handle, query.handle, centres.shape[0],
+                                           _ptr(centres), float(radius), _ptr(col), col.numel(),
+                                           ctypes.byref(t)), self.ctx.handle)
+        self.nq = centres.shape[0]
+        return t.value
+
     def close(self):
         if getattr(self, "handle", None):
             C.lib().abh_search_destroy(self.handle)

@@ -139,6 +139,8 @@ SIGNATURES = {
     "abh_search_total": (c_int, [c_void_p, P(c_int64)]),
     "abh_search_row_ptr": (c_void_p, [c_void_p]),
     "abh_search_fill": (c_int, [c_void_p, c_void_p, c_int64]),
+    "abh_search_run_fused": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double, c_void_p,
+                                     c_int64, P(c_int64)]),
     "abh_sparse_op_create": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double, P(Kernel),
                                      c_void_p, c_void_p, P(c_void_p)]),
     "abh_sparse_op_destroy": (c_int, [c_void_p]),

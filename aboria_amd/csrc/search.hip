@@ -281,7 +281,15 @@ __device__ __forceinline__ bool escapes_stencil(const GridParams &g, const doubl
 // the search kernel
 // ---------------------------------------------------------------------------
 
-enum Mode { kCount = 0, kFill = 1, kMatvec64 = 2, kMatvec32 = 3, kAssemble = 4 };
+enum Mode {
+  kCount = 0,
+  kFill = 1,
+  kMatvec64 = 2,
+  kMatvec32 = 3,
+  kAssemble = 4,
+  kCapacity = 5,   // upper bound of every row: candidates of its visited buckets
+  kFillPadded = 6  // fill rows at capacity offsets and record their counts
+};
 constexpr int kMaxRhs = 8;
 
 struct SearchArgs {
@@ -331,9 +339,11 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
   extern __shared__ __align__(16) unsigned char lds[];
   constexpr bool kMV = (MODE == kMatvec64 || MODE == kMatvec32);
   const int S = a.nslots;
+  // matvec only: candidate tile and right-hand sides in front of the slots
   Tile *tile = reinterpret_cast<Tile *>(lds);
-  double *xt = reinterpret_cast<double *>(tile + 1);  // [kMaxRhs][kTile], matvec only
-  Slot *slots = reinterpret_cast<Slot *>(xt + (kMV ? kTile * kMaxRhs : 0));
+  double *xt = reinterpret_cast<double *>(tile + 1);  // [kMaxRhs][kTile]
+  Slot *slots = kMV ? reinterpret_cast<Slot *>(xt + kTile * kMaxRhs)
+                    : reinterpret_cast<Slot *>(lds);
   int32_t *sorder = reinterpret_cast<int32_t *>(slots + S);
   uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + S);  // [S][64]
   const int lane = threadIdx.x;
@@ -454,7 +464,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         } else {
           qs.valid = 0;
         }
-        if (MODE == kCount && in_range) {
+        if ((MODE == kCount || MODE == kCapacity) && in_range) {
           int c[D];
 #pragma unroll
           for (int d = 0; d < D; ++d) c[d] = it.home[d] + I[d] * g.size[d];
@@ -475,11 +485,20 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         }
       }
     }
-    if (MODE == kCount && escape) atomicOr(a.escape_flag, 1u);
+    if ((MODE == kCount || MODE == kCapacity) && escape) atomicOr(a.escape_flag, 1u);
     __syncthreads();
+    if (MODE == kCapacity) {
+      // candidates of every visited bucket, with multiplicity: no distance
+      // test can accept more
+      int64_t cap = 0;
+      for (int s = 0; s < S; ++s) cap += int64_t(mult[s * kGroup + lane]) * slots[s].len;
+      if (has_row) a.counts[row] = active ? cap : 0;
+      __syncthreads();
+      continue;
+    }
     int cnt = 0;
     int64_t out = 0;
-    constexpr bool kList = (MODE == kFill || MODE == kAssemble);
+    constexpr bool kList = (MODE == kFill || MODE == kAssemble || MODE == kFillPadded);
     if (kList && active) out = a.row_ptr[row];
     double acc64[kMaxRhs];
     float acc32[kMaxRhs], cmp32[kMaxRhs];
@@ -664,7 +683,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       }
     }
     if (has_row) {
-      if (MODE == kCount) a.counts[row] = cnt;
+      if (MODE == kCount || MODE == kFillPadded) a.counts[row] = cnt;
       if (MODE == kMatvec64) {
 #pragma unroll
         for (int r = 0; r < kMaxRhs; ++r)
@@ -818,7 +837,7 @@ static SearchArgs base_args(const QueryPlan &p) {
 }
 
 static size_t lds_bytes(int nslots, bool matvec) {
-  return sizeof(Tile) + (matvec ? sizeof(double) * kTile * kMaxRhs : 0) +
+  return (matvec ? sizeof(Tile) + sizeof(double) * kTile * kMaxRhs : 0) +
          size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
 }
 
@@ -842,6 +861,37 @@ template <int MODE> static void launch_search(const QueryPlan &p, const SearchAr
 
 // Counts neighbours into counts[nq] (device), widening the stencil until no
 // query's visited buckets escape it.
+// col[row_ptr[i] + e] = padded[pad_ptr[i] + e], one wavefront per row
+__global__ __launch_bounds__(256) void k_compact_rows(const int64_t *__restrict__ pad_ptr,
+                                                      const int64_t *__restrict__ row_ptr,
+                                                      int64_t nrows,
+                                                      const int32_t *__restrict__ padded,
+                                                      int32_t *__restrict__ col) {
+  const int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (i >= nrows) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t src = pad_ptr[i], dst = row_ptr[i], n = row_ptr[i + 1] - dst;
+  for (int64_t e = lane; e < n; e += 64) col[dst + e] = padded[src + e];
+}
+
+// Like count_with_escape, with the capacity pass (no distance tests).
+static void capacity_with_escape(QueryPlan &p, int64_t *caps) {
+  p.escape.ensure(4);
+  for (int extra = 0;; ++extra) {
+    if (extra > 0) set_stencil(p, extra);
+    ABH_HIP(hipMemsetAsync(p.escape.p, 0, 4, p.ctx->stream));
+    SearchArgs a = base_args(p);
+    a.counts = caps;
+    a.escape_flag = p.escape.as<uint32_t>();
+    launch_search<kCapacity>(p, a);
+    uint32_t esc = 0;
+    ABH_HIP(hipMemcpyAsync(&esc, p.escape.p, 4, hipMemcpyDeviceToHost, p.ctx->stream));
+    ABH_HIP(hipStreamSynchronize(p.ctx->stream));
+    if (!esc) return;
+    ABH_REQUIRE(extra < 4, ABH_ERR_UNSUPPORTED, "search stencil failed to cover visited buckets");
+  }
+}
+
 static void count_with_escape(QueryPlan &p, int64_t *counts) {
   p.escape.ensure(4);
   for (int extra = 0;; ++extra) {
@@ -865,6 +915,7 @@ struct abh_search {
   abh::QueryPlan plan;
   abh::DevBuf row_ptr, total_buf;
   int64_t total = 0;
+  abh::DevBuf pad_ptr, padded;  // single-traversal search: rows at capacity offsets
 };
 
 struct abh_sparse_op {
@@ -969,6 +1020,56 @@ abh_status abh_search_fill(abh_search *s, int32_t *col, int64_t capacity) {
     a.row_ptr = s->row_ptr.as<int64_t>();
     a.col = col;
     abh::launch_search<kFill>(s->plan, a);
+  });
+}
+
+abh_status abh_search_run_fused(abh_search *s, const abh_cells *cells, int64_t nq,
+                                const double *q, double radius, int32_t *col, int64_t capacity,
+                                int64_t *total_pairs) {
+  if (!s) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = s->plan.ctx;
+  return abh::guard(ctx, [&] {
+    abh::build_plan(s->plan, ctx, cells, nq, q, radius);
+    hipStream_t st = ctx->stream;
+    s->row_ptr.ensure(size_t(nq + 1) * 8);
+    s->pad_ptr.ensure(size_t(nq + 1) * 8);
+    s->total_buf.ensure(16);
+    int64_t *pp = s->pad_ptr.as<int64_t>();
+    int64_t *rp = s->row_ptr.as<int64_t>();
+    ABH_HIP(hipMemsetAsync(pp + nq, 0, 8, st));
+    ABH_HIP(hipMemsetAsync(rp + nq, 0, 8, st));
+    int64_t padded_total = 0;
+    if (nq > 0) {
+      abh::capacity_with_escape(s->plan, pp);
+      abh::scan_exclusive(ctx, pp, pp, nq + 1, s->total_buf.as<int64_t>());
+      ABH_HIP(hipMemcpyAsync(&padded_total, s->total_buf.p, 8, hipMemcpyDeviceToHost, st));
+      ABH_HIP(hipStreamSynchronize(st));
+    }
+    s->padded.ensure(size_t(padded_total) * 4 + 64);
+    if (nq > 0 && padded_total > 0) {
+      SearchArgs a = abh::base_args(s->plan);
+      a.row_ptr = pp;
+      a.col = s->padded.as<int32_t>();
+      a.counts = rp;
+      abh::launch_search<kFillPadded>(s->plan, a);
+    } else if (nq > 0) {
+      ABH_HIP(hipMemsetAsync(rp, 0, size_t(nq) * 8, st));
+    }
+    int64_t total = 0;
+    if (nq > 0) {
+      abh::scan_exclusive(ctx, rp, rp, nq + 1, s->total_buf.as<int64_t>());
+      ABH_HIP(hipMemcpyAsync(&total, s->total_buf.p, 8, hipMemcpyDeviceToHost, st));
+      ABH_HIP(hipStreamSynchronize(st));
+    }
+    s->total = total;
+    if (total_pairs) *total_pairs = total;
+    ABH_REQUIRE(capacity >= total, ABH_ERR_CAPACITY, "neighbour index buffer too small");
+    ABH_REQUIRE(total == 0 || col, ABH_ERR_INVALID_ARG, "col is null");
+    if (total > 0) {
+      abh::k_compact_rows<<<unsigned(abh::ceil_div(nq, 4)), 256, 0, st>>>(
+          pp, rp, nq, s->padded.as<int32_t>(), col);
+      abh::check_launch();
+    }
   });
 }
 

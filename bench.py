@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--sections", default=",".join(DEFAULT_SECTIONS),
                    help="comma-separated subset of %s ('copy' is the calibration copy "
                         "of the HBM traffic counters, not run by default)" % (SECTIONS,))
+    p.add_argument("--fused", action="store_true",
+                   help="time the single-traversal search (abh_search_run_fused) as the step "
+                        "instead of count pass + fill pass")
     p.add_argument("--rbf-iters", type=int, default=100,
                    help="GMRES iteration budget of the batched RBF systems")
     return p.parse_args()
@@ -102,9 +105,13 @@ def bench_search(args, torch, ab, wl, ctx, stream, extra):
         ev[0].record(stream)
         cells.update_positions(pos)
         ev[1].record(stream)
-        search.run(cells, cells.positions(), r)
-        ev[2].record(stream)
-        search.fill(col)
+        if not args.fused:
+            search.run(cells, cells.positions(), r)
+            ev[2].record(stream)
+            search.fill(col)
+        else:
+            ev[2].record(stream)
+            search.run_fused(cells, cells.positions(), r, col)
         ev[3].record(stream)
         if record:
             torch.cuda.synchronize()
@@ -125,8 +132,9 @@ def bench_search(args, torch, ab, wl, ctx, stream, extra):
         step(True)
     total = search.total
     value = total * args.steps / elapsed
-    log("step %.3f ms, build %.3f ms, count %.3f ms, fill %.3f ms" % (
-        ms_per_step, np.median(build_ms), np.median(count_ms), np.median(fill_ms)))
+    log("step %.3f ms, build %.3f ms, count %.3f ms, %s %.3f ms" % (
+        ms_per_step, np.median(build_ms), np.median(count_ms),
+        "fused search" if args.fused else "fill", np.median(fill_ms)))
     nb = cells.n_buckets
     fill_med = float(np.median(fill_ms))
     # algorithmic bytes of the fill launch: positions (24 B) + bucket
@@ -146,7 +154,22 @@ def bench_search(args, torch, ab, wl, ctx, stream, extra):
     roofline["frac"] = roofline["achieved"] / roofline["peak"]
     extra["search"] = {"build_ms": float(np.median(build_ms)),
                        "count_ms": float(np.median(count_ms)), "fill_ms": fill_med,
-                       "pairs": int(total), "grid": list(cells.grid)}
+                       "pairs": int(total), "grid": list(cells.grid),
+                       "mode": "fused" if args.fused else "count + fill"}
+    if not args.fused:
+        # the single-traversal form of the same search, for comparison
+        search.run_fused(cells, cells.positions(), r, col)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ts = []
+        for _ in range(3):
+            ev[0].record(stream)
+            search.run_fused(cells, cells.positions(), r, col)
+            ev[1].record(stream)
+            torch.cuda.synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]))
+        extra["search"]["fused_search_ms"] = float(np.median(ts))
+        log("single-traversal search %.3f ms (count + fill %.3f ms)" % (
+            np.median(ts), np.median(count_ms) + fill_med))
     config = {"workload": "cell-list build + radius search, 3D periodic Gaussian mixture",
               "n": n, "radius": r, "n_particles_in_leaf": n_leaf, "grid": list(cells.grid),
               "n_centres": cfg["n_centres"], "sigma": cfg["sigma"]}

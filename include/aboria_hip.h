@@ -150,6 +150,16 @@ const int64_t *abh_search_row_ptr(const abh_search *s);
  * except where one bucket is reached through two periodic images.
  * col is device memory of at least `capacity` entries. */
 abh_status abh_search_fill(abh_search *s, int32_t *col, int64_t capacity);
+/* The search of abh_search_update + abh_search_fill in one traversal of the
+ * candidates: an upper bound of every row (the candidates of its visited
+ * buckets, no distance tests) places the rows in a padded device buffer
+ * owned by the handle (grow-only; about 4-5x the neighbour lists), the
+ * traversal writes each row there and counts it, and the rows are then
+ * packed into col, with the same row_ptr and lists as the two-pass form.
+ * *total_pairs (host) gets the number of pairs; synchronises the stream. */
+abh_status abh_search_run_fused(abh_search *s, const abh_cells *cells, int64_t nq,
+                                const double *q, double radius, int32_t *col,
+                                int64_t capacity, int64_t *total_pairs);
 
 /* ------------------------------------------------------------------ */
 /* kernel descriptors and sparse operator                              */

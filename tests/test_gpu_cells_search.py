@@ -234,3 +234,23 @@ def test_sparse_assemble(ctx, periodic):
         rows = np.repeat(np.arange(n), np.diff(rpn))
         d2 = ((P[coln] - P[rows]) ** 2).sum(1)
         assert np.allclose(valn, np.exp(-d2 / 0.05 ** 2), rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("case", ["d3_p", "d2_np", "d3_p_wide", "d1_p"])
+def test_fused_search_equals_two_pass(ctx, case):
+    """abh_search_run_fused: same row_ptr and the same lists (same order) as
+    abh_search_update + abh_search_fill."""
+    name, D, N, periodic, n_leaf, r, lo, hi = [c for c in CASES if c[0] == case][0]
+    pos, alive = make_case(D, N, lo, hi, 77)
+    cl, p, a, na = gpu_build(ctx, pos, alive, [lo] * D, [hi] * D, [periodic] * D, n_leaf)
+    sp = cl.positions()
+    s = ab.Search(cl, sp, r)
+    total = s.total
+    col2 = torch.empty(total + 16, dtype=torch.int32, device="cuda")
+    s.fill(col2)
+    rp2 = s.row_ptr().clone()
+    col1 = torch.full((total + 16,), -7, dtype=torch.int32, device="cuda")
+    t1 = s.run_fused(cl, sp, r, col1)
+    assert t1 == total
+    assert torch.equal(s.row_ptr(), rp2)
+    assert torch.equal(col1[:total], col2[:total])
