@@ -556,14 +556,18 @@ class DenseBatchOperator:
               self.ctx.handle)
         return rpt.value, {1: ms[0], 2: ms[1], 4: ms[2]}
 
-    def gmres(self, b, x=None, restart=50, max_iters=1000, tol=1e-8, precision=FP64):
-        """Solves K x = b for every problem; returns (x, stats)."""
+    def gmres(self, b, x=None, restart=50, max_iters=1000, tol=1e-8, precision=FP64,
+              preconditioner=None):
+        """Solves K x = b for every problem, right-preconditioned by
+        `preconditioner` (SchwartzPreconditioner.for_dense_batch) if given;
+        returns (x, stats)."""
         if x is None:
             x = torch.zeros_like(b)
         opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision), 0)
         stats = (C.GmresStats * self.nprob)()
-        check(C.lib().abh_dense_batch_gmres(self.handle, _ptr(b), _ptr(x), ctypes.byref(opts),
-                                            stats), self.ctx.handle)
+        ph = preconditioner.handle if preconditioner is not None else None
+        check(C.lib().abh_dense_batch_gmres_precond(self.handle, ph, _ptr(b), _ptr(x),
+                                                    ctypes.byref(opts), stats), self.ctx.handle)
         return x, [{"iterations": s.iterations, "converged": bool(s.converged),
                     "residual": s.residual} for s in stats]
 
@@ -596,6 +600,27 @@ class SchwartzPreconditioner:
                                          float(radius), float(diag_shift), ctypes.byref(h)),
               ctx.handle)
         self.handle = h
+
+    @classmethod
+    def for_dense_batch(cls, op, low, high, n_particles_in_leaf, diag_shift=0.0):
+        """One restricted additive Schwarz preconditioner per problem of a
+        DenseBatchOperator (K + diag_shift I over all pairs), with the bucket
+        grid a cell list over [low, high] would size for each problem and a
+        signed local factorisation (indefinite kernels allowed)."""
+        self = cls.__new__(cls)
+        self.ctx = op.ctx
+        self.cells = None
+        self.op = op
+        self.n = op.offsets[-1]
+        lo = (ctypes.c_double * op.dim)(*[float(v) for v in low])
+        hi = (ctypes.c_double * op.dim)(*[float(v) for v in high])
+        h = ctypes.c_void_p()
+        check(C.lib().abh_dense_batch_schwarz_create(op.handle, lo, hi,
+                                                     float(n_particles_in_leaf),
+                                                     float(diag_shift), ctypes.byref(h)),
+              op.ctx.handle)
+        self.handle = h
+        return self
 
     @property
     def info(self):

@@ -168,6 +168,10 @@ SIGNATURES = {
     "abh_schwarz_get_info": (c_int, [c_void_p, P(SchwarzInfo)]),
     "abh_schwarz_get_domains": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "abh_schwarz_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "abh_dense_batch_schwarz_create": (c_int, [c_void_p, P(c_double), P(c_double), c_double,
+                                               c_double, P(c_void_p)]),
+    "abh_dense_batch_gmres_precond": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p,
+                                              P(GmresOptions), P(GmresStats)]),
     "abh_knn": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "abh_mlp_create": (c_int, [c_void_p, c_int, c_int, c_int, c_int, ctypes.c_uint64,
                                P(c_void_p)]),

@@ -12,6 +12,7 @@
 
 #include "batch.hpp"
 #include "kernel_fn.hpp"
+#include "solvers.hpp"
 
 using abh::DevBuf;
 using abh::KernelDesc;
@@ -336,9 +337,37 @@ abh_status abh_dense_batch_apply(abh_dense_batch *op, const double *x, double *y
 
 abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x,
                                  const abh_gmres_options *opts, abh_gmres_stats *stats) {
+  return abh_dense_batch_gmres_precond(op, nullptr, b, x, opts, stats);
+}
+
+abh_status abh_dense_batch_schwarz_create(abh_dense_batch *op, const double *low,
+                                          const double *high, double n_particles_in_leaf,
+                                          double diag_shift, abh_schwarz **out) {
+  if (!op || !low || !high || !out) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_context *ctx = op->ctx;
+  abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
+    ABH_REQUIRE(n_particles_in_leaf > 0, ABH_ERR_INVALID_ARG,
+                "n_particles_in_leaf must be positive");
+    for (int d = 0; d < op->dim; ++d)
+      ABH_REQUIRE(high[d] > low[d], ABH_ERR_INVALID_ARG, "high must exceed low");
+    *out = abh::schwarz_create_batch(ctx, op->dim, op->L.off, op->pos, low, high,
+                                     n_particles_in_leaf, op->k, diag_shift);
+  });
+  if (st != ABH_OK) return st;
+  abh::ctx_retain(ctx);
+  return ABH_OK;
+}
+
+abh_status abh_dense_batch_gmres_precond(abh_dense_batch *op, const abh_schwarz *prec,
+                                         const double *b, double *x,
+                                         const abh_gmres_options *opts, abh_gmres_stats *stats) {
   if (!op || !opts) return ABH_ERR_INVALID_ARG;
   abh_context *ctx = op->ctx;
   return abh::guard(ctx, [&] {
+    ABH_REQUIRE(!prec || abh::schwarz_size(prec) == op->L.total(), ABH_ERR_INVALID_ARG,
+                "preconditioner size differs from the batch");
     ABH_REQUIRE(opts->restart >= 1 && opts->max_iters >= 1 && opts->tol > 0, ABH_ERR_INVALID_ARG,
                 "invalid GMRES options");
     ABH_REQUIRE(opts->precision == ABH_FP64 || opts->precision == ABH_FP32_COMPENSATED,
@@ -357,8 +386,11 @@ abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x
                              ctx->stream));
       ABH_HIP(hipStreamSynchronize(ctx->stream));
     };
+    std::function<void(const double *, double *)> f_prec = [&](const double *in, double *out) {
+      abh::schwarz_apply(prec, in, out);
+    };
     abh::GmresResult res;
-    abh::batch_gmres(ctx, op->L, f_op, f_true, nullptr, f_act, b, x, *opts, res);
+    abh::batch_gmres(ctx, op->L, f_op, f_true, prec ? &f_prec : nullptr, f_act, b, x, *opts, res);
     if (stats)
       for (int p = 0; p < op->L.nprob; ++p) stats[p] = res.stats[p];
   });

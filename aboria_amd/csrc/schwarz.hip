@@ -14,13 +14,20 @@
 // the unit columns.  They stay resident: R_g is n x ni column-major, so row a
 // of the local inverse is the contiguous column a.
 //
+// For the dense batch (every problem an independent particle set) the
+// domains come from a per-problem bucket grid instead of a cell list, the
+// kernel is not truncated, and the factorisation is signed, A = L S L^T with
+// S = diag(+-1), since multiquadric matrices are indefinite.
+//
 // Apply (device, HBM-bound): x = b, then for every domain and interior row a
 // x[idx[a]] = <R_g[:, a], b[idx]>, one wavefront per row, the gathered
-// b[idx] staged once per workgroup in LDS.
+// b[idx] staged once per workgroup in LDS.  A domain whose factorisation
+// broke down (counted in failed_domains) is skipped: its rows keep b.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <numeric>
 #include <vector>
 
@@ -51,7 +58,7 @@ struct DomDev {
   int32_t m;        // interior + buffer
   int32_t ni;       // interior
   int32_t ld;       // padded size n of its group (leading dimension of R)
-  int32_t pad;
+  int32_t failed;   // factorisation broke down: apply leaves its rows at b
 };
 
 // A[g] (n x n, column-major) = K + shift I over the domain's particles,
@@ -102,12 +109,18 @@ __global__ __launch_bounds__(256) void k_assemble(const DomDev *__restrict__ dom
 
 // Cholesky of the 64 x 64 diagonal block at (o, o) of every matrix of the
 // group, in LDS; writes L back and L^-1 to dinv (64 x 64 column-major).
+// Signed mode (symmetric matrices with negative eigenvalues, such as
+// multiquadric ones): A = L S L^T with S = diag(+-1); the signs go to
+// sg[g * sg_stride + o + j] and S L^-1 (the panel's right factor) to dinvs.
 __global__ __launch_bounds__(256) void k_potrf_diag(double *__restrict__ A, int n, int o,
                                                     double *__restrict__ dinv,
                                                     int64_t dinv_stride,
-                                                    uint8_t *__restrict__ failed) {
+                                                    uint8_t *__restrict__ failed, int signed_mode,
+                                                    double *__restrict__ sg, int64_t sg_stride,
+                                                    double *__restrict__ dinvs) {
   __shared__ double L[kNb][kNb + 1];
   __shared__ double X[kNb][kNb + 1];
+  __shared__ double S[kNb];
   __shared__ int bad;
   const int g = blockIdx.x;
   const int t = threadIdx.x;
@@ -121,20 +134,30 @@ __global__ __launch_bounds__(256) void k_potrf_diag(double *__restrict__ A, int 
   for (int j = 0; j < kNb; ++j) {
     if (t == 0) {
       double d = L[j][j];
-      if (!(d > 0.0)) {
+      double sj = 1.0;
+      if (signed_mode) {
+        if (!(d != 0.0) || !isfinite(d)) {
+          bad = 1;
+          d = 1.0;
+        }
+        sj = d > 0.0 ? 1.0 : -1.0;
+        d = fabs(d);
+      } else if (!(d > 0.0)) {
         bad = 1;
         d = 1.0;
       }
+      S[j] = sj;
       L[j][j] = sqrt(d);
     }
     __syncthreads();
-    const double djj = L[j][j];
+    const double djj = L[j][j] * S[j];
     if (t > j && t < kNb) L[t][j] = L[t][j] / djj;
     __syncthreads();
     // trailing update of the lower triangle, columns j+1..63
+    const double sj = S[j];
     for (int e = t; e < kNb * kNb; e += 256) {
       const int r = e & 63, c = e >> 6;
-      if (c > j && r >= c) L[r][c] = L[r][c] - L[r][j] * L[c][j];
+      if (c > j && r >= c) L[r][c] = L[r][c] - (L[r][j] * L[c][j]) * sj;
     }
     __syncthreads();
   }
@@ -155,19 +178,34 @@ __global__ __launch_bounds__(256) void k_potrf_diag(double *__restrict__ A, int 
     const int r = e & 63, c = e >> 6;
     Ag[int64_t(c) * n + r] = r >= c ? L[r][c] : 0.0;
     Dg[c * kNb + r] = X[r][c];
+    if (signed_mode) dinvs[int64_t(g) * dinv_stride + c * kNb + r] = X[r][c] * S[r];
   }
+  if (signed_mode && t < kNb) sg[int64_t(g) * sg_stride + o + t] = S[t];
   if (t == 0 && bad) failed[g] = 1;
 }
 
-// dst[g] (rows x cols, ld ldd) = src[g] (ld lds), batched strided copy
+// dst[g] (rows x cols, ld ldd) = src[g] (ld lds) times cs[g * cs_stride + c]
+// per column when cs is given; batched strided copy
 __global__ void k_copy_block(const double *__restrict__ src, int64_t src_stride, int lds,
                              double *__restrict__ dst, int64_t dst_stride, int ldd, int rows,
-                             int cols) {
+                             int cols, const double *__restrict__ cs = nullptr,
+                             int64_t cs_stride = 0) {
   const int g = blockIdx.y;
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= int64_t(rows) * cols) return;
   const int r = int(e % rows), c = int(e / rows);
-  dst[g * dst_stride + int64_t(c) * ldd + r] = src[g * src_stride + int64_t(c) * lds + r];
+  double v = src[g * src_stride + int64_t(c) * lds + r];
+  if (cs) v = v * cs[g * cs_stride + c];
+  dst[g * dst_stride + int64_t(c) * ldd + r] = v;
+}
+
+// Y[g][r][c] *= sg[g * sg_stride + r] (n x ni, column-major)
+__global__ void k_scale_rows(double *__restrict__ Y, int64_t stride, int n, int ni,
+                             const double *__restrict__ sg, int64_t sg_stride) {
+  const int g = blockIdx.y;
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(n) * ni) return;
+  Y[g * stride + e] = Y[g * stride + e] * sg[g * sg_stride + int(e % n)];
 }
 
 // Y[g] (n x ni) = the first ni columns of the identity
@@ -194,6 +232,7 @@ __global__ __launch_bounds__(256) void k_ras_apply(const ApplyChunk *__restrict_
   extern __shared__ double bl[];
   const ApplyChunk ch = chunks[blockIdx.x];
   const DomDev dm = doms[ch.dom];
+  if (dm.failed) return;  // uniform over the workgroup
   const int32_t *id = idx + dm.idx_off;
   if (LDS) {
     for (int j = threadIdx.x; j < dm.m; j += 256) bl[j] = b[id[j]];
@@ -282,18 +321,106 @@ void analyse(abh_schwarz *s, const abh_cells *c, const std::vector<double> &pos,
   }
 }
 
-void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r2,
-               double shift) {
+// Domains of a batch of independent particle sets (problem p owns points
+// off[p] .. off[p+1]-1).  Every problem gets the bucket grid the cell list
+// would size for it (src/CellListOrdered.h:132-186, with n = its particle
+// count); a domain per non-empty bucket, whose interior is the bucket's
+// particles and whose buffer the particles of the other buckets of the 3^dim
+// block around it (src/Preconditioners.h:1169-1230).  Bucket membership is
+// floor((x - low) / side) clamped to the grid, so every particle is interior
+// to exactly one domain.  Lists are in ascending particle order per bucket,
+// neighbour buckets in row-major order; indices are global.
+void analyse_batch(abh_schwarz *s, int D, const std::vector<int64_t> &boff,
+                   const std::vector<double> &pos, const double *lo, const double *hi,
+                   double n_leaf) {
+  s->off.assign(1, 0);
+  s->idx.clear();
+  s->nint.clear();
+  std::vector<int64_t> key, cnt;
+  std::vector<int32_t> sorted;
+  for (size_t p = 0; p + 1 < boff.size(); ++p) {
+    const int64_t p0 = boff[p], np = boff[p + 1] - p0;
+    if (np == 0) continue;
+    int64_t sz[3] = {1, 1, 1};
+    double side[3] = {1, 1, 1};
+    if (n_leaf <= double(np)) {
+      double volume = 1;
+      for (int d = 0; d < D; ++d) volume *= (hi[d] - lo[d]);
+      const double box_side = std::pow(n_leaf / double(np) * volume, 1.0 / D);
+      for (int d = 0; d < D; ++d) {
+        sz[d] = int64_t(std::floor((hi[d] - lo[d]) / box_side));
+        if (sz[d] == 0) sz[d] = 1;
+      }
+    }
+    for (int d = 0; d < D; ++d) side[d] = (hi[d] - lo[d]) / double(sz[d]);
+    const int64_t nb = sz[0] * sz[1] * sz[2];
+    ABH_REQUIRE(nb < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "too many buckets");
+    key.resize(size_t(np));
+    cnt.assign(size_t(nb) + 1, 0);
+    for (int64_t i = 0; i < np; ++i) {
+      int64_t kk = 0;
+      for (int d = 0; d < D; ++d) {
+        const double v = pos[size_t(p0 + i) * D + d];
+        ABH_REQUIRE(std::isfinite(v) && v >= lo[d] && v <= hi[d], ABH_ERR_INVALID_ARG,
+                    "batch particle outside [low, high]");
+        const int64_t c = std::min<int64_t>(int64_t(std::floor((v - lo[d]) / side[d])), sz[d] - 1);
+        kk = kk * sz[d] + c;
+      }
+      key[size_t(i)] = kk;
+      ++cnt[size_t(kk) + 1];
+    }
+    for (int64_t b = 0; b < nb; ++b) cnt[size_t(b) + 1] += cnt[size_t(b)];
+    sorted.resize(size_t(np));
+    {
+      std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+      for (int64_t i = 0; i < np; ++i) sorted[size_t(fill[size_t(key[size_t(i)])]++)] = int32_t(p0 + i);
+    }
+    for (int64_t b = 0; b < nb; ++b) {
+      if (cnt[size_t(b) + 1] == cnt[size_t(b)]) continue;
+      int64_t iv[3] = {0, 0, 0}, r = b;
+      for (int d = D - 1; d >= 0; --d) {
+        iv[d] = r % sz[d];
+        r /= sz[d];
+      }
+      s->idx.insert(s->idx.end(), sorted.begin() + cnt[size_t(b)], sorted.begin() + cnt[size_t(b) + 1]);
+      const int32_t ni = int32_t(cnt[size_t(b) + 1] - cnt[size_t(b)]);
+      int64_t l3[3] = {0, 0, 0}, h3[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) {
+        l3[d] = std::max<int64_t>(iv[d] - 1, 0);
+        h3[d] = std::min<int64_t>(iv[d] + 1, sz[d] - 1);
+      }
+      for (int64_t x0 = l3[0]; x0 <= h3[0]; ++x0)
+        for (int64_t x1 = l3[1]; x1 <= h3[1]; ++x1)
+          for (int64_t x2 = l3[2]; x2 <= h3[2]; ++x2) {
+            const int64_t nbk = (x0 * sz[1] + x1) * sz[2] + x2;
+            if (nbk == b) continue;
+            s->idx.insert(s->idx.end(), sorted.begin() + cnt[size_t(nbk)],
+                          sorted.begin() + cnt[size_t(nbk) + 1]);
+          }
+      s->nint.push_back(ni);
+      s->off.push_back(int64_t(s->idx.size()));
+    }
+  }
+}
+
+// sgn: local matrices may be indefinite; factorise A = L S L^T (S = +-1)
+// instead of requiring positive pivots.
+void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2, double shift,
+               bool sgn) {
   abh_context *ctx = s->ctx;
   hipStream_t st = ctx->stream;
   const int nd = int(s->nint.size());
   if (nd == 0) return;
-  // domains by size, largest first; groups share the padded size of their
-  // first member
+  // domains by padded size, then interior count, largest first; a group
+  // shares the padded size and the interior count of its first member
   std::vector<int> order(nd);
   std::iota(order.begin(), order.end(), 0);
   auto msize = [&](int d) { return int(s->off[d + 1] - s->off[d]); };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return msize(a) > msize(b); });
+  auto npad = [&](int d) { return int(abh::ceil_div(msize(d), kNb) * kNb); };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    if (npad(a) != npad(b)) return npad(a) > npad(b);
+    return s->nint[a] > s->nint[b];
+  });
   size_t free_b = 0, total_b = 0;
   ABH_HIP(hipMemGetInfo(&free_b, &total_b));
   // inverse rows: n_pad x ni_pad per domain
@@ -306,18 +433,19 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
   {
     int i = 0;
     while (i < nd) {
-      const int n = int(abh::ceil_div(msize(order[i]), kNb) * kNb);
+      const int n = npad(order[i]);
       // working set per member: A (n^2) + panel copy (64 n) + dinv (n 64)
-      const double per = double(n) * n * 8.0 + double(n) * kNb * 16.0;
+      // (+ S L^-1 and the signs in signed mode)
+      const double per = double(n) * n * 8.0 + double(n) * kNb * (sgn ? 24.0 : 16.0) +
+                         (sgn ? double(n) * 8.0 : 0.0);
       const double budget = std::min(8.0e9, 0.35 * double(free_b));
       int gmax = int(std::max(1.0, std::min(256.0, budget / per)));
       Group gr{i, 0, n, 0};
-      while (i < nd && gr.count < gmax && int(abh::ceil_div(msize(order[i]), kNb) * kNb) == n) {
-        gr.ni = std::max(gr.ni, s->nint[order[i]]);
+      gr.ni = s->nint[order[i]];
+      while (i < nd && gr.count < gmax && npad(order[i]) == n) {
         ++gr.count;
         ++i;
       }
-      gr.ni = int(abh::ceil_div(gr.ni, kNb) * kNb);
       for (int q = 0; q < gr.count; ++q) {
         const int d = order[gr.first + q];
         dd[d].idx_off = s->off[d];
@@ -325,7 +453,7 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
         dd[d].m = msize(d);
         dd[d].ni = s->nint[d];
         dd[d].ld = n;
-        dd[d].pad = 0;
+        dd[d].failed = 0;
       }
       r_total += int64_t(gr.count) * n * gr.ni;
       groups.push_back(gr);
@@ -371,14 +499,17 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
     nmax = std::max(nmax, gr.n);
     nimax = std::max(nimax, gr.ni);
   }
-  DevBuf A, W, Tm, Dinv, fail;
+  DevBuf A, W, Tm, Dinv, DinvS, Sg, fail;
   A.ensure(size_t(gmax) * nmax * nmax * 8);
   W.ensure(size_t(gmax) * nmax * kNb * 8);
   Tm.ensure(size_t(gmax) * kNb * nimax * 8);
   Dinv.ensure(size_t(gmax) * (nmax / kNb) * kNb * kNb * 8);
+  if (sgn) {
+    DinvS.ensure(size_t(gmax) * (nmax / kNb) * kNb * kNb * 8);
+    Sg.ensure(size_t(gmax) * nmax * 8);
+  }
   fail.ensure(size_t(nd) + 64);
   ABH_HIP(hipMemsetAsync(fail.p, 0, size_t(nd), st));
-  const double *pos = c->pos.as<double>();
   const int32_t *didx = s->d_idx.as<int32_t>();
 
   for (const Group &gr : groups) {
@@ -390,6 +521,9 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
     const rocblas_stride sY = rocblas_stride(n) * ni;
     double *Ap = A.as<double>(), *Wp = W.as<double>(), *Tp = Tm.as<double>();
     double *Dp = Dinv.as<double>();
+    // signed mode: panel right factor S L^-1, signs per member (stride n)
+    double *DSp = sgn ? DinvS.as<double>() : Dp;
+    double *Sp = sgn ? Sg.as<double>() : nullptr;
     double *Y = s->R.as<double>() + dd[order[gr.first]].r_off;
     uint8_t *fl = fail.as<uint8_t>() + gr.first;
     const DomDev *gd = d_dg.as<DomDev>() + gr.first;
@@ -400,27 +534,34 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
       else k_assemble<3><<<grid, 256, 0, st>>>(gd, didx, pos, k, r2, shift, n, Ap);
       abh::check_launch();
     }
-    // blocked Cholesky, lower
+    // blocked Cholesky, lower (signed: A = L S L^T)
     for (int kb = 0; kb < K; ++kb) {
       const int o = kb * kNb;
-      k_potrf_diag<<<G, 256, 0, st>>>(Ap, n, o, Dp + int64_t(kb) * kNb * kNb, sD, fl);
+      k_potrf_diag<<<G, 256, 0, st>>>(Ap, n, o, Dp + int64_t(kb) * kNb * kNb, sD, fl,
+                                      sgn ? 1 : 0, Sp, n, DSp + int64_t(kb) * kNb * kNb);
       abh::check_launch();
       const int r = n - o - kNb;
       if (r == 0) break;
-      // panel P = A[o+64:, o:o+64] * L_kk^-T
+      double *P = Ap + o + kNb + int64_t(o) * n;
+      // panel P = A[o+64:, o:o+64] L_kk^-T (S_k)
       {
         dim3 grid(unsigned(abh::ceil_div(int64_t(r) * kNb, 256)), G);
-        k_copy_block<<<grid, 256, 0, st>>>(Ap + o + kNb + int64_t(o) * n, sA, n, Wp, sW, n, r,
-                                           kNb);
+        k_copy_block<<<grid, 256, 0, st>>>(P, sA, n, Wp, sW, n, r, kNb);
         abh::check_launch();
       }
       ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, kNb, kNb, &one, Wp, n, sW,
-                                             Dp + int64_t(kb) * kNb * kNb, kNb, sD, &zero,
-                                             Ap + o + kNb + int64_t(o) * n, n, sA, G));
-      // trailing A[o+64:, o+64:] -= P P^T
-      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, r, kNb, &mone,
-                                             Ap + o + kNb + int64_t(o) * n, n, sA,
-                                             Ap + o + kNb + int64_t(o) * n, n, sA, &one,
+                                             DSp + int64_t(kb) * kNb * kNb, kNb, sD, &zero, P,
+                                             n, sA, G));
+      // trailing A[o+64:, o+64:] -= P (S_k) P^T
+      const double *Pl = P;
+      if (sgn) {
+        dim3 grid(unsigned(abh::ceil_div(int64_t(r) * kNb, 256)), G);
+        k_copy_block<<<grid, 256, 0, st>>>(P, sA, n, Wp, sW, n, r, kNb, Sp + o, n);
+        abh::check_launch();
+        Pl = Wp;
+      }
+      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, r, kNb, &mone, Pl, n,
+                                             sgn ? sW : sA, P, n, sA, &one,
                                              Ap + o + kNb + int64_t(o + kNb) * n, n, sA, G));
     }
     // Y = L^-1 E (E = first ni unit columns); Y is lower trapezoidal
@@ -446,7 +587,12 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
                                                Ap + o + kNb + int64_t(o) * n, n, sA, Y + o, n,
                                                sY, &one, Y + o + kNb, n, sY, G));
     }
-    // Z = L^-T Y, in place
+    // Z = L^-T (S) Y, in place
+    if (sgn) {
+      dim3 grid(unsigned(abh::ceil_div(int64_t(n) * ni, 256)), G);
+      k_scale_rows<<<grid, 256, 0, st>>>(Y, sY, n, ni, Sp, n);
+      abh::check_launch();
+    }
     for (int kb = K - 1; kb >= 0; --kb) {
       const int o = kb * kNb;
       const int r = n - o - kNb;
@@ -468,6 +614,14 @@ void factorise(abh_schwarz *s, const abh_cells *c, const KernelDesc &k, double r
   ABH_HIP(hipMemcpyAsync(fh.data(), fail.p, size_t(nd), hipMemcpyDeviceToHost, st));
   ABH_HIP(hipStreamSynchronize(st));
   s->failed = std::accumulate(fh.begin(), fh.end(), int64_t(0));
+  if (s->failed) {
+    // a broken-down domain contributes nothing (its interior rows keep b)
+    // rather than non-finite values
+    for (int i = 0; i < nd; ++i) dd[order[i]].failed = fh[i] ? 1 : 0;
+    ABH_HIP(hipMemcpyAsync(s->d_doms.p, dd.data(), size_t(nd) * sizeof(DomDev),
+                           hipMemcpyHostToDevice, st));
+    ABH_HIP(hipStreamSynchronize(st));
+  }
 }
 
 }  // namespace
@@ -491,6 +645,26 @@ void schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
 }
 
 int64_t schwarz_size(const abh_schwarz *s) { return s->n; }
+
+abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<int64_t> &off,
+                                  const double *pos_dev, const double *lo, const double *hi,
+                                  double n_leaf, const KernelDesc &k, double shift) {
+  std::unique_ptr<abh_schwarz> s(new abh_schwarz());
+  s->ctx = ctx;
+  s->dim = dim;
+  s->n = off.back();
+  ABH_REQUIRE(s->n < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "batch too large for int32 indices");
+  std::vector<double> pos(size_t(s->n) * dim);
+  hipStream_t hs = ctx->stream;
+  if (s->n > 0)
+    ABH_HIP(hipMemcpyAsync(pos.data(), pos_dev, pos.size() * 8, hipMemcpyDeviceToHost, hs));
+  ABH_HIP(hipStreamSynchronize(hs));
+  analyse_batch(s.get(), dim, off, pos, lo, hi, n_leaf);
+  for (size_t d = 0; d + 1 < s->off.size(); ++d)
+    s->max_m = std::max(s->max_m, int(s->off[d + 1] - s->off[d]));
+  factorise(s.get(), pos_dev, k, -1.0, shift, true);
+  return s.release();
+}
 
 }  // namespace abh
 
@@ -531,7 +705,7 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
     k.param2 = kernel->param * kernel->param;
     k.w0 = 1.0;
     const double r2 = radius > 0.0 ? radius * radius : -1.0;
-    factorise(s, cells, k, r2, diag_shift);
+    factorise(s, cells->pos.as<double>(), k, r2, diag_shift, false);
   });
   if (st != ABH_OK) {
     delete s;

@@ -2,7 +2,10 @@
 // translation units (search.hip, schwarz.hip, learned.hip, krylov.hip).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
+#include "kernel_fn.hpp"
 
 namespace abh {
 
@@ -15,6 +18,11 @@ abh_context *sparse_context(const abh_sparse_op *op);
 // x = M^-1 b, restricted additive Schwarz (schwarz.hip).
 void schwarz_apply(const abh_schwarz *s, const double *b, double *x);
 int64_t schwarz_size(const abh_schwarz *s);
+// Batched restricted additive Schwarz over independent particle sets
+// (schwarz.hip): signed factorisation, no kernel truncation.  Throws abh::Error.
+abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<int64_t> &off,
+                                  const double *pos_dev, const double *lo, const double *hi,
+                                  double n_leaf, const KernelDesc &k, double shift);
 
 // x = W b with W the approximate-inverse rows (learned.hip).
 void ainv_apply(const abh_ainv *a, const double *b, double *x);

@@ -54,7 +54,8 @@ def normal_rhs(n, nrhs=8, seed=2001):
 
 
 FMM = dict(n=2_000_000, n_leaf=64, order=6, eps=1e-3, nrhs=16, n_check=10_000, seed=3000)
-RBF_BATCH = dict(nprob=512, n_min=1000, n_max=64000, c=0.05, restart=50, tol=1e-8, seed=4000)
+RBF_BATCH = dict(nprob=512, n_min=1000, n_max=64000, c=0.05, restart=50, tol=1e-8, seed=4000,
+                 schwarz_n_leaf=64.0)
 SCHWARZ = dict(nx=632, ny=633, jitter=0.3, sigma=0.01, cutoff=6.0, shift=1e-5, n_leaf=576.0,
                k=32, seed=5000)
 

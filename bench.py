@@ -249,8 +249,55 @@ def bench_rbf(args, torch, ab, wl, ctx, timer, extra):
         log("rbf %s: matvec %.1f ms, %d iters in %.1f s, converged %d/%d, residual median "
             "%.2e max %.2e" % (name, mv, args.rbf_iters, solve_s, conv, c["nprob"],
                                np.median(res), res.max()))
+        out[name]["converged_by_size"] = by_size(sizes, stats)
+    # restricted additive Schwarz per problem (signed local factorisations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sw = ab.SchwartzPreconditioner.for_dense_batch(op, [0.0, 0.0], [1.0, 1.0],
+                                                   c["schwarz_n_leaf"])
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    info = sw.info
+    xs = torch.empty_like(b)
+    ap = timer(lambda: sw.apply(b, xs))
+    out["schwarz"] = {"n_leaf": c["schwarz_n_leaf"], "setup_s": setup_s, "apply_ms": ap,
+                      "n_domains": info["n_domains"], "max_size": info["max_size"],
+                      "factor_bytes": info["factor_bytes"],
+                      "failed_domains": info["failed_domains"]}
+    log("rbf schwarz: %d domains (max %d), %.1f GB inverse rows, setup %.1f s, apply %.2f ms"
+        % (info["n_domains"], info["max_size"], info["factor_bytes"] / 1e9, setup_s, ap))
+    for prec, name in ((ab.FP64, "fp64_schwarz"),):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, stats = op.gmres(b, restart=c["restart"], max_iters=args.rbf_iters, tol=c["tol"],
+                            precision=prec, preconditioner=sw)
+        torch.cuda.synchronize()
+        solve_s = time.perf_counter() - t0
+        res = np.array([s["residual"] for s in stats])
+        conv = int(sum(s["converged"] for s in stats))
+        its = np.array([s["iterations"] for s in stats])
+        out[name] = {"solve_s": solve_s, "converged": conv,
+                     "residual_median": float(np.median(res)), "residual_max": float(res.max()),
+                     "iterations_converged_max": int(its[[s["converged"] for s in stats]].max())
+                     if conv else None,
+                     "converged_by_size": by_size(sizes, stats)}
+        log("rbf %s: %d iters in %.1f s, converged %d/%d, residual median %.2e max %.2e, %s"
+            % (name, args.rbf_iters, solve_s, conv, c["nprob"], np.median(res), res.max(),
+               out[name]["converged_by_size"]))
+    sw.close()
     op.close()
     extra["rbf_batch"] = out
+
+
+def by_size(sizes, stats):
+    """Converged / total problems per size octave [1000, 2000), [2000, 4000) ..."""
+    edges = [1000, 2000, 4000, 8000, 16000, 32000, 64001]
+    out = {}
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = [i for i, n in enumerate(sizes) if lo <= n < hi]
+        out["%d-%d" % (lo, hi - 1 if hi == 64001 else hi)] = "%d/%d" % (
+            sum(stats[i]["converged"] for i in sel), len(sel))
+    return out
 
 
 def bench_schwarz(torch, ab, wl, ctx, extra):

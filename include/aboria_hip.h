@@ -338,13 +338,14 @@ abh_status abh_dense_batch_gmres(abh_dense_batch *op, const double *b, double *x
 typedef struct abh_schwarz abh_schwarz;
 
 typedef struct abh_schwarz_info {
-  int64_t n;               /* rows: particles of the cell list             */
+  int64_t n;               /* rows: particles of the cell list (batch)     */
   int64_t n_domains;       /* domains with a non-empty interior            */
   int64_t min_size, max_size;         /* interior + buffer particles       */
   int64_t min_interior, max_interior; /* interior particles                */
   int64_t factor_bytes;    /* device bytes of the stored inverse rows      */
   int64_t failed_domains;  /* local Cholesky factorisations that met a
-                              non-positive pivot                           */
+                              non-positive pivot (a zero pivot for the
+                              signed factorisation of a dense batch)       */
 } abh_schwarz_info;
 
 /* Replaces SchwartzPreconditioner<Eigen::LLT<MatrixXd>>::analyzePattern +
@@ -376,6 +377,36 @@ abh_status abh_schwarz_get_domains(const abh_schwarz *s, int64_t *offsets,
  * src/Preconditioners.h:1476-1522): x = b, then every domain overwrites its
  * interior rows with its local solution.  b, x: device, length n, distinct. */
 abh_status abh_schwarz_apply(const abh_schwarz *s, const double *b, double *x);
+
+/* SchwartzPreconditioner::analyzePattern + factorize
+ * (src/Preconditioners.h:1169-1316, :1402-1467) applied to every problem of
+ * a dense batch (the reference builds one per particle set): the operator is
+ * K + diag_shift * I with the batch's kernel over all pairs of the problem.
+ * Problem p gets the bucket grid CellListOrdered would size for its particle
+ * count over [low, high] (src/CellListOrdered.h:132-186); one domain per
+ * non-empty bucket, interior = the bucket's particles (membership by
+ * floor((x - low) / side), clamped to the grid), buffer = the particles of
+ * the other buckets of the 3^dim block around it.  Every particle must lie
+ * in [low, high] (host arrays of dim entries).  The local factorisation is
+ * A = L S L^T with S = diag(+-1), the Cholesky factorisation when A is
+ * positive definite, so that indefinite kernels such as the multiquadric
+ * are handled (the reference's LLT meets negative pivots there);
+ * failed_domains counts zero pivots.  Domain indices are global (offsets
+ * of the batch).  Apply with abh_schwarz_apply over the batch vector, free
+ * with abh_schwarz_destroy. */
+abh_status abh_dense_batch_schwarz_create(abh_dense_batch *op, const double *low,
+                                          const double *high,
+                                          double n_particles_in_leaf,
+                                          double diag_shift, abh_schwarz **out);
+/* abh_dense_batch_gmres right-preconditioned by prec (from
+ * abh_dense_batch_schwarz_create; NULL = none): Eigen::GMRES<...,
+ * SchwartzPreconditioner> (tests/rbf_interpolation.h:358-369) for every
+ * problem.  Convergence is judged on the true residual ||b - K x||. */
+abh_status abh_dense_batch_gmres_precond(abh_dense_batch *op,
+                                         const abh_schwarz *prec,
+                                         const double *b, double *x,
+                                         const abh_gmres_options *opts,
+                                         abh_gmres_stats *stats);
 
 /* ------------------------------------------------------------------ */
 /* learned approximate-inverse preconditioner                          */
