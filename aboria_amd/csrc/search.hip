@@ -540,6 +540,9 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         const double r2m = m != 0 ? a.r2 : -1.0;
         const CDouble *pj = cpos + int64_t(beg) * D;
         int j = beg;
+        // one counter per unrolled candidate: each compare feeds a
+        // carry-in add instead of a packed bit count
+        int c4[4] = {0, 0, 0, 0};
         for (; j + 4 <= end; j += 4, pj += 4 * D) {
           double e[4][3];
 #pragma unroll
@@ -554,7 +557,48 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
               const double dx = e[u][d] - qi[d];
               acc = acc + dx * dx;
             }
-            cnt += (acc <= r2m) ? 1 : 0;
+            c4[u] += (acc <= r2m) ? 1 : 0;
+          }
+        }
+        cnt += (c4[0] + c4[1]) + (c4[2] + c4[3]);
+        for (; j < end; ++j, pj += D) {
+          double acc = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const double dx = pj[d] - qi[d];
+            acc = acc + dx * dx;
+          }
+          cnt += (acc <= r2m) ? 1 : 0;
+        }
+      } else if ((MODE == kFill || MODE == kFillPadded) && __ballot(m > 1) == 0) {
+        // every lane visits this bucket at most once: a hit appends one
+        // index (the count is a carry-in add, the address is formed only
+        // under the hit predicate)
+        const double r2m = m != 0 ? a.r2 : -1.0;
+        const CDouble *pj = cpos + int64_t(beg) * D;
+        int32_t *const row0 = a.col + out;
+        int j = beg;
+        for (; j + 4 <= end; j += 4, pj += 4 * D) {
+          double e[4][3];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int d = 0; d < D; ++d) e[u][d] = pj[u * D + d];
+          bool h[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double dx = e[u][d] - qi[d];
+              acc = acc + dx * dx;
+            }
+            h[u] = acc <= r2m;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (h[u]) row0[cnt] = j + u;
+            cnt += h[u] ? 1 : 0;
           }
         }
         for (; j < end; ++j, pj += D) {
@@ -564,7 +608,9 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
             const double dx = pj[d] - qi[d];
             acc = acc + dx * dx;
           }
-          cnt += (acc <= r2m) ? 1 : 0;
+          const bool h = acc <= r2m;
+          if (h) row0[cnt] = j;
+          cnt += h ? 1 : 0;
         }
       } else if (MODE == kCount || kList) {
         const double r2m = m != 0 ? a.r2 : -1.0;
@@ -844,7 +890,11 @@ static size_t lds_bytes(int nslots, bool matvec) {
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
 template <int MODE> static void launch_search(const QueryPlan &p, const SearchArgs &a0) {
-  const unsigned grid = unsigned(std::min<int64_t>(ceil_div(p.nq, kGroup), int64_t(p.ctx->n_cu) * 32));
+  // persistent waves per CU: the list passes run best at 16 (measured on
+  // the 4M-particle mixture: 9.4 / 14.3 ms at 16 against 10.0 / 15.6 ms at
+  // 32; 8 leaves the scalar-load latency exposed)
+  const int wpc = (MODE == kMatvec64 || MODE == kMatvec32) ? 32 : 16;
+  const unsigned grid = unsigned(std::min<int64_t>(ceil_div(p.nq, kGroup), int64_t(p.ctx->n_cu) * wpc));
   if (grid == 0) return;
   SearchArgs a = a0;
   a.next = p.sched.as<uint32_t>();
