@@ -71,28 +71,52 @@ template <int D> __device__ __forceinline__ int home_of(const GridParams &g, con
   return int(t);
 }
 
+// Sort key of a centre: its virtual home cell, then (levels > 0) the Morton
+// code of its sub-cell on a 2^levels lattice inside the home cell, so that
+// consecutive centres of a crowded cell are close together (the radius
+// test culls candidates against the bounding box of a work item).
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_home_keys(const double *__restrict__ q, int64_t n,
-                                                        GridParams g, uint32_t *__restrict__ keys,
+                                                        GridParams g, int levels,
+                                                        uint32_t *__restrict__ keys,
                                                         int32_t *__restrict__ vals) {
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
   double p[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) p[d] = q[i * D + d];
-  uint32_t key = 0;
+  uint32_t key = 0, code = 0;
+  int h[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d)
-    key = key * uint32_t(home_span(g, d)) + uint32_t(home_of<D>(g, p, d) - home_lo(g, d));
+  for (int d = 0; d < D; ++d) {
+    h[d] = home_of<D>(g, p, d);
+    key = key * uint32_t(home_span(g, d)) + uint32_t(h[d] - home_lo(g, d));
+  }
+  if (levels > 0) {
+    const int side = 1 << levels;
+    int sub[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double f = ((p[d] - g.lo[d]) * g.inv[d] - double(h[d])) * double(side);
+      int v = f >= 0.0 ? int(f) : 0;  // NaN and rounding below the cell -> 0
+      sub[d] = v < side ? v : side - 1;
+    }
+    for (int l = levels - 1; l >= 0; --l)
+#pragma unroll
+      for (int d = 0; d < D; ++d) code = (code << 1) | uint32_t((sub[d] >> l) & 1);
+    key = (key << (levels * D)) | code;
+  }
   keys[i] = key;
   if (vals) vals[i] = int32_t(i);
 }
 
+// runs of equal home cell (the key without its sub-cell bits)
 __global__ __launch_bounds__(kThreads) void k_run_flags(const uint32_t *__restrict__ keys,
-                                                        int64_t n, uint32_t *__restrict__ flags) {
+                                                        int64_t n, int shift,
+                                                        uint32_t *__restrict__ flags) {
   const int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (k >= n) return;
-  flags[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1u : 0u;
+  flags[k] = (k == 0 || (keys[k] >> shift) != (keys[k - 1] >> shift)) ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(kThreads) void k_run_starts(const uint32_t *__restrict__ flags,
@@ -309,6 +333,8 @@ struct SearchArgs {
   double *val;  // assemble: kernel value of every stored pair
   uint32_t *escape_flag;
   uint32_t *next;  // dynamic work counter, zero at launch
+  int ring;        // fill: stage each row's entries in LDS and store whole
+                   // 64-byte segments (col is 64-byte aligned)
   // matvec
   KernelDesc k;
   int nrhs;
@@ -317,6 +343,19 @@ struct SearchArgs {
   double *y;
   int64_t ldy;
 };
+
+#ifndef ABH_STAGE_UNROLL
+#define ABH_STAGE_UNROLL 4
+#endif
+// staged candidates tested per step of the count / list passes
+constexpr int kStageUnroll = ABH_STAGE_UNROLL;
+
+// Fill output ring: the last 32 entries of each lane's row (two 16-entry,
+// 64-byte segments); a lane's ring is 36 words so that its segments stay
+// 16-byte aligned for vector LDS reads.
+constexpr int kRing = 32;
+constexpr int kRingStride = 36;
+constexpr int kSeg = 16;
 
 // Staged candidates, structure of arrays: positions, stencil slot and
 // container index of up to kTile candidates.
@@ -330,8 +369,64 @@ struct Tile {
 
 typedef __attribute__((address_space(4))) const double CDouble;
 
+// a wave-uniform double moved to scalar registers
+__device__ __forceinline__ double uniform_f64(double v) {
+  const uint64_t b = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b))));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(b >> 32))));
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+__device__ __forceinline__ double wave_min_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// q + I*L for the image digit I + 1 (the product is exact: I is -1, 0, +1)
+__device__ __forceinline__ double image_of(int digit, double q, double L) {
+  return digit == 1 ? q : q + double(digit - 1) * L;
+}
+
 __device__ __forceinline__ double pick3(int digit, double m, double z, double p) {
   return digit == 0 ? m : (digit == 1 ? z : p);
+}
+
+// Action of the count and list passes on one tested pair (candidate j of a
+// lane's centre, squared distance acc): count it m times, or append j m
+// times (with the kernel value when assembling).  single: no lane of the
+// wave visits the bucket more than once, so a hit appends one entry.
+template <int MODE>
+__device__ __forceinline__ void staged_hit(const SearchArgs &a, bool single, bool hit, double acc,
+                                           int j, int m, double wrow, int64_t out, int &cnt) {
+  if (MODE == kCount) {
+    if (single) cnt += hit ? 1 : 0;
+    else cnt += hit ? m : 0;
+  } else if (MODE == kFill || MODE == kFillPadded) {
+    if (single) {
+      if (hit) a.col[out + cnt] = j;
+      cnt += hit ? 1 : 0;
+    } else if (hit) {
+      for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j;
+      cnt += m;
+    }
+  } else if (MODE == kAssemble) {
+    if (hit) {
+      double w = wrow;
+      if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j];
+      const double f = phi64(a.k, acc) * w;
+      for (int rep = 0; rep < m; ++rep) {
+        a.col[out + cnt + rep] = j;
+        a.val[out + cnt + rep] = f;
+      }
+      cnt += m;
+    }
+  }
 }
 
 template <int D, int MODE>
@@ -346,6 +441,11 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
                     : reinterpret_cast<Slot *>(lds);
   int32_t *sorder = reinterpret_cast<int32_t *>(slots + S);
   uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + S);  // [S][64]
+  // count / list passes: staged candidates (coordinates [D][64], index [64])
+  double *st_x = reinterpret_cast<double *>(
+      lds + ((size_t(S) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(S + 1) + 15) & ~size_t(15)));
+  int32_t *st_j = reinterpret_cast<int32_t *>(st_x + 3 * kGroup);
+  uint32_t *ring = reinterpret_cast<uint32_t *>(st_j + kGroup);  // [64][36], fill only
   const int lane = threadIdx.x;
   const int nwork = int(*a.nwork);
   int wd[3] = {a.w[0], a.w[1], a.w[2]};
@@ -372,14 +472,18 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     // a centre with a non-finite coordinate has no neighbours
     const bool active = lane < it.count && finite;
     const bool has_row = lane < it.count;
-    const int64_t row = a.qperm ? int64_t(a.qperm[qk]) : int64_t(qk);
-    // image points q + I*L for I = -1, 0, +1 (search_iterator, src/Search.h:188-190)
-    double qm[D], qp[D];
+    // box of the work item's active centres (empty when none is active)
+    double qlo[3] = {0.0, 0.0, 0.0}, qhi[3] = {0.0, 0.0, 0.0};
+    if (MODE == kCount || MODE == kFill || MODE == kAssemble || MODE == kFillPadded) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      qm[d] = q[d] + double(-1) * g.L[d];
-      qp[d] = q[d] + double(1) * g.L[d];
+      for (int d = 0; d < D; ++d) {
+        qlo[d] = uniform_f64(wave_min_f64(active ? q[d] : INFINITY));
+        qhi[d] = uniform_f64(wave_max_f64(active ? q[d] : -INFINITY));
+      }
     }
+    const int64_t row = a.qperm ? int64_t(a.qperm[qk]) : int64_t(qk);
+    // image points q + I*L for I = -1, 0, +1 (search_iterator,
+    // src/Search.h:188-190), formed where needed by image_of
 
     // stencil slots: virtual cell v = home + o -> bucket b under image I
     for (int s = lane; s < S; s += kGroup) {
@@ -455,7 +559,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         for (int d = 0; d < D; ++d) pk |= (I[d] + 1) << (2 * d);
         double qi[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) qi[d] = pick3(I[d] + 1, qm[d], q[d], qp[d]);
+        for (int d = 0; d < D; ++d) qi[d] = image_of(I[d] + 1, q[d], g.L[d]);
         const bool in_range = active && !outside_domain<D>(g, qi, a.r2);
         if (__ballot(in_range) == 0) continue;
         QuadState<D> qs;
@@ -500,6 +604,37 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     int64_t out = 0;
     constexpr bool kList = (MODE == kFill || MODE == kAssemble || MODE == kFillPadded);
     if (kList && active) out = a.row_ptr[row];
+    // ring state: entries [fpos, out + cnt) of this lane's row are staged
+    int64_t fpos = out;
+    const bool use_ring = (MODE == kFill || MODE == kFillPadded) && a.ring;
+    uint32_t *rl = ring + lane * kRingStride;
+    // store the segment holding fpos once it is complete (64-byte vector
+    // stores when it starts at its segment boundary, the row's first,
+    // partial segment entry by entry)
+    auto ring_segment = [&]() {
+      const int64_t pos = out + cnt;
+      const int64_t seg = fpos & ~int64_t(kSeg - 1);
+      if (pos >= seg + kSeg) {
+        if (fpos == seg) {
+          const uint4 *src = reinterpret_cast<const uint4 *>(rl + int(seg & (kRing - 1)));
+          uint4 *dst = reinterpret_cast<uint4 *>(a.col + seg);
+          const uint4 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
+          dst[0] = v0;
+          dst[1] = v1;
+          dst[2] = v2;
+          dst[3] = v3;
+        } else {
+          for (int64_t e = fpos; e < seg + kSeg; ++e) a.col[e] = int32_t(rl[int(e & (kRing - 1))]);
+        }
+        fpos = seg + kSeg;
+      }
+    };
+    // store everything staged (row end, or before a bucket written directly)
+    auto ring_drain = [&]() {
+      const int64_t pos = out + cnt;
+      for (int64_t e = fpos; e < pos; ++e) a.col[e] = int32_t(rl[int(e & (kRing - 1))]);
+      fpos = pos;
+    };
     double acc64[kMaxRhs];
     float acc32[kMaxRhs], cmp32[kMaxRhs];
     if (kMV) {
@@ -529,138 +664,141 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       const int img = abh::uniform_i32(slots[s].img);
       double qi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-      for (int d = 0; d < D; ++d) qi[d] = pick3((img >> (2 * d)) & 3, qm[d], q[d], qp[d]);
+      for (int d = 0; d < D; ++d) qi[d] = image_of((img >> (2 * d)) & 3, q[d], g.L[d]);
       const int end = beg + len;
 
-      if (MODE == kCount && __ballot(m > 1) == 0) {
-        // every lane visits this bucket at most once: fold the multiplicity
-        // into the radius test.  acc <= r2 equals the reference's
-        // !(acc > r2) for every finite centre (acc is never NaN there);
-        // lanes with m == 0 compare against -1.
-        const double r2m = m != 0 ? a.r2 : -1.0;
-        const CDouble *pj = cpos + int64_t(beg) * D;
-        int j = beg;
-        // one counter per unrolled candidate: each compare feeds a
-        // carry-in add instead of a packed bit count
-        int c4[4] = {0, 0, 0, 0};
-        for (; j + 4 <= end; j += 4, pj += 4 * D) {
-          double e[4][3];
+      if (MODE == kCount || kList) {
+        // Candidates of this bucket in blocks of 64: each lane loads one and
+        // keeps it if it lies within r of the work item's query box (the
+        // same FP64 operations as the radius test, monotone in the query, so
+        // no accepted pair is dropped); the kept ones are compacted into LDS
+        // in index order and tested against every lane's centre.  acc <= r2
+        // equals the reference's !(acc > r2) for every finite centre (acc is
+        // never NaN there); lanes with m == 0 compare against -1.
+        double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int d = 0; d < D; ++d) e[u][d] = pj[u * D + d];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            double acc = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              const double dx = e[u][d] - qi[d];
-              acc = acc + dx * dx;
-            }
-            c4[u] += (acc <= r2m) ? 1 : 0;
-          }
+        for (int d = 0; d < D; ++d) {
+          const int dg = (img >> (2 * d)) & 3;
+          blo[d] = uniform_f64(
+              pick3(dg, qlo[d] + double(-1) * g.L[d], qlo[d], qlo[d] + double(1) * g.L[d]));
+          bhi[d] = uniform_f64(
+              pick3(dg, qhi[d] + double(-1) * g.L[d], qhi[d], qhi[d] + double(1) * g.L[d]));
         }
-        cnt += (c4[0] + c4[1]) + (c4[2] + c4[3]);
-        for (; j < end; ++j, pj += D) {
-          double acc = 0.0;
+        const double r2m = m != 0 ? a.r2 : -1.0;
+        const bool single = __ballot(m > 1) == 0;
+        for (int c0 = beg; c0 < end; c0 += kGroup) {
+          const int jc = c0 + lane;
+          bool keep = jc < end;
+          double x[3] = {0.0, 0.0, 0.0};
+          if (keep) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) x[d] = a.pos[int64_t(jc) * D + d];
+          }
+          double dd = 0.0;
 #pragma unroll
           for (int d = 0; d < D; ++d) {
-            const double dx = pj[d] - qi[d];
-            acc = acc + dx * dx;
+            double e = 0.0;
+            if (x[d] < blo[d]) e = blo[d] - x[d];
+            else if (x[d] > bhi[d]) e = x[d] - bhi[d];
+            dd = dd + e * e;
           }
-          cnt += (acc <= r2m) ? 1 : 0;
-        }
-      } else if ((MODE == kFill || MODE == kFillPadded) && __ballot(m > 1) == 0) {
-        // every lane visits this bucket at most once: a hit appends one
-        // index (the count is a carry-in add, the address is formed only
-        // under the hit predicate)
-        const double r2m = m != 0 ? a.r2 : -1.0;
-        const CDouble *pj = cpos + int64_t(beg) * D;
-        int32_t *const row0 = a.col + out;
-        int j = beg;
-        for (; j + 4 <= end; j += 4, pj += 4 * D) {
-          double e[4][3];
+          keep = keep && dd <= a.r2;
+          const uint64_t km = __ballot(keep);
+          if (km == 0) continue;
+          const int nk = __popcll(km);
+          if (keep) {
+            const int sl = int(__builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u)));
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int d = 0; d < D; ++d) e[u][d] = pj[u * D + d];
-          bool h[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            double acc = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              const double dx = e[u][d] - qi[d];
-              acc = acc + dx * dx;
-            }
-            h[u] = acc <= r2m;
+            for (int d = 0; d < D; ++d) st_x[d * kGroup + sl] = x[d];
+            st_j[sl] = jc;
           }
+          __syncthreads();  // single-wave group: the staged block is complete
+          if (use_ring && single) {
+            // at most 16 appends per lane between segment stores, so the
+            // ring never holds more than 32 unstored entries
+            for (int t0 = 0; t0 < nk; t0 += kSeg) {
+              const int t1 = nk - t0 < kSeg ? nk : t0 + kSeg;
+              int t = t0;
+              for (; t + kStageUnroll <= t1; t += kStageUnroll) {
+                double e[kStageUnroll][3];
+                int jj[kStageUnroll];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (h[u]) row0[cnt] = j + u;
-            cnt += h[u] ? 1 : 0;
-          }
-        }
-        for (; j < end; ++j, pj += D) {
-          double acc = 0.0;
+                for (int u = 0; u < kStageUnroll; ++u) {
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            const double dx = pj[d] - qi[d];
-            acc = acc + dx * dx;
-          }
-          const bool h = acc <= r2m;
-          if (h) row0[cnt] = j;
-          cnt += h ? 1 : 0;
-        }
-      } else if (MODE == kCount || kList) {
-        const double r2m = m != 0 ? a.r2 : -1.0;
-        const CDouble *pj = cpos + int64_t(beg) * D;
-        int j = beg;
-        for (; j + 4 <= end; j += 4, pj += 4 * D) {
-          double e[4][3];
+                  for (int d = 0; d < D; ++d) e[u][d] = st_x[d * kGroup + t + u];
+                  jj[u] = st_j[t + u];
+                }
+                bool h[kStageUnroll];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < kStageUnroll; ++u) {
+                  double acc = 0.0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) e[u][d] = pj[u * D + d];
+                  for (int d = 0; d < D; ++d) {
+                    const double dx = e[u][d] - qi[d];
+                    acc = acc + dx * dx;
+                  }
+                  h[u] = acc <= r2m;
+                }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            double acc = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              const double dx = e[u][d] - qi[d];
-              acc = acc + dx * dx;
-            }
-            if (acc <= r2m) {
-              if (kList)
-                for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j + u;
-              if (MODE == kAssemble) {
-                double w = wrow;
-                if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j + u];
-                const double f = phi64(a.k, acc) * w;
-                for (int rep = 0; rep < m; ++rep) a.val[out + cnt + rep] = f;
+                for (int u = 0; u < kStageUnroll; ++u) {
+                  if (h[u]) rl[int((out + cnt) & (kRing - 1))] = uint32_t(jj[u]);
+                  cnt += h[u] ? 1 : 0;
+                }
               }
-              cnt += m;
-            }
-          }
-        }
-        for (; j < end; ++j, pj += D) {
-          double acc = 0.0;
+              for (; t < t1; ++t) {
+                double acc = 0.0;
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            const double dx = pj[d] - qi[d];
-            acc = acc + dx * dx;
-          }
-          if (acc <= r2m) {
-            if (kList)
-              for (int rep = 0; rep < m; ++rep) a.col[out + cnt + rep] = j;
-            if (MODE == kAssemble) {
-              double w = wrow;
-              if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j];
-              const double f = phi64(a.k, acc) * w;
-              for (int rep = 0; rep < m; ++rep) a.val[out + cnt + rep] = f;
+                for (int d = 0; d < D; ++d) {
+                  const double dx = st_x[d * kGroup + t] - qi[d];
+                  acc = acc + dx * dx;
+                }
+                if (acc <= r2m) {
+                  rl[int((out + cnt) & (kRing - 1))] = uint32_t(st_j[t]);
+                  ++cnt;
+                }
+              }
+              ring_segment();
             }
-            cnt += m;
+            __syncthreads();  // the next block overwrites the stage
+            continue;
           }
+          if (use_ring) ring_drain();  // several visits of one bucket: direct stores
+          int t = 0;
+          for (; t + kStageUnroll <= nk; t += kStageUnroll) {
+            double e[kStageUnroll][3];
+            int jj[kStageUnroll];
+#pragma unroll
+            for (int u = 0; u < kStageUnroll; ++u) {
+#pragma unroll
+              for (int d = 0; d < D; ++d) e[u][d] = st_x[d * kGroup + t + u];
+              jj[u] = st_j[t + u];
+            }
+            double acc[kStageUnroll];
+#pragma unroll
+            for (int u = 0; u < kStageUnroll; ++u) {
+              acc[u] = 0.0;
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const double dx = e[u][d] - qi[d];
+                acc[u] = acc[u] + dx * dx;
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < kStageUnroll; ++u)
+              staged_hit<MODE>(a, single, acc[u] <= r2m, acc[u], jj[u], m, wrow, out, cnt);
+          }
+          for (; t < nk; ++t) {
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              const double dx = st_x[d * kGroup + t] - qi[d];
+              acc = acc + dx * dx;
+            }
+            staged_hit<MODE>(a, single, acc <= r2m, acc, st_j[t], m, wrow, out, cnt);
+          }
+          if (use_ring) fpos = out + cnt;
+          __syncthreads();  // the next block overwrites the stage
         }
       } else {
         // matvec: chunks of 64 candidates; the radius test streams the
@@ -728,6 +866,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
         }
       }
     }
+    if (use_ring) ring_drain();
     if (has_row) {
       if (MODE == kCount || MODE == kFillPadded) a.counts[row] = cnt;
       if (MODE == kMatvec64) {
@@ -759,14 +898,16 @@ struct QueryPlan {
   const abh_cells *cells = nullptr;
   int D = 3;
   int64_t nq = 0;
-  bool self = false;
+  bool self = false;      // centres are the cell list's own positions
+  bool permuted = false;  // centres were sorted (perm / qbuf hold the order)
+  int sub_levels = 0;     // Morton sub-cell levels inside a home cell
   double radius = 0, r2 = 0;
   int w[3] = {0, 0, 0};
   int nslots = 1;
   const double *q_sorted = nullptr;
   DevBuf qbuf, perm, keys, flags, ridx, run_start, chunks, choff, work, counters, escape, sched;
 
-  const int32_t *qperm() const { return self ? nullptr : perm.as<int32_t>(); }
+  const int32_t *qperm() const { return permuted ? perm.as<int32_t>() : nullptr; }
 };
 
 static void set_stencil(QueryPlan &p, int extra) {
@@ -801,23 +942,28 @@ template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
   const unsigned blocks = unsigned(ceil_div(n, kThreads));
   p.keys.ensure(size_t(n + 1) * 4);
   uint32_t *keys = p.keys.as<uint32_t>();
-  if (p.self) {
+  uint64_t maxkey = 1;
+  for (int d = 0; d < D; ++d)
+    maxkey *= uint64_t(g.periodic[d] ? 3 * g.size[d] : g.size[d] + 2);
+  int bits = 0;
+  while (bits < 64 && ((maxkey - 1) >> bits) != 0) ++bits;
+  // sub-cell levels that fit the 32-bit key (at most 2: 4^D sub-cells)
+  int levels = 0;
+  while (levels < 2 && bits + (levels + 1) * D <= 32) ++levels;
+  p.sub_levels = levels;
+  p.permuted = !p.self || levels > 0;
+  if (!p.permuted) {
     p.q_sorted = q;
-    k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, keys, nullptr);
+    k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, 0, keys, nullptr);
     check_launch();
   } else {
-    p.perm.ensure(size_t(n + 1) * 4);
-    k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, keys, p.perm.as<int32_t>());
-    check_launch();
-    uint64_t maxkey = 1;
-    for (int d = 0; d < D; ++d)
-      maxkey *= uint64_t(g.periodic[d] ? 3 * g.size[d] : g.size[d] + 2);
-    ABH_REQUIRE(maxkey <= (uint64_t(1) << 32), ABH_ERR_UNSUPPORTED,
+    ABH_REQUIRE(bits <= 32, ABH_ERR_UNSUPPORTED,
                 "too many buckets to group arbitrary centres; search from the particle "
                 "positions themselves");
-    int bits = 0;
-    while (bits < 32 && ((maxkey - 1) >> bits) != 0) ++bits;
-    radix_sort_pairs(ctx, keys, p.perm.as<int32_t>(), n, bits);
+    p.perm.ensure(size_t(n + 1) * 4);
+    k_home_keys<D><<<blocks, kThreads, 0, s>>>(q, n, g, levels, keys, p.perm.as<int32_t>());
+    check_launch();
+    radix_sort_pairs(ctx, keys, p.perm.as<int32_t>(), n, bits + levels * D);
     p.qbuf.ensure(size_t(n + 1) * D * 8);
     gather_bytes(ctx, p.perm.as<int32_t>(), n, q, p.qbuf.p, size_t(D) * 8);
     p.q_sorted = p.qbuf.as<double>();
@@ -831,7 +977,7 @@ template <int D> static void build_plan_d(QueryPlan &p, const double *q) {
   p.counters.ensure(16);
   p.sched.ensure(16);
   uint32_t *ctr = p.counters.as<uint32_t>();  // [0] = runs, [1] = work items
-  k_run_flags<<<blocks, kThreads, 0, s>>>(keys, n, p.flags.as<uint32_t>());
+  k_run_flags<<<blocks, kThreads, 0, s>>>(keys, n, p.sub_levels * D, p.flags.as<uint32_t>());
   check_launch();
   scan_exclusive(ctx, p.flags.as<uint32_t>(), p.ridx.as<uint32_t>(), n, ctr + 0);
   k_run_starts<<<blocks, kThreads, 0, s>>>(p.flags.as<uint32_t>(), p.ridx.as<uint32_t>(), n,
@@ -882,9 +1028,11 @@ static SearchArgs base_args(const QueryPlan &p) {
   return a;
 }
 
-static size_t lds_bytes(int nslots, bool matvec) {
-  return (matvec ? sizeof(Tile) + sizeof(double) * kTile * kMaxRhs : 0) +
-         size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
+static size_t lds_bytes(int nslots, bool matvec, bool ring = false) {
+  const size_t slots = size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
+  if (matvec) return sizeof(Tile) + sizeof(double) * kTile * kMaxRhs + slots;
+  return ((slots + 15) & ~size_t(15)) + size_t(kGroup) * (3 * 8 + 4) +
+         (ring ? size_t(kGroup) * kRingStride * 4 : 0);
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
@@ -902,7 +1050,7 @@ template <int MODE> static void launch_search(const QueryPlan &p, const SearchAr
   k_zero_u32<<<1, 1, 0, p.ctx->stream>>>(a.next);
   check_launch();
   const GridParams g = p.cells->params();
-  const size_t lds = lds_bytes(a.nslots, MODE == kMatvec64 || MODE == kMatvec32);
+  const size_t lds = lds_bytes(a.nslots, MODE == kMatvec64 || MODE == kMatvec32, a.ring != 0);
   if (p.D == 1) k_search<1, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
   else if (p.D == 2) k_search<2, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
   else k_search<3, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
@@ -1069,6 +1217,7 @@ abh_status abh_search_fill(abh_search *s, int32_t *col, int64_t capacity) {
     SearchArgs a = abh::base_args(s->plan);
     a.row_ptr = s->row_ptr.as<int64_t>();
     a.col = col;
+    a.ring = (reinterpret_cast<uintptr_t>(col) & 63) == 0 ? 1 : 0;
     abh::launch_search<kFill>(s->plan, a);
   });
 }
@@ -1100,6 +1249,7 @@ abh_status abh_search_run_fused(abh_search *s, const abh_cells *cells, int64_t n
       SearchArgs a = abh::base_args(s->plan);
       a.row_ptr = pp;
       a.col = s->padded.as<int32_t>();
+      a.ring = (reinterpret_cast<uintptr_t>(a.col) & 63) == 0 ? 1 : 0;
       a.counts = rp;
       abh::launch_search<kFillPadded>(s->plan, a);
     } else if (nq > 0) {
