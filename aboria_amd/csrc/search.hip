@@ -350,11 +350,27 @@ struct SearchArgs {
 // staged candidates tested per step of the count / list passes
 constexpr int kStageUnroll = ABH_STAGE_UNROLL;
 
+// A staged candidate: coordinates (unused ones 0) and container index, read
+// back as two 16-byte LDS loads.
+struct __align__(16) StageRec {
+  double x, y, z;
+  int32_t j;
+  int32_t pad;
+};
+
+__device__ __forceinline__ void load_stage(const StageRec &r, double (&e)[3], int &j) {
+  const double2 xy = *reinterpret_cast<const double2 *>(&r.x);
+  const double2 zj = *reinterpret_cast<const double2 *>(&r.z);
+  e[0] = xy.x;
+  e[1] = xy.y;
+  e[2] = zj.x;
+  j = int(uint32_t(uint64_t(__double_as_longlong(zj.y))));
+}
+
 // Fill output ring: the last 32 entries of each lane's row (two 16-entry,
-// 64-byte segments); a lane's ring is 36 words so that its segments stay
-// 16-byte aligned for vector LDS reads.
+// 64-byte segments), slot-major ([slot][lane]) so that the lanes' appends
+// and segment reads fall in distinct LDS banks whatever their positions.
 constexpr int kRing = 32;
-constexpr int kRingStride = 36;
 constexpr int kSeg = 16;
 
 // Staged candidates, structure of arrays: positions, stencil slot and
@@ -442,10 +458,9 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
   int32_t *sorder = reinterpret_cast<int32_t *>(slots + S);
   uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + S);  // [S][64]
   // count / list passes: staged candidates (coordinates [D][64], index [64])
-  double *st_x = reinterpret_cast<double *>(
+  StageRec *st = reinterpret_cast<StageRec *>(
       lds + ((size_t(S) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(S + 1) + 15) & ~size_t(15)));
-  int32_t *st_j = reinterpret_cast<int32_t *>(st_x + 3 * kGroup);
-  uint32_t *ring = reinterpret_cast<uint32_t *>(st_j + kGroup);  // [64][36], fill only
+  uint32_t *ring = reinterpret_cast<uint32_t *>(st + kGroup);  // [32][64], fill only
   const int lane = threadIdx.x;
   const int nwork = int(*a.nwork);
   int wd[3] = {a.w[0], a.w[1], a.w[2]};
@@ -607,7 +622,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     // ring state: entries [fpos, out + cnt) of this lane's row are staged
     int64_t fpos = out;
     const bool use_ring = (MODE == kFill || MODE == kFillPadded) && a.ring;
-    uint32_t *rl = ring + lane * kRingStride;
+    uint32_t *rl = ring + lane;  // slot k of this lane: rl[k * kGroup]
     // store the segment holding fpos once it is complete (64-byte vector
     // stores when it starts at its segment boundary, the row's first,
     // partial segment entry by entry)
@@ -616,15 +631,17 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       const int64_t seg = fpos & ~int64_t(kSeg - 1);
       if (pos >= seg + kSeg) {
         if (fpos == seg) {
-          const uint4 *src = reinterpret_cast<const uint4 *>(rl + int(seg & (kRing - 1)));
+          const uint32_t *src = rl + int(seg & (kRing - 1)) * kGroup;
+          uint32_t v[kSeg];
+#pragma unroll
+          for (int k = 0; k < kSeg; ++k) v[k] = src[k * kGroup];
           uint4 *dst = reinterpret_cast<uint4 *>(a.col + seg);
-          const uint4 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
-          dst[0] = v0;
-          dst[1] = v1;
-          dst[2] = v2;
-          dst[3] = v3;
+#pragma unroll
+          for (int k = 0; k < kSeg / 4; ++k)
+            dst[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
         } else {
-          for (int64_t e = fpos; e < seg + kSeg; ++e) a.col[e] = int32_t(rl[int(e & (kRing - 1))]);
+          for (int64_t e = fpos; e < seg + kSeg; ++e)
+            a.col[e] = int32_t(rl[int(e & (kRing - 1)) * kGroup]);
         }
         fpos = seg + kSeg;
       }
@@ -632,7 +649,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     // store everything staged (row end, or before a bucket written directly)
     auto ring_drain = [&]() {
       const int64_t pos = out + cnt;
-      for (int64_t e = fpos; e < pos; ++e) a.col[e] = int32_t(rl[int(e & (kRing - 1))]);
+      for (int64_t e = fpos; e < pos; ++e) a.col[e] = int32_t(rl[int(e & (kRing - 1)) * kGroup]);
       fpos = pos;
     };
     double acc64[kMaxRhs];
@@ -709,9 +726,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
           if (keep) {
             const int sl = int(__builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32),
                                                          __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u)));
-#pragma unroll
-            for (int d = 0; d < D; ++d) st_x[d * kGroup + sl] = x[d];
-            st_j[sl] = jc;
+            st[sl] = StageRec{x[0], x[1], x[2], jc, 0};
           }
           __syncthreads();  // single-wave group: the staged block is complete
           if (use_ring && single) {
@@ -724,11 +739,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
                 double e[kStageUnroll][3];
                 int jj[kStageUnroll];
 #pragma unroll
-                for (int u = 0; u < kStageUnroll; ++u) {
-#pragma unroll
-                  for (int d = 0; d < D; ++d) e[u][d] = st_x[d * kGroup + t + u];
-                  jj[u] = st_j[t + u];
-                }
+                for (int u = 0; u < kStageUnroll; ++u) load_stage(st[t + u], e[u], jj[u]);
                 bool h[kStageUnroll];
 #pragma unroll
                 for (int u = 0; u < kStageUnroll; ++u) {
@@ -742,19 +753,22 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
                 }
 #pragma unroll
                 for (int u = 0; u < kStageUnroll; ++u) {
-                  if (h[u]) rl[int((out + cnt) & (kRing - 1))] = uint32_t(jj[u]);
+                  if (h[u]) rl[int((out + cnt) & (kRing - 1)) * kGroup] = uint32_t(jj[u]);
                   cnt += h[u] ? 1 : 0;
                 }
               }
               for (; t < t1; ++t) {
+                double e1[3];
+                int j1;
+                load_stage(st[t], e1, j1);
                 double acc = 0.0;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                  const double dx = st_x[d * kGroup + t] - qi[d];
+                  const double dx = e1[d] - qi[d];
                   acc = acc + dx * dx;
                 }
                 if (acc <= r2m) {
-                  rl[int((out + cnt) & (kRing - 1))] = uint32_t(st_j[t]);
+                  rl[int((out + cnt) & (kRing - 1)) * kGroup] = uint32_t(j1);
                   ++cnt;
                 }
               }
@@ -769,11 +783,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
             double e[kStageUnroll][3];
             int jj[kStageUnroll];
 #pragma unroll
-            for (int u = 0; u < kStageUnroll; ++u) {
-#pragma unroll
-              for (int d = 0; d < D; ++d) e[u][d] = st_x[d * kGroup + t + u];
-              jj[u] = st_j[t + u];
-            }
+            for (int u = 0; u < kStageUnroll; ++u) load_stage(st[t + u], e[u], jj[u]);
             double acc[kStageUnroll];
 #pragma unroll
             for (int u = 0; u < kStageUnroll; ++u) {
@@ -789,13 +799,16 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
               staged_hit<MODE>(a, single, acc[u] <= r2m, acc[u], jj[u], m, wrow, out, cnt);
           }
           for (; t < nk; ++t) {
+            double e1[3];
+            int j1;
+            load_stage(st[t], e1, j1);
             double acc = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-              const double dx = st_x[d * kGroup + t] - qi[d];
+              const double dx = e1[d] - qi[d];
               acc = acc + dx * dx;
             }
-            staged_hit<MODE>(a, single, acc <= r2m, acc, st_j[t], m, wrow, out, cnt);
+            staged_hit<MODE>(a, single, acc <= r2m, acc, j1, m, wrow, out, cnt);
           }
           if (use_ring) fpos = out + cnt;
           __syncthreads();  // the next block overwrites the stage
@@ -1031,8 +1044,8 @@ static SearchArgs base_args(const QueryPlan &p) {
 static size_t lds_bytes(int nslots, bool matvec, bool ring = false) {
   const size_t slots = size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
   if (matvec) return sizeof(Tile) + sizeof(double) * kTile * kMaxRhs + slots;
-  return ((slots + 15) & ~size_t(15)) + size_t(kGroup) * (3 * 8 + 4) +
-         (ring ? size_t(kGroup) * kRingStride * 4 : 0);
+  return ((slots + 15) & ~size_t(15)) + size_t(kGroup) * sizeof(StageRec) +
+         (ring ? size_t(kGroup) * kRing * 4 : 0);
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
