@@ -378,6 +378,13 @@ class SparseOperator:
         check(C.lib().abh_sparse_op_nnz(self.handle, ctypes.byref(n)), self.ctx.handle)
         return n.value
 
+    def cache(self, precision=FP64, enable=True):
+        """Keep (or drop) the operator's CSR form for `precision` on the
+        device; applies in that precision then stream it."""
+        check(C.lib().abh_sparse_op_cache(self.handle, int(precision), int(bool(enable))),
+              self.ctx.handle)
+        return self
+
     def assemble(self):
         """The operator as CSR (row_ptr int64, col int32, val float64), the
         triplets of KernelSparse::assemble."""

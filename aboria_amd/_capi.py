@@ -147,6 +147,7 @@ SIGNATURES = {
     "abh_sparse_op_nnz": (c_int, [c_void_p, P(c_int64)]),
     "abh_sparse_op_assemble": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64]),
     "abh_sparse_op_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+    "abh_sparse_op_cache": (c_int, [c_void_p, c_int, c_int]),
     "abh_fmm_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), c_int64, c_void_p,
                                c_double, c_int, P(Kernel), P(c_void_p)]),
     "abh_fmm_destroy": (c_int, [c_void_p]),

@@ -130,6 +130,16 @@ template <class T> __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// XCD-contiguous block numbering: workgroups are dealt to the 8 XCDs round
+// robin (block b runs on XCD b % 8), so logical block b % 8 * per + b / 8
+// gives each XCD (and its L2) one contiguous range of the work; launch
+// xcd_grid(count) blocks and skip logical blocks >= count.
+__device__ __forceinline__ int xcd_block(int count) {
+  const int per = (count + 7) / 8;
+  return int(blockIdx.x % 8) * per + int(blockIdx.x / 8);
+}
+inline unsigned xcd_grid(int count) { return unsigned((count + 7) / 8 * 8); }
+
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
   return uint32_t(__builtin_amdgcn_readfirstlane(int(v)));

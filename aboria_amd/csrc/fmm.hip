@@ -32,6 +32,8 @@
 
 using abh::DevBuf;
 using abh::KernelDesc;
+using abh::xcd_block;
+using abh::xcd_grid;
 
 namespace {
 
@@ -385,11 +387,6 @@ __global__ __launch_bounds__(256) void k_transfer(const int32_t *__restrict__ pa
 // renumbering them so that XCD x runs a contiguous range of tiles keeps
 // neighbouring tiles -- which share source expansions -- on one L2.  The
 // grid is padded to a multiple of 8; indices past `count` do nothing.
-__device__ __forceinline__ int xcd_block(int count) {
-  const int per = (count + 7) / 8;
-  return int(blockIdx.x % 8) * per + int(blockIdx.x / 8);
-}
-inline unsigned xcd_grid(int count) { return unsigned((count + 7) / 8 * 8); }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 

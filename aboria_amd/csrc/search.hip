@@ -330,7 +330,8 @@ struct SearchArgs {
   int64_t *counts;
   const int64_t *row_ptr;
   int32_t *col;
-  double *val;  // assemble: kernel value of every stored pair
+  double *val;   // assemble: kernel value of every stored pair
+  float *val32;  // assemble (optional): the same value as the FP32 matvec forms it
   uint32_t *escape_flag;
   uint32_t *next;  // dynamic work counter, zero at launch
   int ring;        // fill: stage each row's entries in LDS and store whole
@@ -373,17 +374,7 @@ __device__ __forceinline__ void load_stage(const StageRec &r, double (&e)[3], in
 constexpr int kRing = 32;
 constexpr int kSeg = 16;
 
-// Staged candidates, structure of arrays: positions, stencil slot and
-// container index of up to kTile candidates.
-constexpr int kTile = 64;
-struct Tile {
-  double x[kTile], y[kTile], z[kTile];
-  int32_t slot[kTile];
-  int32_t img[kTile];
-  int32_t j[kTile];
-};
 
-typedef __attribute__((address_space(4))) const double CDouble;
 
 // a wave-uniform double moved to scalar registers
 __device__ __forceinline__ double uniform_f64(double v) {
@@ -436,9 +427,11 @@ __device__ __forceinline__ void staged_hit(const SearchArgs &a, bool single, boo
       double w = wrow;
       if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j];
       const double f = phi64(a.k, acc) * w;
+      const float f32 = phi32(a.k, float(acc)) * float(w);
       for (int rep = 0; rep < m; ++rep) {
         a.col[out + cnt + rep] = j;
-        a.val[out + cnt + rep] = f;
+        if (a.val) a.val[out + cnt + rep] = f;
+        if (a.val32) a.val32[out + cnt + rep] = f32;
       }
       cnt += m;
     }
@@ -451,16 +444,14 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
   constexpr bool kMV = (MODE == kMatvec64 || MODE == kMatvec32);
   const int S = a.nslots;
   // matvec only: candidate tile and right-hand sides in front of the slots
-  Tile *tile = reinterpret_cast<Tile *>(lds);
-  double *xt = reinterpret_cast<double *>(tile + 1);  // [kMaxRhs][kTile]
-  Slot *slots = kMV ? reinterpret_cast<Slot *>(xt + kTile * kMaxRhs)
-                    : reinterpret_cast<Slot *>(lds);
+  Slot *slots = reinterpret_cast<Slot *>(lds);
   int32_t *sorder = reinterpret_cast<int32_t *>(slots + S);
   uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + S);  // [S][64]
   // count / list passes: staged candidates (coordinates [D][64], index [64])
   StageRec *st = reinterpret_cast<StageRec *>(
       lds + ((size_t(S) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(S + 1) + 15) & ~size_t(15)));
   uint32_t *ring = reinterpret_cast<uint32_t *>(st + kGroup);  // [32][64], fill only
+  double *xs = reinterpret_cast<double *>(st + kGroup);        // [64][kMaxRhs], matvec only
   const int lane = threadIdx.x;
   const int nwork = int(*a.nwork);
   int wd[3] = {a.w[0], a.w[1], a.w[2]};
@@ -489,7 +480,7 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     const bool has_row = lane < it.count;
     // box of the work item's active centres (empty when none is active)
     double qlo[3] = {0.0, 0.0, 0.0}, qhi[3] = {0.0, 0.0, 0.0};
-    if (MODE == kCount || MODE == kFill || MODE == kAssemble || MODE == kFillPadded) {
+    if (MODE != kCapacity) {
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         qlo[d] = uniform_f64(wave_min_f64(active ? q[d] : INFINITY));
@@ -667,9 +658,6 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       wrow = a.k.w0;
       if (a.k.row_s && active) wrow = wrow + a.k.wr * a.k.row_s[row];
     }
-    // candidate positions through the constant address space: wave-uniform
-    // indices become scalar loads whose values feed the VALU as SGPRs
-    const CDouble *cpos = (const CDouble *)(a.pos);
 
     for (int k = 0; k < S; ++k) {
       const int s = abh::uniform_i32(sorder[k]);
@@ -814,68 +802,81 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
           __syncthreads();  // the next block overwrites the stage
         }
       } else {
-        // matvec: chunks of 64 candidates; the radius test streams the
-        // positions as scalars, the hits are then evaluated lane-compacted
-        // (each iteration every lane takes its next hit) from an LDS copy
+        // matvec: the candidates are staged and culled as in the list
+        // passes, with their right-hand sides; every lane then evaluates the
+        // kernel for the staged candidates within its radius, in candidate
+        // order (y += f x for each of the m visits)
+        double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const int dg = (img >> (2 * d)) & 3;
+          blo[d] = uniform_f64(
+              pick3(dg, qlo[d] + double(-1) * g.L[d], qlo[d], qlo[d] + double(1) * g.L[d]));
+          bhi[d] = uniform_f64(
+              pick3(dg, qhi[d] + double(-1) * g.L[d], qhi[d], qhi[d] + double(1) * g.L[d]));
+        }
         const double r2m = m != 0 ? a.r2 : -1.0;
-        for (int c0 = beg; c0 < end; c0 += kTile) {
-          const int nt = end - c0 < kTile ? end - c0 : kTile;
-          if (lane < nt) {
-            const int j = c0 + lane;
-            tile->x[lane] = a.pos[int64_t(j) * D + 0];
-            if (D > 1) tile->y[lane] = a.pos[int64_t(j) * D + (D > 1 ? 1 : 0)];
-            if (D > 2) tile->z[lane] = a.pos[int64_t(j) * D + (D > 2 ? 2 : 0)];
+        for (int c0 = beg; c0 < end; c0 += kGroup) {
+          const int jc = c0 + lane;
+          bool keep = jc < end;
+          double x[3] = {0.0, 0.0, 0.0};
+          if (keep) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) x[d] = a.pos[int64_t(jc) * D + d];
+          }
+          double dd = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            double e = 0.0;
+            if (x[d] < blo[d]) e = blo[d] - x[d];
+            else if (x[d] > bhi[d]) e = x[d] - bhi[d];
+            dd = dd + e * e;
+          }
+          keep = keep && dd <= a.r2;
+          const uint64_t km = __ballot(keep);
+          if (km == 0) continue;
+          const int nk = __popcll(km);
+          if (keep) {
+            const int sl = int(__builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u)));
+            st[sl] = StageRec{x[0], x[1], x[2], jc, 0};
 #pragma unroll
             for (int r = 0; r < kMaxRhs; ++r)
-              if (r < a.nrhs) xt[r * kTile + lane] = a.x[int64_t(r) * a.ldx + j];
+              xs[sl * kMaxRhs + r] = r < a.nrhs ? a.x[int64_t(r) * a.ldx + jc] : 0.0;
           }
-          uint64_t hits = 0;
-          const CDouble *pc = cpos + int64_t(c0) * D;
-          for (int c = 0; c < nt; ++c, pc += D) {
+          __syncthreads();  // single-wave group: the staged block is complete
+          for (int t = 0; t < nk; ++t) {
+            double e1[3];
+            int j1;
+            load_stage(st[t], e1, j1);
             double acc = 0.0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-              const double dx = pc[d] - qi[d];
+              const double dx = e1[d] - qi[d];
               acc = acc + dx * dx;
             }
-            if (acc <= r2m) hits |= uint64_t(1) << c;
-          }
-          __syncthreads();
-          while (hits) {
-            const int c = __ffsll((unsigned long long)hits) - 1;
-            hits &= hits - 1;
-            double acc = 0.0;
-            {
-              const double dx0 = tile->x[c] - qi[0];
-              acc = acc + dx0 * dx0;
-              if (D > 1) {
-                const double dx1 = tile->y[c] - qi[1];
-                acc = acc + dx1 * dx1;
-              }
-              if (D > 2) {
-                const double dx2 = tile->z[c] - qi[2];
-                acc = acc + dx2 * dx2;
-              }
-            }
-            double w = wrow;
-            if (a.k.col_s) w = w + a.k.wc * a.k.col_s[c0 + c];
-            if (MODE == kMatvec64) {
-              const double f = phi64(a.k, acc) * w;
-              for (int rep = 0; rep < m; ++rep) {
+            if (acc <= r2m) {
+              double w = wrow;
+              if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j1];
+              const double *xr = xs + t * kMaxRhs;
+              if (MODE == kMatvec64) {
+                const double f = phi64(a.k, acc) * w;
+                for (int rep = 0; rep < m; ++rep) {
 #pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r)
-                  if (r < a.nrhs) acc64[r] += f * xt[r * kTile + c];
-              }
-            } else {
-              const float f = phi32(a.k, float(acc)) * float(w);
-              for (int rep = 0; rep < m; ++rep) {
+                  for (int r = 0; r < kMaxRhs; ++r)
+                    if (r < a.nrhs) acc64[r] += f * xr[r];
+                }
+              } else {
+                const float f = phi32(a.k, float(acc)) * float(w);
+                for (int rep = 0; rep < m; ++rep) {
 #pragma unroll
-                for (int r = 0; r < kMaxRhs; ++r)
-                  if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(xt[r * kTile + c]));
+                  for (int r = 0; r < kMaxRhs; ++r)
+                    if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(xr[r]));
+                }
               }
             }
           }
-          __syncthreads();  // tile reused by the next chunk
+          __syncthreads();  // the next block overwrites the stage
         }
       }
     }
@@ -895,6 +896,97 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       }
     }
     __syncthreads();  // slots / mult reused by the next work item
+  }
+}
+
+// xr[j][r] = x[r][j] (r < nrhs, zero beyond): the right-hand sides of one
+// column in one 64-byte segment for the CSR product's gathers
+__global__ __launch_bounds__(256) void k_rhs_rows(const double *__restrict__ x, int64_t ldx,
+                                                  int nrhs, int64_t n, double *__restrict__ xr) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= n) return;
+  double v[kMaxRhs];
+#pragma unroll
+  for (int r = 0; r < kMaxRhs; ++r) v[r] = r < nrhs ? x[int64_t(r) * ldx + j] : 0.0;
+  double2 *o = reinterpret_cast<double2 *>(xr + j * kMaxRhs);
+#pragma unroll
+  for (int r = 0; r < kMaxRhs / 2; ++r) o[r] = make_double2(v[2 * r], v[2 * r + 1]);
+}
+
+// y += K x from the cached CSR form of the operator: one wavefront per row.
+// Four lanes share an entry and each takes two of its eight right-hand
+// sides, so that one 16-byte load per lane reads an entry's whole 64-byte
+// row of xr (one cache line for four lanes); sixteen entries per step, the
+// partial sums then combined over the lanes holding the same pair.  FP64
+// reads val; the FP32 form reads val32 (the values the FP32 matvec forms)
+// and sums each lane's terms with compensation.  xr: the right-hand sides by
+// column (k_rhs_rows).
+template <bool FP32>
+__global__ __launch_bounds__(256) void k_csr_spmv(int64_t nrows, const int64_t *__restrict__ rp,
+                                                  const int32_t *__restrict__ col,
+                                                  const double *__restrict__ val,
+                                                  const float *__restrict__ val32, int nrhs,
+                                                  const double *__restrict__ xr,
+                                                  double *__restrict__ y, int64_t ldy) {
+  const int blk = abh::xcd_block(int((nrows + 3) / 4));
+  const int64_t row = int64_t(blk) * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const int g = lane >> 2, q = lane & 3;  // entry slot, right-hand-side pair
+  const int64_t e0 = rp[row], e1 = rp[row + 1];
+  double s64[2] = {0.0, 0.0};
+  float s32[2] = {0.0f, 0.0f}, c32[2] = {0.0f, 0.0f};
+  // the row's entries are read 64 at a time (one contiguous load of col and
+  // of val per block, two blocks ahead in flight), then handed to the lane
+  // groups by shuffles
+  int64_t b = e0;
+  int32_t jn = 0, jn2 = 0;
+  double fn = 0.0, fn2 = 0.0;
+  if (b + lane < e1) {
+    jn = col[b + lane];
+    fn = FP32 ? double(val32[b + lane]) : val[b + lane];
+  }
+  if (b + 64 + lane < e1) {
+    jn2 = col[b + 64 + lane];
+    fn2 = FP32 ? double(val32[b + 64 + lane]) : val[b + 64 + lane];
+  }
+  for (; b < e1; b += 64) {
+    const int32_t jl = jn;
+    const double fl = fn;
+    jn = jn2;
+    fn = fn2;
+    const int nb = e1 - b < 64 ? int(e1 - b) : 64;
+    const int64_t bn = b + 128 + lane;
+    if (bn < e1) {
+      jn2 = col[bn];
+      fn2 = FP32 ? double(val32[bn]) : val[bn];
+    }
+#pragma unroll
+    for (int s0 = 0; s0 < 64; s0 += 16) {
+      if (s0 >= nb) break;
+      const int src = s0 + g;
+      const int32_t j = __shfl(jl, src, 64);
+      const double f = __shfl(fl, src, 64);
+      if (src < nb) {
+        const double2 xv = *reinterpret_cast<const double2 *>(xr + int64_t(j) * kMaxRhs + 2 * q);
+        if (FP32) {
+          const float f32 = float(f);  // exact: f holds an FP32 value
+          kb_add(s32[0], c32[0], f32 * float(xv.x));
+          kb_add(s32[1], c32[1], f32 * float(xv.y));
+        } else {
+          s64[0] = s64[0] + f * xv.x;
+          s64[1] = s64[1] + f * xv.y;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    double t = FP32 ? double(s32[h]) + double(c32[h]) : s64[h];
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) t += __shfl_xor(t, o, 64);
+    const int r = 2 * q + h;
+    if (g == 0 && r < nrhs) y[int64_t(r) * ldy + row] += t;
   }
 }
 
@@ -1043,9 +1135,9 @@ static SearchArgs base_args(const QueryPlan &p) {
 
 static size_t lds_bytes(int nslots, bool matvec, bool ring = false) {
   const size_t slots = size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
-  if (matvec) return sizeof(Tile) + sizeof(double) * kTile * kMaxRhs + slots;
   return ((slots + 15) & ~size_t(15)) + size_t(kGroup) * sizeof(StageRec) +
-         (ring ? size_t(kGroup) * kRing * 4 : 0);
+         (ring ? size_t(kGroup) * kRing * 4 : 0) +
+         (matvec ? size_t(kGroup) * kMaxRhs * sizeof(double) : 0);
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
@@ -1134,21 +1226,54 @@ struct abh_sparse_op {
   KernelDesc k{};
   int64_t nnz = 0;
   int64_t ncols = 0;
+  // cached CSR form (abh_sparse_op_cache): row pointers, columns and the
+  // kernel values of the FP64 and / or FP32 forms
+  abh::DevBuf c_rp, c_col, c_val, c_val32, c_xr;
+  bool cached64 = false, cached32 = false;
 };
 
 namespace abh {
 
-void sparse_apply(abh_sparse_op *op, const double *x, double *y, bool fp32) {
-  if (op->plan.nq == 0) return;
+// y += K x for nrhs <= kMaxRhs right-hand sides, from the cached CSR form
+// when it holds this precision, else evaluating the kernel on the fly
+static void apply_group(abh_sparse_op *op, int nrhs, const double *x, int64_t ldx, double *y,
+                        int64_t ldy, bool fp32) {
+  const int64_t nq = op->plan.nq;
+  if (nq == 0 || nrhs == 0) return;
+  if (fp32 ? op->cached32 : op->cached64) {
+    hipStream_t st = op->plan.ctx->stream;
+    const int64_t nc = op->ncols;
+    op->c_xr.ensure(size_t(nc) * kMaxRhs * 8 + 64);
+    double *xr = op->c_xr.as<double>();
+    if (nc > 0) {
+      k_rhs_rows<<<unsigned(ceil_div(nc, 256)), 256, 0, st>>>(x, ldx, nrhs, nc, xr);
+      check_launch();
+    }
+    const unsigned blocks = xcd_grid(int(ceil_div(nq, 4)));
+    if (fp32)
+      k_csr_spmv<true><<<blocks, 256, 0, st>>>(nq, op->c_rp.as<int64_t>(), op->c_col.as<int32_t>(),
+                                               nullptr, op->c_val32.as<float>(), nrhs, xr, y,
+                                               ldy);
+    else
+      k_csr_spmv<false><<<blocks, 256, 0, st>>>(nq, op->c_rp.as<int64_t>(),
+                                                op->c_col.as<int32_t>(), op->c_val.as<double>(),
+                                                nullptr, nrhs, xr, y, ldy);
+    check_launch();
+    return;
+  }
   SearchArgs a = base_args(op->plan);
   a.k = op->k;
-  a.nrhs = 1;
+  a.nrhs = nrhs;
   a.x = x;
-  a.ldx = op->ncols;
+  a.ldx = ldx;
   a.y = y;
-  a.ldy = op->plan.nq;
+  a.ldy = ldy;
   if (fp32) launch_search<kMatvec32>(op->plan, a);
   else launch_search<kMatvec64>(op->plan, a);
+}
+
+void sparse_apply(abh_sparse_op *op, const double *x, double *y, bool fp32) {
+  apply_group(op, 1, x, op->ncols, y, op->plan.nq, fp32);
 }
 
 int64_t sparse_rows(const abh_sparse_op *op) { return op->plan.nq; }
@@ -1384,17 +1509,62 @@ abh_status abh_sparse_op_apply(abh_sparse_op *op, int nrhs, const double *x, int
     ABH_REQUIRE(x && y, ABH_ERR_INVALID_ARG, "x or y is null");
     ABH_REQUIRE(ldx >= op->ncols && ldy >= op->plan.nq, ABH_ERR_INVALID_ARG,
                 "leading dimension smaller than the operator");
-    for (int r0 = 0; r0 < nrhs; r0 += kMaxRhs) {
-      SearchArgs a = abh::base_args(op->plan);
-      a.k = op->k;
-      a.nrhs = std::min(kMaxRhs, nrhs - r0);
-      a.x = x + int64_t(r0) * ldx;
-      a.ldx = ldx;
-      a.y = y + int64_t(r0) * ldy;
-      a.ldy = ldy;
-      if (precision == ABH_FP64) abh::launch_search<kMatvec64>(op->plan, a);
-      else abh::launch_search<kMatvec32>(op->plan, a);
+    for (int r0 = 0; r0 < nrhs; r0 += kMaxRhs)
+      abh::apply_group(op, std::min(kMaxRhs, nrhs - r0), x + int64_t(r0) * ldx, ldx,
+                       y + int64_t(r0) * ldy, ldy, precision == ABH_FP32_COMPENSATED);
+  });
+}
+
+abh_status abh_sparse_op_cache(abh_sparse_op *op, abh_precision precision, int enable) {
+  if (!op) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = op->plan.ctx;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
+                "unknown precision");
+    const bool fp32 = precision == ABH_FP32_COMPENSATED;
+    if (!enable) {
+      (fp32 ? op->c_val32 : op->c_val).release();
+      (fp32 ? op->cached32 : op->cached64) = false;
+      if (!op->cached32 && !op->cached64) {
+        op->c_col.release();
+        op->c_rp.release();
+      }
+      return;
     }
+    if (fp32 ? op->cached32 : op->cached64) return;
+    const int64_t nq = op->plan.nq;
+    const bool have = op->cached32 || op->cached64;
+    const size_t need = size_t(op->nnz) * (fp32 ? 4 : 8) +
+                        (have ? 0 : size_t(op->nnz) * 4 + size_t(nq + 1) * 8);
+    size_t free_b = 0, total_b = 0;
+    ABH_HIP(hipMemGetInfo(&free_b, &total_b));
+    ABH_REQUIRE(double(need) <= 0.8 * double(free_b), ABH_ERR_OUT_OF_MEMORY,
+                "not enough device memory to cache the operator");
+    op->c_rp.ensure(size_t(nq + 1) * 8);
+    op->c_col.ensure(size_t(op->nnz) * 4 + 64);
+    (fp32 ? op->c_val32 : op->c_val).ensure(size_t(op->nnz) * (fp32 ? 4 : 8) + 64);
+    if (nq == 0) {
+      (fp32 ? op->cached32 : op->cached64) = true;
+      return;
+    }
+    int64_t *rp = op->c_rp.as<int64_t>();
+    ABH_HIP(hipMemsetAsync(rp + nq, 0, 8, ctx->stream));
+    abh::count_with_escape(op->plan, rp);
+    abh::DevBuf total_buf;
+    total_buf.ensure(8);
+    abh::scan_exclusive(ctx, rp, rp, nq + 1, total_buf.as<int64_t>());
+    SearchArgs a = abh::base_args(op->plan);
+    a.k = op->k;
+    a.row_ptr = rp;
+    a.col = op->c_col.as<int32_t>();
+    a.val = fp32 ? nullptr : op->c_val.as<double>();
+    a.val32 = fp32 ? op->c_val32.as<float>() : nullptr;
+    // values of the other form already cached are rewritten identically
+    if (!fp32 && op->cached32) a.val32 = op->c_val32.as<float>();
+    if (fp32 && op->cached64) a.val = op->c_val.as<double>();
+    abh::launch_search<kAssemble>(op->plan, a);
+    ABH_HIP(hipStreamSynchronize(ctx->stream));
+    (fp32 ? op->cached32 : op->cached64) = true;
   });
 }
 

@@ -188,6 +188,20 @@ def bench_sparse(torch, ab, ctx, timer, st, extra):
         out["matvec8_%s_ms" % name] = timer(lambda: K.apply_colmajor(x, y, prec), reps=3)
     log("sparse matvec 8 rhs: fp64 %.3f ms, fp32c %.3f ms" % (out["matvec8_fp64_ms"],
                                                                out["matvec8_fp32c_ms"]))
+    # cached CSR form: assembled once, then streamed by every apply
+    out["nnz"] = K.nnz
+    for prec, name in ((ab.FP64, "fp64"), (ab.FP32_COMPENSATED, "fp32c")):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        K.cache(prec)
+        torch.cuda.synchronize()
+        out["cache_%s_setup_ms" % name] = (time.perf_counter() - t0) * 1e3
+        K.apply_colmajor(x, y, prec)
+        out["cached_matvec8_%s_ms" % name] = timer(lambda: K.apply_colmajor(x, y, prec), reps=3)
+        K.cache(prec, enable=False)
+    log("sparse matvec 8 rhs, cached CSR (%d pairs): fp64 %.3f ms (setup %.1f ms), fp32c %.3f ms "
+        "(setup %.1f ms)" % (out["nnz"], out["cached_matvec8_fp64_ms"], out["cache_fp64_setup_ms"],
+                             out["cached_matvec8_fp32c_ms"], out["cache_fp32c_setup_ms"]))
     K.close()
     extra["sparse"] = out
 

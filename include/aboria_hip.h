@@ -222,6 +222,17 @@ abh_status abh_sparse_op_assemble(abh_sparse_op *op, int64_t *row_ptr,
 abh_status abh_sparse_op_apply(abh_sparse_op *op, int nrhs, const double *x,
                                int64_t ldx, double *y, int64_t ldy,
                                abh_precision precision);
+/* No counterpart in the reference (which evaluates the kernel at every
+ * apply): keep the operator's CSR form (columns and kernel values of the
+ * given precision, 12 bytes per stored pair for ABH_FP64, 8 for
+ * ABH_FP32_COMPENSATED, plus the row pointers) on the device, so that later
+ * applies in that precision stream it instead of searching and evaluating
+ * again.  The values are exactly those the on-the-fly forms compute; the
+ * sums run in another order (within the same tolerances).  enable = 0 frees
+ * that precision's values.  ABH_ERR_OUT_OF_MEMORY when they would take more
+ * than 80% of the free device memory.  Valid until the cell list changes. */
+abh_status abh_sparse_op_cache(abh_sparse_op *op, abh_precision precision,
+                               int enable);
 
 /* ------------------------------------------------------------------ */
 /* black-box Chebyshev FMM over a HyperOctree                          */

@@ -144,10 +144,12 @@ def test_sparse_kat(ctx):
     assert res.total == 7
 
 
+@pytest.mark.parametrize("cached", [False, True])
 @pytest.mark.parametrize("precision", [ab.FP64, ab.FP32_COMPENSATED])
-def test_mixture_sparse_subset(ctx, precision):
+def test_mixture_sparse_subset(ctx, precision, cached):
     """Periodic Gaussian-mixture workload (reduced N, same cell side and
-    radius): Wendland C2 operator on 8 right-hand sides."""
+    radius): Wendland C2 operator on 8 right-hand sides, evaluated on the fly
+    or from the cached CSR form (10 right-hand sides there, two groups)."""
     n = 400_000
     pos = wl.mixture_positions(n, seed=7)
     r = 0.01
@@ -159,8 +161,10 @@ def test_mixture_sparse_subset(ctx, precision):
     assert cl.grid == oc.grid == (100, 100, 100)
     assert np.array_equal(cl.order().cpu().numpy(), oc.order)
     sp = cl.positions().clone()
-    x = wl.normal_rhs(n, 8, seed=11)
+    x = wl.normal_rhs(n, 10 if cached else 8, seed=11)
     K = ab.SparseOperator(ctx, cl, sp, r, ab.Kernel.wendland_c2(r))
+    if cached:
+        K.cache(precision)
     xt = torch.tensor(x.T.copy(), device="cuda")
     yt = torch.zeros_like(xt)
     K.apply_colmajor(xt, yt, precision)
