@@ -26,7 +26,23 @@ constexpr int kRows = 256;
 constexpr int kVariants = 3;
 constexpr int kRptOf[kVariants] = {1, 2, 4};
 
-template <int D, bool FP32, int RPT>
+// sqrt(s) for a normal positive s (here s >= c^2 > 0): the hardware
+// reciprocal square root refined by one Goldschmidt step and one Newton
+// correction, within about one ulp (the libm square root is correctly
+// rounded, and costs twice the instructions with its range scaling)
+__device__ __forceinline__ double sqrt_pos(double s) {
+  const double y = __builtin_amdgcn_rsq(s);
+  double g = s * y, h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, s);
+  return fma(d, h, g);
+}
+
+// MQ: the multiquadric sqrt(|dx|^2 + c^2) with fused multiply-adds (the
+// batched RBF systems' kernel), instead of the generic profile.
+template <int D, bool FP32, int RPT, bool MQ>
 __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles, int ntiles,
                                                  const int64_t *__restrict__ off,
                                                  const double *__restrict__ pos,
@@ -77,7 +93,23 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
       const double xj = sx[j];
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        if (FP32) {
+        if (MQ && FP32) {
+          float s = float(k.param2);
+#pragma unroll
+          for (int d = D - 1; d >= 0; --d) {
+            const float dx = float(pj[d] - xi[q][d]);
+            s = fmaf(dx, dx, s);
+          }
+          abh::kb_add(a32[q], c32[q], __builtin_amdgcn_sqrtf(s) * float(xj));
+        } else if (MQ) {
+          double s = k.param2;
+#pragma unroll
+          for (int d = D - 1; d >= 0; --d) {
+            const double dx = pj[d] - xi[q][d];
+            s = fma(dx, dx, s);
+          }
+          acc[q] = fma(sqrt_pos(s), xj, acc[q]);
+        } else if (FP32) {
           float s = 0.0f;
 #pragma unroll
           for (int d = 0; d < D; ++d) {
@@ -199,15 +231,20 @@ struct abh_dense_batch {
     const int64_t *o = off.as<int64_t>();
     const int nt = ntiles[v];
     const int ow = overwrite ? 1 : 0;
-#define ABH_DENSE_LAUNCH(DD)                                                                    \
+    const bool mq = k.type == ABH_KERNEL_MULTIQUADRIC && !k.row_s && !k.col_s;
+#define ABH_DENSE_LAUNCH2(DD, MQ)                                                               \
   if (fp32)                                                                                     \
-    k_dense<DD, true, RPT><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);       \
+    k_dense<DD, true, RPT, MQ><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);   \
   else                                                                                          \
-    k_dense<DD, false, RPT><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);
+    k_dense<DD, false, RPT, MQ><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);
+#define ABH_DENSE_LAUNCH(DD)                                                                    \
+  if (mq) { ABH_DENSE_LAUNCH2(DD, true) }                                                       \
+  else { ABH_DENSE_LAUNCH2(DD, false) }
     if (dim == 1) { ABH_DENSE_LAUNCH(1) }
     else if (dim == 2) { ABH_DENSE_LAUNCH(2) }
     else { ABH_DENSE_LAUNCH(3) }
 #undef ABH_DENSE_LAUNCH
+#undef ABH_DENSE_LAUNCH2
     abh::check_launch();
   }
 
