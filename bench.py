@@ -418,10 +418,104 @@ def cpu_baseline(st, value_unit="pairs/s"):
         q_s = time.perf_counter() - tq
     pairs = int(rp[-1])
     log("cpu: %.3e pairs/s" % (pairs / q_s))
+    st["oc"] = oc
     return {"value": pairs / q_s, "unit": value_unit, "cores": threads, "kind": "port",
             "sample": "oracle search of %d random centres of the N=%d mixture cell list "
                       "(%d pairs, %.2f s); serial oracle build of all N: %.2f s"
                       % (sample, n, pairs, q_s, build_s)}
+
+
+def cpu_sparse(st, extra):
+    """C2 operator on the CPU oracle: one right-hand side over a row sample,
+    scaled to all N rows (the reference applies one RHS per call)."""
+    import oracle
+    oc, n, r = st["oc"], st["n"], st["r"]
+    rng = np.random.default_rng(98)
+    sample = 20000
+    x = rng.standard_normal(n)
+    for _ in range(2):
+        idx = rng.choice(n, sample, replace=False)
+        t0 = time.perf_counter()
+        oc.sparse_matvec(oc.positions[idx], r, (2, r), x)
+        t = time.perf_counter() - t0
+        if t >= 5.0:
+            break
+        sample = min(n, int(sample * 8.0 / max(t, 1e-3)))
+    ms = t * 1e3 * n / sample
+    extra.setdefault("sparse", {})["cpu"] = {
+        "ms_per_rhs_all_rows": ms, "cores": oracle.nthreads_default(), "kind": "port",
+        "sample": "oracle Wendland C2 matvec, 1 RHS, %d of %d rows (%.2f s), scaled to all rows"
+                  % (sample, n, t)}
+    log("cpu sparse: %.0f ms per RHS over all rows" % ms)
+
+
+def cpu_fmm(wl, extra):
+    """C3 FMM on the CPU oracle at N = 20,000 (the first points of the C3
+    set; same order, leaf size and kernel), one right-hand side."""
+    import oracle
+    c = wl.FMM
+    n = 20000
+    pos = wl.fmm_positions(c["n"], c["seed"])[:n]
+    q = wl.fmm_rhs(c["n"], 1, c["seed"] + 1)[0][:n]
+    ot = oracle.Octree([0, 0, 0], [1, 1, 1], pos, c["n_leaf"])
+    t0 = time.perf_counter()
+    ot.fmm_matvec(c["order"], (4, c["eps"]), q)
+    t = time.perf_counter() - t0
+    extra.setdefault("fmm", {})["cpu"] = {
+        "n": n, "matvec_per_s": 1.0 / t, "cores": oracle.nthreads_default(), "kind": "port",
+        "sample": "oracle FMM (M2L kernels recomputed per pair, as the reference) at N=%d, "
+                  "1 RHS: %.2f s" % (n, t)}
+    log("cpu fmm N=%d: %.2f s per matvec" % (n, t))
+
+
+def cpu_rbf(wl, extra):
+    """C4 batched matvec on the CPU oracle: the problems in batch order until
+    about 1.5e10 pairs, pairs/s scaled to the batch's sum of N^2."""
+    import oracle
+    c = wl.RBF_BATCH
+    off, pos, f = wl.rbf_batch_problems(c["nprob"], c["n_min"], c["n_max"], c["seed"])
+    sizes = np.diff(off)
+    pairs = t = 0.0
+    used = 0
+    for p in range(len(sizes)):
+        a, b = off[p], off[p + 1]
+        t0 = time.perf_counter()
+        oracle.dense_matvec(pos[a:b], pos[a:b], (3, c["c"]), f[a:b])
+        t += time.perf_counter() - t0
+        pairs += float(sizes[p]) ** 2
+        used += 1
+        if pairs >= 1.5e10:
+            break
+    rate = pairs / t
+    total = float((sizes.astype(np.float64) ** 2).sum())
+    extra.setdefault("rbf_batch", {})["cpu"] = {
+        "pairs_per_s": rate, "batched_matvec_s": total / rate,
+        "cores": oracle.nthreads_default(), "kind": "port",
+        "sample": "oracle dense multiquadric matvec of %d problems (%.3g pairs, %.2f s), "
+                  "scaled to the batch's %.3g pairs" % (used, pairs, t, total)}
+    log("cpu rbf: %.3e pairs/s, %.1f s per batched matvec" % (rate, total / rate))
+
+
+def cpu_schwarz(wl, extra):
+    """C5 plain Schwarz on the CPU oracle, sigma = 0.002 variant (the one the
+    method solves): factorisation and one application over all N."""
+    import oracle
+    s = wl.SCHWARZ
+    pos = wl.jittered_grid(s["nx"], s["ny"], s["jitter"], s["seed"])
+    sigma = 0.002
+    oc = oracle.Cells([0.0, 0.0], [1.0, 1.0], [0, 0], 64.0)
+    oc.update(pos.copy())
+    t0 = time.perf_counter()
+    sw = oc.schwarz((1, sigma), s["cutoff"] * sigma, s["shift"])
+    t1 = time.perf_counter()
+    sw.apply(np.random.default_rng(1).normal(size=oc.n_alive))
+    t2 = time.perf_counter()
+    extra.setdefault("schwarz", {})["cpu"] = {
+        "setup_s": t1 - t0, "apply_ms": (t2 - t1) * 1e3, "n_domains": sw.n_domains,
+        "cores": oracle.nthreads_default(), "kind": "port",
+        "sample": "oracle SchwartzPreconditioner<LLT> on the N=%d jittered grid, sigma=%g, "
+                  "64 particles per bucket" % (oc.n_alive, sigma)}
+    log("cpu schwarz sigma=%g: setup %.1f s, apply %.1f ms" % (sigma, t1 - t0, (t2 - t1) * 1e3))
 
 
 def main():
@@ -452,6 +546,8 @@ def main():
         bench_sparse(torch, ab, ctx, timer, st, extra)
     if st is not None and not args.no_cpu:
         cpu = cpu_baseline(st)
+        if "sparse" in sections:
+            cpu_sparse(st, extra)
     if st is not None:
         st["cells"].close()
         st.clear()
@@ -466,6 +562,14 @@ def main():
         bench_schwarz(torch, ab, wl, ctx, extra)
     if "copy" in sections:
         bench_copy(torch, ctx, timer, extra)
+    if not args.no_cpu:
+        # the CPU oracle beside the GPU numbers (bounded samples, host cores)
+        if "fmm" in sections:
+            cpu_fmm(wl, extra)
+        if "rbf" in sections:
+            cpu_rbf(wl, extra)
+        if "schwarz" in sections:
+            cpu_schwarz(wl, extra)
 
     out = {
         "metric": "pairs/s (cell-list query)",
