@@ -409,18 +409,21 @@ class SparseOperator:
         return self.apply(x, y)
 
     def gmres(self, b, x=None, diag_shift=0.0, preconditioner=None, restart=50, max_iters=1000,
-              tol=1e-8, precision=FP64):
+              tol=1e-8, precision=FP64, stall_restarts=0):
         """Solves (K + diag_shift I) x = b with restarted GMRES, right
         preconditioned by a SchwartzPreconditioner or an ApproxInverse (the
         reference's Eigen::GMRES<MatrixReplacement, SchwartzPreconditioner>).
-        x (if given) is the initial guess.  Returns (x, stats)."""
+        x (if given) is the initial guess; stall_restarts > 0 stops after that
+        many restarts in a row that fail to cut the residual by 10%.  Returns
+        (x, stats)."""
         if x is None:
             x = torch.zeros_like(b)
         sw = preconditioner.handle if isinstance(preconditioner, SchwartzPreconditioner) else None
         ai = preconditioner.handle if isinstance(preconditioner, ApproxInverse) else None
         if preconditioner is not None and sw is None and ai is None:
             raise TypeError("preconditioner must be a SchwartzPreconditioner or an ApproxInverse")
-        opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision), 0)
+        opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision),
+                              int(stall_restarts))
         st = C.GmresStats()
         check(C.lib().abh_sparse_op_gmres(self.handle, float(diag_shift), sw, ai, _ptr(b), _ptr(x),
                                           ctypes.byref(opts), ctypes.byref(st)), self.ctx.handle)
@@ -564,13 +567,15 @@ class DenseBatchOperator:
         return rpt.value, {1: ms[0], 2: ms[1], 4: ms[2]}
 
     def gmres(self, b, x=None, restart=50, max_iters=1000, tol=1e-8, precision=FP64,
-              preconditioner=None):
+              preconditioner=None, stall_restarts=0):
         """Solves K x = b for every problem, right-preconditioned by
-        `preconditioner` (SchwartzPreconditioner.for_dense_batch) if given;
-        returns (x, stats)."""
+        `preconditioner` (SchwartzPreconditioner.for_dense_batch) if given; a
+        problem stops after stall_restarts (> 0) restarts in a row that fail
+        to cut its residual by 10%.  Returns (x, stats)."""
         if x is None:
             x = torch.zeros_like(b)
-        opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision), 0)
+        opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision),
+                              int(stall_restarts))
         stats = (C.GmresStats * self.nprob)()
         ph = preconditioner.handle if preconditioner is not None else None
         check(C.lib().abh_dense_batch_gmres_precond(self.handle, ph, _ptr(b), _ptr(x),

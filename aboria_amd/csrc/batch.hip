@@ -137,8 +137,9 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
   const int64_t n = L.total();
   const int m = std::max(1, opt.restart);
   const int ld = m + 2;
-  DevBuf V, w, z, partial, hdev, sdev;
+  DevBuf V, w, z, partial, hdev, sdev, xprev;
   V.ensure(size_t(m + 1) * size_t(n) * 8 + 64);
+  xprev.ensure(size_t(n) * 8 + 64);
   w.ensure(size_t(n) * 8 + 64);
   z.ensure(size_t(n) * 8 + 64);
   partial.ensure(size_t(std::max(L.nchunks, 1)) * ld * 8);
@@ -160,9 +161,10 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
     ABH_HIP(hipStreamSynchronize(s));
     for (int p = 0; p < np; ++p) out[p] = std::sqrt(std::max(0.0, hh[size_t(p) * ld]));
   };
-  auto residual = [&](double *r) {  // r = b - A x (FP64)
-    std::vector<uint8_t> all(np, 1);
-    set_active(all.data());
+  // r = b - A x (FP64) for the problems still iterating (the rows of the
+  // others are left unspecified)
+  auto residual = [&](double *r) {
+    set_active(active.data());
     op_true(x, r);
     std::vector<double> neg(np, -1.0);
     up(sdev.p, neg.data(), size_t(np) * 8);
@@ -176,6 +178,8 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
   norms(b, bnorm);
   residual(w.as<double>());
   norms(w.as<double>(), beta);
+  std::vector<int> stalls(np, 0);
+  if (n > 0) ABH_HIP(hipMemcpyAsync(xprev.p, x, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
   for (int p = 0; p < np; ++p) {
     res.stats[p].residual = bnorm[p] > 0 ? beta[p] / bnorm[p] : 0.0;
     if (bnorm[p] == 0.0 || beta[p] <= opt.tol * bnorm[p]) {
@@ -283,15 +287,45 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
       batch_combine(ctx, L, Vp, n, kmax, hdev.as<double>(), ld, 1.0, x);
     }
     // true residual in FP64
+    const std::vector<double> beta_prev = beta;
     residual(w.as<double>());
     norms(w.as<double>(), beta);
+    // a problem whose true residual grew over the cycle (rounding has taken
+    // over) goes back to its previous iterate and stops
+    std::vector<double> back(np, 0.0);
+    bool any_back = false;
     for (int p = 0; p < np; ++p) {
+      if (!active[p]) {
+        beta[p] = beta_prev[p];  // not recomputed
+        continue;
+      }
+      if (beta[p] > beta_prev[p]) {
+        back[p] = 1.0;
+        any_back = true;
+        beta[p] = beta_prev[p];
+        active[p] = 0;
+        continue;
+      }
       res.stats[p].residual = bnorm[p] > 0 ? beta[p] / bnorm[p] : 0.0;
       if (beta[p] <= opt.tol * bnorm[p]) {
         res.stats[p].converged = 1;
         active[p] = 0;
+      } else if (opt.stall_restarts > 0) {
+        stalls[p] = beta[p] > 0.9 * beta_prev[p] ? stalls[p] + 1 : 0;
+        if (stalls[p] >= opt.stall_restarts) active[p] = 0;  // stagnated
       }
     }
+    if (any_back) {  // x += back_p (xprev - x); z is free here
+      ABH_HIP(hipMemcpyAsync(z.p, xprev.p, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+      std::vector<double> one(size_t(np) * ld, 0.0);
+      for (int p = 0; p < np; ++p) one[size_t(p) * ld] = 1.0;
+      up(hdev.p, one.data(), one.size() * 8);
+      batch_combine(ctx, L, x, 0, 1, hdev.as<double>(), ld, -1.0, z.as<double>());
+      up(sdev.p, back.data(), size_t(np) * 8);
+      batch_scale(ctx, L, z.as<double>(), sdev.as<double>(), z.as<double>());
+      batch_combine(ctx, L, z.as<double>(), 0, 1, hdev.as<double>(), ld, 1.0, x);
+    }
+    if (n > 0) ABH_HIP(hipMemcpyAsync(xprev.p, x, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
   }
   ABH_HIP(hipStreamSynchronize(s));
 }

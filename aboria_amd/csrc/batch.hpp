@@ -44,7 +44,10 @@ void batch_scale(abh_context *ctx, const BatchLayout &L, const double *in, const
 // The Arnoldi basis is orthogonalised by classical Gram-Schmidt applied twice
 // (CGS2) in FP64; the small Hessenberg systems are reduced with Givens
 // rotations on the host.  Each restart recomputes the true residual with
-// op_true (FP64 operator).
+// op_true (FP64 operator), for the problems still iterating; a problem whose
+// true residual grew over a cycle is put back to its previous iterate and
+// stopped, and with opt.stall_restarts > 0 one whose residual fell by less
+// than 10% on that many restarts in a row stops too.
 struct GmresResult {
   std::vector<abh_gmres_stats> stats;
 };

@@ -280,11 +280,16 @@ def bench_rbf(args, torch, ab, wl, ctx, timer, extra):
                       "failed_domains": info["failed_domains"]}
     log("rbf schwarz: %d domains (max %d), %.1f GB inverse rows, setup %.1f s, apply %.2f ms"
         % (info["n_domains"], info["max_size"], info["factor_bytes"] / 1e9, setup_s, ap))
+    # preconditioned solve until every problem has converged or stagnated
+    # (two restarts in a row without a 10% residual cut: the FP64 floor)
+    budget = max(args.rbf_iters, 150)
+    out["stall_restarts"] = 2
+    out["schwarz_iteration_budget"] = budget
     for prec, name in ((ab.FP64, "fp64_schwarz"),):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        x, stats = op.gmres(b, restart=c["restart"], max_iters=args.rbf_iters, tol=c["tol"],
-                            precision=prec, preconditioner=sw)
+        x, stats = op.gmres(b, restart=c["restart"], max_iters=budget, tol=c["tol"],
+                            precision=prec, preconditioner=sw, stall_restarts=2)
         torch.cuda.synchronize()
         solve_s = time.perf_counter() - t0
         res = np.array([s["residual"] for s in stats])
@@ -295,9 +300,10 @@ def bench_rbf(args, torch, ab, wl, ctx, timer, extra):
                      "iterations_converged_max": int(its[[s["converged"] for s in stats]].max())
                      if conv else None,
                      "converged_by_size": by_size(sizes, stats)}
-        log("rbf %s: %d iters in %.1f s, converged %d/%d, residual median %.2e max %.2e, %s"
-            % (name, args.rbf_iters, solve_s, conv, c["nprob"], np.median(res), res.max(),
-               out[name]["converged_by_size"]))
+        out[name]["iterations_max"] = int(its.max())
+        log("rbf %s: finished in %.1f s (max %d iters), converged %d/%d, residual median %.2e "
+            "max %.2e, %s" % (name, solve_s, int(its.max()), conv, c["nprob"], np.median(res),
+                              res.max(), out[name]["converged_by_size"]))
     sw.close()
     op.close()
     extra["rbf_batch"] = out

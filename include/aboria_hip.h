@@ -325,11 +325,18 @@ typedef struct abh_gmres_options {
                             always FP64.  With ABH_FP32_COMPENSATED every
                             restart recomputes the true residual in FP64
                             (iterative refinement).                          */
-  int32_t reserved;
+  int32_t stall_restarts; /* 0: iterate to max_iters.  k > 0 (no counterpart
+                            in the reference): stop a problem after k
+                            consecutive restarts that each fail to cut its
+                            true residual by 10% (converged stays 0), so a
+                            batch ends once every problem has converged or
+                            stagnated, e.g. at its FP64 residual floor.     */
 } abh_gmres_options;
 
 typedef struct abh_gmres_stats {
-  int32_t iterations;    /* Arnoldi steps taken                              */
+  int32_t iterations;    /* Arnoldi steps taken (a problem whose true
+                            residual grows over a restart cycle is returned
+                            at its previous iterate and stops)              */
   int32_t converged;     /* 1 if the tolerance was met                       */
   double residual;       /* final ||b - A x|| / ||b||, operator in FP64      */
 } abh_gmres_stats;

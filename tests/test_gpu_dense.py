@@ -177,3 +177,26 @@ def test_batch_gmres_with_schwarz_converges(ctx):
         # recomputed in another summation order: rounding of |K| |x| allowed
         assert np.linalg.norm(r) / np.linalg.norm(f[a:e]) <= 2e-8
         assert st[p]["iterations"] < st0[p]["iterations"] or not st0[p]["converged"]
+
+
+def test_batch_gmres_stops_stagnated_problems(ctx):
+    """stall_restarts: a problem whose residual cannot reach the tolerance
+    (multiquadric at N = 6000: FP64 rounding of K x sits far above 1e-8 ||b||)
+    stops after two restarts without a 10% residual cut; a solvable one in
+    the same batch converges as without the option."""
+    sizes = [1200, 6000]
+    pos, off = make_batch(sizes, 26)
+    allp = np.concatenate(pos)
+    f = np.sin(2 * np.pi * allp[:, 0]) * np.cos(2 * np.pi * allp[:, 1])
+    A = ab.DenseBatchOperator(ctx, off, torch.tensor(allp, device="cuda"),
+                              ab.Kernel.multiquadric(0.05))
+    sw = ab.SchwartzPreconditioner.for_dense_batch(A, [0, 0], [1, 1], 64.0)
+    bt = torch.tensor(f, device="cuda")
+    _, st = A.gmres(bt, restart=50, max_iters=3000, preconditioner=sw, stall_restarts=2)
+    _, st0 = A.gmres(bt, restart=50, max_iters=3000, preconditioner=sw)
+    print("stall", st, "no stall", st0, flush=True)
+    assert st[0]["converged"] and st[0]["iterations"] == st0[0]["iterations"]
+    assert not st[1]["converged"] and st[1]["iterations"] < 3000
+    assert st0[1]["iterations"] >= st[1]["iterations"]
+    # never returned worse than its best restart: residual below the start
+    assert st[1]["residual"] <= 1.0 and st0[1]["residual"] <= 1.0
