@@ -26,20 +26,6 @@ constexpr int kRows = 256;
 constexpr int kVariants = 3;
 constexpr int kRptOf[kVariants] = {1, 2, 4};
 
-// sqrt(s) for a normal positive s (here s >= c^2 > 0): the hardware
-// reciprocal square root refined by one Goldschmidt step and one Newton
-// correction, within about one ulp (the libm square root is correctly
-// rounded, and costs twice the instructions with its range scaling)
-__device__ __forceinline__ double sqrt_pos(double s) {
-  const double y = __builtin_amdgcn_rsq(s);
-  double g = s * y, h = 0.5 * y;
-  const double r = fma(-h, g, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  const double d = fma(-g, g, s);
-  return fma(d, h, g);
-}
-
 // MQ: the multiquadric sqrt(|dx|^2 + c^2) with fused multiply-adds (the
 // batched RBF systems' kernel), instead of the generic profile.
 template <int D, bool FP32, int RPT, bool MQ>
@@ -108,7 +94,7 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
             const double dx = pj[d] - xi[q][d];
             s = fma(dx, dx, s);
           }
-          acc[q] = fma(sqrt_pos(s), xj, acc[q]);
+          acc[q] = fma(abh::sqrt_fast(s), xj, acc[q]);
         } else if (FP32) {
           float s = 0.0f;
 #pragma unroll
@@ -316,6 +302,7 @@ abh_status abh_dense_batch_create(abh_context *ctx, int dim, int nprob, const in
     op->k.type = kernel->type;
     op->k.param = kernel->param;
     op->k.param2 = kernel->param * kernel->param;
+    abh::set_inverses(op->k);
     op->L.build(ctx, offsets, nprob);
     ABH_REQUIRE(op->L.total() == 0 || pos, ABH_ERR_INVALID_ARG, "pos is null");
     // tiles, largest problem first
@@ -454,6 +441,7 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const doubl
     k.type = kernel->type;
     k.param = kernel->param;
     k.param2 = kernel->param * kernel->param;
+    abh::set_inverses(k);
     const bool f32 = precision == ABH_FP32_COMPENSATED;
     hipStream_t s = ctx->stream;
     const unsigned grid = unsigned(abh::ceil_div(nrows, kRows));

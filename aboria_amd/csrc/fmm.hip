@@ -1284,6 +1284,7 @@ abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low, const do
     f->kern.type = kernel->type;
     f->kern.param = kernel->param;
     f->kern.param2 = kernel->param * kernel->param;
+    abh::set_inverses(f->kern);
     dispatch_order(order, [&](auto NN) { create_impl<decltype(NN)::value>(f, pos, n_leaf); });
   });
   if (st != ABH_OK) {

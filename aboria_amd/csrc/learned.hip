@@ -958,6 +958,7 @@ abh_status abh_ainv_create_lsq(abh_context *ctx, const abh_cells *cells, const a
     kd.type = kernel->type;
     kd.param = kernel->param;
     kd.param2 = kernel->param * kernel->param;
+    abh::set_inverses(kd);
     DevBuf fail;
     fail.ensure(64);
     hipStream_t s = ctx->stream;

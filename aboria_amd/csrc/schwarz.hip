@@ -703,6 +703,7 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
     k.type = kernel->type;
     k.param = kernel->param;
     k.param2 = kernel->param * kernel->param;
+    abh::set_inverses(k);
     k.w0 = 1.0;
     const double r2 = radius > 0.0 ? radius * radius : -1.0;
     factorise(s, cells->pos.as<double>(), k, r2, diag_shift, false);

@@ -1428,6 +1428,7 @@ abh_status abh_sparse_op_create(abh_context *ctx, const abh_cells *cols, int64_t
     op->k.type = kernel->type;
     op->k.param = kernel->param;
     op->k.param2 = kernel->param * kernel->param;
+    abh::set_inverses(op->k);
     op->k.w0 = kernel->w0;
     op->k.wr = kernel->w_row;
     op->k.wc = kernel->w_col;
