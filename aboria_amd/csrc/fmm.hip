@@ -739,31 +739,41 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
                                               const double *__restrict__ G,
                                               double *__restrict__ ys, KernelDesc k, FmmConsts c) {
   constexpr int NC = Cheb<N>::NC;
-  constexpr int SRC = 256;  // sources staged per round
-  // leaf local expansion, then the [4][64][kRhs] reduction of the P2P parts
-  __shared__ double Gs[NC * kRhs > 4 * 64 * kRhs ? NC * kRhs : 4 * 64 * kRhs];
+  constexpr int SRC = 256;   // sources staged per round
+  constexpr int ECH = 64;    // P2P list entries per chunk
+  // leaf local expansion, then the [parts][slots][kRhs] reduction of the
+  // P2P parts
+  __shared__ double Gs[NC * kRhs > 256 * kRhs ? NC * kRhs : 256 * kRhs];
   __shared__ double Ps[SRC][3];
-  __shared__ double Qs[SRC][kRhs];
+  __shared__ __align__(16) double Qs[SRC][kRhs];
+  __shared__ int ebeg[ECH], epre[ECH + 1];
   const int li = blockIdx.x;
   if (li >= nleaf) return;
   const int node = leaf_nodes[li];
   const int2 rg = leaf_range[li];
-  const int pl = threadIdx.x & 63, part = threadIdx.x >> 6;
   double blo[3], bhi[3];
   for (int d = 0; d < 3; ++d) {
     blo[d] = box[int64_t(node) * 6 + d];
     bhi[d] = box[int64_t(node) * 6 + 3 + d];
   }
+  const int e0 = p2p_ptr[li], e1 = p2p_ptr[li + 1];
   for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
-    const int p = p0 + pl;
-    const bool act = p < rg.y;
+    // targets of this chunk on TS lane slots, the block's 256 threads split
+    // into 256 / TS parts that share out the sources (and, for the local
+    // expansion, the right-hand sides)
+    const int T = rg.y - p0 < 64 ? rg.y - p0 : 64;
+    const int TS = T <= 16 ? 16 : (T <= 32 ? 32 : 64);
+    const int parts = 256 / TS, rp = kRhs / parts;
+    const int tg = threadIdx.x % TS, part = threadIdx.x / TS;
+    const int p = p0 + tg;
+    const bool act = tg < T;
     double x[3] = {0.0, 0.0, 0.0};
     if (act)
       for (int d = 0; d < 3; ++d) x[d] = pos[int64_t(p) * 3 + d];
     __syncthreads();
     for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
     __syncthreads();
-    // L2P for right-hand sides 4*part .. 4*part+3
+    // L2P for right-hand sides rp * part .. rp * part + rp - 1
     double l2p[4] = {0.0, 0.0, 0.0, 0.0};
     if (act) {
       double S[3][N];
@@ -784,11 +794,13 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
             const double R = sxy * S[2][jz];
             const int j = (jx * N + jy) * N + jz;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) l2p[r] += R * Gs[j * kRhs + 4 * part + r];
+            for (int r = 0; r < 4; ++r)
+              if (r < rp) l2p[r] += R * Gs[j * kRhs + rp * part + r];
           }
         }
     }
-    // P2P: this thread takes sources part, part+4, ...
+    // P2P over the concatenated sources of the leaf's P2P list: rounds of
+    // 256 staged sources, this thread taking those congruent to its part
     double acc[kRhs];
     float a32[kRhs], c32[kRhs];
 #pragma unroll
@@ -797,20 +809,42 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
       a32[r] = 0.0f;
       c32[r] = 0.0f;
     }
-    for (int e = p2p_ptr[li]; e < p2p_ptr[li + 1]; ++e) {
-      const int2 sr = leaf_range[p2p_src[e]];
-      for (int s0 = sr.x; s0 < sr.y; s0 += SRC) {
-        const int ns = min(SRC, sr.y - s0);
+    for (int ec = e0; ec < e1; ec += ECH) {
+      const int ne = e1 - ec < ECH ? e1 - ec : ECH;
+      __syncthreads();
+      if (threadIdx.x < ne) {
+        const int2 sr = leaf_range[p2p_src[ec + threadIdx.x]];
+        ebeg[threadIdx.x] = sr.x;
+        epre[threadIdx.x + 1] = sr.y - sr.x;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        epre[0] = 0;
+        for (int i = 0; i < ne; ++i) epre[i + 1] += epre[i];
+      }
+      __syncthreads();
+      const int total = epre[ne];
+      for (int b0 = 0; b0 < total; b0 += SRC) {
+        const int ns = total - b0 < SRC ? total - b0 : SRC;
         __syncthreads();
         if (threadIdx.x < ns) {
-          const int q = s0 + threadIdx.x;
+          const int gsrc = b0 + threadIdx.x;
+          int lo = 0, hi = ne - 1;  // entry i with epre[i] <= gsrc < epre[i + 1]
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (epre[mid] <= gsrc) lo = mid;
+            else hi = mid - 1;
+          }
+          const int q = ebeg[lo] + (gsrc - epre[lo]);
           for (int d = 0; d < 3; ++d) Ps[threadIdx.x][d] = pos[int64_t(q) * 3 + d];
+          const double2 *qx = reinterpret_cast<const double2 *>(xs + int64_t(q) * kRhs);
+          double2 *qd = reinterpret_cast<double2 *>(&Qs[threadIdx.x][0]);
+#pragma unroll
+          for (int r = 0; r < kRhs / 2; ++r) qd[r] = qx[r];
         }
-        for (int t = threadIdx.x; t < ns * kRhs; t += 256)
-          Qs[t / kRhs][t % kRhs] = xs[int64_t(s0 + t / kRhs) * kRhs + t % kRhs];
         __syncthreads();
         if (act) {
-          for (int j = part; j < ns; j += 4) {
+          for (int j = part; j < ns; j += parts) {
             double s2 = 0.0;
             for (int d = 0; d < 3; ++d) {
               const double dx = Ps[j][d] - x[d];
@@ -829,19 +863,20 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
         }
       }
     }
-    // combine the four partial P2P sums (Gs is free now)
+    // combine the partial P2P sums of the parts (Gs is free now)
     __syncthreads();
-    double *red = Gs;  // [4][64][kRhs]
+    double *red = Gs;  // [parts][TS][kRhs]
 #pragma unroll
     for (int r = 0; r < kRhs; ++r)
-      red[(part * 64 + pl) * kRhs + r] = acc[r] + (FP32 ? double(a32[r]) + double(c32[r]) : 0.0);
+      red[(part * TS + tg) * kRhs + r] = acc[r] + (FP32 ? double(a32[r]) + double(c32[r]) : 0.0);
     __syncthreads();
     if (act) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int rr = 4 * part + r;
+        if (r >= rp) break;
+        const int rr = rp * part + r;
         double v = l2p[r];
-        for (int q = 0; q < 4; ++q) v += red[(q * 64 + pl) * kRhs + rr];
+        for (int q = 0; q < parts; ++q) v += red[(q * TS + tg) * kRhs + rr];
         ys[int64_t(p) * kRhs + rr] = v;
       }
     }
