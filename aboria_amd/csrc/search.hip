@@ -376,6 +376,14 @@ constexpr int kSeg = 16;
 
 
 
+// Writes v to this lane's word of the ring slot whose number k8 holds in
+// bits 8..12 (slot-major ring, 256 bytes per slot).
+static_assert(kRing == 32 && kGroup == 64, "ring_put assumes 32 slots of 64 lanes");
+__device__ __forceinline__ void ring_put(uint32_t *ring, uint32_t k8, int lane, uint32_t v) {
+  *reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(ring) +
+                                ((k8 & 0x1f00u) | (uint32_t(lane) << 2))) = v;
+}
+
 // a wave-uniform double moved to scalar registers
 __device__ __forceinline__ double uniform_f64(double v) {
   const uint64_t b = uint64_t(__double_as_longlong(v));
@@ -443,15 +451,22 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
   extern __shared__ __align__(16) unsigned char lds[];
   constexpr bool kMV = (MODE == kMatvec64 || MODE == kMatvec32);
   const int S = a.nslots;
-  // matvec only: candidate tile and right-hand sides in front of the slots
+  // fill passes: the output ring, [32][64] words in static LDS (a lane's
+  // ring address is a constant plus (slot << 8) | (lane << 2))
+  constexpr bool kFillMode = (MODE == kFill || MODE == kFillPadded);
+  const bool use_ring = kFillMode && a.ring;
+  uint32_t *ring = nullptr;
+  if constexpr (kFillMode) {
+    __shared__ __align__(16) uint32_t ring_lds[kRing * kGroup];
+    ring = ring_lds;
+  }
   Slot *slots = reinterpret_cast<Slot *>(lds);
   int32_t *sorder = reinterpret_cast<int32_t *>(slots + S);
   uint8_t *mult = reinterpret_cast<uint8_t *>(sorder + S);  // [S][64]
   // count / list passes: staged candidates (coordinates [D][64], index [64])
   StageRec *st = reinterpret_cast<StageRec *>(
       lds + ((size_t(S) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(S + 1) + 15) & ~size_t(15)));
-  uint32_t *ring = reinterpret_cast<uint32_t *>(st + kGroup);  // [32][64], fill only
-  double *xs = reinterpret_cast<double *>(st + kGroup);        // [64][kMaxRhs], matvec only
+  double *xs = reinterpret_cast<double *>(st + kGroup);  // [64][kMaxRhs], matvec only
   const int lane = threadIdx.x;
   const int nwork = int(*a.nwork);
   int wd[3] = {a.w[0], a.w[1], a.w[2]};
@@ -612,7 +627,6 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
     if (kList && active) out = a.row_ptr[row];
     // ring state: entries [fpos, out + cnt) of this lane's row are staged
     int64_t fpos = out;
-    const bool use_ring = (MODE == kFill || MODE == kFillPadded) && a.ring;
     uint32_t *rl = ring + lane;  // slot k of this lane: rl[k * kGroup]
     // store the segment holding fpos once it is complete (64-byte vector
     // stores when it starts at its segment boundary, the row's first,
@@ -718,10 +732,16 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
           }
           __syncthreads();  // single-wave group: the staged block is complete
           if (use_ring && single) {
-            // at most 16 appends per lane between segment stores, so the
-            // ring never holds more than 32 unstored entries
+            // at most 16 appends per lane between segment stores: a block
+            // starts with the next entry in the segment of the first
+            // unstored one, so fewer than 32 entries are ever unstored and
+            // the slot of the next entry is free.  Every candidate is
+            // written there and the position advances on a hit only.
             for (int t0 = 0; t0 < nk; t0 += kSeg) {
               const int t1 = nk - t0 < kSeg ? nk : t0 + kSeg;
+              // byte offset of the next entry's ring slot in bits 8..12
+              const uint32_t k8s = uint32_t(out + cnt) << 8;
+              uint32_t k8 = k8s;
               int t = t0;
               for (; t + kStageUnroll <= t1; t += kStageUnroll) {
                 double e[kStageUnroll][3];
@@ -741,8 +761,8 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
                 }
 #pragma unroll
                 for (int u = 0; u < kStageUnroll; ++u) {
-                  if (h[u]) rl[int((out + cnt) & (kRing - 1)) * kGroup] = uint32_t(jj[u]);
-                  cnt += h[u] ? 1 : 0;
+                  ring_put(ring, k8, lane, uint32_t(jj[u]));
+                  k8 += h[u] ? 256u : 0u;
                 }
               }
               for (; t < t1; ++t) {
@@ -755,12 +775,45 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
                   const double dx = e1[d] - qi[d];
                   acc = acc + dx * dx;
                 }
-                if (acc <= r2m) {
-                  rl[int((out + cnt) & (kRing - 1)) * kGroup] = uint32_t(j1);
-                  ++cnt;
-                }
+                ring_put(ring, k8, lane, uint32_t(j1));
+                k8 += acc <= r2m ? 256u : 0u;
               }
+              cnt += int((k8 - k8s) >> 8);
               ring_segment();
+            }
+            __syncthreads();  // the next block overwrites the stage
+            continue;
+          }
+          if (MODE == kCount) {
+            // a hit adds m (1 whenever no lane visits the bucket twice)
+            int t = 0;
+            for (; t + kStageUnroll <= nk; t += kStageUnroll) {
+              double e[kStageUnroll][3];
+              int jj[kStageUnroll];
+#pragma unroll
+              for (int u = 0; u < kStageUnroll; ++u) load_stage(st[t + u], e[u], jj[u]);
+#pragma unroll
+              for (int u = 0; u < kStageUnroll; ++u) {
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                  const double dx = e[u][d] - qi[d];
+                  acc = acc + dx * dx;
+                }
+                cnt += acc <= r2m ? m : 0;
+              }
+            }
+            for (; t < nk; ++t) {
+              double e1[3];
+              int j1;
+              load_stage(st[t], e1, j1);
+              double acc = 0.0;
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const double dx = e1[d] - qi[d];
+                acc = acc + dx * dx;
+              }
+              cnt += acc <= r2m ? m : 0;
             }
             __syncthreads();  // the next block overwrites the stage
             continue;
@@ -1133,10 +1186,10 @@ static SearchArgs base_args(const QueryPlan &p) {
   return a;
 }
 
-static size_t lds_bytes(int nslots, bool matvec, bool ring = false) {
+// dynamic LDS of k_search (the fill passes add their static output ring)
+static size_t lds_bytes(int nslots, bool matvec) {
   const size_t slots = size_t(nslots) * (sizeof(Slot) + 4 + kGroup) + 4 * size_t(nslots + 1);
   return ((slots + 15) & ~size_t(15)) + size_t(kGroup) * sizeof(StageRec) +
-         (ring ? size_t(kGroup) * kRing * 4 : 0) +
          (matvec ? size_t(kGroup) * kMaxRhs * sizeof(double) : 0);
 }
 
@@ -1155,7 +1208,7 @@ template <int MODE> static void launch_search(const QueryPlan &p, const SearchAr
   k_zero_u32<<<1, 1, 0, p.ctx->stream>>>(a.next);
   check_launch();
   const GridParams g = p.cells->params();
-  const size_t lds = lds_bytes(a.nslots, MODE == kMatvec64 || MODE == kMatvec32, a.ring != 0);
+  const size_t lds = lds_bytes(a.nslots, MODE == kMatvec64 || MODE == kMatvec32);
   if (p.D == 1) k_search<1, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
   else if (p.D == 2) k_search<2, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
   else k_search<3, MODE><<<grid, kGroup, lds, p.ctx->stream>>>(a, g);
