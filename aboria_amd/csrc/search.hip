@@ -898,7 +898,48 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
               xs[sl * kMaxRhs + r] = r < a.nrhs ? a.x[int64_t(r) * a.ldx + jc] : 0.0;
           }
           __syncthreads();  // single-wave group: the staged block is complete
-          for (int t = 0; t < nk; ++t) {
+          // radius tests of the whole block first, this lane's hits as bits
+          // (candidate t at bit nk - 1 - t), then the kernel for the hits
+          // only, in candidate order: a round evaluates one hit per lane,
+          // so lanes do not wait on the kernels of candidates they miss
+          uint64_t hm = 0;
+          {
+            int t = 0;
+            for (; t + kStageUnroll <= nk; t += kStageUnroll) {
+              double e[kStageUnroll][3];
+              int jj[kStageUnroll];
+#pragma unroll
+              for (int u = 0; u < kStageUnroll; ++u) load_stage(st[t + u], e[u], jj[u]);
+#pragma unroll
+              for (int u = 0; u < kStageUnroll; ++u) {
+                double acc = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                  const double dx = e[u][d] - qi[d];
+                  acc = acc + dx * dx;
+                }
+                hm = (hm << 1) + (acc <= r2m ? 1ull : 0ull);
+              }
+            }
+            for (; t < nk; ++t) {
+              double e1[3];
+              int j1;
+              load_stage(st[t], e1, j1);
+              double acc = 0.0;
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                const double dx = e1[d] - qi[d];
+                acc = acc + dx * dx;
+              }
+              hm = (hm << 1) + (acc <= r2m ? 1ull : 0ull);
+            }
+          }
+          // (a divergent loop: the wave runs as many rounds as its lane
+          // with the most hits)
+          while (hm != 0) {
+            const int p = 63 - __builtin_clzll(hm);
+            hm ^= 1ull << p;
+            const int t = nk - 1 - p;
             double e1[3];
             int j1;
             load_stage(st[t], e1, j1);
@@ -908,24 +949,22 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
               const double dx = e1[d] - qi[d];
               acc = acc + dx * dx;
             }
-            if (acc <= r2m) {
-              double w = wrow;
-              if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j1];
-              const double *xr = xs + t * kMaxRhs;
-              if (MODE == kMatvec64) {
-                const double f = phi64(a.k, acc) * w;
-                for (int rep = 0; rep < m; ++rep) {
+            double w = wrow;
+            if (a.k.col_s) w = w + a.k.wc * a.k.col_s[j1];
+            const double *xr = xs + t * kMaxRhs;
+            if (MODE == kMatvec64) {
+              const double f = phi64(a.k, acc) * w;
+              for (int rep = 0; rep < m; ++rep) {
 #pragma unroll
-                  for (int r = 0; r < kMaxRhs; ++r)
-                    if (r < a.nrhs) acc64[r] += f * xr[r];
-                }
-              } else {
-                const float f = phi32(a.k, float(acc)) * float(w);
-                for (int rep = 0; rep < m; ++rep) {
+                for (int r = 0; r < kMaxRhs; ++r)
+                  if (r < a.nrhs) acc64[r] += f * xr[r];
+              }
+            } else {
+              const float f = phi32(a.k, float(acc)) * float(w);
+              for (int rep = 0; rep < m; ++rep) {
 #pragma unroll
-                  for (int r = 0; r < kMaxRhs; ++r)
-                    if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(xr[r]));
-                }
+                for (int r = 0; r < kMaxRhs; ++r)
+                  if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(xr[r]));
               }
             }
           }

@@ -94,6 +94,34 @@ def test_build_and_search_parity(ctx, case):
             assert np.array_equal(np.sort(a), np.sort(b)), (name, i)
 
 
+@pytest.mark.parametrize("precision", [ab.FP64, ab.FP32_COMPENSATED])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_sparse_matvec_cases(ctx, case, precision):
+    """On-the-fly Wendland C2 operator on three right-hand sides over the
+    search cases (tiny periodic grids visit a bucket under several images,
+    so a pair can count more than once)."""
+    name, D, N, per, n_leaf, r, lo, hi = case
+    print("case", name, precision, flush=True)
+    pos, alive = make_case(D, N, lo, hi, sum(name.encode()) * 7 + N)
+    low, high, periodic = [lo] * D, [hi] * D, [per] * D
+    cl, gp, ga, gna = gpu_build(ctx, pos, alive, low, high, periodic, n_leaf)
+    oc, op, oa, ona = orc_build(pos, alive, low, high, periodic, n_leaf)
+    sp = cl.positions().clone()
+    n = sp.shape[0]
+    x = np.random.default_rng(3).standard_normal((n, 3))
+    K = ab.SparseOperator(ctx, cl, sp, r, ab.Kernel.wendland_c2(r))
+    xt = torch.tensor(x.T.copy(), device="cuda")
+    yt = torch.zeros_like(xt)
+    K.apply_colmajor(xt, yt, precision)
+    y = yt.cpu().numpy().T
+    kern = (2, r)
+    y_o = oc.sparse_matvec(oc.positions, r, kern, x)
+    scale = oc.sparse_matvec(oc.positions, r, kern, np.abs(x))
+    tol = 1e-13 if precision == ab.FP64 else 2e-6
+    err = np.abs(y - y_o)
+    assert np.all(err <= tol * scale + 1e-300), float((err / (scale + 1e-300)).max())
+
+
 def test_uniform_workload_full(ctx):
     """The N=100,000 uniform workload: build bit-exact, every neighbour list
     equal to the oracle's, sparse Gaussian matvec within tolerance."""
