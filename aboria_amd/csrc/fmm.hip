@@ -473,35 +473,52 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
     }
     if (!any) continue;
     const double *Mo = Mf + int64_t(o) * F::PER_OFFSET;
-    // operand fragments of k step ks+1 are loaded while step ks runs on
-    // the matrix cores (register double buffer)
-    double a[RPW], b[4], an[RPW], bn[4];
+    // operand fragments are loaded three k steps ahead of the matrix cores
+    // (three register buffers).  Every load is unconditional, from a
+    // clamped, valid address (missing children read a present child's
+    // expansion, k steps past the end the last one), so that the loads stay
+    // in flight across the steps; the products of missing children and of
+    // row blocks past the end are not issued.
+    int svalid = -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) svalid = sc[c] >= 0 ? sc[c] : svalid;
+    int sl[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sl[c] = sc[c] >= 0 ? sc[c] : svalid;
     auto load = [&](int ks, double *av, double *bv) {
+      const int kk = ks < KS ? ks : KS - 1;
 #pragma unroll
       for (int i = 0; i < RPW; ++i) {
-        const int rb = rq + 4 * i;
-        av[i] = rb < RB ? Mo[(int64_t(ks) * RB + rb) * 64 + lane] : 0.0;
+        const int rb = rq + 4 * i < RB ? rq + 4 * i : RB - 1;
+        av[i] = Mo[(int64_t(kk) * RB + rb) * 64 + lane];
       }
-      const int k = ks * 4 + kl;
+      const int k = kk * 4 + kl < NC ? kk * 4 + kl : NC - 1;
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        bv[c] = (sc[c] >= 0 && k < NC) ? W[(int64_t(sc[c]) * NC + k) * kRhs + rl] : 0.0;
+      for (int c = 0; c < 4; ++c) bv[c] = W[(int64_t(sl[c]) * NC + k) * kRhs + rl];
     };
-    load(0, a, b);
-    for (int ks = 0; ks < KS; ++ks) {
-      if (ks + 1 < KS) load(ks + 1, an, bn);
+    auto step = [&](const double *av, const double *bv) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (sc[c] < 0) continue;
 #pragma unroll
         for (int i = 0; i < RPW; ++i)
           if (rq + 4 * i < RB)
-            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[c], acc[i][c], 0, 0, 0);
+            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[c], acc[i][c], 0, 0, 0);
       }
-#pragma unroll
-      for (int i = 0; i < RPW; ++i) a[i] = an[i];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) b[c] = bn[c];
+    };
+    // k steps past KS hold zero operands in no product: A rows are zero
+    // padded only up to KS, so the steps are guarded
+    double a0[RPW], b0[4], a1[RPW], b1[4], a2[RPW], b2[4];
+    load(0, a0, b0);
+    load(1, a1, b1);
+    load(2, a2, b2);
+    for (int ks = 0; ks < KS; ks += 3) {
+      step(a0, b0);
+      load(ks + 3, a0, b0);
+      if (ks + 1 < KS) step(a1, b1);
+      load(ks + 4, a1, b1);
+      if (ks + 2 < KS) step(a2, b2);
+      load(ks + 5, a2, b2);
     }
   }
 #pragma unroll
