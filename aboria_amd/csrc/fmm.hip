@@ -685,36 +685,55 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
     if (!any) continue;
     const bf16x8 *mh = reinterpret_cast<const bf16x8 *>(Mh) + int64_t(o) * (kBfPerOffset / 8);
     const bf16x8 *ml = reinterpret_cast<const bf16x8 *>(Ml) + int64_t(o) * (kBfPerOffset / 8);
+    // every fragment of a k step is loaded before its products are issued,
+    // unconditionally from a valid address (row tiles past the end read the
+    // last one, targets without a source read a present source and zero
+    // the fragment), so the loads are in flight together
+    int svalid = 0;
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int p = p0 + q;
+        svalid = src[tt][2 * p] >= 0 ? src[tt][2 * p] : svalid;
+        svalid = src[tt][2 * p + 1] >= 0 ? src[tt][2 * p + 1] : svalid;
+      }
     for (int ks = 0; ks < kKs16; ++ks) {
       bf16x8 ah[2], al[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const int rt = rq + 4 * i;
-        if (rt < kRt) {
-          ah[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
-          al[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
-        }
+        const int rt = rq + 4 * i < kRt ? rq + 4 * i : kRt - 1;
+        ah[i] = mh[(int64_t(ks) * kRt + rt) * 64 + lane];
+        al[i] = ml[(int64_t(ks) * kRt + rt) * 64 + lane];
       }
+      bf16x8 bh[2][2], bl[2][2];
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int sv = sl[tt][q] >= 0 ? sl[tt][q] : svalid;
+          const int64_t e = ((int64_t(sv) * kKs16 + ks) * kRhs + r) * 16 + 8 * half;
+          bh[tt][q] = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
+          bl[tt][q] = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
+        }
 #pragma unroll
       for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (!act[tt][q]) continue;
-          bf16x8 bh, bl;
-          if (sl[tt][q] >= 0) {
-            const int64_t e = ((int64_t(sl[tt][q]) * kKs16 + ks) * kRhs + r) * 16 + 8 * half;
-            bh = *reinterpret_cast<const bf16x8 *>(&Wh[e]);
-            bl = *reinterpret_cast<const bf16x8 *>(&Wl[e]);
-          } else {
+          if (sl[tt][q] < 0) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) bh[j] = bl[j] = 0;
+            for (int j = 0; j < 8; ++j) bh[tt][q][j] = bl[tt][q][j] = 0;
           }
 #pragma unroll
           for (int i = 0; i < 2; ++i) {
             if (rq + 4 * i >= kRt) continue;
-            acc[i][tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh, acc[i][tt][q], 0, 0, 0);
-            acc[i][tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl, acc[i][tt][q], 0, 0, 0);
-            acc[i][tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh, acc[i][tt][q], 0, 0, 0);
+            acc[i][tt][q] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[tt][q], acc[i][tt][q], 0, 0, 0);
+            acc[i][tt][q] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[tt][q], acc[i][tt][q], 0, 0, 0);
+            acc[i][tt][q] =
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[tt][q], acc[i][tt][q], 0, 0, 0);
           }
         }
     }
