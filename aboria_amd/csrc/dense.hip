@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
             const double dx = pj[d] - xi[q][d];
             s = fma(dx, dx, s);
           }
-          acc[q] = fma(abh::sqrt_fast(s), xj, acc[q]);
+          acc[q] = fma(abh::sqrt_pos(s), xj, acc[q]);  // s >= c^2 >= 1e-280
         } else if (FP32) {
           float s = 0.0f;
 #pragma unroll
@@ -217,7 +217,8 @@ struct abh_dense_batch {
     const int64_t *o = off.as<int64_t>();
     const int nt = ntiles[v];
     const int ow = overwrite ? 1 : 0;
-    const bool mq = k.type == ABH_KERNEL_MULTIQUADRIC && !k.row_s && !k.col_s;
+    const bool mq =
+        k.type == ABH_KERNEL_MULTIQUADRIC && !k.row_s && !k.col_s && k.param2 >= 1e-280;
 #define ABH_DENSE_LAUNCH2(DD, MQ)                                                               \
   if (fp32)                                                                                     \
     k_dense<DD, true, RPT, MQ><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, y, act, k, alpha, ow);   \

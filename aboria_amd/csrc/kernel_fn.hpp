@@ -21,18 +21,23 @@ inline void set_inverses(KernelDesc &k) {
   k.inv2 = k.param2 != 0.0 ? 1.0 / k.param2 : 0.0;
 }
 
-// sqrt(s) for s >= 0: the hardware reciprocal square root refined by one
-// Goldschmidt step and one Newton correction (about one ulp; the libm square
-// root is correctly rounded and costs twice the instructions with its range
-// scaling); s below 1e-280 gives 0
-__device__ __forceinline__ double sqrt_fast(double s) {
+// sqrt(s) for s >= 1e-280: the hardware reciprocal square root refined by
+// one Goldschmidt step and one Newton correction (about one ulp; the libm
+// square root is correctly rounded and costs twice the instructions with its
+// range scaling)
+__device__ __forceinline__ double sqrt_pos(double s) {
   const double y = __builtin_amdgcn_rsq(s);
   double g = s * y, h = 0.5 * y;
   const double r = fma(-h, g, 0.5);
   g = fma(g, r, g);
   h = fma(h, r, h);
   const double d = fma(-g, g, s);
-  g = fma(d, h, g);
+  return fma(d, h, g);
+}
+
+// sqrt(s) for s >= 0 (s below 1e-280 gives 0)
+__device__ __forceinline__ double sqrt_fast(double s) {
+  const double g = sqrt_pos(s);
   return s >= 1e-280 ? g : 0.0;
 }
 
