@@ -143,15 +143,23 @@ def bench_search(args, torch, ab, wl, ctx, stream, extra):
     # measured HBM bytes per fill launch (rocprofv3 FETCH_SIZE / WRITE_SIZE,
     # scaled by a plain copy of known size), recorded beside the profiles
     traffic = None
+    valu_insts = None
     tpath = os.path.join(ROOT, "profiles", "k_search_fill_traffic.json")
     if os.path.exists(tpath) and n == 4_000_000:
         with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            tj = json.load(fh)
+        traffic = tj.get("hbm_bytes_per_launch")
+        valu_insts = tj.get("valu_insts_per_launch")
     roofline = {"kernel": "k_search<3,fill>", "bound": "hbm",
                 "achieved": fill_bytes / (fill_med * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "traffic": traffic,
                 "bytes_per_launch": int(fill_bytes), "launch_ms": fill_med}
     roofline["frac"] = roofline["achieved"] / roofline["peak"]
+    if valu_insts:
+        # the fill's radius tests are FP64 vector work: its SQ_INSTS_VALU per
+        # launch (rocprofv3) at 4 cycles per wave64 instruction on the 1,024
+        # SIMDs at 2.4 GHz, over this run's launch time
+        roofline["valu_issue_frac"] = valu_insts * 4 / (1024 * 2.4e9 * fill_med * 1e-3)
     extra["search"] = {"build_ms": float(np.median(build_ms)),
                        "count_ms": float(np.median(count_ms)), "fill_ms": fill_med,
                        "pairs": int(total), "grid": list(cells.grid),
