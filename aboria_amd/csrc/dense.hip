@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
   for (int64_t c0 = 0; c0 < n; c0 += kRows) {
     const int m = int(n - c0 < kRows ? n - c0 : kRows);
     __syncthreads();
-    if (threadIdx.x < m) {
+    if (int(threadIdx.x) < m) {
       const int64_t j = base + c0 + threadIdx.x;
 #pragma unroll
       for (int d = 0; d < D; ++d) sp[threadIdx.x][d] = pos[j * D + d];
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kRows) void k_dense_rect(int64_t nrows, const doubl
   for (int64_t c0 = 0; c0 < ncols; c0 += kRows) {
     const int m = int(ncols - c0 < kRows ? ncols - c0 : kRows);
     __syncthreads();
-    if (threadIdx.x < m) {
+    if (int(threadIdx.x) < m) {
       const int64_t j = c0 + threadIdx.x;
 #pragma unroll
       for (int d = 0; d < D; ++d) sp[threadIdx.x][d] = cp[j * D + d];

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaf_no
   for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
     const int np = min(64, rg.y - p0);
     __syncthreads();
-    if (threadIdx.x < np) {
+    if (int(threadIdx.x) < np) {
       const int p = p0 + threadIdx.x;
       for (int d = 0; d < 3; ++d) {
         const double span = bhi[d] - blo[d];
@@ -848,7 +848,7 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
     for (int ec = e0; ec < e1; ec += ECH) {
       const int ne = e1 - ec < ECH ? e1 - ec : ECH;
       __syncthreads();
-      if (threadIdx.x < ne) {
+      if (int(threadIdx.x) < ne) {
         const int2 sr = leaf_range[p2p_src[ec + threadIdx.x]];
         ebeg[threadIdx.x] = sr.x;
         epre[threadIdx.x + 1] = sr.y - sr.x;
@@ -863,7 +863,7 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
       for (int b0 = 0; b0 < total; b0 += SRC) {
         const int ns = total - b0 < SRC ? total - b0 : SRC;
         __syncthreads();
-        if (threadIdx.x < ns) {
+        if (int(threadIdx.x) < ns) {
           const int gsrc = b0 + threadIdx.x;
           int lo = 0, hi = ne - 1;  // entry i with epre[i] <= gsrc < epre[i + 1]
           while (lo < hi) {
