@@ -993,16 +993,16 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
 
 // xr[j][r] = x[r][j] (r < nrhs, zero beyond): the right-hand sides of one
 // column in one 64-byte segment for the CSR product's gathers
+template <typename T>
 __global__ __launch_bounds__(256) void k_rhs_rows(const double *__restrict__ x, int64_t ldx,
-                                                  int nrhs, int64_t n, double *__restrict__ xr) {
+                                                  int nrhs, int64_t n, T *__restrict__ xr) {
   const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (j >= n) return;
-  double v[kMaxRhs];
+  T v[kMaxRhs];
 #pragma unroll
-  for (int r = 0; r < kMaxRhs; ++r) v[r] = r < nrhs ? x[int64_t(r) * ldx + j] : 0.0;
-  double2 *o = reinterpret_cast<double2 *>(xr + j * kMaxRhs);
+  for (int r = 0; r < kMaxRhs; ++r) v[r] = r < nrhs ? T(x[int64_t(r) * ldx + j]) : T(0);
 #pragma unroll
-  for (int r = 0; r < kMaxRhs / 2; ++r) o[r] = make_double2(v[2 * r], v[2 * r + 1]);
+  for (int r = 0; r < kMaxRhs; ++r) xr[j * kMaxRhs + r] = v[r];
 }
 
 // y += K x from the cached CSR form of the operator: one wavefront per row.
@@ -1012,14 +1012,17 @@ __global__ __launch_bounds__(256) void k_rhs_rows(const double *__restrict__ x, 
 // partial sums then combined over the lanes holding the same pair.  FP64
 // reads val; the FP32 form reads val32 (the values the FP32 matvec forms)
 // and sums each lane's terms with compensation.  xr: the right-hand sides by
-// column (k_rhs_rows).
+// column (k_rhs_rows), in FP64, or in FP32 for the FP32 form (which rounds
+// every x to FP32 anyway: a 32-byte row per entry).
 template <bool FP32>
 __global__ __launch_bounds__(256) void k_csr_spmv(int64_t nrows, const int64_t *__restrict__ rp,
                                                   const int32_t *__restrict__ col,
                                                   const double *__restrict__ val,
                                                   const float *__restrict__ val32, int nrhs,
-                                                  const double *__restrict__ xr,
+                                                  const void *__restrict__ xrv,
                                                   double *__restrict__ y, int64_t ldy) {
+  const double *xr = static_cast<const double *>(xrv);
+  const float *xr32 = static_cast<const float *>(xrv);
   const int blk = abh::xcd_block(int((nrows + 3) / 4));
   const int64_t row = int64_t(blk) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -1048,27 +1051,49 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int64_t nrows, const int64_t *
     jn = jn2;
     fn = fn2;
     const int nb = e1 - b < 64 ? int(e1 - b) : 64;
+    // the four gathers of x for the block's 64 entries are issued together,
+    // before the next block's col / val loads (slots past the row's end read
+    // a column of the row, or column 0, and add nothing), then accumulated
+    // in entry order
+    int32_t j[4];
+    double f[4];
+    double2 xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = __shfl(jl, 16 * u + g, 64);
+      f[u] = __shfl(fl, 16 * u + g, 64);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (FP32) {
+        const float2 v = *reinterpret_cast<const float2 *>(xr32 + int64_t(j[u]) * kMaxRhs + 2 * q);
+        xv[u] = make_double2(double(v.x), double(v.y));
+      } else {
+        xv[u] = *reinterpret_cast<const double2 *>(xr + int64_t(j[u]) * kMaxRhs + 2 * q);
+      }
+    }
     const int64_t bn = b + 128 + lane;
     if (bn < e1) {
       jn2 = col[bn];
       fn2 = FP32 ? double(val32[bn]) : val[bn];
     }
 #pragma unroll
-    for (int s0 = 0; s0 < 64; s0 += 16) {
-      if (s0 >= nb) break;
-      const int src = s0 + g;
-      const int32_t j = __shfl(jl, src, 64);
-      const double f = __shfl(fl, src, 64);
-      if (src < nb) {
-        const double2 xv = *reinterpret_cast<const double2 *>(xr + int64_t(j) * kMaxRhs + 2 * q);
-        if (FP32) {
-          const float f32 = float(f);  // exact: f holds an FP32 value
-          kb_add(s32[0], c32[0], f32 * float(xv.x));
-          kb_add(s32[1], c32[1], f32 * float(xv.y));
-        } else {
-          s64[0] = s64[0] + f * xv.x;
-          s64[1] = s64[1] + f * xv.y;
-        }
+    for (int u = 0; u < 4; ++u) {
+      const bool ok = 16 * u + g < nb;
+      if (FP32) {
+        const float f32 = float(f[u]);  // exact: f holds an FP32 value
+        float sa = s32[0], ca = c32[0], sb = s32[1], cb = c32[1];
+        kb_add(sa, ca, f32 * float(xv[u].x));
+        kb_add(sb, cb, f32 * float(xv[u].y));
+        s32[0] = ok ? sa : s32[0];
+        c32[0] = ok ? ca : c32[0];
+        s32[1] = ok ? sb : s32[1];
+        c32[1] = ok ? cb : c32[1];
+      } else {
+        const double ta = s64[0] + f[u] * xv[u].x;
+        const double tb = s64[1] + f[u] * xv[u].y;
+        s64[0] = ok ? ta : s64[0];
+        s64[1] = ok ? tb : s64[1];
       }
     }
   }
@@ -1336,9 +1361,14 @@ static void apply_group(abh_sparse_op *op, int nrhs, const double *x, int64_t ld
     hipStream_t st = op->plan.ctx->stream;
     const int64_t nc = op->ncols;
     op->c_xr.ensure(size_t(nc) * kMaxRhs * 8 + 64);
-    double *xr = op->c_xr.as<double>();
+    void *xr = op->c_xr.p;
     if (nc > 0) {
-      k_rhs_rows<<<unsigned(ceil_div(nc, 256)), 256, 0, st>>>(x, ldx, nrhs, nc, xr);
+      if (fp32)
+        k_rhs_rows<float><<<unsigned(ceil_div(nc, 256)), 256, 0, st>>>(x, ldx, nrhs, nc,
+                                                                         op->c_xr.as<float>());
+      else
+        k_rhs_rows<double><<<unsigned(ceil_div(nc, 256)), 256, 0, st>>>(x, ldx, nrhs, nc,
+                                                                          op->c_xr.as<double>());
       check_launch();
     }
     const unsigned blocks = xcd_grid(int(ceil_div(nq, 4)));
