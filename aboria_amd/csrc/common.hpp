@@ -67,6 +67,7 @@ struct abh_context {
   // every primitive is enqueued on the one context stream)
   abh::DevBuf scan_blocks;                     // per-tile sums of scan_exclusive
   abh::DevBuf sort_keys, sort_vals, sort_hist;  // radix sort ping-pong + histograms
+  abh::DevBuf dense_part;                       // column-split partial sums (abh_dense_apply)
   int n_cu = 256;
   // Handles created on this context keep it alive: abh_context_destroy only
   // marks it closed while any of them exists, the last one frees it.

@@ -126,7 +126,10 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
 }
 
 // Rectangular operator y += K x over separate row / column point sets,
-// up to kRectRhs right-hand sides per pass (column-major x, y).
+// up to kRectRhs right-hand sides per pass (column-major x, y).  Block
+// (bx, by) takes rows bx * kRows .. and columns by * cspan ..; with several
+// column ranges each writes its partial sums to part[by][r][row], which
+// k_rect_reduce adds to y in range order (deterministic).
 constexpr int kRectRhs = 16;
 
 template <int D, bool FP32>
@@ -134,7 +137,8 @@ __global__ __launch_bounds__(kRows) void k_dense_rect(int64_t nrows, const doubl
                                                       int64_t ncols, const double *__restrict__ cp,
                                                       KernelDesc k, int nrhs,
                                                       const double *__restrict__ x, int64_t ldx,
-                                                      double *__restrict__ y, int64_t ldy) {
+                                                      double *__restrict__ y, int64_t ldy,
+                                                      int64_t cspan, double *__restrict__ part) {
   __shared__ double sp[kRows][D];
   __shared__ double sx[kRectRhs][kRows];
   const int64_t i = int64_t(blockIdx.x) * kRows + threadIdx.x;
@@ -150,8 +154,10 @@ __global__ __launch_bounds__(kRows) void k_dense_rect(int64_t nrows, const doubl
     a32[r] = 0.0f;
     c32[r] = 0.0f;
   }
-  for (int64_t c0 = 0; c0 < ncols; c0 += kRows) {
-    const int m = int(ncols - c0 < kRows ? ncols - c0 : kRows);
+  const int64_t cb = int64_t(blockIdx.y) * cspan;
+  const int64_t ce = cb + cspan < ncols ? cb + cspan : ncols;
+  for (int64_t c0 = cb; c0 < ce; c0 += kRows) {
+    const int m = int(ce - c0 < kRows ? ce - c0 : kRows);
     __syncthreads();
     if (int(threadIdx.x) < m) {
       const int64_t j = c0 + threadIdx.x;
@@ -192,8 +198,22 @@ __global__ __launch_bounds__(kRows) void k_dense_rect(int64_t nrows, const doubl
     for (int r = 0; r < kRectRhs; ++r)
       if (r < nrhs) {
         const double v = FP32 ? double(a32[r]) + double(c32[r]) : acc[r];
-        y[int64_t(r) * ldy + i] += v;
+        if (part) part[(int64_t(blockIdx.y) * nrhs + r) * nrows + i] = v;
+        else y[int64_t(r) * ldy + i] += v;
       }
+}
+
+// y[r][i] += sum over the column ranges s (in order) of part[s][r][i]
+__global__ __launch_bounds__(256) void k_rect_reduce(int64_t nrows, int nrhs, int ns,
+                                                     const double *__restrict__ part,
+                                                     double *__restrict__ y, int64_t ldy) {
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= nrows * nrhs) return;
+  const int r = int(t / nrows);
+  const int64_t i = t % nrows;
+  double v = 0.0;
+  for (int s = 0; s < ns; ++s) v += part[(int64_t(s) * nrhs + r) * nrows + i];
+  y[int64_t(r) * ldy + i] += v;
 }
 
 }  // namespace
@@ -445,7 +465,22 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const doubl
     abh::set_inverses(k);
     const bool f32 = precision == ABH_FP32_COMPENSATED;
     hipStream_t s = ctx->stream;
-    const unsigned grid = unsigned(abh::ceil_div(nrows, kRows));
+    const int64_t grows = abh::ceil_div(nrows, kRows);
+    // few row blocks (a tall operator has plenty): split the columns into
+    // ranges of at least 2,048 so that about 8 blocks per CU run
+    int64_t ns = 1;
+    const int64_t want = int64_t(ctx->n_cu) * 8;
+    if (grows < want && ncols > 0)
+      ns = std::max<int64_t>(1, std::min(abh::ceil_div(want, grows), ncols / 2048));
+    const int64_t cspan =
+        ncols > 0 ? abh::ceil_div(abh::ceil_div(ncols, ns), int64_t(kRows)) * kRows : 1;
+    ns = ncols > 0 ? abh::ceil_div(ncols, cspan) : 1;
+    double *part = nullptr;
+    if (ns > 1) {
+      ctx->dense_part.ensure(size_t(ns) * kRectRhs * size_t(nrows) * sizeof(double));
+      part = ctx->dense_part.as<double>();
+    }
+    const dim3 grid{unsigned(grows), unsigned(ns), 1u};
     for (int r0 = 0; r0 < nrhs; r0 += kRectRhs) {
       const int nr = std::min(kRectRhs, nrhs - r0);
       const double *xr = x + int64_t(r0) * ldx;
@@ -453,15 +488,20 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const doubl
 #define ABH_RECT(DD)                                                                            \
   if (f32)                                                                                      \
     k_dense_rect<DD, true><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr, ldx, \
-                                                  yr, ldy);                                     \
+                                                  yr, ldy, cspan, part);                        \
   else                                                                                          \
     k_dense_rect<DD, false><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr,   \
-                                                   ldx, yr, ldy);
+                                                   ldx, yr, ldy, cspan, part);
       if (dim == 1) { ABH_RECT(1) }
       else if (dim == 2) { ABH_RECT(2) }
       else { ABH_RECT(3) }
 #undef ABH_RECT
       abh::check_launch();
+      if (part) {
+        k_rect_reduce<<<unsigned(abh::ceil_div(nrows * nr, int64_t(256))), 256, 0, s>>>(
+            nrows, nr, int(ns), part, yr, ldy);
+        abh::check_launch();
+      }
     }
   });
 }

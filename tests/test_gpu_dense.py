@@ -83,6 +83,27 @@ def test_rectangular_dense_apply_matches_oracle(ctx, dim):
     assert np.max(np.abs(y32 - ref) / scale) < 2e-6
 
 
+def test_rectangular_dense_apply_column_split(ctx):
+    """Few rows, many columns: the columns are split over blocks and the
+    partial sums added in range order (deterministic, within the FP64 bar)."""
+    rng = np.random.default_rng(77)
+    a = rng.random((300, 3))
+    b = rng.random((20000, 3))
+    x = rng.normal(size=(20, 20000))
+    kern = (KM, 0.05)
+    ref = oracle.dense_matvec(a, b, kern, x.T).T
+    scale = np.abs(oracle.dense_matvec(a, b, kern, np.abs(x.T)).T)
+    at, bt = torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda")
+    xt = torch.tensor(x, device="cuda")
+    y = ab.dense_apply(ctx, at, bt, ab.Kernel.multiquadric(0.05), xt).cpu().numpy()
+    assert np.max(np.abs(y - ref) / scale) < 1e-13
+    y2 = ab.dense_apply(ctx, at, bt, ab.Kernel.multiquadric(0.05), xt).cpu().numpy()
+    assert np.array_equal(y, y2)
+    y32 = ab.dense_apply(ctx, at, bt, ab.Kernel.multiquadric(0.05), xt,
+                         precision=ab.FP32_COMPENSATED).cpu().numpy()
+    assert np.max(np.abs(y32 - ref) / scale) < 2e-6
+
+
 def test_dense_tile_variants_agree_and_tuner_picks_one(ctx):
     """Every P2P tile shape gives the same FP64 rows (same summation order
     per row) and the FP32c bar; the autotuner returns one of them."""
