@@ -299,7 +299,10 @@ abh_status abh_dense_batch_apply(abh_dense_batch *op, const double *x, double *y
  * src/Kernels.h:212-267): y += K x, K_ij = phi(|x_col_j - x_row_i|), for nrhs
  * right-hand sides.  row_pos (nrows x dim), col_pos (ncols x dim): device;
  * x: column-major ncols x nrhs (leading dimension ldx); y: nrows x nrhs
- * (ldy).  precision: ABH_FP64 or ABH_FP32_COMPENSATED. */
+ * (ldy).  precision: ABH_FP64 or ABH_FP32_COMPENSATED.  With fewer rows
+ * than 2,048 per CU the columns are split into ranges whose partial sums the
+ * context keeps in a device buffer (ranges x 16 x nrows doubles, kept for
+ * later calls) and adds in range order: results are deterministic. */
 abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows,
                            const double *row_pos, int64_t ncols,
                            const double *col_pos, const abh_kernel *kernel,
