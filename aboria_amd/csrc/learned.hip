@@ -320,19 +320,25 @@ __device__ __forceinline__ float silu_grad(float z) {
 // C[M x N] = op(A) op(B) (+ C if beta) (+ bias[n]); Z = pre-activation copy;
 // C = act(pre-activation).  Row-major operands; op(A)[m][k] =
 // TA ? A[k*lda + m] : A[m*lda + k], op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n].
+// With part set, block z takes k in [z * kchunk, (z + 1) * kchunk) and writes
+// its raw partial product to part[z][m][n] (no epilogue); k_split_sum adds
+// the partials in z order.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *__restrict__ A,
                                               int lda, const float *__restrict__ B, int ldb,
                                               float *__restrict__ C, int ldc, int beta,
                                               const float *__restrict__ bias, int act,
-                                              float *__restrict__ Z) {
+                                              float *__restrict__ Z, int kchunk = 0,
+                                              float *__restrict__ part = nullptr) {
   __shared__ float As[16][64 + 4];
   __shared__ float Bs[16][64 + 4];
   const int t = threadIdx.x;
   const int tx = t & 15, ty = t >> 4;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   float acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  const int kb = part ? int(blockIdx.z) * kchunk : 0;
+  const int ke = part ? min(K, kb + kchunk) : K;
+  for (int k0 = kb; k0 < ke; k0 += 16) {
     for (int e = t; e < 1024; e += 256) {
       int kk, mm;
       if (TA) {
@@ -343,7 +349,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
         mm = e >> 4;
       }
       const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < M && k < K) ? (TA ? A[int64_t(k) * lda + m] : A[int64_t(m) * lda + k]) : 0.0f;
+      As[kk][mm] = (m < M && k < ke) ? (TA ? A[int64_t(k) * lda + m] : A[int64_t(m) * lda + k]) : 0.0f;
     }
     for (int e = t; e < 1024; e += 256) {
       int kk, nn;
@@ -355,7 +361,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
         kk = e >> 6;
       }
       const int nx = n0 + nn, k = k0 + kk;
-      Bs[kk][nn] = (nx < N && k < K) ? (TB ? B[int64_t(nx) * ldb + k] : B[int64_t(k) * ldb + nx]) : 0.0f;
+      Bs[kk][nn] = (nx < N && k < ke) ? (TB ? B[int64_t(nx) * ldb + k] : B[int64_t(k) * ldb + nx]) : 0.0f;
     }
     __syncthreads();
 #pragma unroll
@@ -381,6 +387,10 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
       const int nx = n0 + tx * 4 + c;
       if (nx >= N) continue;
       float v = acc[r][c];
+      if (part) {
+        part[(int64_t(blockIdx.z) * M + m) * N + nx] = v;
+        continue;
+      }
       float *cp = &C[int64_t(m) * ldc + nx];
       if (beta) v = v + *cp;
       if (bias) v = v + bias[nx];
@@ -396,13 +406,32 @@ __global__ void k_silu_back(float *__restrict__ dA, const float *__restrict__ Z,
   if (e < m) dA[e] = dA[e] * silu_grad(Z[e]);
 }
 
-// db[c] = sum over rows of dZ[r][c] (fixed order)
-__global__ void k_colsum(const float *__restrict__ dZ, int rows, int cols, float *__restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+// C[m][n] = sum over z (in order) of part[z][m][n]
+__global__ void k_split_sum(int M, int N, int nz, const float *__restrict__ part,
+                            float *__restrict__ C, int ldc) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(M) * N) return;
+  float v = 0.0f;
+  for (int z = 0; z < nz; ++z) v = v + part[int64_t(z) * M * N + e];
+  C[(e / N) * ldc + e % N] = v;
+}
+
+// part[z][c] = sum of dZ[r][c] over rows r of chunk z (64 columns x 4 row
+// phases per block, the phases combined in order); k_split_sum then adds
+// the chunks in order (fixed order throughout)
+__global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ dZ, int rows, int cols,
+                                                int rchunk, float *__restrict__ part) {
+  __shared__ float red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + tx;
+  const int r0 = blockIdx.y * rchunk, r1 = min(rows, r0 + rchunk);
   float s = 0.0f;
-  for (int r = 0; r < rows; ++r) s = s + dZ[int64_t(r) * cols + c];
-  db[c] = s;
+  if (c < cols)
+    for (int r = r0 + ty; r < r1; r += 4) s = s + dZ[int64_t(r) * cols + c];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && c < cols)
+    part[int64_t(blockIdx.y) * cols + c] = ((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx];
 }
 
 // dY = 2 (Y - T) / (rows * cols); partial[block] = sum (Y - T)^2 in FP64
@@ -562,7 +591,8 @@ struct abh_mlp {
   std::vector<int64_t> w_off, b_off;
   int64_t nparams = 0;
   DevBuf params, grads, am, av, wbf, loss;
-  DevBuf ws;  // activations of the training / FP32 forward passes
+  DevBuf ws;      // activations of the training / FP32 forward passes
+  DevBuf gsplit;  // split-K partial weight / bias gradients
   int64_t adam_t = 0;
   int layers() const { return int(dims.size()) - 1; }
 };
@@ -671,17 +701,31 @@ void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, float *
                                                                             rows * dout);
       abh::check_launch();
     }
-    // dW = dZ^T act_in  (dout x din)
+    // dW = dZ^T act_in  (dout x din): the rows (the k dimension) are split
+    // into chunks of 256 over the grid's z, the partial products added in
+    // chunk order; db = column sums of dZ, split the same way
     {
-      dim3 grid(unsigned(abh::ceil_div(din, 64)), unsigned(abh::ceil_div(dout, 64)));
+      const int kz = int(std::min<int64_t>(abh::ceil_div(rows, 256), 512));
+      const int kchunk = int(abh::ceil_div(abh::ceil_div(rows, kz), 16) * 16);
+      const int nz = int(abh::ceil_div(rows, kchunk));
+      m->gsplit.ensure(size_t(nz) * (size_t(dout) * din + dout) * sizeof(float));
+      float *pw = m->gsplit.as<float>();
+      float *pb = pw + int64_t(nz) * dout * din;
+      dim3 grid(unsigned(abh::ceil_div(din, 64)), unsigned(abh::ceil_div(dout, 64)), unsigned(nz));
       k_gemm<true, false><<<grid, 256, 0, s>>>(dout, din, int(rows), dA, dout, f.act[l], din,
-                                               grad + m->w_off[l], din, 0, nullptr, kNone,
-                                               nullptr);
+                                               nullptr, din, 0, nullptr, kNone, nullptr, kchunk,
+                                               pw);
+      abh::check_launch();
+      k_split_sum<<<unsigned(abh::ceil_div(int64_t(dout) * din, 256)), 256, 0, s>>>(
+          dout, din, nz, pw, grad + m->w_off[l], din);
+      abh::check_launch();
+      dim3 gb(unsigned(abh::ceil_div(dout, 64)), unsigned(nz));
+      k_colsum<<<gb, 256, 0, s>>>(dA, int(rows), dout, kchunk, pb);
+      abh::check_launch();
+      k_split_sum<<<unsigned(abh::ceil_div(dout, 256)), 256, 0, s>>>(1, dout, nz, pb,
+                                                                      grad + m->b_off[l], dout);
       abh::check_launch();
     }
-    k_colsum<<<unsigned(abh::ceil_div(dout, 256)), 256, 0, s>>>(dA, int(rows), dout,
-                                                                 grad + m->b_off[l]);
-    abh::check_launch();
     if (l > 0) {  // dact_in = dZ W  (rows x din)
       dim3 grid(unsigned(abh::ceil_div(din, 64)), unsigned(abh::ceil_div(rows, 64)));
       k_gemm<false, false><<<grid, 256, 0, s>>>(int(rows), din, dout, dA, dout,
