@@ -874,9 +874,19 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
           const int q = ebeg[lo] + (gsrc - epre[lo]);
           for (int d = 0; d < 3; ++d) Ps[threadIdx.x][d] = pos[int64_t(q) * 3 + d];
           const double2 *qx = reinterpret_cast<const double2 *>(xs + int64_t(q) * kRhs);
-          double2 *qd = reinterpret_cast<double2 *>(&Qs[threadIdx.x][0]);
+          if (FP32) {
+            // FP32 P2P: the right-hand sides staged as FP32, rounded once
+            float2 *qf = reinterpret_cast<float2 *>(&Qs[0][0]) + threadIdx.x * (kRhs / 2);
 #pragma unroll
-          for (int r = 0; r < kRhs / 2; ++r) qd[r] = qx[r];
+            for (int r = 0; r < kRhs / 2; ++r) {
+              const double2 v = qx[r];
+              qf[r] = make_float2(float(v.x), float(v.y));
+            }
+          } else {
+            double2 *qd = reinterpret_cast<double2 *>(&Qs[threadIdx.x][0]);
+#pragma unroll
+            for (int r = 0; r < kRhs / 2; ++r) qd[r] = qx[r];
+          }
         }
         __syncthreads();
         if (act) {
@@ -889,7 +899,10 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
             if (FP32) {
               const float f = abh::phi32(k, float(s2));
 #pragma unroll
-              for (int r = 0; r < kRhs; ++r) abh::kb_add(a32[r], c32[r], f * float(Qs[j][r]));
+              for (int r = 0; r < kRhs; ++r) {
+                const float qv = reinterpret_cast<const float *>(&Qs[0][0])[j * kRhs + r];
+                abh::kb_add(a32[r], c32[r], f * qv);
+              }
             } else {
               const double f = abh::phi64(k, s2);
 #pragma unroll
