@@ -893,9 +893,18 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
             const int sl = int(__builtin_amdgcn_mbcnt_hi(uint32_t(km >> 32),
                                                          __builtin_amdgcn_mbcnt_lo(uint32_t(km), 0u)));
             st[sl] = StageRec{x[0], x[1], x[2], jc, 0};
+            if (MODE == kMatvec32) {
+              // the FP32 form stages its right-hand sides in FP32 (rounded
+              // once here instead of at every use)
+              float *xf = reinterpret_cast<float *>(xs);
 #pragma unroll
-            for (int r = 0; r < kMaxRhs; ++r)
-              xs[sl * kMaxRhs + r] = r < a.nrhs ? a.x[int64_t(r) * a.ldx + jc] : 0.0;
+              for (int r = 0; r < kMaxRhs; ++r)
+                xf[sl * kMaxRhs + r] = r < a.nrhs ? float(a.x[int64_t(r) * a.ldx + jc]) : 0.0f;
+            } else {
+#pragma unroll
+              for (int r = 0; r < kMaxRhs; ++r)
+                xs[sl * kMaxRhs + r] = r < a.nrhs ? a.x[int64_t(r) * a.ldx + jc] : 0.0;
+            }
           }
           __syncthreads();  // single-wave group: the staged block is complete
           // radius tests of the whole block first, this lane's hits as bits
@@ -961,10 +970,11 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
               }
             } else {
               const float f = phi32(a.k, float(acc)) * float(w);
+              const float *xf = reinterpret_cast<const float *>(xs) + t * kMaxRhs;
               for (int rep = 0; rep < m; ++rep) {
 #pragma unroll
                 for (int r = 0; r < kMaxRhs; ++r)
-                  if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * float(xr[r]));
+                  if (r < a.nrhs) kb_add(acc32[r], cmp32[r], f * xf[r]);
               }
             }
           }
