@@ -4,7 +4,7 @@ Bar: FP64 batched matvec within 1e-13 * sum_j |f_ij x_j| of the oracle's
 KernelDense::evaluate restatement for every problem; FP32-compensated within
 2e-6 of the same scale; GMRES reaches ||b - A x|| <= 1e-8 ||b|| with the
 residual recomputed independently by the oracle.  (Iteration counts of
-Eigen::GMRES are not reproduced: parity unpinned, the residual is the judge.)
+Eigen::GMRES are not reproduced: parity unpinned, the true residual decides.)
 """
 import numpy as np
 import pytest
