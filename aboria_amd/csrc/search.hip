@@ -1292,6 +1292,8 @@ template <int MODE> static void launch_search(const QueryPlan &p, const SearchAr
 // Counts neighbours into counts[nq] (device), widening the stencil until no
 // query's visited buckets escape it.
 // col[row_ptr[i] + e] = padded[pad_ptr[i] + e], one wavefront per row
+// (the row's body is stored 16 bytes per lane from its first 16-byte
+// aligned address, head and tail element by element)
 __global__ __launch_bounds__(256) void k_compact_rows(const int64_t *__restrict__ pad_ptr,
                                                       const int64_t *__restrict__ row_ptr,
                                                       int64_t nrows,
@@ -1301,7 +1303,17 @@ __global__ __launch_bounds__(256) void k_compact_rows(const int64_t *__restrict_
   if (i >= nrows) return;
   const int lane = threadIdx.x & 63;
   const int64_t src = pad_ptr[i], dst = row_ptr[i], n = row_ptr[i + 1] - dst;
-  for (int64_t e = lane; e < n; e += 64) col[dst + e] = padded[src + e];
+  const uintptr_t a0 = reinterpret_cast<uintptr_t>(col + dst);
+  const int64_t head = std::min<int64_t>(n, int64_t(((16 - (a0 & 15)) & 15) >> 2));
+  if (lane < head) col[dst + lane] = padded[src + lane];
+  const int64_t d0 = dst + head, s0 = src + head;
+  const int64_t nb4 = (n - head) >> 2;
+  for (int64_t c = lane; c < nb4; c += 64) {
+    const int32_t *sp = padded + s0 + 4 * c;
+    *reinterpret_cast<int4 *>(col + d0 + 4 * c) = make_int4(sp[0], sp[1], sp[2], sp[3]);
+  }
+  const int64_t rem = (n - head) - 4 * nb4;
+  if (lane < rem) col[d0 + 4 * nb4 + lane] = padded[s0 + 4 * nb4 + lane];
 }
 
 // Like count_with_escape, with the capacity pass (no distance tests).
