@@ -617,7 +617,9 @@ __global__ __launch_bounds__(kGroup) void k_search(SearchArgs a, GridParams g) {
       // test can accept more
       int64_t cap = 0;
       for (int s = 0; s < S; ++s) cap += int64_t(mult[s * kGroup + lane]) * slots[s].len;
-      if (has_row) a.counts[row] = active ? cap : 0;
+      // rounded up to 16 entries: every padded row then starts on a 64-byte
+      // boundary, where the fill's ring stores whole segments from the first
+      if (has_row) a.counts[row] = active ? ((cap + 15) & ~int64_t(15)) : 0;
       __syncthreads();
       continue;
     }
