@@ -243,7 +243,20 @@ __global__ __launch_bounds__(256) void k_ras_apply(const ApplyChunk *__restrict_
   for (int a = wave; a < rows; a += 4) {
     const double *Ra = R + dm.r_off + int64_t(ch.row0 + a) * dm.ld;
     double s = 0.0;
-    for (int j = lane; j < dm.m; j += 64) s = s + Ra[j] * (LDS ? bl[j] : b[id[j]]);
+    int j = lane;
+    // four row loads in flight per step, summed in the same order
+    for (; j + 192 < dm.m; j += 256) {
+      const double r0 = Ra[j], r1 = Ra[j + 64], r2 = Ra[j + 128], r3 = Ra[j + 192];
+      const double b0 = LDS ? bl[j] : b[id[j]];
+      const double b1 = LDS ? bl[j + 64] : b[id[j + 64]];
+      const double b2 = LDS ? bl[j + 128] : b[id[j + 128]];
+      const double b3 = LDS ? bl[j + 192] : b[id[j + 192]];
+      s = s + r0 * b0;
+      s = s + r1 * b1;
+      s = s + r2 * b2;
+      s = s + r3 * b3;
+    }
+    for (; j < dm.m; j += 64) s = s + Ra[j] * (LDS ? bl[j] : b[id[j]]);
     s = abh::wave_sum(s);
     if (lane == 0) x[id[ch.row0 + a]] = s;
   }
