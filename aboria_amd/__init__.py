@@ -245,6 +245,7 @@ class Search:
         self.nq = centres.shape[0]
 
     def run(self, query, centres, radius):
+        _require_cuda(centres, torch.float64, "centres")
         check(C.lib().abh_search_update(self.handle, query.handle, centres.shape[0],
                                         _ptr(centres), float(radius)), self.ctx.handle)
         self.nq = centres.shape[0]
@@ -261,6 +262,7 @@ class Search:
                      self.ctx.device)
 
     def fill(self, col):
+        _require_cuda(col, torch.int32, "col")
         check(C.lib().abh_search_fill(self.handle, _ptr(col), col.numel()), self.ctx.handle)
         return col
 
@@ -268,6 +270,8 @@ class Search:
         """Counts and fills in one traversal of the candidates (rows staged at
         upper-bound offsets in a handle-owned buffer, then packed into col).
         Returns the number of pairs; row_ptr() is valid afterwards."""
+        _require_cuda(centres, torch.float64, "centres")
+        _require_cuda(col, torch.int32, "col")
         t = ctypes.c_int64()
         check(C.lib().abh_search_run_fused(self.handle, query.handle, centres.shape[0],
                                            _ptr(centres), float(radius), _ptr(col), col.numel(),

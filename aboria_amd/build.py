@@ -28,8 +28,9 @@ FLAGS = [
 ]
 
 
-# rocBLAS supplies the plain DGEMMs of the Schwarz local factorisations
-LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas"]
+# no vendor libraries: every kernel, the GEMMs of the Schwarz local
+# factorisations included, is in csrc/
+LIBS = []
 
 
 def sources():

@@ -56,6 +56,16 @@ struct DevBuf {
   template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Synchronises a stream when it goes out of scope.  Declared after the host
+// buffers of asynchronous copies, it is destroyed before them, so an error
+// thrown while a copy is in flight does not free memory the copy still uses.
+struct StreamDrain {
+  hipStream_t st;
+  StreamDrain(const StreamDrain &) = delete;
+  StreamDrain &operator=(const StreamDrain &) = delete;
+  ~StreamDrain() { (void)hipStreamSynchronize(st); }
+};
+
 }  // namespace abh
 
 struct abh_context {

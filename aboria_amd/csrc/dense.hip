@@ -8,6 +8,7 @@
 // first so the long rows of the big problems start early.
 #include <algorithm>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "batch.hpp"
@@ -301,6 +302,84 @@ struct abh_dense_batch {
   }
 };
 
+namespace {
+
+// y += K x for the rectangular operator.  part_ext == nullptr: the column
+// split's partial sums live in the context's grow-only buffer; otherwise in
+// part_ext (part_cap doubles), and a split that needs more is refused.
+void dense_apply_impl(abh_context *ctx, int dim, int64_t nrows, const double *row_pos,
+                      int64_t ncols, const double *col_pos, const abh_kernel *kernel, int nrhs,
+                      const double *x, int64_t ldx, double *y, int64_t ldy,
+                      abh_precision precision, double *part_ext, size_t part_cap) {
+  ABH_REQUIRE(dim >= 1 && dim <= 3, ABH_ERR_INVALID_ARG, "dim must be 1..3");
+  ABH_REQUIRE(nrows >= 0 && ncols >= 0 && nrhs >= 0, ABH_ERR_INVALID_ARG, "negative size");
+  ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+              ABH_ERR_UNSUPPORTED, "unknown kernel type");
+  ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
+              "dense precision must be ABH_FP64 or ABH_FP32_COMPENSATED");
+  if (nrows == 0 || nrhs == 0) return;
+  ABH_REQUIRE(row_pos && y && (ncols == 0 || (col_pos && x)), ABH_ERR_INVALID_ARG,
+              "null pointer");
+  ABH_REQUIRE(ldx >= ncols && ldy >= nrows, ABH_ERR_INVALID_ARG,
+              "leading dimension smaller than the operator");
+  KernelDesc k{};
+  k.type = kernel->type;
+  k.param = kernel->param;
+  k.param2 = kernel->param * kernel->param;
+  abh::set_inverses(k);
+  const bool f32 = precision == ABH_FP32_COMPENSATED;
+  hipStream_t s = ctx->stream;
+  const int64_t grows = abh::ceil_div(nrows, kRows);
+  // few row blocks (a tall operator has plenty): split the columns into
+  // ranges of at least 2,048 so that about 8 blocks per CU run
+  int64_t ns = 1;
+  const int64_t want = int64_t(ctx->n_cu) * 8;
+  if (grows < want && ncols > 0)
+    ns = std::max<int64_t>(1, std::min(abh::ceil_div(want, grows), ncols / 2048));
+  const int64_t cspan =
+      ncols > 0 ? abh::ceil_div(abh::ceil_div(ncols, ns), int64_t(kRows)) * kRows : 1;
+  ns = ncols > 0 ? abh::ceil_div(ncols, cspan) : 1;
+  double *part = nullptr;
+  if (ns > 1) {
+    const size_t need = size_t(ns) * kRectRhs * size_t(nrows);
+    if (part_ext) {
+      ABH_REQUIRE(need <= part_cap, ABH_ERR_CAPACITY,
+                  "column split needs " + std::to_string(need * 8) +
+                      " bytes of partial sums, above the " + std::to_string(part_cap * 8) +
+                      "-byte static scratch of the allocation-free entry point");
+      part = part_ext;
+    } else {
+      ctx->dense_part.ensure(need * sizeof(double));
+      part = ctx->dense_part.as<double>();
+    }
+  }
+  const dim3 grid{unsigned(grows), unsigned(ns), 1u};
+  for (int r0 = 0; r0 < nrhs; r0 += kRectRhs) {
+    const int nr = std::min(kRectRhs, nrhs - r0);
+    const double *xr = x + int64_t(r0) * ldx;
+    double *yr = y + int64_t(r0) * ldy;
+#define ABH_RECT(DD)                                                                            \
+  if (f32)                                                                                      \
+    k_dense_rect<DD, true><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr, ldx, \
+                                                  yr, ldy, cspan, part);                        \
+  else                                                                                          \
+    k_dense_rect<DD, false><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr,   \
+                                                   ldx, yr, ldy, cspan, part);
+    if (dim == 1) { ABH_RECT(1) }
+    else if (dim == 2) { ABH_RECT(2) }
+    else { ABH_RECT(3) }
+#undef ABH_RECT
+    abh::check_launch();
+    if (part) {
+      k_rect_reduce<<<unsigned(abh::ceil_div(nrows * nr, int64_t(256))), 256, 0, s>>>(
+          nrows, nr, int(ns), part, yr, ldy);
+      abh::check_launch();
+    }
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 abh_status abh_dense_batch_create(abh_context *ctx, int dim, int nprob, const int64_t *offsets,
@@ -447,62 +526,8 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const doubl
                            abh_precision precision) {
   if (!ctx || !kernel) return ABH_ERR_INVALID_ARG;
   return abh::guard(ctx, [&] {
-    ABH_REQUIRE(dim >= 1 && dim <= 3, ABH_ERR_INVALID_ARG, "dim must be 1..3");
-    ABH_REQUIRE(nrows >= 0 && ncols >= 0 && nrhs >= 0, ABH_ERR_INVALID_ARG, "negative size");
-    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
-                ABH_ERR_UNSUPPORTED, "unknown kernel type");
-    ABH_REQUIRE(precision == ABH_FP64 || precision == ABH_FP32_COMPENSATED, ABH_ERR_UNSUPPORTED,
-                "dense precision must be ABH_FP64 or ABH_FP32_COMPENSATED");
-    if (nrows == 0 || nrhs == 0) return;
-    ABH_REQUIRE(row_pos && y && (ncols == 0 || (col_pos && x)), ABH_ERR_INVALID_ARG,
-                "null pointer");
-    ABH_REQUIRE(ldx >= ncols && ldy >= nrows, ABH_ERR_INVALID_ARG,
-                "leading dimension smaller than the operator");
-    KernelDesc k{};
-    k.type = kernel->type;
-    k.param = kernel->param;
-    k.param2 = kernel->param * kernel->param;
-    abh::set_inverses(k);
-    const bool f32 = precision == ABH_FP32_COMPENSATED;
-    hipStream_t s = ctx->stream;
-    const int64_t grows = abh::ceil_div(nrows, kRows);
-    // few row blocks (a tall operator has plenty): split the columns into
-    // ranges of at least 2,048 so that about 8 blocks per CU run
-    int64_t ns = 1;
-    const int64_t want = int64_t(ctx->n_cu) * 8;
-    if (grows < want && ncols > 0)
-      ns = std::max<int64_t>(1, std::min(abh::ceil_div(want, grows), ncols / 2048));
-    const int64_t cspan =
-        ncols > 0 ? abh::ceil_div(abh::ceil_div(ncols, ns), int64_t(kRows)) * kRows : 1;
-    ns = ncols > 0 ? abh::ceil_div(ncols, cspan) : 1;
-    double *part = nullptr;
-    if (ns > 1) {
-      ctx->dense_part.ensure(size_t(ns) * kRectRhs * size_t(nrows) * sizeof(double));
-      part = ctx->dense_part.as<double>();
-    }
-    const dim3 grid{unsigned(grows), unsigned(ns), 1u};
-    for (int r0 = 0; r0 < nrhs; r0 += kRectRhs) {
-      const int nr = std::min(kRectRhs, nrhs - r0);
-      const double *xr = x + int64_t(r0) * ldx;
-      double *yr = y + int64_t(r0) * ldy;
-#define ABH_RECT(DD)                                                                            \
-  if (f32)                                                                                      \
-    k_dense_rect<DD, true><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr, ldx, \
-                                                  yr, ldy, cspan, part);                        \
-  else                                                                                          \
-    k_dense_rect<DD, false><<<grid, kRows, 0, s>>>(nrows, row_pos, ncols, col_pos, k, nr, xr,   \
-                                                   ldx, yr, ldy, cspan, part);
-      if (dim == 1) { ABH_RECT(1) }
-      else if (dim == 2) { ABH_RECT(2) }
-      else { ABH_RECT(3) }
-#undef ABH_RECT
-      abh::check_launch();
-      if (part) {
-        k_rect_reduce<<<unsigned(abh::ceil_div(nrows * nr, int64_t(256))), 256, 0, s>>>(
-            nrows, nr, int(ns), part, yr, ldy);
-        abh::check_launch();
-      }
-    }
+    dense_apply_impl(ctx, dim, nrows, row_pos, ncols, col_pos, kernel, nrhs, x, ldx, y, ldy,
+                     precision, nullptr, 0);
   });
 }
 

@@ -9,10 +9,11 @@
 // A_g = K + shift I (identity in the padding) is assembled in HBM,
 // factorised by a right-looking blocked Cholesky (64-wide panels: the
 // diagonal block in LDS, panel and trailing updates as strided-batched
-// DGEMMs), and the interior columns of A_g^-1 (= the interior rows, A_g is
-// symmetric) are obtained by a blocked forward and backward substitution on
-// the unit columns.  They stay resident: R_g is n x ni column-major, so row a
-// of the local inverse is the contiguous column a.
+// FP64 MFMA GEMMs, dgemm.hip), and the interior columns of A_g^-1 (= the
+// interior rows, A_g is symmetric) are obtained by a blocked forward and
+// backward substitution on the unit columns.  They stay resident: R_g is
+// n x ni column-major, so row a of the local inverse is the contiguous
+// column a.
 //
 // For the dense batch (every problem an independent particle set) the
 // domains come from a per-problem bucket grid instead of a cell list, the
@@ -23,8 +24,6 @@
 // x[idx[a]] = <R_g[:, a], b[idx]>, one wavefront per row, the gathered
 // b[idx] staged once per workgroup in LDS.  A domain whose factorisation
 // broke down (counted in failed_domains) is skipped: its rows keep b.
-#include <rocblas/rocblas.h>
-
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -33,6 +32,7 @@
 
 #include "cells.hpp"
 #include "kernel_fn.hpp"
+#include "schwarz_analysis.hpp"
 #include "solvers.hpp"
 
 using abh::DevBuf;
@@ -43,14 +43,6 @@ namespace {
 constexpr int kNb = 64;            // Cholesky panel width
 constexpr int kApplyRows = 64;     // interior rows per apply workgroup
 constexpr int kApplyLds = 7936;    // max domain size staged in LDS by apply (62 KiB)
-
-#define ABH_BLAS(expr)                                                                   \
-  do {                                                                                   \
-    rocblas_status s_ = (expr);                                                          \
-    if (s_ != rocblas_status_success)                                                    \
-      throw ::abh::Error{ABH_ERR_HIP, std::string(#expr) + " failed: " +                 \
-                                          rocblas_status_to_string(s_)};                 \
-  } while (0)
 
 struct DomDev {
   int64_t idx_off;  // first entry in the index array
@@ -280,140 +272,34 @@ struct abh_schwarz {
 
 namespace {
 
-void analyse(abh_schwarz *s, const abh_cells *c, const std::vector<double> &pos,
-             const std::vector<uint32_t> &beg, const std::vector<uint32_t> &end) {
-  const int D = c->dim;
-  int64_t sz[3] = {1, 1, 1};
-  for (int d = 0; d < D; ++d) sz[d] = c->size[d];
-  s->off.assign(1, 0);
-  s->idx.clear();
-  s->nint.clear();
-  std::vector<int32_t> buf;
-  for (int64_t b = 0; b < c->nb; ++b) {
-    if (end[b] <= beg[b]) continue;  // empty bucket: no domain
-    // bucket index vector, last dimension fastest
-    int64_t iv[3] = {0, 0, 0}, r = b;
-    for (int d = D - 1; d >= 0; --d) {
-      iv[d] = r % sz[d];
-      r /= sz[d];
-    }
-    double bmin[3], bmax[3];
-    for (int d = 0; d < D; ++d) {
-      bmin[d] = double(iv[d]) * c->side[d] + c->lo[d];
-      bmax[d] = double(iv[d] + 1) * c->side[d] + c->lo[d];
-    }
-    buf.clear();
-    const size_t first = s->idx.size();
-    int64_t lo3[3] = {0, 0, 0}, hi3[3] = {0, 0, 0};
-    for (int d = 0; d < D; ++d) {
-      lo3[d] = std::max<int64_t>(iv[d] - 1, 0);
-      hi3[d] = std::min<int64_t>(iv[d] + 1, sz[d] - 1);
-    }
-    for (int64_t x0 = lo3[0]; x0 <= hi3[0]; ++x0)
-      for (int64_t x1 = lo3[1]; x1 <= hi3[1]; ++x1)
-        for (int64_t x2 = lo3[2]; x2 <= hi3[2]; ++x2) {
-          const int64_t nbk = (x0 * sz[1] + x1) * sz[2] + x2;
-          for (uint32_t p = beg[nbk]; p < end[nbk]; ++p) {
-            bool inside = true;
-            for (int d = 0; d < D; ++d) {
-              const double v = pos[size_t(p) * D + d];
-              if (v < bmin[d] || v >= bmax[d]) inside = false;
-            }
-            if (inside) s->idx.push_back(int32_t(p));
-            else buf.push_back(int32_t(p));
-          }
-        }
-    const int32_t ni = int32_t(s->idx.size() - first);
-    if (ni == 0) {  // no interior: the domain writes nothing
-      s->idx.resize(first);
-      continue;
-    }
-    s->idx.insert(s->idx.end(), buf.begin(), buf.end());
-    s->nint.push_back(ni);
-    s->off.push_back(int64_t(s->idx.size()));
-  }
+void take(abh_schwarz *s, abh::SchwarzDomains &dm) {
+  s->off.swap(dm.off);
+  s->idx.swap(dm.idx);
+  s->nint.swap(dm.nint);
 }
 
-// Domains of a batch of independent particle sets (problem p owns points
-// off[p] .. off[p+1]-1).  Every problem gets the bucket grid the cell list
-// would size for it (src/CellListOrdered.h:132-186, with n = its particle
-// count); a domain per non-empty bucket, whose interior is the bucket's
-// particles and whose buffer the particles of the other buckets of the 3^dim
-// block around it (src/Preconditioners.h:1169-1230).  Bucket membership is
-// floor((x - low) / side) clamped to the grid, so every particle is interior
-// to exactly one domain.  Lists are in ascending particle order per bucket,
-// neighbour buckets in row-major order; indices are global.
-void analyse_batch(abh_schwarz *s, int D, const std::vector<int64_t> &boff,
+void analyse_into(abh_schwarz *s, const abh_cells *c, const std::vector<double> &pos,
+             const std::vector<uint32_t> &beg, const std::vector<uint32_t> &end) {
+  abh::GridView g;
+  g.dim = c->dim;
+  for (int d = 0; d < c->dim; ++d) {
+    g.size[d] = c->size[d];
+    g.side[d] = c->side[d];
+    g.lo[d] = c->lo[d];
+  }
+  abh::SchwarzDomains dm;
+  const char *err = abh::analyse_cells(g, pos, beg, end, dm);
+  ABH_REQUIRE(!err, ABH_ERR_INVALID_ARG, err ? err : "");
+  take(s, dm);
+}
+
+void analyse_batch_into(abh_schwarz *s, int D, const std::vector<int64_t> &boff,
                    const std::vector<double> &pos, const double *lo, const double *hi,
                    double n_leaf) {
-  s->off.assign(1, 0);
-  s->idx.clear();
-  s->nint.clear();
-  std::vector<int64_t> key, cnt;
-  std::vector<int32_t> sorted;
-  for (size_t p = 0; p + 1 < boff.size(); ++p) {
-    const int64_t p0 = boff[p], np = boff[p + 1] - p0;
-    if (np == 0) continue;
-    int64_t sz[3] = {1, 1, 1};
-    double side[3] = {1, 1, 1};
-    if (n_leaf <= double(np)) {
-      double volume = 1;
-      for (int d = 0; d < D; ++d) volume *= (hi[d] - lo[d]);
-      const double box_side = std::pow(n_leaf / double(np) * volume, 1.0 / D);
-      for (int d = 0; d < D; ++d) {
-        sz[d] = int64_t(std::floor((hi[d] - lo[d]) / box_side));
-        if (sz[d] == 0) sz[d] = 1;
-      }
-    }
-    for (int d = 0; d < D; ++d) side[d] = (hi[d] - lo[d]) / double(sz[d]);
-    const int64_t nb = sz[0] * sz[1] * sz[2];
-    ABH_REQUIRE(nb < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "too many buckets");
-    key.resize(size_t(np));
-    cnt.assign(size_t(nb) + 1, 0);
-    for (int64_t i = 0; i < np; ++i) {
-      int64_t kk = 0;
-      for (int d = 0; d < D; ++d) {
-        const double v = pos[size_t(p0 + i) * D + d];
-        ABH_REQUIRE(std::isfinite(v) && v >= lo[d] && v <= hi[d], ABH_ERR_INVALID_ARG,
-                    "batch particle outside [low, high]");
-        const int64_t c = std::min<int64_t>(int64_t(std::floor((v - lo[d]) / side[d])), sz[d] - 1);
-        kk = kk * sz[d] + c;
-      }
-      key[size_t(i)] = kk;
-      ++cnt[size_t(kk) + 1];
-    }
-    for (int64_t b = 0; b < nb; ++b) cnt[size_t(b) + 1] += cnt[size_t(b)];
-    sorted.resize(size_t(np));
-    {
-      std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-      for (int64_t i = 0; i < np; ++i) sorted[size_t(fill[size_t(key[size_t(i)])]++)] = int32_t(p0 + i);
-    }
-    for (int64_t b = 0; b < nb; ++b) {
-      if (cnt[size_t(b) + 1] == cnt[size_t(b)]) continue;
-      int64_t iv[3] = {0, 0, 0}, r = b;
-      for (int d = D - 1; d >= 0; --d) {
-        iv[d] = r % sz[d];
-        r /= sz[d];
-      }
-      s->idx.insert(s->idx.end(), sorted.begin() + cnt[size_t(b)], sorted.begin() + cnt[size_t(b) + 1]);
-      const int32_t ni = int32_t(cnt[size_t(b) + 1] - cnt[size_t(b)]);
-      int64_t l3[3] = {0, 0, 0}, h3[3] = {0, 0, 0};
-      for (int d = 0; d < D; ++d) {
-        l3[d] = std::max<int64_t>(iv[d] - 1, 0);
-        h3[d] = std::min<int64_t>(iv[d] + 1, sz[d] - 1);
-      }
-      for (int64_t x0 = l3[0]; x0 <= h3[0]; ++x0)
-        for (int64_t x1 = l3[1]; x1 <= h3[1]; ++x1)
-          for (int64_t x2 = l3[2]; x2 <= h3[2]; ++x2) {
-            const int64_t nbk = (x0 * sz[1] + x1) * sz[2] + x2;
-            if (nbk == b) continue;
-            s->idx.insert(s->idx.end(), sorted.begin() + cnt[size_t(nbk)],
-                          sorted.begin() + cnt[size_t(nbk) + 1]);
-          }
-      s->nint.push_back(ni);
-      s->off.push_back(int64_t(s->idx.size()));
-    }
-  }
+  abh::SchwarzDomains dm;
+  const char *err = abh::analyse_batch(D, boff, pos, lo, hi, n_leaf, dm);
+  ABH_REQUIRE(!err, ABH_ERR_INVALID_ARG, err ? err : "");
+  take(s, dm);
 }
 
 // sgn: local matrices may be indefinite; factorise A = L S L^T (S = +-1)
@@ -436,8 +322,14 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
   });
   size_t free_b = 0, total_b = 0;
   ABH_HIP(hipMemGetInfo(&free_b, &total_b));
+  // host staging of the asynchronous copies below; the drain is destroyed
+  // before them, so an error thrown while a copy is pending cannot free its
+  // source or destination under it
+  std::vector<DomDev> dd(nd), dg(nd);
+  std::vector<ApplyChunk> ch;
+  std::vector<uint8_t> fh(nd);
+  abh::StreamDrain drain{st};
   // inverse rows: n_pad x ni_pad per domain
-  std::vector<DomDev> dd(nd);
   struct Group {
     int first, count, n, ni;
   };
@@ -477,7 +369,6 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
   s->d_idx.ensure(s->idx.size() * 4 + 64);
   ABH_HIP(hipMemcpyAsync(s->d_idx.p, s->idx.data(), s->idx.size() * 4, hipMemcpyHostToDevice, st));
   // domain table in group order (the group's members are contiguous)
-  std::vector<DomDev> dg(nd);
   for (int i = 0; i < nd; ++i) dg[i] = dd[order[i]];
   DevBuf d_dg;
   d_dg.ensure(size_t(nd) * sizeof(DomDev) + 64);
@@ -487,24 +378,13 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
   s->d_doms.ensure(size_t(nd) * sizeof(DomDev) + 64);
   ABH_HIP(hipMemcpyAsync(s->d_doms.p, dd.data(), size_t(nd) * sizeof(DomDev),
                          hipMemcpyHostToDevice, st));
-  std::vector<ApplyChunk> ch;
   for (int d = 0; d < nd; ++d)
     for (int r = 0; r < s->nint[d]; r += kApplyRows) ch.push_back(ApplyChunk{d, r});
   s->nchunks = int(ch.size());
   s->d_chunks.ensure(ch.size() * sizeof(ApplyChunk) + 64);
   ABH_HIP(hipMemcpyAsync(s->d_chunks.p, ch.data(), ch.size() * sizeof(ApplyChunk),
                          hipMemcpyHostToDevice, st));
-
-  rocblas_handle h = nullptr;
-  ABH_BLAS(rocblas_create_handle(&h));
-  struct HandleGuard {
-    rocblas_handle h;
-    ~HandleGuard() { rocblas_destroy_handle(h); }
-  } hg{h};
-  ABH_BLAS(rocblas_set_stream(h, st));
-  ABH_BLAS(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host));
-  const double one = 1.0, zero = 0.0, mone = -1.0;
-  const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
+  using abh::dgemm_strided_batched;
 
   int gmax = 0, nmax = 0, nimax = 0;
   for (const Group &gr : groups) {
@@ -527,11 +407,11 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
 
   for (const Group &gr : groups) {
     const int G = gr.count, n = gr.n, ni = gr.ni, K = n / kNb;
-    const rocblas_stride sA = rocblas_stride(n) * n;
-    const rocblas_stride sD = rocblas_stride(K) * kNb * kNb;
-    const rocblas_stride sW = rocblas_stride(n) * kNb;
-    const rocblas_stride sT = rocblas_stride(kNb) * ni;
-    const rocblas_stride sY = rocblas_stride(n) * ni;
+    const int64_t sA = int64_t(n) * n;
+    const int64_t sD = int64_t(K) * kNb * kNb;
+    const int64_t sW = int64_t(n) * kNb;
+    const int64_t sT = int64_t(kNb) * ni;
+    const int64_t sY = int64_t(n) * ni;
     double *Ap = A.as<double>(), *Wp = W.as<double>(), *Tp = Tm.as<double>();
     double *Dp = Dinv.as<double>();
     // signed mode: panel right factor S L^-1, signs per member (stride n)
@@ -562,10 +442,10 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
         k_copy_block<<<grid, 256, 0, st>>>(P, sA, n, Wp, sW, n, r, kNb);
         abh::check_launch();
       }
-      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, kNb, kNb, &one, Wp, n, sW,
-                                             DSp + int64_t(kb) * kNb * kNb, kNb, sD, &zero, P,
-                                             n, sA, G));
-      // trailing A[o+64:, o+64:] -= P (S_k) P^T
+      dgemm_strided_batched(st, false, true, r, kNb, kNb, 1.0, Wp, n, sW,
+                            DSp + int64_t(kb) * kNb * kNb, kNb, sD, 0.0, P, n, sA, G);
+      // trailing A[o+64:, o+64:] -= P (S_k) P^T, lower triangle (the upper
+      // one is never read again)
       const double *Pl = P;
       if (sgn) {
         dim3 grid(unsigned(abh::ceil_div(int64_t(r) * kNb, 256)), G);
@@ -573,9 +453,8 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
         abh::check_launch();
         Pl = Wp;
       }
-      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, T, r, r, kNb, &mone, Pl, n,
-                                             sgn ? sW : sA, P, n, sA, &one,
-                                             Ap + o + kNb + int64_t(o + kNb) * n, n, sA, G));
+      dgemm_strided_batched(st, false, true, r, r, kNb, -1.0, Pl, n, sgn ? sW : sA, P, n, sA,
+                            1.0, Ap + o + kNb + int64_t(o + kNb) * n, n, sA, G, true);
     }
     // Y = L^-1 E (E = first ni unit columns); Y is lower trapezoidal
     {
@@ -591,14 +470,12 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
         k_copy_block<<<grid, 256, 0, st>>>(Y + o, sY, n, Tp, sT, kNb, kNb, nc);
         abh::check_launch();
       }
-      ABH_BLAS(rocblas_dgemm_strided_batched(h, N, N, kNb, nc, kNb, &one,
-                                             Dp + int64_t(kb) * kNb * kNb, kNb, sD, Tp, kNb, sT,
-                                             &zero, Y + o, n, sY, G));
+      dgemm_strided_batched(st, false, false, kNb, nc, kNb, 1.0, Dp + int64_t(kb) * kNb * kNb,
+                            kNb, sD, Tp, kNb, sT, 0.0, Y + o, n, sY, G);
       const int r = n - o - kNb;
       if (r > 0)
-        ABH_BLAS(rocblas_dgemm_strided_batched(h, N, N, r, nc, kNb, &mone,
-                                               Ap + o + kNb + int64_t(o) * n, n, sA, Y + o, n,
-                                               sY, &one, Y + o + kNb, n, sY, G));
+        dgemm_strided_batched(st, false, false, r, nc, kNb, -1.0, Ap + o + kNb + int64_t(o) * n,
+                              n, sA, Y + o, n, sY, 1.0, Y + o + kNb, n, sY, G);
     }
     // Z = L^-T (S) Y, in place
     if (sgn) {
@@ -610,20 +487,17 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
       const int o = kb * kNb;
       const int r = n - o - kNb;
       if (r > 0)
-        ABH_BLAS(rocblas_dgemm_strided_batched(h, T, N, kNb, ni, r, &mone,
-                                               Ap + o + kNb + int64_t(o) * n, n, sA,
-                                               Y + o + kNb, n, sY, &one, Y + o, n, sY, G));
+        dgemm_strided_batched(st, true, false, kNb, ni, r, -1.0, Ap + o + kNb + int64_t(o) * n,
+                              n, sA, Y + o + kNb, n, sY, 1.0, Y + o, n, sY, G);
       {
         dim3 grid(unsigned(abh::ceil_div(int64_t(kNb) * ni, 256)), G);
         k_copy_block<<<grid, 256, 0, st>>>(Y + o, sY, n, Tp, sT, kNb, kNb, ni);
         abh::check_launch();
       }
-      ABH_BLAS(rocblas_dgemm_strided_batched(h, T, N, kNb, ni, kNb, &one,
-                                             Dp + int64_t(kb) * kNb * kNb, kNb, sD, Tp, kNb, sT,
-                                             &zero, Y + o, n, sY, G));
+      dgemm_strided_batched(st, true, false, kNb, ni, kNb, 1.0, Dp + int64_t(kb) * kNb * kNb,
+                            kNb, sD, Tp, kNb, sT, 0.0, Y + o, n, sY, G);
     }
   }
-  std::vector<uint8_t> fh(nd);
   ABH_HIP(hipMemcpyAsync(fh.data(), fail.p, size_t(nd), hipMemcpyDeviceToHost, st));
   ABH_HIP(hipStreamSynchronize(st));
   s->failed = std::accumulate(fh.begin(), fh.end(), int64_t(0));
@@ -672,7 +546,7 @@ abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<i
   if (s->n > 0)
     ABH_HIP(hipMemcpyAsync(pos.data(), pos_dev, pos.size() * 8, hipMemcpyDeviceToHost, hs));
   ABH_HIP(hipStreamSynchronize(hs));
-  analyse_batch(s.get(), dim, off, pos, lo, hi, n_leaf);
+  analyse_batch_into(s.get(), dim, off, pos, lo, hi, n_leaf);
   for (size_t d = 0; d + 1 < s->off.size(); ++d)
     s->max_m = std::max(s->max_m, int(s->off[d + 1] - s->off[d]));
   factorise(s.get(), pos_dev, k, -1.0, shift, true);
@@ -703,13 +577,14 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
     std::vector<double> pos(size_t(s->n) * D);
     std::vector<uint32_t> beg(size_t(cells->nb)), end(size_t(cells->nb));
     hipStream_t hs = ctx->stream;
+    abh::StreamDrain drain{hs};
     if (s->n > 0) {
       ABH_HIP(hipMemcpyAsync(pos.data(), cells->pos.p, pos.size() * 8, hipMemcpyDeviceToHost, hs));
       ABH_HIP(hipMemcpyAsync(beg.data(), cells->begin.p, beg.size() * 4, hipMemcpyDeviceToHost, hs));
       ABH_HIP(hipMemcpyAsync(end.data(), cells->end.p, end.size() * 4, hipMemcpyDeviceToHost, hs));
     }
     ABH_HIP(hipStreamSynchronize(hs));
-    if (s->n > 0) analyse(s, cells, pos, beg, end);
+    if (s->n > 0) analyse_into(s, cells, pos, beg, end);
     for (size_t d = 0; d + 1 < s->off.size(); ++d)
       s->max_m = std::max(s->max_m, int(s->off[d + 1] - s->off[d]));
     KernelDesc k{};

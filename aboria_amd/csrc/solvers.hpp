@@ -24,6 +24,15 @@ abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<i
                                   const double *pos_dev, const double *lo, const double *hi,
                                   double n_leaf, const KernelDesc &k, double shift);
 
+// Strided-batched FP64 GEMM on the matrix cores, column-major
+// (dgemm.hip): C[g] = alpha op(A[g]) op(B[g]) + beta C[g] for g < batch;
+// beta == 0 does not read C.  lower_only skips the 64 x 64 tiles of C that
+// lie strictly above its diagonal.  Enqueued on st; throws abh::Error.
+void dgemm_strided_batched(hipStream_t st, bool ta, bool tb, int m, int n, int k, double alpha,
+                           const double *A, int lda, int64_t sA, const double *B, int ldb,
+                           int64_t sB, double beta, double *C, int ldc, int64_t sC, int batch,
+                           bool lower_only = false);
+
 // x = W b with W the approximate-inverse rows (learned.hip).
 void ainv_apply(const abh_ainv *a, const double *b, double *x);
 int64_t ainv_size(const abh_ainv *a);

@@ -152,11 +152,18 @@ const int64_t *abh_search_row_ptr(const abh_search *s);
 abh_status abh_search_fill(abh_search *s, int32_t *col, int64_t capacity);
 /* The search of abh_search_update + abh_search_fill in one traversal of the
  * candidates: an upper bound of every row (the candidates of its visited
- * buckets, no distance tests) places the rows in a padded device buffer
- * owned by the handle (grow-only; about 4-5x the neighbour lists), the
- * traversal writes each row there and counts it, and the rows are then
- * packed into col, with the same row_ptr and lists as the two-pass form.
- * *total_pairs (host) gets the number of pairs; synchronises the stream. */
+ * buckets, no distance tests, rounded up to a multiple of 16 entries so
+ * that every padded row starts on a 64-byte boundary) places the rows in a
+ * padded device buffer owned by the handle, the traversal writes each row
+ * there and counts it, and the rows are then packed into col, with the same
+ * row_ptr and lists as the two-pass form.  The padded buffer is grow-only
+ * and holds at most 4 * (sum of the row bounds + 15 * nq) bytes: about 4-5x the
+ * neighbour lists at dozens of neighbours per centre, dominated by the
+ * 15 * nq rounding term when centres have few neighbours.
+ * *total_pairs (host) gets the number of pairs; synchronises the stream.
+ * When capacity < total this returns ABH_ERR_CAPACITY after *total_pairs,
+ * abh_search_total and abh_search_row_ptr have been updated: the caller can
+ * size col from them and call abh_search_fill (no second count pass). */
 abh_status abh_search_run_fused(abh_search *s, const abh_cells *cells, int64_t nq,
                                 const double *q, double radius, int32_t *col,
                                 int64_t capacity, int64_t *total_pairs);

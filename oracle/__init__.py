@@ -51,6 +51,8 @@ def lib():
         l.orc_search_count.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p, ctypes.c_int]
         l.orc_search_fill.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p, _i32p,
                                       ctypes.c_int]
+        l.orc_search_digest.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p,
+                                        ctypes.c_int]
         l.orc_sparse_matvec.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, P(KernelC),
                                         vp, vp, ctypes.c_int, _dp, ctypes.c_int64, _dp,
                                         ctypes.c_int64, ctypes.c_int]
@@ -72,6 +74,8 @@ def lib():
         l.orc_schwarz_create.restype = vp
         l.orc_schwarz_create.argtypes = [vp, P(KernelC), ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double]
+        l.orc_schwarz_analyse.restype = vp
+        l.orc_schwarz_analyse.argtypes = [vp, ctypes.c_double]
         l.orc_schwarz_destroy.argtypes = [vp]
         l.orc_schwarz_info.argtypes = [vp, _i64p]
         l.orc_schwarz_domains.argtypes = [vp, _i64p, _i32p, _i32p]
@@ -134,6 +138,17 @@ class Cells:
         lib().orc_search_fill(self.h, nq, c.reshape(-1), float(radius), row_ptr, col, nt)
         return row_ptr, col[:total]
 
+    def search_digest(self, centres, radius, nthreads=None):
+        """(nq, 4) int64 per-centre digest of the neighbour lists: count, sum
+        of indices, sum of squared indices, sum of a 64-bit mix of each
+        index (wrapping).  Order-independent: equal for equal multisets."""
+        c = np.ascontiguousarray(centres, np.float64)
+        nq = c.shape[0]
+        out = np.zeros((nq, 4), np.int64)
+        lib().orc_search_digest(self.h, nq, c.reshape(-1), float(radius), out.reshape(-1),
+                                nthreads or nthreads_default())
+        return out
+
     def sparse_matvec(self, centres, radius, kernel, x, y=None, row_s=None, col_s=None,
                       nthreads=None):
         """y += K x, x (ncols,) or (ncols, nrhs)."""
@@ -159,6 +174,22 @@ class Cells:
     def schwarz(self, kernel, radius, shift, buffer=1e5 * np.finfo(float).eps):
         """Restricted additive Schwarz preconditioner of A = K + shift I."""
         return Schwarz(self, kernel, radius, shift, buffer)
+
+    def schwarz_domains(self, buffer=1e5 * np.finfo(float).eps):
+        """(offsets, indices, n_interior) of the Schwarz domains
+        (analyzePattern only, no factorisation)."""
+        h = lib().orc_schwarz_analyse(self.h, float(buffer))
+        try:
+            info = np.zeros(3, np.int64)
+            lib().orc_schwarz_info(h, info)
+            nd, total = int(info[0]), int(info[1])
+            off = np.zeros(nd + 1, np.int64)
+            idx = np.zeros(max(total, 1), np.int32)
+            nint = np.zeros(max(nd, 1), np.int32)
+            lib().orc_schwarz_domains(h, off, idx, nint)
+        finally:
+            lib().orc_schwarz_destroy(h)
+        return off, idx[:total], nint[:nd]
 
     def knn(self, k):
         """(n, k) neighbour indices (self first) and squared distances."""

@@ -42,6 +42,15 @@ void init(AnyCells *h, const double *lo, const double *hi, const uint8_t *per, d
   size_grid<D>(g, 0);
 }
 
+// 64-bit finaliser of SplitMix64 (Steele, Lea, Flood 2014) applied to an
+// index: the per-entry term of the neighbour-list digest's fourth column.
+inline uint64_t digest_mix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
 template <int N> void fmm_n(Octree<3> *t, const Kernel *k, const double *q, double *y) {
   FMM<3, N> f(*t, *k);
   f.matvec(q, y);
@@ -140,6 +149,36 @@ void orc_search_fill(void *p, int64_t nq, const double *q, double r, const int64
       int64_t k = row_ptr[i];
       euclidean_search<D>(c, q + i * D, r,
                           [&](uint32_t j, const double *, double) { col[k++] = int32_t(j); });
+    }
+    return 0;
+  });
+}
+
+// Order-independent digest of every neighbour list, so that lists of
+// billions of pairs can be compared without storing them: per centre the
+// count, the sum of the neighbour indices and the sum of their squares, and
+// a sum of a 64-bit mix of each index (modulo 2^64), all in int64.
+// digest: nq x 4 int64, row-major.
+void orc_search_digest(void *p, int64_t nq, const double *q, double r, int64_t *digest,
+                       int nthreads) {
+  auto *h = static_cast<AnyCells *>(p);
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Cells<D> &c = get<D>(h);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int64_t i = 0; i < nq; ++i) {
+      int64_t cnt = 0, s1 = 0, s2 = 0;
+      uint64_t sh = 0;
+      euclidean_search<D>(c, q + i * D, r, [&](uint32_t j, const double *, double) {
+        ++cnt;
+        s1 += int64_t(j);
+        s2 += int64_t(j) * int64_t(j);
+        sh += digest_mix(j);
+      });
+      digest[4 * i] = cnt;
+      digest[4 * i + 1] = s1;
+      digest[4 * i + 2] = s2;
+      digest[4 * i + 3] = int64_t(sh);
     }
     return 0;
   });
@@ -267,6 +306,19 @@ void *orc_schwarz_create(void *p, const Kernel *k, double radius, double shift, 
     constexpr int D = decltype(DD)::value;
     schwarz_analyse<D>(get<D>(h), buffer, sh->s);
     schwarz_factorise<D>(get<D>(h), *k, r2, shift, sh->s);
+    return 0;
+  });
+  return sh;
+}
+
+// analyzePattern only (src/Preconditioners.h:1169-1316): the domain lists,
+// no factorisation (orc_schwarz_apply must not be called on the handle)
+void *orc_schwarz_analyse(void *p, double buffer) {
+  auto *h = static_cast<AnyCells *>(p);
+  auto *sh = new SchwarzH();
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    schwarz_analyse<D>(get<D>(h), buffer, sh->s);
     return 0;
   });
   return sh;

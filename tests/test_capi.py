@@ -50,3 +50,20 @@ def test_cpp_layer_compiles_and_links():
     import __graft_entry__ as ge
     out = ge.build_cpp_example()
     assert os.path.exists(out)
+
+
+def test_search_rejects_host_buffers():
+    """Search.fill / run / run_fused check their tensors before any call into
+    the library: a host tensor raises ValueError (no GPU needed)."""
+    import pytest
+    import torch
+    import aboria_amd as ab
+    s = ab.Search.__new__(ab.Search)
+    s.handle = None
+    with pytest.raises(ValueError):
+        s.fill(torch.empty(16, dtype=torch.int32))
+    with pytest.raises(ValueError):
+        s.run(None, torch.empty((4, 3), dtype=torch.float64), 0.1)
+    with pytest.raises(ValueError):
+        s.run_fused(None, torch.empty((4, 3), dtype=torch.float64), 0.1,
+                    torch.empty(16, dtype=torch.int32))

@@ -268,10 +268,14 @@ def test_sparse_assemble(ctx, periodic):
         assert np.allclose(valn, np.exp(-d2 / 0.05 ** 2), rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("shift", [0, 1, 2, 3])
 @pytest.mark.parametrize("case", ["d3_p", "d2_np", "d3_p_wide", "d1_p"])
-def test_fused_search_equals_two_pass(ctx, case):
+def test_fused_search_equals_two_pass(ctx, case, shift):
     """abh_search_run_fused: same row_ptr and the same lists (same order) as
-    abh_search_update + abh_search_fill."""
+    abh_search_update + abh_search_fill, into a column buffer starting at
+    every 4-byte offset of a 16-byte line (the row compaction copies 16 bytes
+    per lane from each row's first aligned address); the guard entries past
+    the pairs are left untouched."""
     name, D, N, periodic, n_leaf, r, lo, hi = [c for c in CASES if c[0] == case][0]
     pos, alive = make_case(D, N, lo, hi, 77)
     cl, p, a, na = gpu_build(ctx, pos, alive, [lo] * D, [hi] * D, [periodic] * D, n_leaf)
@@ -281,8 +285,28 @@ def test_fused_search_equals_two_pass(ctx, case):
     col2 = torch.empty(total + 16, dtype=torch.int32, device="cuda")
     s.fill(col2)
     rp2 = s.row_ptr().clone()
-    col1 = torch.full((total + 16,), -7, dtype=torch.int32, device="cuda")
+    buf = torch.full((total + 16 + shift,), -7, dtype=torch.int32, device="cuda")
+    col1 = buf[shift:]
+    assert (col1.data_ptr() - buf.data_ptr()) == 4 * shift
     t1 = s.run_fused(cl, sp, r, col1)
     assert t1 == total
     assert torch.equal(s.row_ptr(), rp2)
     assert torch.equal(col1[:total], col2[:total])
+    assert torch.all(col1[total:] == -7)
+    assert torch.all(buf[:shift] == -7)
+
+
+def test_search_rejects_wrong_buffers():
+    """Search.fill / run / run_fused check their tensors before any launch:
+    a host tensor or a wrong dtype raises ValueError."""
+    s = ab.Search.__new__(ab.Search)
+    s.handle = None
+    with pytest.raises(ValueError):
+        s.fill(torch.empty(16, dtype=torch.int32))
+    with pytest.raises(ValueError):
+        s.fill(torch.empty(16, dtype=torch.int64, device="cuda"))
+    with pytest.raises(ValueError):
+        s.run(None, torch.empty((4, 3), dtype=torch.float64), 0.1)
+    with pytest.raises(ValueError):
+        s.run_fused(None, torch.empty((4, 3), dtype=torch.float32, device="cuda"), 0.1,
+                    torch.empty(16, dtype=torch.int32, device="cuda"))

@@ -206,3 +206,107 @@ def test_learned_preconditioner_gain(ctx):
           flush=True)
     assert res["learned"]["converged"] and res["learned"]["true_residual"] <= 1.01e-8
     assert res["learned"]["iterations"] < 0.5 * res["none"]["iterations"]
+
+
+def _c5_ratio_problem(ctx, n1=128, seed=5000):
+    """The 400,056-point Schwarz workload scaled down to n1 x n1 points with
+    its ratios kept: jitter +-0.3 h, Gaussian width sigma = 0.01 * 632.5 h
+    (6.3 h), cut off at 6 sigma, diagonal shift 1e-5, 576 particles per
+    bucket (bucket side 25.6 h here, 24 h at full size)."""
+    c = wl.SCHWARZ
+    h = 1.0 / n1
+    sigma = c["sigma"] * np.sqrt(c["nx"] * c["ny"]) * h
+    radius, shift = c["cutoff"] * sigma, c["shift"]
+    pos = wl.jittered_grid(n1, n1, c["jitter"], seed=seed)
+    cl, oc = build(ctx, pos, c["n_leaf"])
+    return cl, oc, sigma, radius, shift
+
+
+def test_c5_ratio_schwarz_domains_apply_gmres(ctx):
+    """Schwarz at the configured kernel width (sigma = 6.3 h, +-0.3 h jitter)
+    on a 128 x 128 grid: domains bit-exact against the oracle's analysis, the
+    apply against dense local solves (numpy LAPACK) of every domain, and
+    GMRES(50) to 1e-8 with the true residual recomputed by the oracle."""
+    cl, oc, sigma, radius, shift = _c5_ratio_problem(ctx)
+    print("c5 ratio: n=%d sigma=%.4f radius=%.4f grid=%s" % (cl.n_alive, sigma, radius, cl.grid),
+          flush=True)
+    kern = ab.Kernel.gaussian(sigma)
+    sw = ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift)
+    off, idx, nint = sw.domains()
+    ooff, oidx, onint = oc.schwarz_domains()
+    assert np.array_equal(off, ooff) and np.array_equal(idx, oidx) and np.array_equal(nint, onint)
+    assert sw.info["failed_domains"] == 0
+    # apply: x = b, then every domain's interior rows from its local solve
+    P = oc.positions
+    b = np.random.default_rng(11).normal(size=cl.n_alive)
+    x = sw.solve(torch.tensor(b, device="cuda")).cpu().numpy()
+    ref = b.copy()
+    for d in range(len(nint)):
+        ids = idx[off[d]:off[d + 1]]
+        q = P[ids]
+        d2 = ((q[:, None, :] - q[None, :, :]) ** 2).sum(-1)
+        A = np.where(d2 <= radius * radius, np.exp(-d2 / sigma ** 2), 0.0)
+        A[np.diag_indices_from(A)] += shift
+        ref[ids[:nint[d]]] = np.linalg.solve(A, b[ids])[:nint[d]]
+    err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+    print("c5 ratio apply rel err vs numpy local solves %.2e" % err, flush=True)
+    assert err < 1e-6
+    # GMRES(50) with and without the preconditioner
+    op = ab.SparseOperator(ctx, cl, cl.positions(), radius, kern)
+    bt = torch.tensor(b, device="cuda")
+    xs, st = op.gmres(bt, diag_shift=shift, preconditioner=sw, restart=50, max_iters=400)
+    _, st0 = op.gmres(bt, diag_shift=shift, restart=50, max_iters=400)
+    print("c5 ratio gmres: schwarz", st, "none", st0, flush=True)
+    assert st["converged"]
+    r = op_matvec(oc, (GAUSS, sigma), radius, shift, xs.cpu().numpy()) - b
+    true_res = np.linalg.norm(r) / np.linalg.norm(b)
+    print("c5 ratio true residual %.3e" % true_res, flush=True)
+    assert true_res <= 1.01e-8
+    assert not st0["converged"] or st0["iterations"] > st["iterations"]
+
+
+def test_c5_ratio_learned_vs_schwarz(ctx):
+    """The learned approximate inverse at the configured jitter (+-0.3 h) and
+    kernel width, trained on seeded point sets, against plain Schwarz (its
+    fallback): the iteration counts of both are recorded and the solve that
+    ends through the fallback meets 1e-8 on the true residual."""
+    cl, oc, sigma, radius, shift = _c5_ratio_problem(ctx)
+    c = wl.SCHWARZ
+    k = c["k"]
+    kern = ab.Kernel.gaussian(sigma)
+    sets = [wl.jittered_grid(128, 128, c["jitter"], seed=200 + s) for s in range(2)]
+    X, T = learned.training_set(ctx, sets, kern, radius, shift, 16.0, k, feature_scale=sigma)
+    net, scale, losses = learned.train(ctx, X, T, epochs=30, batch=512)
+    print("c5 ratio training loss %.3e -> %.3e" % (losses[0], losses[-1]), flush=True)
+    op = ab.SparseOperator(ctx, cl, cl.positions(), radius, kern)
+    b = torch.tensor(np.random.default_rng(12).normal(size=cl.n_alive), device="cuda")
+    sw = ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift)
+    ai = ab.ApproxInverse.learned(ctx, cl, net, k, sigma, scale)
+    _, st_sw = op.gmres(b, diag_shift=shift, preconditioner=sw, restart=50, max_iters=400)
+    x, rep = ab.gmres_with_fallback(op, b, shift, ai, sw, restart=50, max_iters=400,
+                                    learned_iters=100)
+    print("c5 ratio: schwarz %s | learned %s fallback %s" % (st_sw, rep["learned"],
+                                                              rep["fallback"]), flush=True)
+    final = rep["fallback"] or rep["learned"]
+    assert final["converged"]
+    r = op_matvec(oc, (GAUSS, sigma), radius, shift, x.cpu().numpy()) - b.cpu().numpy()
+    assert np.linalg.norm(r) / torch.linalg.norm(b).item() <= 1.01e-8
+
+
+def test_schwarz_apply_gather_path(ctx):
+    """Domains larger than the apply's LDS staging (7,936 particles) take
+    the gathered form of the apply: 9,000 points in 2D with 2 x 2 buckets, so
+    every domain covers every particle and the preconditioner is the exact
+    inverse; A x = b is checked with the oracle's operator."""
+    pos = np.random.default_rng(41).random((9000, 2))
+    cl, oc = build(ctx, pos, 2000.0)
+    assert cl.grid == (2, 2)
+    kern, sigma, radius, shift = ab.Kernel.gaussian(0.01), 0.01, 0.06, 1.0
+    sw = ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift)
+    assert sw.info["max_size"] == 9000 > 7936
+    b = np.random.default_rng(42).normal(size=9000)
+    x = sw.solve(torch.tensor(b, device="cuda")).cpu().numpy()
+    r = op_matvec(oc, (GAUSS, sigma), radius, shift, x) - b
+    err = np.linalg.norm(r) / np.linalg.norm(b)
+    print("gather-path apply residual %.2e" % err, flush=True)
+    assert err < 1e-10
