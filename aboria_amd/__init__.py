@@ -18,7 +18,7 @@ __all__ = [
     "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
     "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
     "DenseBatchOperator", "SchwartzPreconditioner", "knn", "Mlp", "ApproxInverse",
-    "gmres_with_fallback", "dense_apply",
+    "gmres_with_fallback", "dense_apply", "IdMap",
 ]
 
 FP64 = C.FP64
@@ -105,6 +105,38 @@ class CellListOrdered:
                                        float(n_particles_in_leaf), ctypes.byref(h)), ctx.handle)
         self.handle = h
         self.n_alive = 0
+        self.id_map = None
+
+    def find(self, ids):
+        """CellListOrderedQuery::find: container indices of the particles with
+        these ids (n_alive for a missing id); needs init_id_search on the
+        particle set that owns this cell list."""
+        if self.id_map is None:
+            raise RuntimeError("no id map: call init_id_search on the particle set")
+        return self.id_map.find(ids)
+
+    def bucket_pairs(self):
+        """get_neighbouring_buckets(query): the bucket_pair_iterator sequence
+        as device tensors (bi, bj, image); the position offset of pair p is
+        image[p] * (high - low)."""
+        n = ctypes.c_int64()
+        check(C.lib().abh_cells_bucket_pairs(self.handle, 0, None, None, None, ctypes.byref(n)),
+              self.ctx.handle)
+        dev = self.ctx.device
+        bi = torch.empty(max(n.value, 1), dtype=torch.int32, device=dev)
+        bj = torch.empty(max(n.value, 1), dtype=torch.int32, device=dev)
+        im = torch.empty((max(n.value, 1), self.dim), dtype=torch.int8, device=dev)
+        check(C.lib().abh_cells_bucket_pairs(self.handle, n.value, _ptr(bi), _ptr(bj), _ptr(im),
+                                             ctypes.byref(n)), self.ctx.handle)
+        return bi[:n.value], bj[:n.value], im[:n.value]
+
+    def pair_counts(self, radius):
+        """Neighbours of every particle (strict radius, itself included) by
+        the symmetric bucket-pair traversal; needs bucket sides >= radius."""
+        out = torch.empty(self.n_alive, dtype=torch.int32, device=self.ctx.device)
+        check(C.lib().abh_cells_pair_counts(self.handle, float(radius), _ptr(out)),
+              self.ctx.handle)
+        return out
 
     def update_positions(self, positions, alive=None):
         """Wraps/kills in place, sorts by bucket.  Returns the alive count."""
@@ -191,6 +223,7 @@ class Particles:
         self.alive = torch.ones(n, dtype=torch.uint8, device=ctx.device)
         self.columns = {k: v.to(ctx.device).contiguous() for k, v in columns.items()}
         self.search = None
+        self.id_map = None
 
     @property
     def dimension(self):
@@ -211,11 +244,77 @@ class Particles:
         self.id = s.gather(self.id)
         self.alive = torch.ones(s.n_alive, dtype=torch.uint8, device=self.ctx.device)
         self.columns = {k: s.gather(v) for k, v in self.columns.items()}
+        if self.id_map is not None:
+            self.id_map.update(self.id)
+        s.id_map = self.id_map
+
+    def init_id_search(self):
+        """Particles::init_id_search (src/Particles.h:460-465): keeps a
+        find-by-id map of this set, rebuilt by every update_positions."""
+        if self.id_map is None:
+            self.id_map = IdMap(self.ctx, self.id)
+        else:
+            self.id_map.update(self.id)
+        if self.search is not None:
+            self.search.id_map = self.id_map
+
+    def find(self, ids):
+        """Container index of the particle with each id, or size() when no
+        particle has it (CellListOrderedQuery::find)."""
+        if self.id_map is None:
+            raise RuntimeError("init_id_search not called on this particle set")
+        return self.id_map.find(ids)
 
     def get_query(self):
         if self.search is None:
             raise RuntimeError("init_neighbour_search not called on this particle set")
         return self.search
+
+
+class IdMap:
+    """Find-by-id map: the (id, index) pairs of a particle set sorted by id
+    (neighbour_search_base's m_id_map_key / m_id_map_value)."""
+
+    def __init__(self, ctx, ids):
+        _require_cuda(ids, torch.int64, "ids")
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(C.lib().abh_id_map_create(ctx.handle, ids.numel(), _ptr(ids), ctypes.byref(h)),
+              ctx.handle)
+        self.handle = h
+        self.n = ids.numel()
+
+    def update(self, ids):
+        _require_cuda(ids, torch.int64, "ids")
+        check(C.lib().abh_id_map_update(self.handle, ids.numel(), _ptr(ids)), self.ctx.handle)
+        self.n = ids.numel()
+
+    def keys(self):
+        return _view(C.lib().abh_id_map_keys(self.handle), (self.n,), torch.int64,
+                     self.ctx.device)
+
+    def values(self):
+        return _view(C.lib().abh_id_map_values(self.handle), (self.n,), torch.int32,
+                     self.ctx.device)
+
+    def find(self, ids):
+        """(nq,) int32 container indices of the given ids; n for a missing id."""
+        ids = torch.as_tensor(ids, dtype=torch.int64, device=self.ctx.device).contiguous()
+        out = torch.empty(ids.shape, dtype=torch.int32, device=self.ctx.device)
+        check(C.lib().abh_id_map_find(self.handle, ids.numel(), _ptr(ids), _ptr(out)),
+              self.ctx.handle)
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_id_map_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SearchResult:

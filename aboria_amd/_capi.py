@@ -162,6 +162,16 @@ SIGNATURES = {
                                       P(GmresStats)]),
     "abh_dense_apply": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int64, c_void_p, P(Kernel),
                                 c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+    "abh_cells_bucket_pairs": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                       P(c_int64)]),
+    "abh_cells_pair_counts": (c_int, [c_void_p, c_double, c_void_p]),
+    "abh_id_map_create": (c_int, [c_void_p, c_int64, c_void_p, P(c_void_p)]),
+    "abh_id_map_update": (c_int, [c_void_p, c_int64, c_void_p]),
+    "abh_id_map_destroy": (c_int, [c_void_p]),
+    "abh_id_map_size": (c_int, [c_void_p, P(c_int64)]),
+    "abh_id_map_keys": (c_void_p, [c_void_p]),
+    "abh_id_map_values": (c_void_p, [c_void_p]),
+    "abh_id_map_find": (c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
     "abh_dense_batch_tune": (c_int, [c_void_p, c_int, P(c_int), P(ctypes.c_float)]),
     "abh_schwarz_create": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double,
                                    P(c_void_p)]),

@@ -32,6 +32,9 @@ struct abh_cells {
   int64_t nb = 1;
   abh::DevBuf order, keys, begin, end, pos;
   abh::DevBuf flags, offs, counters;  // build temporaries
+  // bucket-pair sequence (pairs.hip): per-source run lengths, total, pairs
+  abh::DevBuf pair_src, pair_total, pair_i, pair_j, pair_img;
+  int64_t npairs = 0;
 
   abh::GridParams params() const {
     abh::GridParams g{};

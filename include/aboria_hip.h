@@ -116,10 +116,64 @@ const uint32_t *abh_cells_bucket_begin(const abh_cells *cells); /* n_buckets*/
 const uint32_t *abh_cells_bucket_end(const abh_cells *cells);   /* n_buckets*/
 const double *abh_cells_positions(const abh_cells *cells); /* n_alive x dim */
 
+/* get_neighbouring_buckets(query) (src/Search.h:857-860): the sequence of
+ * (bucket i, bucket j, periodic image) triples bucket_pair_iterator yields
+ * (src/Search.h:498-770), in its order -- every bucket but the last against
+ * the buckets of its 3^dim block that come after it, then for each periodic
+ * quadrant after 0 the face buckets against the block around their ghost
+ * image -- so that each pair of adjacent buckets is visited once (the
+ * symmetric half-shell of pair loops).  Pair p's position offset is
+ * image[p * dim + d] * (high - low)[d], added to bucket i's particles.
+ * Buckets are collapsed indices.  *npairs (host) gets the count (this
+ * synchronises); with bi, bj, image all NULL nothing else is written,
+ * otherwise they are device arrays of capacity (image: capacity * dim)
+ * entries, ABH_ERR_CAPACITY when that is below the count. */
+abh_status abh_cells_bucket_pairs(abh_cells *cells, int64_t capacity, int32_t *bi,
+                                  int32_t *bj, int8_t *image, int64_t *npairs);
+/* The neighbour counts of the reference's symmetric pair loop over those
+ * bucket pairs (tests/neighbours.h:892-951): counts[k] (device, n_alive,
+ * gather order) = 1 + the particles b != k with |x_k - x_b + offset|^2 <
+ * radius^2 (strict), each pair tested once and counted for both.  Needs
+ * every bucket side >= radius (ABH_ERR_INVALID_ARG otherwise), as the 3^dim
+ * traversal does. */
+abh_status abh_cells_pair_counts(abh_cells *cells, double radius, int32_t *counts);
+
 /* dst[k] = src[order[k]] for k < n, elements of elem_bytes bytes (the
  * per-column gather of Particles::reorder, src/Particles.h:694-732). */
 abh_status abh_gather(abh_context *ctx, const int32_t *order, int64_t n,
                       const void *src, void *dst, size_t elem_bytes);
+
+/* ------------------------------------------------------------------ */
+/* find-by-id map                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_id_map abh_id_map;
+
+/* Replaces neighbour_search_base::init_id_map and the id-map maintenance of
+ * update_positions (src/NeighbourSearchBase.h:294-298, :440-486), switched
+ * on by Particles::init_id_search (src/Particles.h:460-465): the (id,
+ * container index) pairs of a particle set sorted by id (ids compared as
+ * unsigned 64-bit values, as the reference's size_t; equal ids keep
+ * container order).  ids: device, n int64 -- the id column in container
+ * order (after abh_gather of the id column by the cell list's order when
+ * the particles were reordered).  The map keeps no reference to ids. */
+abh_status abh_id_map_create(abh_context *ctx, int64_t n, const int64_t *ids,
+                             abh_id_map **out);
+/* Rebuilds the map for a new id column (reusing its device buffers). */
+abh_status abh_id_map_update(abh_id_map *map, int64_t n, const int64_t *ids);
+abh_status abh_id_map_destroy(abh_id_map *map);
+abh_status abh_id_map_size(const abh_id_map *map, int64_t *n);
+/* Device arrays owned by the map (m_id_map_key / m_id_map_value), n entries,
+ * valid until the next update/destroy. */
+const int64_t *abh_id_map_keys(const abh_id_map *map);
+const int32_t *abh_id_map_values(const abh_id_map *map);
+/* CellListOrderedQuery::find (src/CellListOrdered.h:379-388) for every
+ * query: index[i] = container index of the particle whose id is ids[i] (the
+ * lowest such index when several share it), or n -- the end of the
+ * container -- when no particle has that id.  ids, index: device, nq
+ * entries.  Asynchronous on the context stream. */
+abh_status abh_id_map_find(const abh_id_map *map, int64_t nq, const int64_t *ids,
+                           int32_t *index);
 
 /* ------------------------------------------------------------------ */
 /* radius search (euclidean_search, bulk form)                         */

@@ -51,6 +51,9 @@ def lib():
         l.orc_search_count.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p, ctypes.c_int]
         l.orc_search_fill.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p, _i32p,
                                       ctypes.c_int]
+        l.orc_bucket_pairs.restype = ctypes.c_int64
+        l.orc_bucket_pairs.argtypes = [vp, vp, vp, vp]
+        l.orc_pair_counts.argtypes = [vp, ctypes.c_double, _i32p]
         l.orc_search_digest.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, _i64p,
                                         ctypes.c_int]
         l.orc_sparse_matvec.argtypes = [vp, ctypes.c_int64, _dp, ctypes.c_double, P(KernelC),
@@ -137,6 +140,23 @@ class Cells:
         col = np.zeros(max(total, 1), np.int32)
         lib().orc_search_fill(self.h, nq, c.reshape(-1), float(radius), row_ptr, col, nt)
         return row_ptr, col[:total]
+
+    def bucket_pairs(self):
+        """The bucket_pair_iterator sequence: (bi, bj, q) with q (npairs, dim)
+        the periodic quadrant of each pair (offset q * (high - low))."""
+        n = lib().orc_bucket_pairs(self.h, None, None, None)
+        bi = np.zeros(max(n, 1), np.int32)
+        bj = np.zeros(max(n, 1), np.int32)
+        q = np.zeros((max(n, 1), 3), np.int32)
+        lib().orc_bucket_pairs(self.h, bi.ctypes.data, bj.ctypes.data, q.ctypes.data)
+        return bi[:n], bj[:n], q[:n, :self.dim]
+
+    def pair_counts(self, radius):
+        """neighbour counts (strict radius, self included) by bucket-pair
+        traversal, in container order"""
+        out = np.zeros(self.n_alive, np.int32)
+        lib().orc_pair_counts(self.h, float(radius), out)
+        return out
 
     def search_digest(self, centres, radius, nthreads=None):
         """(nq, 4) int64 per-centre digest of the neighbour lists: count, sum

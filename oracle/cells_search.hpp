@@ -269,3 +269,191 @@ void euclidean_search(const Cells<D> &c, const double *centre, double radius, Vi
 }
 
 }  // namespace orc
+
+namespace orc {
+
+// lattice_iterator<D> over [start, end) (src/LatticeIterator.h:48-295): an
+// odometer with the last dimension fastest; invalid once it runs past the
+// last point (or when built empty).
+template <int D> struct Lattice {
+  int start[D], end[D], cur[D];
+  bool valid = false;
+  Lattice() = default;
+  Lattice(const int *s, const int *e) {
+    valid = true;
+    for (int d = 0; d < D; ++d) {
+      start[d] = s[d];
+      end[d] = e[d];
+      cur[d] = s[d];
+      if (e[d] <= s[d]) valid = false;
+    }
+  }
+  void set(const int *p) {
+    for (int d = 0; d < D; ++d) cur[d] = p[d];
+  }
+  void inc() {
+    for (int d = D - 1; d >= 0; --d) {
+      if (++cur[d] < end[d]) return;
+      cur[d] = start[d];
+    }
+    valid = false;
+  }
+  bool has_next() const {  // !(it + 1 == false)
+    Lattice t = *this;
+    t.inc();
+    return t.valid;
+  }
+};
+
+template <int D> struct BucketPair {
+  int i[D], j[D], q[D];
+};
+
+// get_neighbouring_buckets(query, bucket) (src/Search.h:688-724): the 3^D
+// block around `bucket` clipped to the grid, empty when it misses the grid.
+template <int D> Lattice<D> neighbouring_buckets(const Grid<D> &g, const int *b) {
+  int s[D], e[D];
+  bool none = false;
+  for (int d = 0; d < D; ++d) {
+    s[d] = b[d] - 1;
+    e[d] = b[d] + 1;
+    if (s[d] < 0) s[d] = 0;
+    else if (s[d] > g.end_bucket[d]) { none = true; s[d] = g.end_bucket[d]; }
+    if (e[d] < 0) { none = true; e[d] = 0; }
+    else if (e[d] > g.end_bucket[d]) e[d] = g.end_bucket[d];
+    e[d] += 1;
+  }
+  if (none) return Lattice<D>();
+  return Lattice<D>(s, e);
+}
+
+// get_regular_buckets(query, quadrant) (src/Search.h:739-770): every bucket
+// for quadrant 0; the face a periodic quadrant's ghosts touch otherwise.
+template <int D> Lattice<D> regular_buckets(const Grid<D> &g, const int *q) {
+  int s[D], e[D];
+  for (int d = 0; d < D; ++d) {
+    s[d] = 0;
+    e[d] = g.end_bucket[d];
+    if (g.periodic[d]) {
+      if (q[d] > 0) { s[d] = 0; e[d] = 0; }
+      else if (q[d] < 0) { s[d] = g.end_bucket[d]; e[d] = g.end_bucket[d]; }
+    }
+    e[d] += 1;
+  }
+  return Lattice<D>(s, e);
+}
+
+// The sequence bucket_pair_iterator<Query> yields (src/Search.h:498-680,
+// get_neighbouring_buckets(query) :857-860): the constructor and increment
+// state machine restated over the lattice iterators above.  Each item is
+// (bucket i, bucket j, periodic quadrant q); the position offset the
+// reference yields is q * (high - low).
+template <int D> void bucket_pairs(const Grid<D> &g, std::vector<BucketPair<D>> &out) {
+  out.clear();
+  int ps[D], pe[D], zero[D];
+  for (int d = 0; d < D; ++d) {
+    ps[d] = g.periodic[d] ? -1 : 0;
+    pe[d] = g.periodic[d] ? 2 : 1;
+    zero[d] = 0;
+  }
+  Lattice<D> per(ps, pe);
+  per.set(zero);
+  bool dd = true;
+  Lattice<D> mi = regular_buckets<D>(g, per.cur), mj;
+  auto ghost_block = [&]() {
+    int b[D];
+    for (int d = 0; d < D; ++d) b[d] = mi.cur[d] + per.cur[d] * (g.end_bucket[d] + 1);
+    return neighbouring_buckets<D>(g, b);
+  };
+  if (!mi.has_next()) {
+    dd = false;
+    per.inc();
+    if (!per.valid) return;
+    mi = regular_buckets<D>(g, per.cur);
+    mj = ghost_block();
+  } else {
+    mj = neighbouring_buckets<D>(g, mi.cur);
+    mj.set(mi.cur);
+    mj.inc();
+  }
+  for (;;) {
+    BucketPair<D> p;
+    for (int d = 0; d < D; ++d) {
+      p.i[d] = mi.cur[d];
+      p.j[d] = mj.cur[d];
+      p.q[d] = per.cur[d];
+    }
+    out.push_back(p);
+    // increment
+    mj.inc();
+    if (!mj.valid) {
+      mi.inc();
+      if (dd ? !mi.has_next() : !mi.valid) {
+        dd = false;
+        per.inc();
+        if (!per.valid) return;
+        mi = regular_buckets<D>(g, per.cur);
+        mj = ghost_block();
+      } else {
+        mj = ghost_block();
+        if (dd) {
+          mj.set(mi.cur);
+          mj.inc();
+        }
+      }
+    }
+  }
+}
+
+// Neighbour counts by bucket-pair traversal, as the reference's
+// aboria_fast_bucketsearch_check_neighbour + _check_self functors do
+// (tests/neighbours.h:892-951): for every pair (i, j, q) and particles
+// a in i, b in j, |(x_a + q (high - low)) - x_b|^2 < r^2 adds one to both;
+// within a bucket every particle counts itself and each later particle of
+// the bucket within r adds one to both.  Sums in dimension order.
+template <int D> void pair_counts(const Cells<D> &c, double r, int32_t *counts) {
+  const Grid<D> &g = c.g;
+  const double r2 = r * r;
+  const size_t n = c.order.size();
+  for (size_t k = 0; k < n; ++k) counts[k] = 0;
+  std::vector<BucketPair<D>> pairs;
+  bucket_pairs<D>(g, pairs);
+  for (const BucketPair<D> &p : pairs) {
+    const uint32_t bi = collapse<D>(g, p.i), bj = collapse<D>(g, p.j);
+    double off[D];
+    for (int d = 0; d < D; ++d) off[d] = double(p.q[d]) * (g.hi[d] - g.lo[d]);
+    for (uint32_t a = c.begin[bi]; a < c.end[bi]; ++a) {
+      double pa[D];
+      for (int d = 0; d < D; ++d) pa[d] = c.pos[size_t(a) * D + d] + off[d];
+      for (uint32_t b = c.begin[bj]; b < c.end[bj]; ++b) {
+        double s = 0;
+        for (int d = 0; d < D; ++d) {
+          const double dx = pa[d] - c.pos[size_t(b) * D + d];
+          s = s + dx * dx;
+        }
+        if (s < r2) {
+          ++counts[a];
+          ++counts[b];
+        }
+      }
+    }
+  }
+  const size_t nb = g.nbuckets();
+  for (size_t b = 0; b < nb; ++b)
+    for (uint32_t a = c.begin[b]; a < c.end[b]; ++a) {
+      ++counts[a];
+      for (uint32_t e = a + 1; e < c.end[b]; ++e) {
+        double s = 0;
+        for (int d = 0; d < D; ++d) {
+          const double dx = c.pos[size_t(a) * D + d] - c.pos[size_t(e) * D + d];
+          s = s + dx * dx;
+        }
+        if (s < r2) {
+          ++counts[a];
+          ++counts[e];
+        }
+      }
+    }
+}
+
+}  // namespace orc

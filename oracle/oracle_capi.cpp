@@ -154,6 +154,36 @@ void orc_search_fill(void *p, int64_t nq, const double *q, double r, const int64
   });
 }
 
+// The bucket_pair_iterator sequence: returns the number of pairs; when bi is
+// non-null writes collapsed bucket indices bi[p], bj[p] and the periodic
+// quadrant q[p * 3 + d] (zero-padded to 3 entries) of each.
+int64_t orc_bucket_pairs(void *p, int32_t *bi, int32_t *bj, int32_t *q) {
+  auto *h = static_cast<AnyCells *>(p);
+  return dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    const Cells<D> &c = get<D>(h);
+    std::vector<BucketPair<D>> pairs;
+    bucket_pairs<D>(c.g, pairs);
+    if (bi)
+      for (size_t k = 0; k < pairs.size(); ++k) {
+        bi[k] = int32_t(collapse<D>(c.g, pairs[k].i));
+        bj[k] = int32_t(collapse<D>(c.g, pairs[k].j));
+        for (int d = 0; d < 3; ++d) q[k * 3 + d] = d < D ? pairs[k].q[d] : 0;
+      }
+    return int64_t(pairs.size());
+  });
+}
+
+// neighbour counts by bucket-pair traversal (strict r), container order
+void orc_pair_counts(void *p, double r, int32_t *counts) {
+  auto *h = static_cast<AnyCells *>(p);
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    pair_counts<D>(get<D>(h), r, counts);
+    return 0;
+  });
+}
+
 // Order-independent digest of every neighbour list, so that lists of
 // billions of pairs can be compared without storing them: per centre the
 // count, the sum of the neighbour indices and the sum of their squares, and
