@@ -18,7 +18,7 @@ __all__ = [
     "Kernel", "SparseOperator", "create_sparse_operator", "AbhError",
     "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
     "DenseBatchOperator", "SchwartzPreconditioner", "knn", "Mlp", "ApproxInverse",
-    "gmres_with_fallback", "dense_apply", "IdMap",
+    "gmres_with_fallback", "dense_apply", "IdMap", "BlockKernel", "SparseBlockOperator",
 ]
 
 FP64 = C.FP64
@@ -137,6 +137,13 @@ class CellListOrdered:
         check(C.lib().abh_cells_pair_counts(self.handle, float(radius), _ptr(out)),
               self.ctx.handle)
         return out
+
+    def query_view(self):
+        """The abh_query_view POD (ctypes) to pass to HIP kernels that use
+        include/aboria_hip_device.hpp."""
+        v = C.QueryView()
+        check(C.lib().abh_cells_query_view(self.handle, ctypes.byref(v)))
+        return v
 
     def update_positions(self, positions, alive=None):
         """Wraps/kills in place, sorts by bucket.  Returns the alive count."""
@@ -545,11 +552,116 @@ class SparseOperator:
             pass
 
 
+class BlockKernel:
+    """Matrix-valued kernel: entry (ii, jj) of the block of pair (i, j) is
+    phi(|dx|) * (w0[ii][jj] + w_row[ii][jj] a_i + w_col[ii][jj] b_j), with
+    phi the profile of a Kernel (its weights are ignored) and a, b the row and
+    column scalars (zero when not given)."""
+
+    def __init__(self, profile, w0, w_row=None, w_col=None):
+        import numpy as np
+        w0 = np.atleast_2d(np.asarray(w0, np.float64))
+        self.block_rows, self.block_cols = w0.shape
+        z = np.zeros_like(w0)
+        self.profile = profile
+        self.w = [w0, np.atleast_2d(np.asarray(w_row if w_row is not None else z, np.float64)),
+                  np.atleast_2d(np.asarray(w_col if w_col is not None else z, np.float64))]
+        for w in self.w:
+            if w.shape != w0.shape:
+                raise ValueError("weight blocks must have the same shape")
+
+    def to_c(self):
+        k = C.BlockKernel()
+        k.type, k.block_rows, k.block_cols = self.profile.kind, self.block_rows, self.block_cols
+        k.param = self.profile.param
+        for arr, w in zip((k.w0, k.w_row, k.w_col), self.w):
+            for e, v in enumerate(w.ravel()):
+                arr[e] = float(v)
+        return k
+
+
+class SparseBlockOperator:
+    """General sparse kernel operator: a radius per row (radius_function of
+    create_sparse_operator) and/or a block-valued kernel."""
+
+    def __init__(self, ctx, cols, row_positions, radius, kernel, row_radius=None,
+                 row_scalar=None, col_scalar=None):
+        _require_cuda(row_positions, torch.float64, "row_positions")
+        if row_radius is not None:
+            _require_cuda(row_radius, torch.float64, "row_radius")
+        if not isinstance(kernel, BlockKernel):
+            kernel = BlockKernel(kernel, [[1.0]])
+        self.ctx = ctx
+        self.kernel = kernel
+        self.nrows = row_positions.shape[0]
+        self.ncols = cols.n_alive
+        self._keep = (row_positions, row_radius, row_scalar, col_scalar)
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_sparse_block_op_create(
+            ctx.handle, cols.handle, self.nrows, _ptr(row_positions), float(radius),
+            _ptr(row_radius), ctypes.byref(kd), _ptr(row_scalar), _ptr(col_scalar),
+            ctypes.byref(h)), ctx.handle)
+        self.handle = h
+
+    @property
+    def pairs(self):
+        n = ctypes.c_int64()
+        check(C.lib().abh_sparse_block_op_pairs(self.handle, ctypes.byref(n)), self.ctx.handle)
+        return n.value
+
+    def apply(self, x, y):
+        """y += K x; x (ncols * block_cols,), y (nrows * block_rows,)"""
+        _require_cuda(x, torch.float64, "x")
+        _require_cuda(y, torch.float64, "y")
+        check(C.lib().abh_sparse_block_op_apply(self.handle, _ptr(x), _ptr(y)), self.ctx.handle)
+        return y
+
+    def __matmul__(self, x):
+        y = torch.zeros(self.nrows * self.kernel.block_rows, dtype=torch.float64,
+                        device=x.device)
+        return self.apply(x.contiguous(), y)
+
+    def assemble(self):
+        """CSR (row_ptr int64, col int32, val float64) of the scalar matrix."""
+        dev = self.ctx.device
+        ne = self.pairs * self.kernel.block_rows * self.kernel.block_cols
+        rp = torch.empty(self.nrows * self.kernel.block_rows + 1, dtype=torch.int64, device=dev)
+        col = torch.empty(max(ne, 1), dtype=torch.int32, device=dev)
+        val = torch.empty(max(ne, 1), dtype=torch.float64, device=dev)
+        check(C.lib().abh_sparse_block_op_assemble(self.handle, _ptr(rp), _ptr(col), _ptr(val),
+                                                   ne), self.ctx.handle)
+        return rp, col[:ne], val[:ne]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_sparse_block_op_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def create_sparse_operator(row_particles, col_particles, radius, kernel, row_scalar=None,
                            col_scalar=None):
     """create_sparse_operator(rows, cols, radius, f) -> SparseOperator."""
     rs = row_particles.columns[row_scalar] if isinstance(row_scalar, str) else row_scalar
     cs = col_particles.columns[col_scalar] if isinstance(col_scalar, str) else col_scalar
+    if isinstance(kernel, BlockKernel) or callable(radius) or isinstance(radius, torch.Tensor):
+        # a radius per row (a tensor, or a function of the row particle set)
+        # and/or a block kernel: the general operator
+        rr = radius(row_particles) if callable(radius) else radius
+        if isinstance(rr, torch.Tensor):
+            return SparseBlockOperator(col_particles.ctx, col_particles.get_query(),
+                                       row_particles.position, 0.0, kernel,
+                                       row_radius=rr.to(torch.float64).contiguous(),
+                                       row_scalar=rs, col_scalar=cs)
+        return SparseBlockOperator(col_particles.ctx, col_particles.get_query(),
+                                   row_particles.position, float(rr), kernel, row_scalar=rs,
+                                   col_scalar=cs)
     return SparseOperator(col_particles.ctx, col_particles.get_query(), row_particles.position,
                           radius, kernel, rs, cs)
 

@@ -65,6 +65,23 @@ class Kernel(ctypes.Structure):
     ]
 
 
+class BlockKernel(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("block_rows", ctypes.c_int32),
+                ("block_cols", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("param", ctypes.c_double), ("w0", ctypes.c_double * 16),
+                ("w_row", ctypes.c_double * 16), ("w_col", ctypes.c_double * 16)]
+
+
+class QueryView(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("periodic", ctypes.c_int32 * 3),
+                ("size", ctypes.c_int32 * 3), ("end_bucket", ctypes.c_int32 * 3),
+                ("low", ctypes.c_double * 3), ("high", ctypes.c_double * 3),
+                ("side", ctypes.c_double * 3), ("inv_side", ctypes.c_double * 3),
+                ("pos", ctypes.c_void_p), ("bucket_begin", ctypes.c_void_p),
+                ("bucket_end", ctypes.c_void_p), ("n", ctypes.c_int64),
+                ("n_buckets", ctypes.c_int64)]
+
+
 class FmmInfo(ctypes.Structure):
     _fields_ = [
         ("order", ctypes.c_int32),
@@ -165,6 +182,14 @@ SIGNATURES = {
     "abh_cells_bucket_pairs": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                        P(c_int64)]),
     "abh_cells_pair_counts": (c_int, [c_void_p, c_double, c_void_p]),
+    "abh_cells_query_view": (c_int, [c_void_p, P(QueryView)]),
+    "abh_sparse_block_op_create": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_double,
+                                           c_void_p, P(BlockKernel), c_void_p, c_void_p,
+                                           P(c_void_p)]),
+    "abh_sparse_block_op_destroy": (c_int, [c_void_p]),
+    "abh_sparse_block_op_pairs": (c_int, [c_void_p, P(c_int64)]),
+    "abh_sparse_block_op_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "abh_sparse_block_op_assemble": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64]),
     "abh_id_map_create": (c_int, [c_void_p, c_int64, c_void_p, P(c_void_p)]),
     "abh_id_map_update": (c_int, [c_void_p, c_int64, c_void_p]),
     "abh_id_map_destroy": (c_int, [c_void_p]),

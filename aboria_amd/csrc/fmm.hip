@@ -433,7 +433,11 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
                                                     double *__restrict__ G) {
   using F = Mfma64<N>;
   constexpr int NC = F::NC, RB = F::RB, KS = F::KS;
-  constexpr int RPW = (RB + 3) / 4;  // row blocks per wave
+  // row blocks in NP passes over the offsets (two for orders >= 7, so that
+  // the accumulators of 8 children fit the registers of a 512-thread block)
+  constexpr int NP = RB > 16 ? 2 : 1;
+  constexpr int RBP = (RB + NP - 1) / NP;  // row blocks per pass
+  constexpr int RPW = (RBP + 3) / 4;       // row blocks per wave and pass
   __shared__ int src[8];
   const int tile = xcd_block(ntiles);
   if (tile >= ntiles) return;
@@ -442,99 +446,104 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rq = wave & 3, c0 = 4 * (wave >> 2);
   const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
-  f64x4 acc[RPW][4];
-#pragma unroll
-  for (int i = 0; i < RPW; ++i)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[i][c] = f64x4{0.0, 0.0, 0.0, 0.0};
   const int kl = lane >> 4, rl = lane & 15;
-  for (int o = 0; o < kOffsets; ++o) {
-    const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
-    if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
-    __syncthreads();
-    if (threadIdx.x < 8) {
-      const int ch = threadIdx.x;
-      const int qx = ((ch >> 2) & 1) + ox, qy = ((ch >> 1) & 1) + oy, qz = (ch & 1) + oz;
-      int s = -1;
-      if (qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
-        const int sx = 2 * pc[0] + qx, sy = 2 * pc[1] + qy, sz = 2 * pc[2] + qz;
-        if (sx >= 0 && sx < side && sy >= 0 && sy < side && sz >= 0 && sz < side)
-          s = map[(int64_t(sx) * side + sy) * side + sz];
+  for (int pass = 0; pass < NP; ++pass) {
+    // row block of slot i in this pass (valid when rq + 4 i < RBP and < RB)
+    auto rb_of = [&](int i) { return pass * RBP + rq + 4 * i; };
+    auto rb_ok = [&](int i) { return rq + 4 * i < RBP && rb_of(i) < RB; };
+    f64x4 acc[RPW][4];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[i][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int o = 0; o < kOffsets; ++o) {
+      const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
+      if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+      __syncthreads();
+      if (threadIdx.x < 8) {
+        const int ch = threadIdx.x;
+        const int qx = ((ch >> 2) & 1) + ox, qy = ((ch >> 1) & 1) + oy, qz = (ch & 1) + oz;
+        int s = -1;
+        if (qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
+          const int sx = 2 * pc[0] + qx, sy = 2 * pc[1] + qy, sz = 2 * pc[2] + qz;
+          if (sx >= 0 && sx < side && sy >= 0 && sy < side && sz >= 0 && sz < side)
+            s = map[(int64_t(sx) * side + sy) * side + sz];
+        }
+        src[ch] = s;
       }
-      src[ch] = s;
-    }
-    __syncthreads();
-    int sc[4];
-    bool any = false;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      sc[c] = src[c0 + c];
-      any |= sc[c] >= 0;
-    }
-    if (!any) continue;
-    const double *Mo = Mf + int64_t(o) * F::PER_OFFSET;
-    // operand fragments are loaded three k steps ahead of the matrix cores
-    // (three register buffers).  Every load is unconditional, from a
-    // clamped, valid address (missing children read a present child's
-    // expansion, k steps past the end the last one), so that the loads stay
-    // in flight across the steps; the products of missing children and of
-    // row blocks past the end are not issued.
-    int svalid = -1;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) svalid = sc[c] >= 0 ? sc[c] : svalid;
-    int sl[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) sl[c] = sc[c] >= 0 ? sc[c] : svalid;
-    auto load = [&](int ks, double *av, double *bv) {
-      const int kk = ks < KS ? ks : KS - 1;
-#pragma unroll
-      for (int i = 0; i < RPW; ++i) {
-        const int rb = rq + 4 * i < RB ? rq + 4 * i : RB - 1;
-        av[i] = Mo[(int64_t(kk) * RB + rb) * 64 + lane];
-      }
-      const int k = kk * 4 + kl < NC ? kk * 4 + kl : NC - 1;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = W[(int64_t(sl[c]) * NC + k) * kRhs + rl];
-    };
-    auto step = [&](const double *av, const double *bv) {
+      __syncthreads();
+      int sc[4];
+      bool any = false;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        if (sc[c] < 0) continue;
-#pragma unroll
-        for (int i = 0; i < RPW; ++i)
-          if (rq + 4 * i < RB)
-            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[c], acc[i][c], 0, 0, 0);
+        sc[c] = src[c0 + c];
+        any |= sc[c] >= 0;
       }
-    };
-    // k steps past KS hold zero operands in no product: A rows are zero
-    // padded only up to KS, so the steps are guarded
-    double a0[RPW], b0[4], a1[RPW], b1[4], a2[RPW], b2[4];
-    load(0, a0, b0);
-    load(1, a1, b1);
-    load(2, a2, b2);
-    for (int ks = 0; ks < KS; ks += 3) {
-      step(a0, b0);
-      load(ks + 3, a0, b0);
-      if (ks + 1 < KS) step(a1, b1);
-      load(ks + 4, a1, b1);
-      if (ks + 2 < KS) step(a2, b2);
-      load(ks + 5, a2, b2);
-    }
-  }
+      if (!any) continue;
+      const double *Mo = Mf + int64_t(o) * F::PER_OFFSET;
+      // operand fragments are loaded three k steps ahead of the matrix cores
+      // (three register buffers).  Every load is unconditional, from a
+      // clamped, valid address (missing children read a present child's
+      // expansion, k steps past the end the last one), so that the loads stay
+      // in flight across the steps; the products of missing children and of
+      // row blocks past the end are not issued.
+      int svalid = -1;
 #pragma unroll
-  for (int i = 0; i < RPW; ++i) {
-    const int rb = rq + 4 * i;
-    if (rb >= RB) continue;
+      for (int c = 0; c < 4; ++c) svalid = sc[c] >= 0 ? sc[c] : svalid;
+      int sl[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int t = first + c0 + c;
+      for (int c = 0; c < 4; ++c) sl[c] = sc[c] >= 0 ? sc[c] : svalid;
+      auto load = [&](int ks, double *av, double *bv) {
+        const int kk = ks < KS ? ks : KS - 1;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = rb * 16 + kl + 4 * v;
-        if (row < NC) G[(int64_t(t) * NC + row) * kRhs + rl] += acc[i][c][v];
+        for (int i = 0; i < RPW; ++i) {
+          const int rb = rb_ok(i) ? rb_of(i) : RB - 1;
+          av[i] = Mo[(int64_t(kk) * RB + rb) * 64 + lane];
+        }
+        const int k = kk * 4 + kl < NC ? kk * 4 + kl : NC - 1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bv[c] = W[(int64_t(sl[c]) * NC + k) * kRhs + rl];
+      };
+      auto step = [&](const double *av, const double *bv) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (sc[c] < 0) continue;
+#pragma unroll
+          for (int i = 0; i < RPW; ++i)
+            if (rb_ok(i))
+              acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[c], acc[i][c], 0, 0, 0);
+        }
+      };
+      // k steps past KS hold zero operands in no product: A rows are zero
+      // padded only up to KS, so the steps are guarded
+      double a0[RPW], b0[4], a1[RPW], b1[4], a2[RPW], b2[4];
+      load(0, a0, b0);
+      load(1, a1, b1);
+      load(2, a2, b2);
+      for (int ks = 0; ks < KS; ks += 3) {
+        step(a0, b0);
+        load(ks + 3, a0, b0);
+        if (ks + 1 < KS) step(a1, b1);
+        load(ks + 4, a1, b1);
+        if (ks + 2 < KS) step(a2, b2);
+        load(ks + 5, a2, b2);
       }
     }
-  }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      if (!rb_ok(i)) continue;
+      const int rb = rb_of(i);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int t = first + c0 + c;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rb * 16 + kl + 4 * v;
+          if (row < NC) G[(int64_t(t) * NC + row) * kRhs + rl] += acc[i][c][v];
+        }
+      }
+    }
+  }  // pass
 }
 
 // ---------------------------------------------------------------------------
@@ -1334,7 +1343,9 @@ template <class F> void dispatch_order(int order, F &&fn) {
     case 4: fn(std::integral_constant<int, 4>()); break;
     case 5: fn(std::integral_constant<int, 5>()); break;
     case 6: fn(std::integral_constant<int, 6>()); break;
-    default: throw abh::Error{ABH_ERR_UNSUPPORTED, "Chebyshev order must be 3..6"};
+    case 7: fn(std::integral_constant<int, 7>()); break;
+    case 8: fn(std::integral_constant<int, 8>()); break;
+    default: throw abh::Error{ABH_ERR_UNSUPPORTED, "Chebyshev order must be 3..8"};
   }
 }
 

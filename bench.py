@@ -215,30 +215,48 @@ def bench_sparse(torch, ab, ctx, timer, st, extra):
 
 
 def bench_fmm(torch, ab, wl, ctx, timer, extra):
+    """Chebyshev FMM on N=2,000,000 uniform points, 16 right-hand sides, at the
+    configured order 6 and at orders 7 and 8, FP64 and MIXED: error against
+    the FP64 direct sum on 10,000 seeded targets and matvec/s (one matvec =
+    one right-hand side).  'at_1e-6' names the fastest form whose error is at
+    most 1e-6."""
     c = wl.FMM
     pos = torch.tensor(wl.fmm_positions(c["n"], c["seed"]), device=ctx.device)
     kern = ab.Kernel.smooth_inv_r(c["eps"])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    f = ab.FmmOperator(ctx, pos, [0] * 3, [1] * 3, c["n_leaf"], c["order"], kern)
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
     x = torch.tensor(wl.fmm_rhs(c["n"], c["nrhs"], c["seed"] + 1), device=ctx.device)
     rng = np.random.default_rng(c["seed"] + 2)
     idx = torch.tensor(np.sort(rng.choice(c["n"], c["n_check"], replace=False)),
                        device=ctx.device)
     ref = ab.dense_apply(ctx, pos[idx].contiguous(), pos, kern, x)  # FP64 direct sum
-    out = {"setup_s": setup_s, "info": f.info, "n_check": c["n_check"]}
-    for prec, name in ((ab.FP64, "fp64"), (ab.MIXED, "mixed")):
-        y = torch.zeros_like(x)
-        f.apply_colmajor(x, y, prec)
-        err = float(torch.linalg.norm(y[:, idx] - ref) / torch.linalg.norm(ref))
-        ms = timer(lambda: f.apply_colmajor(x, torch.zeros_like(x), prec))
-        out[name] = {"apply_ms": ms, "matvec_per_s": c["nrhs"] / (ms * 1e-3),
-                     "rel_err_vs_direct": err}
-        log("fmm %s: %d rhs in %.1f ms (%.1f matvec/s), rel err %.2e" % (
-            name, c["nrhs"], ms, c["nrhs"] / (ms * 1e-3), err))
-    f.close()
+    out = {"n_check": c["n_check"]}
+    best = None
+    for order in (c["order"], 7, 8):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = ab.FmmOperator(ctx, pos, [0] * 3, [1] * 3, c["n_leaf"], order, kern)
+        torch.cuda.synchronize()
+        o = {"setup_s": time.perf_counter() - t0, "info": f.info}
+        for prec, name in ((ab.FP64, "fp64"), (ab.MIXED, "mixed")):
+            y = torch.zeros_like(x)
+            f.apply_colmajor(x, y, prec)
+            err = float(torch.linalg.norm(y[:, idx] - ref) / torch.linalg.norm(ref))
+            del y
+            ms = timer(lambda: f.apply_colmajor(x, torch.zeros_like(x), prec),
+                       reps=3 if order <= 6 else 1)
+            mps = c["nrhs"] / (ms * 1e-3)
+            o[name] = {"apply_ms": ms, "matvec_per_s": mps, "rel_err_vs_direct": err}
+            log("fmm order %d %s: %d rhs in %.1f ms (%.1f matvec/s), rel err %.2e" % (
+                order, name, c["nrhs"], ms, mps, err))
+            if err <= 1e-6 and (best is None or mps > best["matvec_per_s"]):
+                best = {"order": order, "precision": name, "matvec_per_s": mps,
+                        "rel_err_vs_direct": err}
+        out["order%d" % order] = o
+        f.close()
+        torch.cuda.empty_cache()
+    # the configured order's figures at the top level as well
+    out.update({k: out["order%d" % c["order"]][k] for k in ("setup_s", "info", "fp64", "mixed")})
+    out["at_1e-6"] = best
+    log("fmm at 1e-6: %s" % best)
     extra["fmm"] = out
 
 

@@ -116,6 +116,29 @@ const uint32_t *abh_cells_bucket_begin(const abh_cells *cells); /* n_buckets*/
 const uint32_t *abh_cells_bucket_end(const abh_cells *cells);   /* n_buckets*/
 const double *abh_cells_positions(const abh_cells *cells); /* n_alive x dim */
 
+/* The query as a plain struct of device pointers and grid scalars, the
+ * counterpart of CellListOrderedQuery (src/CellListOrdered.h:284-364), which
+ * the reference copies by value into device lambdas (tests/parallel.h:
+ * 220-228).  include/aboria_hip_device.hpp walks it from HIP kernels with the
+ * reference's visit rule.  Valid until the next update/destroy of the cell
+ * list. */
+typedef struct abh_query_view {
+  int32_t dim;
+  int32_t periodic[3];
+  int32_t size[3];        /* buckets per dimension                         */
+  int32_t end_bucket[3];  /* size - 1                                      */
+  double low[3], high[3];
+  double side[3];         /* bucket side                                   */
+  double inv_side[3];     /* 1 / side, as the bucket index uses it         */
+  const double *pos;      /* n x dim, gather order                         */
+  const uint32_t *bucket_begin, *bucket_end;
+  int64_t n;              /* particles                                     */
+  int64_t n_buckets;
+} abh_query_view;
+
+/* Fills *view from the cell list's current state (no device work). */
+abh_status abh_cells_query_view(const abh_cells *cells, abh_query_view *view);
+
 /* get_neighbouring_buckets(query) (src/Search.h:857-860): the sequence of
  * (bucket i, bucket j, periodic image) triples bucket_pair_iterator yields
  * (src/Search.h:498-770), in its order -- every bucket but the last against
@@ -295,6 +318,54 @@ abh_status abh_sparse_op_apply(abh_sparse_op *op, int nrhs, const double *x,
 abh_status abh_sparse_op_cache(abh_sparse_op *op, abh_precision precision,
                                int enable);
 
+/* Matrix-valued kernel: the reference's functor may return an Eigen block
+ * of BlockRows x BlockCols entries (src/detail/Kernels.h:336-367, e.g. the
+ * 2-vector kernel of tests/operators.h:904-933).  Entry (ii, jj), e = ii *
+ * block_cols + jj, of the block of pair (i, j) is
+ *   phi(|dx|) * (w0[e] + w_row[e] * row_scalar[i] + w_col[e] * col_scalar[j])
+ * (a missing scalar array counts as zeros).  block_rows, block_cols in 1..4. */
+typedef struct abh_block_kernel {
+  int32_t type;         /* abh_kernel_type of phi                          */
+  int32_t block_rows;
+  int32_t block_cols;
+  int32_t reserved;
+  double param;
+  double w0[16], w_row[16], w_col[16];
+} abh_block_kernel;
+
+typedef struct abh_sparse_block_op abh_sparse_block_op;
+
+/* Replaces create_sparse_operator(rows, cols, radius_function, f) and
+ * (radius, f) with a block-valued f (src/Operators.h:472-516, KernelSparse
+ * src/Kernels.h:578-751): row i couples to the euclidean_search neighbours
+ * of row_pos[i] within row_radius[i] (device, nrows; NULL = radius for every
+ * row), each through the kernel's block.  The general form of
+ * abh_sparse_op: one thread per row walking the neighbours with the device
+ * query (include/aboria_hip_device.hpp).  row_pos, row_radius, row_scalar,
+ * col_scalar stay referenced (not copied) and must not change while the
+ * operator lives; the neighbour counts are fixed at creation (synchronises). */
+abh_status abh_sparse_block_op_create(abh_context *ctx, const abh_cells *cols,
+                                      int64_t nrows, const double *row_pos,
+                                      double radius, const double *row_radius,
+                                      const abh_block_kernel *kernel,
+                                      const double *row_scalar,
+                                      const double *col_scalar,
+                                      abh_sparse_block_op **out);
+abh_status abh_sparse_block_op_destroy(abh_sparse_block_op *op);
+/* neighbour pairs (the operator stores block_rows * block_cols entries each) */
+abh_status abh_sparse_block_op_pairs(const abh_sparse_block_op *op, int64_t *npairs);
+/* y += K x (KernelSparse::evaluate, src/Kernels.h:727-751): x device,
+ * ncols * block_cols; y device, nrows * block_rows (segment of particle i at
+ * i * block_rows). */
+abh_status abh_sparse_block_op_apply(abh_sparse_block_op *op, const double *x, double *y);
+/* KernelSparse::assemble (src/Kernels.h:629-685) as CSR of the scalar
+ * matrix (nrows * block_rows rows, row_ptr device of that + 1): block row
+ * i * block_rows + ii holds, per neighbour j in search order, the entries of
+ * columns j * block_cols .. + block_cols - 1.  col / val: device, capacity >=
+ * pairs * block_rows * block_cols. */
+abh_status abh_sparse_block_op_assemble(abh_sparse_block_op *op, int64_t *row_ptr,
+                                        int32_t *col, double *val, int64_t capacity);
+
 /* ------------------------------------------------------------------ */
 /* black-box Chebyshev FMM over a HyperOctree                          */
 /* ------------------------------------------------------------------ */
@@ -315,8 +386,9 @@ typedef struct abh_fmm_info {
  * (src/Operators.h:396-407, KernelFMM src/Kernels.h:533-576) for a particle
  * set searched by a HyperOctree (src/OctTree.h:102-475) built here with
  * n_particles_in_leaf.  dim must be 3; order (Chebyshev nodes per dimension)
- * in 3..6.  pos: device, n x 3.  Builds the tree, the interaction lists and
- * the M2L operators (synchronises). */
+ * in 3..8.  pos: device, n x 3.  Builds the tree, the interaction lists and
+ * the M2L operators (synchronises; the M2L operators take
+ * levels x 343 x order^6 x 8 bytes, 3.6 GB for order 8 and 5 levels). */
 abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low,
                           const double *high, int64_t n, const double *pos,
                           double n_particles_in_leaf, int order,
@@ -328,7 +400,9 @@ const int32_t *abh_fmm_order(const abh_fmm *fmm);
 /* y += K x (FastMultipoleMethod::matrix_vector_multiply,
  * src/FastMultipoleMethod.h:110-167) for nrhs right-hand sides; x, y are
  * indexed like the positions given at creation (column-major, leading
- * dimensions ldx / ldy).  precision: ABH_FP64 or ABH_MIXED. */
+ * dimensions ldx / ldy).  precision: ABH_FP64 or ABH_MIXED (for orders 7
+ * and 8 MIXED keeps the M2L in FP64 -- its BF16 split would floor the error
+ * near 1e-6 -- and runs P2P in compensated FP32). */
 abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
                          double *y, int64_t ldy, abh_precision precision);
 

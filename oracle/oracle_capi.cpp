@@ -312,6 +312,7 @@ int orc_fmm_matvec(void *p, int order, const Kernel *k, const double *q, double 
     case 4: fmm_n<4>(t, k, q, y); return 0;
     case 5: fmm_n<5>(t, k, q, y); return 0;
     case 6: fmm_n<6>(t, k, q, y); return 0;
+    case 7: fmm_n<7>(t, k, q, y); return 0;
     case 8: fmm_n<8>(t, k, q, y); return 0;
     default: return -1;
   }

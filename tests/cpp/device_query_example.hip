@@ -1,0 +1,27 @@
+// A kernel written outside the library against include/aboria_hip_device.hpp
+// (built with hipcc's default flags, FP contraction on): counts and sums the
+// neighbour indices of every centre from an abh_query_view passed by value,
+// the way the reference's device lambdas use a copied CellListOrderedQuery.
+#include "aboria_hip_device.hpp"
+
+__global__ void k_count(abh_query_view q, const double *centres, int64_t nq, double r,
+                        int64_t *counts, int64_t *index_sums) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  int64_t c = 0, s = 0;
+  abh::for_each_neighbour(q, centres + i * q.dim, r, [&](uint32_t j, const double *, double) {
+    ++c;
+    s += j;
+  });
+  counts[i] = c;
+  index_sums[i] = s;
+}
+
+extern "C" int example_count_neighbours(const abh_query_view *q, const double *centres,
+                                        int64_t nq, double r, int64_t *counts,
+                                        int64_t *index_sums) {
+  if (nq <= 0) return 0;
+  k_count<<<unsigned((nq + 255) / 256), 256>>>(*q, centres, nq, r, counts, index_sums);
+  if (hipGetLastError() != hipSuccess) return 1;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : 2;
+}

@@ -23,17 +23,19 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("order", [3, 4, 6])
+@pytest.mark.parametrize("order", [3, 4, 6, 7, 8])
 def test_fmm_matches_oracle(ctx, order):
     print("fmm oracle order", order, flush=True)
-    n = 3000
+    # orders 7 and 8: fewer, fuller leaves (three levels) keep the CPU
+    # oracle to seconds
+    n, leaf = (3000, 16) if order <= 6 else (2000, 40)
     rng = np.random.default_rng(40 + order)
     pos = rng.random((n, 3))
     q = rng.uniform(-1, 1, n)
-    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0] * 3, [1] * 3, 16, order,
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0] * 3, [1] * 3, leaf, order,
                        ab.Kernel.smooth_inv_r(1e-3))
     y = (f @ torch.tensor(q, device="cuda")).cpu().numpy()
-    t = oracle.Octree([0, 0, 0], [1, 1, 1], pos, 16)
+    t = oracle.Octree([0, 0, 0], [1, 1, 1], pos, leaf)
     assert np.array_equal(f.order().cpu().numpy(), t.order)
     y_o_sorted = t.fmm_matvec(order, (KI, 1e-3), q[t.order])
     y_o = np.empty(n)
@@ -78,3 +80,31 @@ def test_fmm_multi_rhs_and_mixed(ctx):
     idx = rng.choice(n, 500, replace=False)
     ref = oracle.dense_matvec(pos[idx], pos, (KI, 1e-3), x[0])
     assert rel(y64[0][idx], ref) < 1e-4
+
+
+@pytest.mark.parametrize("order", [6, 7, 8])
+def test_fmm_high_order_accuracy(ctx, order):
+    """Error against the FP64 direct sum on 1,000 targets of 40,000 uniform
+    points (the 1/(r + 1e-3) kernel, leaves of at most 64): it falls with the
+    order, and order 8 is below 1e-6 -- the accuracy the FMM throughput is
+    quoted at.  MIXED at orders above 6 keeps the M2L in FP64 (its BF16 split
+    would floor the error near 1e-6) and runs P2P in compensated FP32."""
+    n = 40_000
+    rng = np.random.default_rng(11)
+    pos = rng.random((n, 3))
+    x = rng.uniform(-1, 1, n)
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0] * 3, [1] * 3, 64, order,
+                       ab.Kernel.smooth_inv_r(1e-3))
+    xt = torch.tensor(x, device="cuda")
+    y = (f @ xt).cpu().numpy()
+    idx = rng.choice(n, 1000, replace=False)
+    ref = oracle.dense_matvec(pos[idx], pos, (KI, 1e-3), x)
+    err = rel(y[idx], ref)
+    ym = f.apply_colmajor(xt[None, :].contiguous(), torch.zeros((1, n), dtype=torch.float64,
+                                                               device="cuda"), ab.MIXED)
+    errm = rel(ym.cpu().numpy()[0][idx], ref)
+    print("order %d: fp64 err %.3e mixed err %.3e" % (order, err, errm), flush=True)
+    bound = {6: 2e-5, 7: 5e-6, 8: 1e-6}[order]
+    assert err < bound
+    if order > 6:
+        assert errm < bound
