@@ -179,6 +179,9 @@ SIGNATURES = {
                                       P(GmresStats)]),
     "abh_dense_apply": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int64, c_void_p, P(Kernel),
                                 c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
+    "abh_dense_apply_noalloc": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int64, c_void_p,
+                                        P(Kernel), c_int, c_void_p, c_int64, c_void_p, c_int64,
+                                        c_int]),
     "abh_cells_bucket_pairs": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
                                        P(c_int64)]),
     "abh_cells_pair_counts": (c_int, [c_void_p, c_double, c_void_p]),

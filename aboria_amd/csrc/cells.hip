@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "cells.hpp"
+#include "solvers.hpp"
 
 using abh::DevBuf;
 using abh::GridParams;
@@ -191,6 +192,8 @@ abh_status abh_context_create(int device, void *stream, abh_context **out) {
     hipDeviceProp_t prop;
     ABH_HIP(hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
+    // the device's copy of the allocation-free dense apply's scratch
+    ctx->dense_scratch = abh::dense_scratch_address();
   });
   if (st != ABH_OK) {
     delete ctx;

@@ -79,6 +79,9 @@ struct abh_context {
   abh::DevBuf sort_keys, sort_vals, sort_hist;  // radix sort ping-pong + histograms
   abh::DevBuf dense_part;                       // column-split partial sums (abh_dense_apply)
   int n_cu = 256;
+  // device address of the static scratch of abh_dense_apply_noalloc on this
+  // context's device (hipGetSymbolAddress at context creation)
+  double *dense_scratch = nullptr;
   // Handles created on this context keep it alive: abh_context_destroy only
   // marks it closed while any of them exists, the last one frees it.
   int refs = 0;

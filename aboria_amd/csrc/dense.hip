@@ -14,6 +14,7 @@
 #include "batch.hpp"
 #include "kernel_fn.hpp"
 #include "solvers.hpp"
+#include "solvers.hpp"
 
 using abh::DevBuf;
 using abh::KernelDesc;
@@ -219,6 +220,24 @@ __global__ __launch_bounds__(256) void k_rect_reduce(int64_t nrows, int nrhs, in
 
 }  // namespace
 
+// Static scratch of abh_dense_apply_noalloc (the column split's partial
+// sums), one copy per device for the life of the process.  Host code uses
+// the device address that hipGetSymbolAddress returns for it (kept in the
+// context), never the array's name.
+constexpr size_t kNoAllocScratchDoubles = size_t(4) << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace abh {
+
+double *dense_scratch_address() {
+  void *p = nullptr;
+  ABH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  ABH_REQUIRE(p != nullptr, ABH_ERR_HIP, "no device address for the dense apply scratch");
+  return static_cast<double *>(p);
+}
+
+}  // namespace abh
+
 struct abh_dense_batch {
   abh_context *ctx = nullptr;
   int dim = 2;
@@ -341,12 +360,16 @@ void dense_apply_impl(abh_context *ctx, int dim, int64_t nrows, const double *ro
   ns = ncols > 0 ? abh::ceil_div(ncols, cspan) : 1;
   double *part = nullptr;
   if (ns > 1) {
-    const size_t need = size_t(ns) * kRectRhs * size_t(nrows);
+    // part[s][r][i] for the ranges s, the right-hand sides r of one pass and
+    // the rows i; every entry is written by k_dense_rect before k_rect_reduce
+    // reads it, so the buffer needs no clearing
+    const size_t need = size_t(ns) * size_t(std::min(nrhs, kRectRhs)) * size_t(nrows);
     if (part_ext) {
       ABH_REQUIRE(need <= part_cap, ABH_ERR_CAPACITY,
                   "column split needs " + std::to_string(need * 8) +
                       " bytes of partial sums, above the " + std::to_string(part_cap * 8) +
-                      "-byte static scratch of the allocation-free entry point");
+                      "-byte static scratch of the allocation-free entry point (use "
+                      "abh_dense_apply, or fewer right-hand sides per call)");
       part = part_ext;
     } else {
       ctx->dense_part.ensure(need * sizeof(double));
@@ -528,6 +551,18 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows, const doubl
   return abh::guard(ctx, [&] {
     dense_apply_impl(ctx, dim, nrows, row_pos, ncols, col_pos, kernel, nrhs, x, ldx, y, ldy,
                      precision, nullptr, 0);
+  });
+}
+
+abh_status abh_dense_apply_noalloc(abh_context *ctx, int dim, int64_t nrows,
+                                   const double *row_pos, int64_t ncols, const double *col_pos,
+                                   const abh_kernel *kernel, int nrhs, const double *x,
+                                   int64_t ldx, double *y, int64_t ldy, abh_precision precision) {
+  if (!ctx || !kernel) return ABH_ERR_INVALID_ARG;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(ctx->dense_scratch, ABH_ERR_INVALID_ARG, "context has no dense scratch");
+    dense_apply_impl(ctx, dim, nrows, row_pos, ncols, col_pos, kernel, nrhs, x, ldx, y, ldy,
+                     precision, ctx->dense_scratch, kNoAllocScratchDoubles);
   });
 }
 

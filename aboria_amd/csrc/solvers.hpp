@@ -15,6 +15,10 @@ int64_t sparse_rows(const abh_sparse_op *op);
 int64_t sparse_cols(const abh_sparse_op *op);
 abh_context *sparse_context(const abh_sparse_op *op);
 
+// Device address, on the current device, of the static scratch of the
+// allocation-free dense apply (dense.hip); throws abh::Error.
+double *dense_scratch_address();
+
 // x = M^-1 b, restricted additive Schwarz (schwarz.hip).
 void schwarz_apply(const abh_schwarz *s, const double *b, double *x);
 int64_t schwarz_size(const abh_schwarz *s);

@@ -444,6 +444,24 @@ abh_status abh_dense_apply(abh_context *ctx, int dim, int64_t nrows,
                            int nrhs, const double *x, int64_t ldx, double *y,
                            int64_t ldy, abh_precision precision);
 
+/* abh_dense_apply without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as abh_dense_apply (bit for bit); the column split's
+ * partial sums live in a static 32 MiB device array (one per device, held
+ * until the process ends) instead of the context's grow-only buffer.  A
+ * shape whose split needs more (ranges x min(nrhs, 16) x nrows doubles
+ * above 4 Mi; on a 256-CU device only when nrows x min(nrhs, 16) exceeds
+ * about 2 million) returns ABH_ERR_CAPACITY before any launch.  Every call
+ * on one device shares the array: calls that overlap on one GPU (other
+ * streams, other contexts) race. */
+abh_status abh_dense_apply_noalloc(abh_context *ctx, int dim, int64_t nrows,
+                                   const double *row_pos, int64_t ncols,
+                                   const double *col_pos,
+                                   const abh_kernel *kernel, int nrhs,
+                                   const double *x, int64_t ldx, double *y,
+                                   int64_t ldy, abh_precision precision);
+
 /* Autotuner of the batched dense operator (no counterpart in the
  * reference): times every P2P tile shape (256 x {1, 2, 4} rows per
  * workgroup, each source staged in LDS serving 1, 2 or 4 rows per thread) on
