@@ -692,6 +692,18 @@ class FmmOperator:
     def order(self):
         return _view(C.lib().abh_fmm_order(self.handle), (self.n,), torch.int32, self.ctx.device)
 
+    def compress(self, eps):
+        """Low-rank M2L operators (KernelH2::compress(tol)): singular values
+        of each level's stacked operators below eps times the largest are
+        dropped; eps <= 0 restores the full operators.  Returns (padded rank,
+        [rank per M2L level])."""
+        levels = self.info["levels"]
+        kp = ctypes.c_int()
+        ranks = (ctypes.c_int * max(1, levels - 1))()
+        check(C.lib().abh_fmm_compress(self.handle, float(eps), ctypes.byref(kp), ranks),
+              self.ctx.handle)
+        return kp.value, [ranks[i] for i in range(max(0, levels - 1))] if kp.value else []
+
     def apply_colmajor(self, xt, yt, precision=FP64):
         """yt += (K x)^T for (nrhs, n) contiguous xt, yt."""
         check(C.lib().abh_fmm_apply(self.handle, xt.shape[0], _ptr(xt), xt.shape[1], _ptr(yt),

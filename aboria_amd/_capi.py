@@ -170,6 +170,7 @@ SIGNATURES = {
     "abh_fmm_destroy": (c_int, [c_void_p]),
     "abh_fmm_get_info": (c_int, [c_void_p, P(FmmInfo)]),
     "abh_fmm_order": (c_void_p, [c_void_p]),
+    "abh_fmm_compress": (c_int, [c_void_p, c_double, P(c_int), c_void_p]),
     "abh_fmm_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
     "abh_dense_batch_create": (c_int, [c_void_p, c_int, c_int, P(c_int64), c_void_p, P(Kernel),
                                        P(c_void_p)]),

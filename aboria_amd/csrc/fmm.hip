@@ -26,9 +26,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "kernel_fn.hpp"
+#include "solvers.hpp"
+#include "symeig.hpp"
 
 using abh::DevBuf;
 using abh::KernelDesc;
@@ -391,29 +394,93 @@ __global__ __launch_bounds__(256) void k_transfer(const int32_t *__restrict__ pa
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 
-template <int N> struct Mfma64 {
-  static constexpr int NC = Cheb<N>::NC;
+// fragment geometry of a DIM x DIM operator (DIM = NC, or the rank of the
+// compressed operators)
+template <int DIM> struct Mfma64 {
+  static constexpr int NC = DIM;
   static constexpr int RB = (NC + 15) / 16;  // 16-row blocks
   static constexpr int KS = (NC + 3) / 4;    // k steps of 4
   static constexpr int64_t PER_OFFSET = int64_t(KS) * RB * 64;
 };
 
-// M (FP64 [lev][343][NC][NC]) -> Mf [lev][343][KS][RB][64]: the A fragment
-// of (k step, row block) for every lane, zero padded
-template <int N>
-__global__ __launch_bounds__(256) void k_pack_m64(const double *__restrict__ M, int nlev,
-                                                  double *__restrict__ Mf) {
-  using F = Mfma64<N>;
-  constexpr int NC = F::NC;
-  const int64_t total = int64_t(nlev) * kOffsets * F::PER_OFFSET;
+// A-operand fragments of `count` R x K matrices: entry (i, k) of matrix m is
+// src[m * ms + i * rs + k * cs]; dst[m][KS][RB][64] holds, for k step ks and
+// row block rb, lane l's value (row rb * 16 + l % 16, column ks * 4 + l / 16),
+// zero outside R x K (KS = ceil(K / 4), RB = ceil(R / 16))
+__global__ __launch_bounds__(256) void k_pack_frag(const double *__restrict__ src, int64_t count,
+                                                   int R, int K, int64_t ms, int64_t rs,
+                                                   int64_t cs, double *__restrict__ dst) {
+  const int RB = (R + 15) / 16, KS = (K + 3) / 4;
+  const int64_t per = int64_t(KS) * RB * 64;
+  const int64_t total = count * per;
   for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
        t += int64_t(gridDim.x) * 256) {
     const int lane = int(t % 64);
-    const int rb = int((t / 64) % F::RB);
-    const int ks = int((t / (64 * F::RB)) % F::KS);
-    const int64_t lo = t / F::PER_OFFSET;  // level * 343 + offset
+    const int rb = int((t / 64) % RB);
+    const int ks = int((t / (64 * int64_t(RB))) % KS);
+    const int64_t m = t / per;
     const int i = rb * 16 + lane % 16, k = ks * 4 + lane / 16;
-    Mf[t] = (i < NC && k < NC) ? M[(lo * NC + i) * NC + k] : 0.0;
+    dst[t] = (i < R && k < K) ? src[m * ms + int64_t(i) * rs + int64_t(k) * cs] : 0.0;
+  }
+}
+
+// out[b] (+)= Op in[b] for the 8 children b of each tile parent, on the
+// matrix cores: Op is DOUT x DIN in A fragments (k_pack_frag layout), in /
+// out hold DIN / DOUT coefficients x 16 right-hand sides per box.  A
+// 512-thread block per tile; wave w takes the children 4 (w / 4) .. +3 and
+// row blocks w % 4 + 4 i of each pass of 16 row blocks (dimensions are
+// runtime: the FMM's compressed-basis transforms W^ = U^T W and G += U G^).
+__global__ __launch_bounds__(512) void k_box_op64(const int32_t *__restrict__ tile_parent,
+                                                  int ntiles, const int32_t *__restrict__ child0,
+                                                  int DOUT, int DIN, const double *__restrict__ Op,
+                                                  const double *__restrict__ in,
+                                                  double *__restrict__ out, int accumulate) {
+  constexpr int RPW = 4;  // row blocks per wave and pass (16 per pass)
+  const int tile = xcd_block(ntiles);
+  if (tile >= ntiles) return;
+  const int first = child0[tile_parent[tile]];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rq = wave & 3, c0 = 4 * (wave >> 2);
+  const int kl = lane >> 4, rl = lane & 15;
+  const int RB = (DOUT + 15) / 16, KS = (DIN + 3) / 4;
+  for (int p0 = 0; p0 < RB; p0 += 4 * RPW) {
+    f64x4 acc[RPW][4];
+#pragma unroll
+    for (int i = 0; i < RPW; ++i)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[i][c] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = ks * 4 + kl;
+      double bv[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        bv[c] = k < DIN ? in[(int64_t(first + c0 + c) * DIN + k) * kRhs + rl] : 0.0;
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const int rb = p0 + rq + 4 * i;
+        if (rb >= RB) continue;  // wave-uniform
+        const double av = Op[(int64_t(ks) * RB + rb) * 64 + lane];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[i][c], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int rb = p0 + rq + 4 * i;
+      if (rb >= RB) continue;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int64_t b = first + c0 + c;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = rb * 16 + kl + 4 * v;
+          if (row >= DOUT) continue;
+          double &o = out[(b * DOUT + row) * kRhs + rl];
+          o = accumulate ? o + acc[i][c][v] : acc[i][c][v];
+        }
+      }
+    }
   }
 }
 
@@ -423,7 +490,7 @@ __global__ __launch_bounds__(256) void k_pack_m64(const double *__restrict__ M, 
 // each active child's source expansion is the B operand and the offset's
 // packed M2L matrix the A operand, both read straight from L2 in the
 // fragment layout (coalesced 512-byte rows).
-template <int N>
+template <int DIM>
 __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ tile_parent,
                                                     int ntiles, const int32_t *__restrict__ child0,
                                                     const int32_t *__restrict__ coord,
@@ -431,7 +498,7 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
                                                     const double *__restrict__ Mf,
                                                     const double *__restrict__ W,
                                                     double *__restrict__ G) {
-  using F = Mfma64<N>;
+  using F = Mfma64<DIM>;
   constexpr int NC = F::NC, RB = F::RB, KS = F::KS;
   // row blocks in NP passes over the offsets (two for orders >= 7, so that
   // the accumulators of 8 children fit the registers of a 512-thread block)
@@ -972,6 +1039,13 @@ struct abh_fmm {
   bool split_ready = false;
   DevBuf Mf;              // FP64 M2L operators in the MFMA fragment layout
   bool pack_ready = false;
+  // compressed M2L (abh_fmm_compress): per level an orthonormal basis U
+  // (NC x comp_kp, the leading eigenvectors of the level's stacked
+  // operators, zero columns past the level's rank) and the operators
+  // C_o = U^T K_o U; W^ = U^T W, G^ the compressed local expansions
+  int comp_kp = 0;                 // padded rank (0: uncompressed)
+  std::vector<int> comp_rank;      // rank per M2L level (2..nlevels)
+  DevBuf Uf, UTf, Cf, Wc, Gc;
 };
 
 namespace {
@@ -1237,6 +1311,139 @@ template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_lea
   f->info.p2p_interactions = inter;
 }
 
+// doubles of the A fragments of an R x K operator (k_pack_frag layout)
+inline int64_t frag_size(int R, int K) { return int64_t((K + 3) / 4) * ((R + 15) / 16) * 64; }
+
+// padded ranks of the compressed M2L (each a k_m2l_mfma64 instantiation)
+constexpr int kCompRanks[] = {32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 448, 512};
+
+template <int KP>
+void m2l_compressed_kp(const int32_t *par, int npar, const int32_t *child0, const int32_t *coord,
+                       const int32_t *map, int side, const double *Cf, const double *Wc,
+                       double *Gc, hipStream_t s) {
+  k_m2l_mfma64<KP><<<xcd_grid(npar), 512, 0, s>>>(par, npar, child0, coord, map, side, Cf, Wc, Gc);
+}
+
+void m2l_compressed(int KP, const int32_t *par, int npar, const int32_t *child0,
+                    const int32_t *coord, const int32_t *map, int side, const double *Cf,
+                    const double *Wc, double *Gc, hipStream_t s) {
+  switch (KP) {
+#define ABH_KP(V)                                                                       \
+  case V:                                                                               \
+    m2l_compressed_kp<V>(par, npar, child0, coord, map, side, Cf, Wc, Gc, s);           \
+    break;
+    ABH_KP(32) ABH_KP(64) ABH_KP(96) ABH_KP(128) ABH_KP(160) ABH_KP(192) ABH_KP(224)
+    ABH_KP(256) ABH_KP(320) ABH_KP(384) ABH_KP(448) ABH_KP(512)
+#undef ABH_KP
+    default: throw abh::Error{ABH_ERR_UNSUPPORTED, "unsupported compressed rank"};
+  }
+}
+
+// Compression of the M2L operators of every level (the black-box FMM's SVD
+// compression, Fong & Darve 2009, in the form KernelH2::compress(tol) offers,
+// src/Kernels.h:523): the operators of one level satisfy K_{-o} = K_o^T (a
+// symmetric kernel, a symmetric offset set), so one orthonormal basis U --
+// the eigenvectors of the Gram matrix sum_o K_o K_o^T of the 316 far offsets
+// whose eigenvalues exceed eps^2 times the largest, i.e. the left singular
+// vectors of [K_o] with singular values above eps times the largest --
+// serves both sides: K_o ~ U (U^T K_o U) U^T.
+template <int N> void compress_impl(abh_fmm *f, double eps) {
+  constexpr int NC = Cheb<N>::NC;
+  abh_context *ctx = f->ctx;
+  hipStream_t s = ctx->stream;
+  f->comp_kp = 0;
+  f->comp_rank.clear();
+  if (eps <= 0.0) return;  // off
+  const int nlev = f->nlevels - 1;
+  if (nlev <= 0) return;
+  const double *M = f->M.as<double>();
+  const int64_t per_lev = int64_t(kOffsets) * NC * NC;
+  // Gram matrices (column-major, symmetric): C += X_o^T X_o with X_o the
+  // row-major K_o read as a column-major matrix (= K_o^T)
+  DevBuf gram;
+  gram.ensure(size_t(nlev) * NC * NC * 8);
+  ABH_HIP(hipMemsetAsync(gram.p, 0, size_t(nlev) * NC * NC * 8, s));
+  for (int o = 0; o < kOffsets; ++o) {
+    const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
+    if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+    abh::dgemm_strided_batched(s, true, false, NC, NC, NC, 1.0, M + int64_t(o) * NC * NC, NC,
+                               per_lev, M + int64_t(o) * NC * NC, NC, per_lev, 1.0,
+                               gram.as<double>(), NC, int64_t(NC) * NC, nlev, false);
+  }
+  std::vector<double> gh(size_t(nlev) * NC * NC);
+  ABH_HIP(hipMemcpyAsync(gh.data(), gram.p, gh.size() * 8, hipMemcpyDeviceToHost, s));
+  ABH_HIP(hipStreamSynchronize(s));
+  std::vector<std::vector<double>> w(nlev), v(nlev);
+  std::vector<int> ok(nlev, 0);
+  {
+    std::vector<std::thread> th;
+    for (int li = 0; li < nlev; ++li)
+      th.emplace_back([&, li] {
+        std::vector<double> a(gh.begin() + int64_t(li) * NC * NC,
+                              gh.begin() + int64_t(li + 1) * NC * NC);
+        ok[li] = abh::sym_eig(NC, a, w[li], v[li]) ? 1 : 0;
+      });
+    for (auto &t : th) t.join();
+  }
+  int kmax = 1;
+  for (int li = 0; li < nlev; ++li) {
+    ABH_REQUIRE(ok[li], ABH_ERR_NUMERICAL, "M2L compression: eigensolver did not converge");
+    int k = 0;
+    const double cut = eps * eps * w[li][0];
+    while (k < NC && w[li][k] > cut) ++k;
+    k = std::max(k, 1);
+    f->comp_rank.push_back(k);
+    kmax = std::max(kmax, k);
+  }
+  int KP = 0;
+  for (int c : kCompRanks)
+    if (c >= kmax) {
+      KP = c;
+      break;
+    }
+  ABH_REQUIRE(KP > 0, ABH_ERR_UNSUPPORTED, "M2L compression: rank above 512");
+  // U (NC x KP column-major per level): leading eigenvectors, zero columns
+  // past the level's rank
+  std::vector<double> uh(size_t(nlev) * NC * KP, 0.0);
+  for (int li = 0; li < nlev; ++li)
+    for (int c = 0; c < f->comp_rank[li]; ++c)
+      for (int a = 0; a < NC; ++a)
+        uh[(size_t(li) * KP + c) * NC + a] = v[li][size_t(a) * NC + c];
+  DevBuf U, T, Cm;
+  U.ensure(uh.size() * 8);
+  ABH_HIP(hipMemcpyAsync(U.p, uh.data(), uh.size() * 8, hipMemcpyHostToDevice, s));
+  T.ensure(size_t(kOffsets) * NC * KP * 8);
+  Cm.ensure(size_t(kOffsets) * KP * KP * 8);
+  f->Cf.ensure(size_t(nlev) * kOffsets * frag_size(KP, KP) * 8);
+  f->UTf.ensure(size_t(nlev) * frag_size(KP, NC) * 8);
+  f->Uf.ensure(size_t(nlev) * frag_size(NC, KP) * 8);
+  for (int li = 0; li < nlev; ++li) {
+    const double *Ul = U.as<double>() + int64_t(li) * NC * KP;
+    // T_o = K_o U (K_o = X_o^T), C_o = U^T T_o, both column-major
+    abh::dgemm_strided_batched(s, true, false, NC, KP, NC, 1.0, M + li * per_lev, NC,
+                               int64_t(NC) * NC, Ul, NC, 0, 0.0, T.as<double>(), NC,
+                               int64_t(NC) * KP, kOffsets, false);
+    abh::dgemm_strided_batched(s, true, false, KP, KP, NC, 1.0, Ul, NC, 0, T.as<double>(), NC,
+                               int64_t(NC) * KP, 0.0, Cm.as<double>(), KP, int64_t(KP) * KP,
+                               kOffsets, false);
+    k_pack_frag<<<8192, 256, 0, s>>>(Cm.as<double>(), kOffsets, KP, KP, int64_t(KP) * KP, 1, KP,
+                                     f->Cf.as<double>() + int64_t(li) * kOffsets *
+                                                              frag_size(KP, KP));
+    abh::check_launch();
+    // U^T: row c, column a = U[a + NC c];  U: row a, column c
+    k_pack_frag<<<1024, 256, 0, s>>>(Ul, 1, KP, NC, 0, NC, 1,
+                                     f->UTf.as<double>() + int64_t(li) * frag_size(KP, NC));
+    abh::check_launch();
+    k_pack_frag<<<1024, 256, 0, s>>>(Ul, 1, NC, KP, 0, 1, NC,
+                                     f->Uf.as<double>() + int64_t(li) * frag_size(NC, KP));
+    abh::check_launch();
+  }
+  f->Wc.ensure(size_t(f->n_nodes) * KP * kRhs * 8);
+  f->Gc.ensure(size_t(f->n_nodes) * KP * kRhs * 8);
+  ABH_HIP(hipStreamSynchronize(s));  // U, T, Cm, gram are freed below
+  f->comp_kp = KP;
+}
+
 template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
                                  int64_t ldy, abh_precision prec) {
   constexpr int NC = Cheb<N>::NC;
@@ -1269,7 +1476,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
       abh::check_launch();
     }
     const bool mixed = prec == ABH_MIXED;
-    if (mixed && NC <= kPad && f->nlevels >= 2) {
+    if (mixed && NC <= kPad && f->nlevels >= 2 && f->comp_kp == 0) {
       if (!f->split_ready) {
         const int nlev = f->nlevels - 1;
         f->Mh.ensure(size_t(nlev) * kOffsets * kBfPerOffset * 2);
@@ -1285,6 +1492,20 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                         f->Wl.as<uint16_t>());
       abh::check_launch();
     }
+    const int KP = f->comp_kp;
+    if (KP > 0) {
+      // compressed expansions of every level's boxes (tiles = the parents
+      // one level up), zeroed compressed locals
+      for (int l = 2; l <= f->nlevels; ++l) {
+        if (npar(l - 1) == 0) continue;
+        k_box_op64<<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
+            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), KP, NC,
+            f->UTf.as<double>() + int64_t(l - 2) * frag_size(KP, NC), f->W.as<double>(),
+            f->Wc.as<double>(), 0);
+        abh::check_launch();
+      }
+      ABH_HIP(hipMemsetAsync(f->Gc.p, 0, size_t(nn) * KP * kRhs * 8, s));
+    }
     // downward: level l receives L2L from level l-1 and its M2L
     for (int l = 2; l <= f->nlevels; ++l) {
       if (npar(l - 1) == 0) continue;
@@ -1292,7 +1513,18 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
                                                        f->child0.as<int32_t>(),
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
-      if (mixed && NC <= kPad) {
+      if (KP > 0) {
+        // C_o W^ into G^, then G += U G^
+        m2l_compressed(KP, par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(),
+                       f->coord.as<int32_t>(), f->maps.as<int32_t>() + f->map_off[l], 1 << l,
+                       f->Cf.as<double>() + int64_t(l - 2) * kOffsets * frag_size(KP, KP),
+                       f->Wc.as<double>(), f->Gc.as<double>(), s);
+        abh::check_launch();
+        k_box_op64<<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
+            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), NC, KP,
+            f->Uf.as<double>() + int64_t(l - 2) * frag_size(NC, KP), f->Gc.as<double>(),
+            f->G.as<double>(), 1);
+      } else if (mixed && NC <= kPad) {
         k_m2l_mfma<N><<<xcd_grid((npar(l - 1) + 1) / 2), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
@@ -1300,15 +1532,16 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
             f->Ml.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset, f->Wh.as<uint16_t>(),
             f->Wl.as<uint16_t>(), f->G.as<double>());
       } else {
-        using F = Mfma64<N>;
+        using F = Mfma64<NC>;
         if (!f->pack_ready) {
           const int nlev = f->nlevels - 1;
           f->Mf.ensure(size_t(nlev) * kOffsets * F::PER_OFFSET * 8);
-          k_pack_m64<N><<<8192, 256, 0, s>>>(f->M.as<double>(), nlev, f->Mf.as<double>());
+          k_pack_frag<<<8192, 256, 0, s>>>(f->M.as<double>(), int64_t(nlev) * kOffsets, NC, NC,
+                                           int64_t(NC) * NC, NC, 1, f->Mf.as<double>());
           abh::check_launch();
           f->pack_ready = true;
         }
-        k_m2l_mfma64<N><<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
+        k_m2l_mfma64<NC><<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mf.as<double>() + int64_t(l - 2) * kOffsets * F::PER_OFFSET, f->W.as<double>(),
@@ -1389,6 +1622,18 @@ abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low, const do
   abh::ctx_retain(ctx);
   *out = f;
   return ABH_OK;
+}
+
+abh_status abh_fmm_compress(abh_fmm *f, double eps, int *padded_rank, int *ranks) {
+  if (!f) return ABH_ERR_INVALID_ARG;
+  return abh::guard(f->ctx, [&] {
+    ABH_REQUIRE(std::isfinite(eps) && eps < 1.0, ABH_ERR_INVALID_ARG,
+                "compression tolerance must be finite and below 1");
+    dispatch_order(f->order, [&](auto NN) { compress_impl<decltype(NN)::value>(f, eps); });
+    if (padded_rank) *padded_rank = f->comp_kp;
+    if (ranks)
+      for (size_t i = 0; i < f->comp_rank.size(); ++i) ranks[i] = f->comp_rank[i];
+  });
 }
 
 abh_status abh_fmm_destroy(abh_fmm *f) {

@@ -216,10 +216,11 @@ def bench_sparse(torch, ab, ctx, timer, st, extra):
 
 def bench_fmm(torch, ab, wl, ctx, timer, extra):
     """Chebyshev FMM on N=2,000,000 uniform points, 16 right-hand sides, at the
-    configured order 6 and at orders 7 and 8, FP64 and MIXED: error against
-    the FP64 direct sum on 10,000 seeded targets and matvec/s (one matvec =
-    one right-hand side).  'at_1e-6' names the fastest form whose error is at
-    most 1e-6."""
+    configured order 6 and at orders 7 and 8, FP64 and MIXED, each also with
+    its M2L operators compressed (abh_fmm_compress, tolerances 1e-6 and
+    1e-7): error against the FP64 direct sum on 10,000 seeded targets and
+    matvec/s (one matvec = one right-hand side).  'at_1e-6' names the fastest
+    form whose error is at most 1e-6."""
     c = wl.FMM
     pos = torch.tensor(wl.fmm_positions(c["n"], c["seed"]), device=ctx.device)
     kern = ab.Kernel.smooth_inv_r(c["eps"])
@@ -236,20 +237,32 @@ def bench_fmm(torch, ab, wl, ctx, timer, extra):
         f = ab.FmmOperator(ctx, pos, [0] * 3, [1] * 3, c["n_leaf"], order, kern)
         torch.cuda.synchronize()
         o = {"setup_s": time.perf_counter() - t0, "info": f.info}
-        for prec, name in ((ab.FP64, "fp64"), (ab.MIXED, "mixed")):
-            y = torch.zeros_like(x)
-            f.apply_colmajor(x, y, prec)
-            err = float(torch.linalg.norm(y[:, idx] - ref) / torch.linalg.norm(ref))
-            del y
-            ms = timer(lambda: f.apply_colmajor(x, torch.zeros_like(x), prec),
-                       reps=3 if order <= 6 else 1)
-            mps = c["nrhs"] / (ms * 1e-3)
-            o[name] = {"apply_ms": ms, "matvec_per_s": mps, "rel_err_vs_direct": err}
-            log("fmm order %d %s: %d rhs in %.1f ms (%.1f matvec/s), rel err %.2e" % (
-                order, name, c["nrhs"], ms, mps, err))
-            if err <= 1e-6 and (best is None or mps > best["matvec_per_s"]):
-                best = {"order": order, "precision": name, "matvec_per_s": mps,
-                        "rel_err_vs_direct": err}
+        for tol in (None, 1e-6, 1e-7):
+            tag = ""
+            if tol is not None:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                kp, ranks = f.compress(tol)
+                torch.cuda.synchronize()
+                tag = "_compressed%.0e" % tol
+                o["compressed%.0e" % tol] = {"setup_s": time.perf_counter() - t0,
+                                             "padded_rank": kp, "ranks": ranks}
+            for prec, name in ((ab.FP64, "fp64"), (ab.MIXED, "mixed")):
+                if tol is None and order > 6 and prec == ab.MIXED:
+                    continue  # the same FP64 M2L at these orders
+                y = torch.zeros_like(x)
+                f.apply_colmajor(x, y, prec)
+                err = float(torch.linalg.norm(y[:, idx] - ref) / torch.linalg.norm(ref))
+                del y
+                ms = timer(lambda: f.apply_colmajor(x, torch.zeros_like(x), prec),
+                           reps=3 if (order <= 6 or tol is not None) else 1)
+                mps = c["nrhs"] / (ms * 1e-3)
+                o[name + tag] = {"apply_ms": ms, "matvec_per_s": mps, "rel_err_vs_direct": err}
+                log("fmm order %d %s%s: %d rhs in %.1f ms (%.1f matvec/s), rel err %.2e" % (
+                    order, name, tag, c["nrhs"], ms, mps, err))
+                if err <= 1e-6 and (best is None or mps > best["matvec_per_s"]):
+                    best = {"order": order, "precision": name + tag, "matvec_per_s": mps,
+                            "rel_err_vs_direct": err}
         out["order%d" % order] = o
         f.close()
         torch.cuda.empty_cache()

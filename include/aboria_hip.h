@@ -406,6 +406,19 @@ const int32_t *abh_fmm_order(const abh_fmm *fmm);
 abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
                          double *y, int64_t ldy, abh_precision precision);
 
+/* Replaces KernelH2::compress(tol) (src/Kernels.h:523, the H2 matrix's
+ * low-rank M2L blocks): per level, the M2L operators K_o become
+ * U (U^T K_o U) U^T with U the left singular vectors of the level's stacked
+ * far-offset operators whose singular values exceed eps times the largest
+ * (eps in (0, 1); eps <= 0 restores the uncompressed operators).  Later
+ * applies (either precision) contract the rank-k operators in FP64 on the
+ * matrix cores.  padded_rank (host, may be NULL): the rank the kernels run
+ * at (the largest level rank rounded up to 32, 64, ..., 256, 320, ...,
+ * 512); ranks (host, levels - 1 ints, may be NULL): the rank per M2L level
+ * 2..levels.  Synchronises. */
+abh_status abh_fmm_compress(abh_fmm *fmm, double eps, int *padded_rank,
+                            int *ranks);
+
 /* ------------------------------------------------------------------ */
 /* batched dense kernel operators and GMRES                            */
 /* ------------------------------------------------------------------ */

@@ -108,3 +108,61 @@ def test_fmm_high_order_accuracy(ctx, order):
     assert err < bound
     if order > 6:
         assert errm < bound
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_fmm_compressed_full_rank_equals_uncompressed(ctx, order):
+    """abh_fmm_compress with a tolerance at the rounding of the operators
+    (1e-15) keeps every singular vector at orders 3 and 4 (rank = order^3,
+    U U^T = I): the compressed apply (W^ = U^T W, C_o = U^T K_o U,
+    G += U G^) equals the uncompressed one to 1e-11 relative -- the layouts
+    and transposes of the three transforms are exact.  (At higher orders the
+    stacked operators' singular values fall below what the FP64 Gram matrix
+    resolves, about 1e-8 of the largest; the accuracy test below covers
+    them at working tolerances.)"""
+    print("fmm compressed full rank order", order, flush=True)
+    n, leaf = 6000, 16
+    rng = np.random.default_rng(60 + order)
+    pos = rng.random((n, 3))
+    x = rng.uniform(-1, 1, (3, n))
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0] * 3, [1] * 3, leaf, order,
+                       ab.Kernel.smooth_inv_r(1e-3))
+    xt = torch.tensor(x, device="cuda")
+    y0 = f.apply_colmajor(xt, torch.zeros_like(xt)).cpu().numpy()
+    kp, ranks = f.compress(1e-15)
+    print("padded rank", kp, "ranks", ranks, flush=True)
+    assert ranks and max(ranks) <= order ** 3 and kp >= max(ranks)
+    assert min(ranks) == order ** 3  # every direction the operators use
+    y1 = f.apply_colmajor(xt, torch.zeros_like(xt)).cpu().numpy()
+    assert rel(y1, y0) < 1e-11
+    kp0, _ = f.compress(0.0)
+    assert kp0 == 0
+    y2 = f.apply_colmajor(xt, torch.zeros_like(xt)).cpu().numpy()
+    assert np.array_equal(y2, y0)
+
+
+@pytest.mark.parametrize("order,eps,bound", [(6, 1e-7, 2e-5), (8, 1e-7, 1e-6)])
+def test_fmm_compressed_accuracy(ctx, order, eps, bound):
+    """The compressed operators at tolerance eps (ranks well below order^3)
+    against the FP64 direct sum on 1,000 targets of 40,000 uniform points:
+    within the uncompressed FMM's bound for the order, and within 20 eps
+    (relative, 2-norm) of the uncompressed apply."""
+    print("fmm compressed order %d eps %g" % (order, eps), flush=True)
+    n = 40_000
+    rng = np.random.default_rng(11)
+    pos = rng.random((n, 3))
+    x = rng.uniform(-1, 1, n)
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0] * 3, [1] * 3, 64, order,
+                       ab.Kernel.smooth_inv_r(1e-3))
+    xt = torch.tensor(x, device="cuda")
+    y = (f @ xt).cpu().numpy()
+    kp, ranks = f.compress(eps)
+    yc = (f @ xt).cpu().numpy()
+    idx = rng.choice(n, 1000, replace=False)
+    ref = oracle.dense_matvec(pos[idx], pos, (KI, 1e-3), x)
+    err, errc = rel(y[idx], ref), rel(yc[idx], ref)
+    print("order %d eps %g: ranks %s (padded %d), err %.3e compressed %.3e, vs uncompressed "
+          "%.3e" % (order, eps, ranks, kp, err, errc, rel(yc, y)), flush=True)
+    assert kp < order ** 3
+    assert errc < bound
+    assert rel(yc, y) < 20 * eps
