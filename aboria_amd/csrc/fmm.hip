@@ -853,9 +853,14 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
   constexpr int NC = Cheb<N>::NC;
   constexpr int SRC = 256;   // sources staged per round
   constexpr int ECH = 64;    // P2P list entries per chunk
-  // leaf local expansion, then the [parts][slots][kRhs] reduction of the
-  // P2P parts
-  __shared__ double Gs[NC * kRhs > 256 * kRhs ? NC * kRhs : 256 * kRhs];
+  // the leaf's local expansion is staged in LDS up to 256 nodes (order 6);
+  // larger ones (64 KB at order 8) would leave one block per CU, so the
+  // L2P reads them from the cache instead
+  constexpr bool kStageG = NC <= 256;
+  // leaf local expansion (staged orders), then the [parts][slots][kRhs]
+  // reduction of the P2P parts
+  static_assert(!kStageG || NC <= 256, "staged expansions fit the reduction buffer");
+  __shared__ double Gs[256 * kRhs];
   __shared__ double Ps[SRC][3];
   __shared__ __align__(16) double Qs[SRC][kRhs];
   __shared__ int ebeg[ECH], epre[ECH + 1];
@@ -883,7 +888,9 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
     if (act)
       for (int d = 0; d < 3; ++d) x[d] = pos[int64_t(p) * 3 + d];
     __syncthreads();
-    for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
+    if (kStageG)
+      for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
+    const double *Gl = kStageG ? Gs : G + int64_t(node) * NC * kRhs;
     __syncthreads();
     // L2P for right-hand sides rp * part .. rp * part + rp - 1
     double l2p[4] = {0.0, 0.0, 0.0, 0.0};
@@ -907,7 +914,7 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
             const int j = (jx * N + jy) * N + jz;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-              if (r < rp) l2p[r] += R * Gs[j * kRhs + rp * part + r];
+              if (r < rp) l2p[r] += R * Gl[j * kRhs + rp * part + r];
           }
         }
     }
@@ -1315,7 +1322,8 @@ template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_lea
 inline int64_t frag_size(int R, int K) { return int64_t((K + 3) / 4) * ((R + 15) / 16) * 64; }
 
 // padded ranks of the compressed M2L (each a k_m2l_mfma64 instantiation)
-constexpr int kCompRanks[] = {32, 64, 96, 128, 160, 192, 224, 256, 320, 384, 448, 512};
+constexpr int kCompRanks[] = {32,  48,  64,  80,  96,  112, 128, 144, 160, 176,
+                               192, 224, 256, 320, 384, 448, 512};
 
 template <int KP>
 void m2l_compressed_kp(const int32_t *par, int npar, const int32_t *child0, const int32_t *coord,
@@ -1332,8 +1340,9 @@ void m2l_compressed(int KP, const int32_t *par, int npar, const int32_t *child0,
   case V:                                                                               \
     m2l_compressed_kp<V>(par, npar, child0, coord, map, side, Cf, Wc, Gc, s);           \
     break;
-    ABH_KP(32) ABH_KP(64) ABH_KP(96) ABH_KP(128) ABH_KP(160) ABH_KP(192) ABH_KP(224)
-    ABH_KP(256) ABH_KP(320) ABH_KP(384) ABH_KP(448) ABH_KP(512)
+    ABH_KP(32) ABH_KP(48) ABH_KP(64) ABH_KP(80) ABH_KP(96) ABH_KP(112) ABH_KP(128)
+    ABH_KP(144) ABH_KP(160) ABH_KP(176) ABH_KP(192) ABH_KP(224) ABH_KP(256) ABH_KP(320)
+    ABH_KP(384) ABH_KP(448) ABH_KP(512)
 #undef ABH_KP
     default: throw abh::Error{ABH_ERR_UNSUPPORTED, "unsupported compressed rank"};
   }

@@ -413,8 +413,8 @@ abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
  * (eps in (0, 1); eps <= 0 restores the uncompressed operators).  Later
  * applies (either precision) contract the rank-k operators in FP64 on the
  * matrix cores.  padded_rank (host, may be NULL): the rank the kernels run
- * at (the largest level rank rounded up to 32, 64, ..., 256, 320, ...,
- * 512); ranks (host, levels - 1 ints, may be NULL): the rank per M2L level
+ * at (the largest level rank rounded up to a multiple of 16 up to 192, then
+ * to 224, 256, 320, 384, 448 or 512); ranks (host, levels - 1 ints, may be NULL): the rank per M2L level
  * 2..levels.  Synchronises. */
 abh_status abh_fmm_compress(abh_fmm *fmm, double eps, int *padded_rank,
                             int *ranks);
