@@ -853,10 +853,12 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
   constexpr int NC = Cheb<N>::NC;
   constexpr int SRC = 256;   // sources staged per round
   constexpr int ECH = 64;    // P2P list entries per chunk
-  // the leaf's local expansion is staged in LDS up to 256 nodes (order 6);
-  // larger ones (64 KB at order 8) would leave one block per CU, so the
-  // L2P reads them from the cache instead
+  // the leaf's local expansion is staged in LDS whole up to 256 nodes
+  // (order 6); larger ones (64 KB at order 8) would leave one block per CU,
+  // so they pass through the same 256-node buffer in slabs of JXC planes of
+  // constant first index
   constexpr bool kStageG = NC <= 256;
+  constexpr int JXC = kStageG ? N : 256 / (N * N);
   // leaf local expansion (staged orders), then the [parts][slots][kRhs]
   // reduction of the P2P parts
   static_assert(!kStageG || NC <= 256, "staged expansions fit the reduction buffer");
@@ -887,36 +889,50 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
     double x[3] = {0.0, 0.0, 0.0};
     if (act)
       for (int d = 0; d < 3; ++d) x[d] = pos[int64_t(p) * 3 + d];
-    __syncthreads();
-    if (kStageG)
-      for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
-    const double *Gl = kStageG ? Gs : G + int64_t(node) * NC * kRhs;
-    __syncthreads();
     // L2P for right-hand sides rp * part .. rp * part + rp - 1
     double l2p[4] = {0.0, 0.0, 0.0, 0.0};
+    double S[3][N], u0 = 0.0;
     if (act) {
-      double S[3][N];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const double span = bhi[d] - blo[d];
         const double u = span > 0.0 ? (2 * x[d] - blo[d] - bhi[d]) / span : 0.0;
+        if (d == 0) u0 = u;
 #pragma unroll
         for (int m = 0; m < N; ++m) S[d][m] = sn_dev<N>(c, u, m);
       }
-#pragma unroll
-      for (int jx = 0; jx < N; ++jx)
+    }
+    if (kStageG) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
+      __syncthreads();
+    }
+    // S[1], S[2] are indexed by the unrolled jy, jz only; the first
+    // dimension's value is recomputed per jx (one Clenshaw sum), so the jx
+    // loop need not unroll for S to stay in registers
+    for (int jx = 0; jx < N; ++jx) {
+      if (!kStageG && jx % JXC == 0) {
+        const int nj = (jx + JXC < N ? JXC : N - jx) * N * N;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nj * kRhs; t += 256)
+          Gs[t] = G[(int64_t(node) * NC + int64_t(jx) * N * N) * kRhs + t];
+        __syncthreads();
+      }
+      if (act) {
+        const double sx = sn_dev<N>(c, u0, jx);
 #pragma unroll
         for (int jy = 0; jy < N; ++jy) {
-          const double sxy = S[0][jx] * S[1][jy];
+          const double sxy = sx * S[1][jy];
 #pragma unroll
           for (int jz = 0; jz < N; ++jz) {
             const double R = sxy * S[2][jz];
-            const int j = (jx * N + jy) * N + jz;
+            const int j = ((jx % JXC) * N + jy) * N + jz;
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-              if (r < rp) l2p[r] += R * Gl[j * kRhs + rp * part + r];
+              if (r < rp) l2p[r] += R * Gs[j * kRhs + rp * part + r];
           }
         }
+      }
     }
     // P2P over the concatenated sources of the leaf's P2P list: rounds of
     // 256 staged sources, this thread taking those congruent to its part
