@@ -484,14 +484,27 @@ __global__ __launch_bounds__(512) void k_box_op64(const int32_t *__restrict__ ti
   }
 }
 
-// One 512-thread block per sibling tile (the 8 children of a level-(l-1)
-// node).  Wave w owns row blocks {w % 4, w % 4 + 4, ...} of the 8 x 16
-// output and the four children 4 (w / 4) .. 4 (w / 4) + 3; per box offset
+// Row groups of the M2L block for operand dimension DIM: 4, or 3 when that
+// divides the row blocks of a pass evenly and 4 does not (144 rows: 9
+// blocks), so that every wave issues the same number of products.
+// Measured on the order-8 C3 apply: 3 groups at 144 rows 317 -> 307 ms;
+// 5 groups at 160 rows (10-wave blocks) 343 -> 372 ms, so 4 stay there;
+// 7 groups for 14 blocks would cap the 14-wave block at 128 registers and
+// spill.
+constexpr int m2l_row_groups(int dim) {
+  const int rb = (dim + 15) / 16;
+  const int rbp = rb > 16 ? (rb + 1) / 2 : rb;
+  return rbp % 4 == 0 ? 4 : (rbp % 3 == 0 ? 3 : 4);
+}
+
+// One block of 2 x RG wavefronts per sibling tile (the 8 children of a
+// level-(l-1) node).  Wave w owns row blocks {w % RG, w % RG + RG, ...} of
+// the 8 x 16 output and the four children 4 (w / RG) .. + 3; per box offset
 // each active child's source expansion is the B operand and the offset's
 // packed M2L matrix the A operand, both read straight from L2 in the
 // fragment layout (coalesced 512-byte rows).
 template <int DIM>
-__global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ tile_parent,
+__global__ __launch_bounds__(128 * m2l_row_groups(DIM)) void k_m2l_mfma64(const int32_t *__restrict__ tile_parent,
                                                     int ntiles, const int32_t *__restrict__ child0,
                                                     const int32_t *__restrict__ coord,
                                                     const int32_t *__restrict__ map, int side,
@@ -504,20 +517,21 @@ __global__ __launch_bounds__(512) void k_m2l_mfma64(const int32_t *__restrict__ 
   // the accumulators of 8 children fit the registers of a 512-thread block)
   constexpr int NP = RB > 16 ? 2 : 1;
   constexpr int RBP = (RB + NP - 1) / NP;  // row blocks per pass
-  constexpr int RPW = (RBP + 3) / 4;       // row blocks per wave and pass
+  constexpr int RG = m2l_row_groups(DIM);  // row groups (waves per child group)
+  constexpr int RPW = (RBP + RG - 1) / RG; // row blocks per wave and pass
   __shared__ int src[8];
   const int tile = xcd_block(ntiles);
   if (tile >= ntiles) return;
   const int parent = tile_parent[tile];
   const int first = child0[parent];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rq = wave & 3, c0 = 4 * (wave >> 2);
+  const int rq = wave % RG, c0 = 4 * (wave / RG);
   const int pc[3] = {coord[parent * 3 + 0], coord[parent * 3 + 1], coord[parent * 3 + 2]};
   const int kl = lane >> 4, rl = lane & 15;
   for (int pass = 0; pass < NP; ++pass) {
-    // row block of slot i in this pass (valid when rq + 4 i < RBP and < RB)
-    auto rb_of = [&](int i) { return pass * RBP + rq + 4 * i; };
-    auto rb_ok = [&](int i) { return rq + 4 * i < RBP && rb_of(i) < RB; };
+    // row block of slot i in this pass (valid when rq + RG i < RBP and < RB)
+    auto rb_of = [&](int i) { return pass * RBP + rq + RG * i; };
+    auto rb_ok = [&](int i) { return rq + RG * i < RBP && rb_of(i) < RB; };
     f64x4 acc[RPW][4];
 #pragma unroll
     for (int i = 0; i < RPW; ++i)
@@ -1345,7 +1359,8 @@ template <int KP>
 void m2l_compressed_kp(const int32_t *par, int npar, const int32_t *child0, const int32_t *coord,
                        const int32_t *map, int side, const double *Cf, const double *Wc,
                        double *Gc, hipStream_t s) {
-  k_m2l_mfma64<KP><<<xcd_grid(npar), 512, 0, s>>>(par, npar, child0, coord, map, side, Cf, Wc, Gc);
+  k_m2l_mfma64<KP><<<xcd_grid(npar), 128 * m2l_row_groups(KP), 0, s>>>(par, npar, child0, coord,
+                                                                       map, side, Cf, Wc, Gc);
 }
 
 void m2l_compressed(int KP, const int32_t *par, int npar, const int32_t *child0,
@@ -1566,7 +1581,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
           abh::check_launch();
           f->pack_ready = true;
         }
-        k_m2l_mfma64<NC><<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
+        k_m2l_mfma64<NC><<<xcd_grid(npar(l - 1)), 128 * m2l_row_groups(NC), 0, s>>>(
             par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mf.as<double>() + int64_t(l - 2) * kOffsets * F::PER_OFFSET, f->W.as<double>(),
