@@ -57,6 +57,10 @@ def parse():
     p.add_argument("--fused", action="store_true",
                    help="time the single-traversal search (abh_search_run_fused) as the step "
                         "instead of count pass + fill pass")
+    p.add_argument("--fmm-orders", default="6,7,8",
+                   help="Chebyshev orders of the fmm section (the first is reported at the top)")
+    p.add_argument("--fmm-tols", default="none,1e-6,2e-7,1e-7",
+                   help="M2L compression tolerances of the fmm section ('none': uncompressed)")
     p.add_argument("--rbf-iters", type=int, default=100,
                    help="GMRES iteration budget of the batched RBF systems")
     return p.parse_args()
@@ -214,7 +218,7 @@ def bench_sparse(torch, ab, ctx, timer, st, extra):
     extra["sparse"] = out
 
 
-def bench_fmm(torch, ab, wl, ctx, timer, extra):
+def bench_fmm(args, torch, ab, wl, ctx, timer, extra):
     """Chebyshev FMM on N=2,000,000 uniform points, 16 right-hand sides, at the
     configured order 6 and at orders 7 and 8, FP64 and MIXED, each also with
     its M2L operators compressed (abh_fmm_compress, tolerances 1e-6, 2e-7
@@ -231,13 +235,15 @@ def bench_fmm(torch, ab, wl, ctx, timer, extra):
     ref = ab.dense_apply(ctx, pos[idx].contiguous(), pos, kern, x)  # FP64 direct sum
     out = {"n_check": c["n_check"]}
     best = None
-    for order in (c["order"], 7, 8):
+    orders = [int(v) for v in args.fmm_orders.split(",") if v]
+    tols = [None if v == "none" else float(v) for v in args.fmm_tols.split(",") if v]
+    for order in orders:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         f = ab.FmmOperator(ctx, pos, [0] * 3, [1] * 3, c["n_leaf"], order, kern)
         torch.cuda.synchronize()
         o = {"setup_s": time.perf_counter() - t0, "info": f.info}
-        for tol in (None, 1e-6, 2e-7, 1e-7):
+        for tol in tols:
             tag = ""
             if tol is not None:
                 torch.cuda.synchronize()
@@ -266,8 +272,9 @@ def bench_fmm(torch, ab, wl, ctx, timer, extra):
         out["order%d" % order] = o
         f.close()
         torch.cuda.empty_cache()
-    # the configured order's figures at the top level as well
-    out.update({k: out["order%d" % c["order"]][k] for k in ("setup_s", "info", "fp64", "mixed")})
+    # the first order's figures at the top level as well
+    first = out["order%d" % orders[0]]
+    out.update({k: first[k] for k in ("setup_s", "info", "fp64", "mixed") if k in first})
     out["at_1e-6"] = best
     log("fmm at 1e-6: %s" % best)
     extra["fmm"] = out
@@ -598,7 +605,7 @@ def main():
         st.clear()
     torch.cuda.empty_cache()
     if "fmm" in sections:
-        bench_fmm(torch, ab, wl, ctx, timer, extra)
+        bench_fmm(args, torch, ab, wl, ctx, timer, extra)
         torch.cuda.empty_cache()
     if "rbf" in sections:
         bench_rbf(args, torch, ab, wl, ctx, timer, extra)
