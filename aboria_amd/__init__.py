@@ -277,6 +277,25 @@ class Particles:
             raise RuntimeError("init_neighbour_search not called on this particle set")
         return self.search
 
+    def to_host(self):
+        """position, id, alive and the user columns as numpy arrays (the
+        input of aboria_amd.io's writers)."""
+        return {"position": self.position.cpu().numpy(), "id": self.id.cpu().numpy(),
+                "alive": self.alive.cpu().numpy(),
+                "columns": {k: v.cpu().numpy() for k, v in self.columns.items()}}
+
+    @classmethod
+    def from_host(cls, ctx, position, id=None, alive=None, columns=None):
+        """A particle set on the context's device from host arrays (what
+        aboria_amd.io's readers return); ids are kept when given."""
+        p = cls(ctx, torch.as_tensor(np.ascontiguousarray(position), dtype=torch.float64),
+                **{k: torch.as_tensor(np.ascontiguousarray(v)) for k, v in (columns or {}).items()})
+        if id is not None:
+            p.id = torch.as_tensor(np.asarray(id, dtype=np.int64), device=ctx.device)
+        if alive is not None:
+            p.alive = torch.as_tensor(np.asarray(alive, dtype=np.uint8), device=ctx.device)
+        return p
+
 
 class IdMap:
     """Find-by-id map: the (id, index) pairs of a particle set sorted by id
