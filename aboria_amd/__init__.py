@@ -8,6 +8,7 @@ streams are torch CUDA tensors; all compute goes through libaboria_hip.so.
 """
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _capi as C
@@ -858,6 +859,15 @@ class SchwartzPreconditioner:
                                          float(radius), float(diag_shift), ctypes.byref(h)),
               ctx.handle)
         self.handle = h
+
+    def factorize(self, kernel, radius, diag_shift=0.0):
+        """factorize() alone on the analysed domains: new kernel values
+        (kernel, radius, diagonal shift) for the same particles."""
+        if self.cells is None:
+            raise ValueError("refactorisation is for preconditioners built on a cell list")
+        kd = kernel.to_c()
+        check(C.lib().abh_schwarz_factorize(self.handle, self.cells.handle, ctypes.byref(kd),
+                                            float(radius), float(diag_shift)), self.ctx.handle)
 
     @classmethod
     def for_dense_batch(cls, op, low, high, n_particles_in_leaf, diag_shift=0.0):

@@ -204,6 +204,7 @@ SIGNATURES = {
     "abh_dense_batch_tune": (c_int, [c_void_p, c_int, P(c_int), P(ctypes.c_float)]),
     "abh_schwarz_create": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double,
                                    P(c_void_p)]),
+    "abh_schwarz_factorize": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double]),
     "abh_schwarz_destroy": (c_int, [c_void_p]),
     "abh_schwarz_get_info": (c_int, [c_void_p, P(SchwarzInfo)]),
     "abh_schwarz_get_domains": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -268,6 +268,7 @@ struct abh_schwarz {
   DevBuf d_idx, d_doms, d_chunks, R;
   int nchunks = 0;
   size_t factor_bytes = 0;
+  bool batch = false;  // built per problem of a dense batch (signed factors)
 };
 
 namespace {
@@ -549,6 +550,7 @@ abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<i
   analyse_batch_into(s.get(), dim, off, pos, lo, hi, n_leaf);
   for (size_t d = 0; d + 1 < s->off.size(); ++d)
     s->max_m = std::max(s->max_m, int(s->off[d + 1] - s->off[d]));
+  s->batch = true;
   factorise(s.get(), pos_dev, k, -1.0, shift, true);
   return s.release();
 }
@@ -603,6 +605,28 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
   abh::ctx_retain(ctx);
   *out = s;
   return ABH_OK;
+}
+
+abh_status abh_schwarz_factorize(abh_schwarz *s, const abh_cells *cells,
+                                 const abh_kernel *kernel, double radius, double diag_shift) {
+  if (!s || !cells || !kernel) return ABH_ERR_INVALID_ARG;
+  return abh::guard(s->ctx, [&] {
+    ABH_REQUIRE(!s->batch, ABH_ERR_UNSUPPORTED,
+                "refactorisation is for preconditioners built on a cell list");
+    ABH_REQUIRE(cells->ctx == s->ctx && cells->dim == s->dim && cells->n_alive == s->n,
+                ABH_ERR_INVALID_ARG, "cell list does not match the analysed pattern");
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    KernelDesc k{};
+    k.type = kernel->type;
+    k.param = kernel->param;
+    k.param2 = kernel->param * kernel->param;
+    abh::set_inverses(k);
+    k.w0 = 1.0;
+    const double r2 = radius > 0.0 ? radius * radius : -1.0;
+    s->failed = 0;
+    factorise(s, cells->pos.as<double>(), k, r2, diag_shift, false);
+  });
 }
 
 abh_status abh_schwarz_destroy(abh_schwarz *s) {

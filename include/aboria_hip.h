@@ -551,6 +551,16 @@ typedef struct abh_schwarz_info {
 abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells,
                               const abh_kernel *kernel, double radius,
                               double diag_shift, abh_schwarz **out);
+/* Replaces SchwartzPreconditioner::factorize alone (src/Preconditioners.h:
+ * 1402-1467) on the pattern analyzePattern left: the domains of `s` are
+ * kept and their local matrices rebuilt and factorised for a new kernel,
+ * radius or diagonal shift (the reference's compute() after a change of the
+ * operator's values).  cells must hold the particles `s` was analysed on
+ * (same count, gather order); preconditioners of a dense batch return
+ * ABH_ERR_UNSUPPORTED.  Synchronises. */
+abh_status abh_schwarz_factorize(abh_schwarz *s, const abh_cells *cells,
+                                 const abh_kernel *kernel, double radius,
+                                 double diag_shift);
 abh_status abh_schwarz_destroy(abh_schwarz *s);
 abh_status abh_schwarz_get_info(const abh_schwarz *s, abh_schwarz_info *info);
 /* Domain lists as host arrays (any pointer may be NULL): offsets has
