@@ -19,7 +19,10 @@ workload):
               seeded targets
   rbf_batch   512 uneven 2D multiquadric (c=0.05) RBF systems, batched
               GMRES(50) on a bounded iteration budget, FP64 and FP32
-              compensated operator (FP64 residual / orthogonalisation)
+              compensated operator (FP64 residual / orthogonalisation);
+              Schwarz-preconditioned FP64 GMRES over the whole batch, and the
+              time to 1e-8 (set-up + solve) of the problems with N < 2,000,
+              the ones whose FP64 residual can reach 1e-8
   schwarz     2D jittered grid N=400,056, Gaussian sigma=0.01 (+1e-5 I) and a
               narrow variant sigma=0.002, GMRES to 1e-8 on a 300-iteration
               budget: plain Schwarz vs the learned approximate inverse (with
