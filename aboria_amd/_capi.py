@@ -138,6 +138,7 @@ SIGNATURES = {
     "abh_free": (c_int, [c_void_p, c_void_p]),
     "abh_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "abh_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
+    "abh_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t]),
     "abh_memset": (c_int, [c_void_p, c_void_p, c_int, c_size_t]),
     "abh_cells_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), P(ctypes.c_uint8),
                                  c_double, P(c_void_p)]),

@@ -256,6 +256,12 @@ abh_status abh_memcpy_d2h(abh_context *ctx, void *dst, const void *src, size_t b
     ABH_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
+abh_status abh_memcpy_d2d(abh_context *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return ABH_ERR_INVALID_ARG;
+  return abh::guard(ctx, [&] {
+    if (bytes) ABH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
 abh_status abh_memset(abh_context *ctx, void *dst, int value, size_t bytes) {
   if (!ctx) return ABH_ERR_INVALID_ARG;
   return abh::guard(ctx, [&] { ABH_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream)); });

@@ -13,6 +13,9 @@ Also reported under "extra" (each outside the timed steps, each its own
 workload):
   sparse      the Wendland C2 sparse operator of the same cell list on 8
               right-hand sides, FP64 and FP32-compensated
+  c1          the reference's CPU-runnable case: 100,000 uniform points,
+              radius 0.04, build + search and a Gaussian sparse matvec,
+              beside the CPU oracle
   fmm         black-box Chebyshev FMM, N=2,000,000 uniform, 1/(r+1e-3),
               leaves <= 64, order 6, 16 RHS: matvec/s in FP64 and mixed
               precision, relative error against an FP64 direct sum on 10,000
@@ -43,8 +46,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SECTIONS = ("search", "sparse", "fmm", "rbf", "schwarz", "copy")
-DEFAULT_SECTIONS = ("search", "sparse", "fmm", "rbf", "schwarz")
+SECTIONS = ("search", "sparse", "c1", "fmm", "rbf", "schwarz", "copy")
+DEFAULT_SECTIONS = ("search", "sparse", "c1", "fmm", "rbf", "schwarz")
 
 
 def parse():
@@ -219,6 +222,74 @@ def bench_sparse(torch, ab, ctx, timer, st, extra):
                              out["cached_matvec8_fp32c_ms"], out["cache_fp32c_setup_ms"]))
     K.close()
     extra["sparse"] = out
+
+
+def bench_c1(args, torch, ab, wl, ctx, stream, extra):
+    """The reference's CPU-runnable case: 100,000 uniform points in the unit
+    cube (non-periodic), cell side = radius = 0.04 (25^3 buckets), a radius
+    search for every particle and one Gaussian sparse matvec
+    exp(-|dx|^2 / 0.02^2) on b ~ U[0,1); GPU step = build + count + fill,
+    and the operator apply, beside the CPU oracle's build + search + matvec
+    (all host threads)."""
+    c = wl.UNIFORM
+    pos_np = wl.uniform_positions(c["n"], seed=c["seed"])
+    b_np = wl.uniform_rhs(c["n"], seed=c["seed"] + 1)[:, 0].copy()
+    pos = torch.tensor(pos_np, device=ctx.device)
+    cells = ab.CellListOrdered(ctx, c["low"], c["high"], c["periodic"], c["n_leaf"])
+    cells.update_positions(pos)
+    search = ab.Search(cells, cells.positions(), c["radius"])
+    col = torch.empty(search.total + 1024, dtype=torch.int32, device=ctx.device)
+    kern = ab.Kernel.gaussian(c["gauss_scale"])
+    op = ab.SparseOperator(ctx, cells, cells.positions(), c["radius"], kern)
+    b = torch.tensor(b_np, device=ctx.device)
+
+    def step():
+        cells.update_positions(pos)
+        search.run(cells, cells.positions(), c["radius"])
+        search.fill(col)
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / reps
+    y = torch.zeros_like(b)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    op.apply(b, y)
+    ev[0].record(stream)
+    for _ in range(reps):
+        op.apply(b, y)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    mv_ms = ev[0].elapsed_time(ev[1]) / reps
+    out = {"n": c["n"], "pairs": int(search.total), "grid": list(cells.grid),
+           "build_search_ms": step_ms, "pairs_per_s": search.total / (step_ms * 1e-3),
+           "matvec_ms": mv_ms}
+    log("c1: build + search %.3f ms (%d pairs, %.3g pairs/s), Gaussian matvec %.3f ms" % (
+        step_ms, search.total, out["pairs_per_s"], mv_ms))
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oc = oracle.Cells(c["low"], c["high"], c["periodic"], c["n_leaf"])
+        oc.update(pos_np.copy())
+        rp, _ = oc.search(oc.positions, c["radius"])
+        t1 = time.perf_counter()
+        oc.sparse_matvec(oc.positions, c["radius"], (1, c["gauss_scale"]), b_np[oc.order])
+        t2 = time.perf_counter()
+        out["cpu"] = {"build_search_ms": (t1 - t0) * 1e3, "matvec_ms": (t2 - t1) * 1e3,
+                      "cores": oracle.nthreads_default(), "kind": "port",
+                      "sample": "oracle build (serial) + search + Gaussian matvec of all "
+                                "100,000 particles"}
+        log("c1 cpu: build + search %.1f ms, matvec %.1f ms" % ((t1 - t0) * 1e3,
+                                                               (t2 - t1) * 1e3))
+    op.close()
+    search.close()
+    cells.close()
+    extra["c1"] = out
 
 
 def bench_fmm(args, torch, ab, wl, ctx, timer, extra):
@@ -636,6 +707,8 @@ def main():
         st["cells"].close()
         st.clear()
     torch.cuda.empty_cache()
+    if "c1" in sections:
+        bench_c1(args, torch, ab, wl, ctx, stream, extra)
     if "fmm" in sections:
         bench_fmm(args, torch, ab, wl, ctx, timer, extra)
         torch.cuda.empty_cache()

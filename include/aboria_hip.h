@@ -65,6 +65,9 @@ abh_status abh_memcpy_h2d(abh_context *ctx, void *dst_dev, const void *src_host,
                           size_t bytes);
 abh_status abh_memcpy_d2h(abh_context *ctx, void *dst_host, const void *src_dev,
                           size_t bytes);
+/* Device-to-device copy, asynchronous on the context stream. */
+abh_status abh_memcpy_d2d(abh_context *ctx, void *dst_dev, const void *src_dev,
+                          size_t bytes);
 abh_status abh_memset(abh_context *ctx, void *dst_dev, int value, size_t bytes);
 
 /* ------------------------------------------------------------------ */

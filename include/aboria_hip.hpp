@@ -162,9 +162,8 @@ SearchResult euclidean_search(const CellListOrdered<D> &query, const double *cen
   check(abh_search_total(s, &r.total), ctx.get());
   r.row_ptr = Context::Buffer<int64_t>(ctx, size_t(nq + 1));
   r.col = Context::Buffer<int32_t>(ctx, size_t(r.total));
-  std::vector<int64_t> rp(size_t(nq + 1));
-  check(abh_memcpy_d2h(ctx.get(), rp.data(), abh_search_row_ptr(s), rp.size() * 8), ctx.get());
-  r.row_ptr.upload(rp);
+  check(abh_memcpy_d2d(ctx.get(), r.row_ptr.p, abh_search_row_ptr(s), size_t(nq + 1) * 8),
+        ctx.get());
   check(abh_search_fill(s, r.col.p, r.total), ctx.get());
   return r;
 }

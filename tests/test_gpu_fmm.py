@@ -166,3 +166,56 @@ def test_fmm_compressed_accuracy(ctx, order, eps, bound):
     assert kp < order ** 3
     assert errc < bound
     assert rel(yc, y) < 20 * eps
+
+
+def test_fmm_tree_leaf_rule(ctx):
+    """The device-built tree keeps the inclusive leaf rule: on the 10-point
+    known answer of tests/test_oracle_fmm.py (an octant of exactly
+    n_leaf = 4 particles is one leaf, one of 5 is split) the GPU tree has the
+    oracle's order and number of non-empty leaves, and its FMM equals the
+    direct sum to 1e-11 (every pair is near: P2P only)."""
+    octant0 = [[0.1, 0.1, 0.1], [0.2, 0.3, 0.1], [0.4, 0.2, 0.3], [0.3, 0.4, 0.4]]
+    octant7 = [[0.6, 0.6, 0.6], [0.9, 0.9, 0.9], [0.6, 0.9, 0.6], [0.9, 0.6, 0.9],
+               [0.7, 0.7, 0.8]]
+    pos = np.array(octant0 + octant7 + [[0.1, 0.1, 0.9]], dtype=np.float64)
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0] * 3, [1] * 3, 4, 3,
+                       ab.Kernel.smooth_inv_r(1e-3))
+    t = oracle.Octree([0, 0, 0], [1, 1, 1], pos, 4)
+    assert np.array_equal(f.order().cpu().numpy(), t.order)
+    nonempty = int(((t.leaves[:, 1] - t.leaves[:, 0]) > 0).sum())
+    assert f.info["n_leaves"] == nonempty
+    q = np.linspace(-1.0, 1.0, len(pos))
+    y = (f @ torch.tensor(q, device="cuda")).cpu().numpy()
+    assert rel(y, oracle.dense_matvec(pos, pos, (KI, 1e-3), q)) < 1e-11
+
+
+def test_fmm_c3_scale(ctx):
+    """The C3 workload itself (N = 2,000,000 uniform, leaves of at most 64:
+    6 levels, 107,417 leaves, the leaf-level M2L tiles): FP64 order 6 within
+    2e-5 of the FP64 direct sum on 300 seeded targets (the CPU oracle's),
+    MIXED within 5e-5 of FP64, and order 8 compressed at 2e-7 within 1e-6."""
+    from aboria_amd import workloads as wl
+    c = wl.FMM
+    print("fmm C3 scale", flush=True)
+    pos = wl.fmm_positions(c["n"], c["seed"])
+    x = wl.fmm_rhs(c["n"], 2, c["seed"] + 1)
+    gp = torch.tensor(pos, device="cuda")
+    xt = torch.tensor(x, device="cuda")
+    idx = np.sort(np.random.default_rng(5).choice(c["n"], 300, replace=False))
+    ref = np.stack([oracle.dense_matvec(pos[idx], pos, (KI, c["eps"]), x[r]) for r in range(2)])
+    f = ab.FmmOperator(ctx, gp, [0] * 3, [1] * 3, c["n_leaf"], 6, ab.Kernel.smooth_inv_r(c["eps"]))
+    info = f.info
+    assert info["levels"] == 6 and info["n_leaves"] == 107417
+    y = f.apply_colmajor(xt, torch.zeros_like(xt)).cpu().numpy()
+    ym = f.apply_colmajor(xt, torch.zeros_like(xt), ab.MIXED).cpu().numpy()
+    e64, emix = rel(y[:, idx], ref), rel(ym, y)
+    f.close()
+    f8 = ab.FmmOperator(ctx, gp, [0] * 3, [1] * 3, c["n_leaf"], 8,
+                        ab.Kernel.smooth_inv_r(c["eps"]))
+    f8.compress(2e-7)
+    y8 = f8.apply_colmajor(xt, torch.zeros_like(xt)).cpu().numpy()
+    e8 = rel(y8[:, idx], ref)
+    f8.close()
+    print("C3 scale: fp64 order 6 %.2e, mixed vs fp64 %.2e, order 8 compressed %.2e" % (
+        e64, emix, e8), flush=True)
+    assert e64 < 2e-5 and emix < 5e-5 and e8 < 1e-6

@@ -102,3 +102,28 @@ def test_fmm_smooth_inverse_order6():
     # lower order is less accurate
     y4 = t.fmm_matvec(4, kern, q)
     assert np.linalg.norm(y4 - ref) > np.linalg.norm(y - ref)
+
+
+def test_octree_leaf_rule_is_inclusive():
+    """HyperOctree's leaf rule (src/OctTree.h:328-475): a box holding at most
+    n_particles_in_leaf particles is a leaf.  With n_leaf = 4, the octant
+    holding exactly 4 particles is one leaf at level 1 (a strict '<' would
+    split it), the octant holding 5 is split, and the lone particle of a
+    third octant is a leaf of its own."""
+    octant0 = [[0.1, 0.1, 0.1], [0.2, 0.3, 0.1], [0.4, 0.2, 0.3], [0.3, 0.4, 0.4]]
+    octant7 = [[0.6, 0.6, 0.6], [0.9, 0.9, 0.9], [0.6, 0.9, 0.6], [0.9, 0.6, 0.9],
+               [0.7, 0.7, 0.8]]
+    lone = [[0.1, 0.1, 0.9]]
+    pos = np.array(octant0 + octant7 + lone, dtype=np.float64)
+    t = oracle.Octree([0, 0, 0], [1, 1, 1], pos, 4)
+    sizes = t.leaves[:, 1] - t.leaves[:, 0]
+    # the four particles of octant 0 come first in tag order: one leaf of 4
+    leaf0 = [tuple(l) for l in t.leaves if l[0] == 0 and l[1] > 0]
+    assert leaf0 == [(0, 4)]
+    assert sorted(t.order[:4]) == [0, 1, 2, 3]
+    assert sizes.max() == 4
+    # octant 7's five particles end up in several leaves, none holding 5
+    in7 = [i for i in range(len(pos)) if np.all(pos[t.order[i]] >= 0.5)]
+    leaves7 = {tuple(l) for l in t.leaves for i in in7 if l[0] <= i < l[1]}
+    assert len(leaves7) >= 2
+    assert t.levels >= 2
