@@ -67,6 +67,8 @@ def parse():
                    help="Chebyshev orders of the fmm section (the first is reported at the top)")
     p.add_argument("--fmm-tols", default="none,1e-6,2e-7,1e-7",
                    help="M2L compression tolerances of the fmm section ('none': uncompressed)")
+    p.add_argument("--no-rbf-schwarz", action="store_true",
+                   help="skip the batched Schwarz part of the rbf section")
     p.add_argument("--rbf-iters", type=int, default=100,
                    help="GMRES iteration budget of the batched RBF systems")
     return p.parse_args()
@@ -384,6 +386,10 @@ def bench_rbf(args, torch, ab, wl, ctx, timer, extra):
             "%.2e max %.2e" % (name, mv, args.rbf_iters, solve_s, conv, c["nprob"],
                                np.median(res), res.max()))
         out[name]["converged_by_size"] = by_size(sizes, stats)
+    if args.no_rbf_schwarz:
+        op.close()
+        extra["rbf_batch"] = out
+        return
     # restricted additive Schwarz per problem (signed local factorisations)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
