@@ -20,6 +20,7 @@ __all__ = [
     "FP64", "FP32_COMPENSATED", "MIXED", "FmmOperator", "create_fmm_operator",
     "DenseBatchOperator", "SchwartzPreconditioner", "knn", "Mlp", "ApproxInverse",
     "gmres_with_fallback", "dense_apply", "IdMap", "BlockKernel", "SparseBlockOperator",
+    "HyperOctree",
 ]
 
 FP64 = C.FP64
@@ -210,6 +211,104 @@ class CellListOrdered:
     def close(self):
         if getattr(self, "handle", None):
             C.lib().abh_cells_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HyperOctree:
+    """HyperOctree (src/OctTree.h:102-911) built on the device: tags, stable
+    sort and the level-by-level node / leaf structure.  Doubles as its own
+    query object (query_view() gives the HyperOctreeQuery POD)."""
+
+    def __init__(self, ctx, low, high, periodic=None, n_particles_in_leaf=10.0):
+        self.ctx = ctx
+        self.dim = len(low)
+        lo = (ctypes.c_double * self.dim)(*[float(v) for v in low])
+        hi = (ctypes.c_double * self.dim)(*[float(v) for v in high])
+        per = (ctypes.c_uint8 * self.dim)(*[1 if p else 0 for p in (periodic or [0] * self.dim)])
+        h = ctypes.c_void_p()
+        check(C.lib().abh_octree_create(ctx.handle, self.dim, lo, hi, per,
+                                        float(n_particles_in_leaf), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        self.n_alive = 0
+
+    def update_positions(self, positions, alive=None):
+        """Wraps/kills in place like the cell list, then tags, sorts and
+        builds the tree.  Returns the alive count."""
+        _require_cuda(positions, torch.float64, "positions")
+        if positions.dim() != 2 or positions.shape[1] != self.dim:
+            raise ValueError("positions must be (n, %d)" % self.dim)
+        if alive is not None:
+            _require_cuda(alive, torch.uint8, "alive")
+        na = ctypes.c_int64()
+        check(C.lib().abh_octree_update(self.handle, positions.shape[0], _ptr(positions),
+                                        _ptr(alive), ctypes.byref(na)), self.ctx.handle)
+        self.n_alive = na.value
+        return na.value
+
+    @property
+    def info(self):
+        i = C.OctreeInfo()
+        check(C.lib().abh_octree_get_info(self.handle, ctypes.byref(i)))
+        return i
+
+    def order(self):
+        return _view(C.lib().abh_octree_order(self.handle), (self.n_alive,), torch.int32,
+                     self.ctx.device)
+
+    def positions(self):
+        return _view(C.lib().abh_octree_positions(self.handle), (self.n_alive, self.dim),
+                     torch.float64, self.ctx.device)
+
+    def tags(self):
+        return _view(C.lib().abh_octree_tags(self.handle), (self.n_alive,), "uint32",
+                     self.ctx.device)
+
+    def nodes(self):
+        return _view(C.lib().abh_octree_nodes(self.handle), (self.info.n_nodes,), torch.int32,
+                     self.ctx.device)
+
+    def leaves(self):
+        return _view(C.lib().abh_octree_leaves(self.handle), (self.info.n_leaves, 2),
+                     torch.int32, self.ctx.device)
+
+    def query_view(self):
+        """The abh_octree_view POD (ctypes) for HIP kernels that use
+        include/aboria_hip_device.hpp."""
+        v = C.OctreeView()
+        check(C.lib().abh_octree_query_view(self.handle, ctypes.byref(v)))
+        return v
+
+    def interaction_lists(self):
+        """The dual traversal's lists (3D): dict of device int32 tensors
+        p2p_ptr, p2p_src (per non-empty leaf, its source leaves), m2l_ptr,
+        m2l_src (per node, its M2L source nodes) and the P2P particle-pair
+        count."""
+        counts = (ctypes.c_int64 * 3)()
+        check(C.lib().abh_octree_interaction_lists(self.handle, None, None, None, None, counts),
+              self.ctx.handle)
+        info = self.info
+        dev = self.ctx.device
+        out = {"p2p_ptr": torch.empty(info.n_leaves + 1, dtype=torch.int32, device=dev),
+               "p2p_src": torch.empty(max(counts[0], 1), dtype=torch.int32, device=dev),
+               "m2l_ptr": torch.empty(info.n_nodes + 1, dtype=torch.int32, device=dev),
+               "m2l_src": torch.empty(max(counts[1], 1), dtype=torch.int32, device=dev)}
+        check(C.lib().abh_octree_interaction_lists(
+            self.handle, _ptr(out["p2p_ptr"]), _ptr(out["p2p_src"]), _ptr(out["m2l_ptr"]),
+            _ptr(out["m2l_src"]), counts), self.ctx.handle)
+        out["p2p_src"] = out["p2p_src"][:counts[0]]
+        out["m2l_src"] = out["m2l_src"][:counts[1]]
+        out["interactions"] = int(counts[2])
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            C.lib().abh_octree_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -82,6 +82,20 @@ class QueryView(ctypes.Structure):
                 ("n_buckets", ctypes.c_int64)]
 
 
+class OctreeInfo(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("levels", ctypes.c_int32), ("n_alive", ctypes.c_int64),
+                ("n_nodes", ctypes.c_int64), ("n_leaves", ctypes.c_int64)]
+
+
+class OctreeView(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("periodic", ctypes.c_int32 * 3), ("reserved", ctypes.c_int32),
+                ("low", ctypes.c_double * 3), ("high", ctypes.c_double * 3),
+                ("nodes", ctypes.c_void_p), ("leaves", ctypes.c_void_p), ("pos", ctypes.c_void_p),
+                ("order", ctypes.c_void_p), ("n", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
+                ("n_leaves", ctypes.c_int64)]
+
+
 class FmmInfo(ctypes.Structure):
     _fields_ = [
         ("order", ctypes.c_int32),
@@ -166,6 +180,19 @@ SIGNATURES = {
     "abh_sparse_op_assemble": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64]),
     "abh_sparse_op_apply": (c_int, [c_void_p, c_int, c_void_p, c_int64, c_void_p, c_int64, c_int]),
     "abh_sparse_op_cache": (c_int, [c_void_p, c_int, c_int]),
+    "abh_octree_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), P(ctypes.c_uint8),
+                                  c_double, P(c_void_p)]),
+    "abh_octree_destroy": (c_int, [c_void_p]),
+    "abh_octree_update": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, P(c_int64)]),
+    "abh_octree_get_info": (c_int, [c_void_p, P(OctreeInfo)]),
+    "abh_octree_order": (c_void_p, [c_void_p]),
+    "abh_octree_positions": (c_void_p, [c_void_p]),
+    "abh_octree_tags": (c_void_p, [c_void_p]),
+    "abh_octree_nodes": (c_void_p, [c_void_p]),
+    "abh_octree_leaves": (c_void_p, [c_void_p]),
+    "abh_octree_query_view": (c_int, [c_void_p, P(OctreeView)]),
+    "abh_octree_interaction_lists": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                             P(c_int64)]),
     "abh_fmm_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), c_int64, c_void_p,
                                c_double, c_int, P(Kernel), P(c_void_p)]),
     "abh_fmm_destroy": (c_int, [c_void_p]),

@@ -40,10 +40,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxWrap = 64;  // periodic wrap iterations before a particle is dropped
 
-struct BuildCounters {
-  uint32_t n_alive;
-  uint32_t n_wrap_fail;
-};
+using abh::BuildCounters;
 
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_enforce_domain(double *__restrict__ pos,
@@ -73,8 +70,10 @@ __global__ __launch_bounds__(kThreads) void k_enforce_domain(double *__restrict_
       ok = false;
     }
   }
+  // only a periodic dimension can change; the others are left unwritten
 #pragma unroll
-  for (int d = 0; d < D; ++d) pos[i * D + d] = r[d];
+  for (int d = 0; d < D; ++d)
+    if (g.periodic[d]) pos[i * D + d] = r[d];
   if (alive) alive[i] = ok ? 1 : 0;
   flags[i] = ok ? 1u : 0u;
   if (wrap_fail) atomicAdd(&counters->n_wrap_fail, 1u);
@@ -127,10 +126,11 @@ __global__ __launch_bounds__(kThreads) void k_bucket_bounds(const uint32_t *__re
   for (int64_t c = prev2; c < cur2; ++c) end[c] = uint32_t(t);
 }
 
-template <int D> void launch_enforce(abh_cells *c, int64_t n, double *pos, uint8_t *alive,
-                                     uint32_t *flags, BuildCounters *ctr) {
-  k_enforce_domain<D><<<unsigned(abh::ceil_div(n, kThreads)), kThreads, 0, c->ctx->stream>>>(
-      pos, alive, flags, n, c->params(), ctr);
+template <int D> void launch_enforce(abh_context *ctx, const GridParams &g, int64_t n,
+                                     double *pos, uint8_t *alive, uint32_t *flags,
+                                     BuildCounters *ctr) {
+  k_enforce_domain<D><<<unsigned(abh::ceil_div(n, kThreads)), kThreads, 0, ctx->stream>>>(
+      pos, alive, flags, n, g, ctr);
 }
 template <int D> void launch_compact(abh_cells *c, int64_t n, const double *pos,
                                      const uint32_t *flags, const uint32_t *offs) {
@@ -172,6 +172,17 @@ int bits_for(uint64_t v) {
 }
 
 }  // namespace
+
+namespace abh {
+void enforce_domain(abh_context *ctx, int dim, const GridParams &g, int64_t n, double *pos,
+                    uint8_t *alive, uint32_t *flags, BuildCounters *counters) {
+  if (n <= 0) return;
+  if (dim == 1) launch_enforce<1>(ctx, g, n, pos, alive, flags, counters);
+  else if (dim == 2) launch_enforce<2>(ctx, g, n, pos, alive, flags, counters);
+  else launch_enforce<3>(ctx, g, n, pos, alive, flags, counters);
+  check_launch();
+}
+}  // namespace abh
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -325,10 +336,7 @@ abh_status abh_cells_update(abh_cells *c, int64_t n, double *pos, uint8_t *alive
     uint32_t *flags = c->flags.as<uint32_t>();
     uint32_t *offs = c->offs.as<uint32_t>();
     if (n > 0) {
-      if (D == 1) launch_enforce<1>(c, n, pos, alive, flags, ctr);
-      else if (D == 2) launch_enforce<2>(c, n, pos, alive, flags, ctr);
-      else launch_enforce<3>(c, n, pos, alive, flags, ctr);
-      abh::check_launch();
+      abh::enforce_domain(ctx, D, c->params(), n, pos, alive, flags, ctr);
     }
     abh::scan_exclusive(ctx, flags, offs, n, &ctr->n_alive);
     BuildCounters h{};

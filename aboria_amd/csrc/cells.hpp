@@ -17,6 +17,19 @@ struct GridParams {
   int64_t nb;
 };
 
+struct BuildCounters {
+  uint32_t n_alive;
+  uint32_t n_wrap_fail;
+};
+
+// enforce_domain_lambda (src/NeighbourSearchBase.h:185-238) over n
+// particles: periodic dimensions of pos wrapped in place (the others are not
+// written), flags[i] = 1 for a kept particle, 0 for a non-finite one or one
+// outside a non-periodic dimension; alive (may be null) gets the same;
+// counters->n_wrap_fail counts wraps abandoned after 64 steps.
+void enforce_domain(abh_context *ctx, int dim, const GridParams &g, int64_t n, double *pos,
+                    uint8_t *alive, uint32_t *flags, BuildCounters *counters);
+
 }  // namespace abh
 
 struct abh_cells {

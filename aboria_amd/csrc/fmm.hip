@@ -1,14 +1,12 @@
 // Black-box Chebyshev FMM over a HyperOctree (3D).
 //
 // Set-up (abh_fmm_create):
-//   - octree tags on the GPU (point_to_tag, src/detail/SpatialUtil.h:255-286),
-//     stable radix sort, tree order gather;
-//   - the node / leaf structure on the host from the sorted tags, level by
-//     level exactly as HyperOctree::build_tree (src/OctTree.h:328-475);
+//   - the HyperOctree built on the device (octree.hip: tags, stable radix
+//     sort, tree order gather, level-by-level build, src/OctTree.h:328-475);
 //   - the reference's dual traversal (src/FastMultipoleMethod.h:442-548 with
 //     theta_condition, src/detail/FastMultipoleMethod.h:1249-1279) on the
-//     host to list the P2P (target leaf, source leaf) pairs and count the
-//     M2L pairs;
+//     device, level by level, listing the P2P (target leaf, source leaf)
+//     pairs and counting the M2L pairs;
 //   - per level a dense map from box coordinates to node, which the M2L
 //     kernel uses to enumerate each target's interaction list (same-level
 //     boxes whose parents touch the target's parent, minus the boxes that
@@ -30,6 +28,7 @@
 #include <vector>
 
 #include "kernel_fn.hpp"
+#include "octree.hpp"
 #include "solvers.hpp"
 #include "symeig.hpp"
 
@@ -68,70 +67,13 @@ struct Box {
   double lo[3], hi[3];
 };
 
-Box child_box(const Box &b, int c) {
-  Box r = b;
-  for (int i = 0; i < 3; ++i) {
-    const bool high = c & (1 << (2 - i));
-    if (high) r.lo[i] = 0.5 * (r.hi[i] + r.lo[i]);
-    else r.hi[i] = 0.5 * (r.hi[i] + r.lo[i]);
-  }
-  return r;
-}
-
-// theta_condition::check: true = too close for M2L
-bool theta_close(const Box &t, const Box &s) {
-  double md = std::numeric_limits<double>::min();
-  for (int i = 0; i < 3; ++i)
-    if (t.hi[i] - t.lo[i] > md) md = t.hi[i] - t.lo[i];
-  double sq = 0.0;
-  for (int i = 0; i < 3; ++i) {
-    double dist = 0.0;
-    if (t.hi[i] < s.lo[i]) dist = s.lo[i] - t.hi[i];
-    else if (s.hi[i] < t.lo[i]) dist = t.lo[i] - s.hi[i];
-    if (s.hi[i] - s.lo[i] > md) md = s.hi[i] - s.lo[i];
-    sq += dist * dist;
-  }
-  return md * md > 1.0 * sq;
-}
-
 // ---------------------------------------------------------------------------
-// device: tags
+// device: the domain passed by value
 // ---------------------------------------------------------------------------
 
 struct Bounds {
   double lo[3], hi[3];
 };
-
-__global__ __launch_bounds__(256) void k_tags(const double *__restrict__ pos, int64_t n, Bounds b,
-                                              uint32_t *__restrict__ keys,
-                                              int32_t *__restrict__ vals) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  double p[3], lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) {
-    p[d] = pos[i * 3 + d];
-    lo[d] = b.lo[d];
-    hi[d] = b.hi[d];
-  }
-  int result = 0;
-  for (int level = 1; level <= kMaxLevel; ++level) {
-    int hh[3];
-    double mid[3];
-    for (int d = 0; d < 3; ++d) {
-      mid[d] = 0.5 * (lo[d] + hi[d]);
-      hh[d] = (p[d] < mid[d]) ? 0 : 1;
-      result |= hh[d];
-      result <<= 1;
-    }
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = hh[d] ? mid[d] : lo[d];
-      hi[d] = hh[d] ? hi[d] : mid[d];
-    }
-  }
-  result >>= 1;
-  keys[i] = uint32_t(result);
-  vals[i] = int32_t(i);
-}
 
 // ---------------------------------------------------------------------------
 // device: operators
@@ -1060,16 +1002,13 @@ struct abh_fmm {
   FmmConsts consts{};
   int nlevels = 0;
   abh_fmm_info info{};
-  // host tree
-  std::vector<int32_t> nodes;
-  std::vector<std::pair<int, int>> leaves;
-  // device
-  DevBuf order_buf, pos, keys, box, child0, coord, leaf_nodes, leaf_range, p2p_ptr, p2p_src,
-      M, W, G, xs, ys, maps;
+  // the tree (order, positions in tree order, child codes, leaves, the
+  // internal nodes level by level with their offsets par_off) and the P2P
+  // lists of its dual traversal
+  abh::Octree tree;
+  abh::OctreeLists lists;
+  DevBuf box, coord, M, W, G, xs, ys, maps;
   std::vector<int64_t> map_off;          // per level offset into maps
-  std::vector<int64_t> par_off;          // per level offset into parents
-  DevBuf parents;                        // non-leaf nodes, level by level
-  std::vector<std::vector<int32_t>> parents_at;  // non-leaf nodes per level
   int nleaf = 0;
   int64_t n_nodes = 0;
   DevBuf Mh, Ml, Wh, Wl;  // BF16 hi / lo parts for the MFMA M2L
@@ -1087,118 +1026,18 @@ struct abh_fmm {
 
 namespace {
 
-struct HostBuild {
-  std::vector<int32_t> nodes;
-  std::vector<std::pair<int, int>> leaves;
-  std::vector<int8_t> level;
-  std::vector<int32_t> coord;  // 3 per node
-  std::vector<Box> boxes;
-  std::vector<int32_t> leaf_of;  // node -> leaf index (or -1)
-  int nlevels = 0;
-};
-
-// HyperOctree::build_tree (src/OctTree.h:328-475) from sorted tags.
-void build_tree(HostBuild &hb, const std::vector<uint32_t> &tags, double n_leaf, const Box &root) {
-  constexpr int D = 3, nchild = 8;
-  struct Active {
-    int tag;
-    int node;  // -1 for the root
-  };
-  std::vector<Active> active{{0, -1}};
-  for (int level = 1; !active.empty(); ++level) {
-    ABH_REQUIRE(level <= kMaxLevel, ABH_ERR_INVALID_ARG, "octree exceeded its maximum depth");
-    const int children_begin = int(hb.nodes.size());
-    const int nch = int(active.size()) * nchild;
-    hb.nodes.resize(hb.nodes.size() + nch);
-    hb.level.resize(hb.nodes.size());
-    hb.coord.resize(hb.nodes.size() * 3);
-    hb.boxes.resize(hb.nodes.size());
-    hb.leaf_of.resize(hb.nodes.size(), -1);
-    const int length = (1 << (kMaxLevel - level) * D) - 1;
-    std::vector<Active> next;
-    int node_idx = 0;
-    for (int c = 0; c < nch; ++c) {
-      const Active &a = active[c / nchild];
-      const int which = c & (nchild - 1);
-      const int child_tag = a.tag | (which << ((kMaxLevel - level) * D));
-      const int k = children_begin + c;
-      const int lb = int(std::lower_bound(tags.begin(), tags.end(), uint32_t(child_tag)) - tags.begin());
-      const int ub = int(std::upper_bound(tags.begin(), tags.end(), uint32_t(child_tag + length)) -
-                         tags.begin());
-      const int count = ub - lb;
-      hb.level[k] = int8_t(level);
-      const Box pb = a.node < 0 ? root : hb.boxes[a.node];
-      hb.boxes[k] = child_box(pb, which);
-      for (int d = 0; d < 3; ++d) {
-        const int pcrd = a.node < 0 ? 0 : hb.coord[a.node * 3 + d];
-        hb.coord[k * 3 + d] = 2 * pcrd + ((which >> (2 - d)) & 1);
-      }
-      if (count == 0) {
-        hb.nodes[k] = int32_t(0xffffffff);
-      } else if (level == kMaxLevel || count <= n_leaf) {
-        hb.leaf_of[k] = int32_t(hb.leaves.size());
-        hb.nodes[k] = int32_t(0x80000000u | uint32_t(hb.leaves.size()));
-        hb.leaves.push_back({lb, ub});
-      } else {
-        hb.nodes[k] = int32_t(hb.nodes.size()) + nchild * node_idx;
-        ++node_idx;
-        next.push_back({child_tag, k});
-      }
-    }
-    active.swap(next);
-    hb.nlevels = level;
-  }
-}
-
-bool is_leaf_code(int32_t id) { return id < 0; }
-
-// The reference's downward traversal (calculate_dive_M2L_and_L2L) reduced to
-// its interaction lists: P2P pairs (target leaf, source leaf) and the M2L
-// pair count.
-struct Lists {
-  std::vector<std::vector<int32_t>> p2p;  // per leaf index: source leaf indices
-  int64_t m2l = 0;
-};
-
-void collect_leaves(const HostBuild &hb, int k, std::vector<int32_t> &out) {
-  const int32_t id = hb.nodes[k];
-  if (is_leaf_code(id)) {
-    if (hb.leaf_of[k] >= 0) out.push_back(hb.leaf_of[k]);
-    return;
-  }
-  for (int c = 0; c < 8; ++c) collect_leaves(hb, id + c, out);
-}
-
-void traverse(const HostBuild &hb, int t, const std::vector<int32_t> &conn_parent, Lists &L) {
-  std::vector<int32_t> conn;
-  const Box &tb = hb.boxes[t];
-  if (conn_parent.empty()) {
-    for (int c = 0; c < 8; ++c) {
-      if (theta_close(tb, hb.boxes[c])) conn.push_back(c);
-      else ++L.m2l;
-    }
-  } else {
-    for (int32_t src : conn_parent) {
-      const int32_t sid = hb.nodes[src];
-      if (is_leaf_code(sid)) {
-        conn.push_back(src);
-      } else {
-        for (int c = 0; c < 8; ++c) {
-          if (theta_close(tb, hb.boxes[sid + c])) conn.push_back(sid + c);
-          else ++L.m2l;
-        }
-      }
-    }
-  }
-  const int32_t tid = hb.nodes[t];
-  if (!is_leaf_code(tid)) {
-    for (int c = 0; c < 8; ++c) traverse(hb, tid + c, conn, L);
-    return;
-  }
-  const int tl = hb.leaf_of[t];
-  if (tl < 0) return;  // empty target
-  std::vector<int32_t> &dst = L.p2p[tl];
-  for (int32_t s : conn) collect_leaves(hb, s, dst);
+// the non-empty nodes of one level into that level's dense coordinate map
+// (empty boxes carry no expansion and stay -1)
+__global__ __launch_bounds__(256) void k_level_map(int64_t b, int64_t cnt,
+                                                   const int32_t *__restrict__ nodes,
+                                                   const int32_t *__restrict__ coord, int side,
+                                                   int32_t *__restrict__ map) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= cnt) return;
+  const int64_t k = b + i;
+  if (nodes[k] == int32_t(0xffffffffu)) return;
+  const int32_t *c = coord + k * 3;
+  map[(int64_t(c[0]) * side + c[1]) * side + c[2]] = int32_t(k);
 }
 
 template <int N> void make_consts(FmmConsts &c) {
@@ -1221,114 +1060,51 @@ template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_lea
   hipStream_t s = ctx->stream;
   const int64_t n = f->n;
   make_consts<N>(f->consts);
-  // tags, sort, gather
-  f->keys.ensure(size_t(n + 1) * 4);
-  f->order_buf.ensure(size_t(n + 1) * 4);
-  f->pos.ensure(size_t(n + 1) * 24);
+  // the tree, built on the device
+  abh::Octree &t = f->tree;
+  t.dim = 3;
+  t.max_level = kMaxLevel;
+  t.n_leaf = n_leaf;
+  for (int d = 0; d < 3; ++d) {
+    t.lo[d] = f->bounds.lo[d];
+    t.hi[d] = f->bounds.hi[d];
+    t.periodic[d] = false;
+  }
+  // non-periodic: the domain enforcement only reads the positions
+  abh::octree_update(ctx, t, n, const_cast<double *>(pos_in), nullptr);
+  ABH_REQUIRE(t.n_alive == n, ABH_ERR_INVALID_ARG,
+              "every position must be finite and inside [low, high)");
+  f->nlevels = t.levels;
+  const int64_t nn = t.n_nodes;
+  f->n_nodes = nn;
+  f->nleaf = int(t.n_leaves);
+  f->box.ensure(size_t(nn + 1) * 48);
+  f->coord.ensure(size_t(nn + 1) * 12);
+  abh::octree_geometry(ctx, t, f->box.as<double>(), f->coord.as<int32_t>());
+  abh::octree_lists(ctx, t, f->box.as<double>(), f->lists, false);
+  // per level the map from box coordinates to non-empty nodes
+  f->map_off.assign(size_t(t.levels) + 2, 0);
+  for (int l = 1; l <= t.levels; ++l) {
+    const int64_t side = int64_t(1) << l;
+    f->map_off[l + 1] = f->map_off[l] + side * side * side;
+  }
+  f->maps.ensure(size_t(f->map_off[t.levels + 1] + 1) * 4);
+  ABH_HIP(hipMemsetAsync(f->maps.p, 0xff, size_t(f->map_off[t.levels + 1]) * 4, s));
+  for (int l = 1; l <= t.levels; ++l) {
+    const int64_t b0 = t.level_begin[size_t(l) - 1], cnt = t.level_begin[size_t(l)] - b0;
+    k_level_map<<<unsigned(abh::ceil_div(cnt, 256)), 256, 0, s>>>(
+        b0, cnt, t.nodes.as<int32_t>(), f->coord.as<int32_t>(), 1 << l,
+        f->maps.as<int32_t>() + f->map_off[l]);
+    abh::check_launch();
+  }
+  // M2L operators for levels 2..nlevels
+  constexpr int NC = Cheb<N>::NC;
   Bounds b;
   for (int d = 0; d < 3; ++d) {
     b.lo[d] = f->bounds.lo[d];
     b.hi[d] = f->bounds.hi[d];
   }
-  if (n > 0) {
-    k_tags<<<unsigned(abh::ceil_div(n, 256)), 256, 0, s>>>(pos_in, n, b, f->keys.as<uint32_t>(),
-                                                          f->order_buf.as<int32_t>());
-    abh::check_launch();
-    abh::radix_sort_pairs(ctx, f->keys.as<uint32_t>(), f->order_buf.as<int32_t>(), n,
-                          3 * kMaxLevel);
-    abh::gather_bytes(ctx, f->order_buf.as<int32_t>(), n, pos_in, f->pos.p, 24);
-  }
-  std::vector<uint32_t> tags(static_cast<size_t>(n));
-  if (n > 0)
-    ABH_HIP(hipMemcpyAsync(tags.data(), f->keys.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
-  ABH_HIP(hipStreamSynchronize(s));
-  HostBuild hb;
-  build_tree(hb, tags, n_leaf, f->bounds);
-  f->nlevels = hb.nlevels;
-  const int64_t nn = int64_t(hb.nodes.size());
-  f->n_nodes = nn;
-  // traversal lists
-  Lists L;
-  L.p2p.resize(hb.leaves.size());
-  {
-    std::vector<int32_t> none;
-    for (int c = 0; c < 8; ++c) traverse(hb, c, none, L);
-  }
-  // device tree arrays
-  std::vector<double> boxv(size_t(nn) * 6);
-  std::vector<int32_t> child0(static_cast<size_t>(nn)), coordv(static_cast<size_t>(nn) * 3);
-  for (int64_t k = 0; k < nn; ++k) {
-    for (int d = 0; d < 3; ++d) {
-      boxv[k * 6 + d] = hb.boxes[k].lo[d];
-      boxv[k * 6 + 3 + d] = hb.boxes[k].hi[d];
-      coordv[k * 3 + d] = hb.coord[k * 3 + d];
-    }
-    child0[k] = is_leaf_code(hb.nodes[k]) ? -1 : hb.nodes[k];
-  }
-  // non-empty leaves, their node and particle range; P2P CSR
-  std::vector<int32_t> leaf_nodes, leaf_index_of(hb.leaves.size(), -1);
-  std::vector<int2> ranges;
-  for (int64_t k = 0; k < nn; ++k) {
-    const int l = hb.leaf_of[k];
-    if (l < 0) continue;
-    leaf_index_of[l] = int32_t(leaf_nodes.size());
-    leaf_nodes.push_back(int32_t(k));
-    ranges.push_back(make_int2(hb.leaves[l].first, hb.leaves[l].second));
-  }
-  const int nleaf = int(leaf_nodes.size());
-  f->nleaf = nleaf;
-  std::vector<int32_t> pptr(size_t(nleaf) + 1, 0), psrc;
-  int64_t inter = 0;
-  for (int li = 0; li < nleaf; ++li) {
-    const int l = hb.leaf_of[leaf_nodes[li]];
-    for (int32_t sl : L.p2p[l]) {
-      psrc.push_back(leaf_index_of[sl]);
-      inter += int64_t(ranges[li].y - ranges[li].x) *
-               (hb.leaves[sl].second - hb.leaves[sl].first);
-    }
-    pptr[li + 1] = int32_t(psrc.size());
-  }
-  // parents per level (non-leaf nodes) and level maps
-  f->parents_at.assign(size_t(hb.nlevels) + 1, {});
-  f->map_off.assign(size_t(hb.nlevels) + 2, 0);
-  for (int l = 1; l <= hb.nlevels; ++l) {
-    const int64_t side = int64_t(1) << l;
-    f->map_off[l + 1] = f->map_off[l] + side * side * side;
-  }
-  std::vector<int32_t> maps(size_t(f->map_off[hb.nlevels + 1]), -1);
-  for (int64_t k = 0; k < nn; ++k) {
-    const int l = hb.level[k];
-    if (!is_leaf_code(hb.nodes[k])) f->parents_at[l].push_back(int32_t(k));
-    const int64_t side = int64_t(1) << l;
-    if (hb.nodes[k] != int32_t(0xffffffff))  // empty boxes carry no expansion
-      maps[size_t(f->map_off[l] + (hb.coord[k * 3] * side + hb.coord[k * 3 + 1]) * side +
-                  hb.coord[k * 3 + 2])] = int32_t(k);
-  }
-  auto up = [&](DevBuf &buf, const void *src, size_t bytes) {
-    buf.ensure(bytes + 16);
-    if (bytes) ABH_HIP(hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, s));
-  };
-  up(f->box, boxv.data(), boxv.size() * 8);
-  up(f->child0, child0.data(), child0.size() * 4);
-  up(f->coord, coordv.data(), coordv.size() * 4);
-  up(f->leaf_nodes, leaf_nodes.data(), leaf_nodes.size() * 4);
-  up(f->leaf_range, ranges.data(), ranges.size() * sizeof(int2));
-  up(f->p2p_ptr, pptr.data(), pptr.size() * 4);
-  up(f->p2p_src, psrc.data(), psrc.size() * 4);
-  up(f->maps, maps.data(), maps.size() * 4);
-  {
-    std::vector<int32_t> flat;
-    f->par_off.assign(size_t(hb.nlevels) + 2, 0);
-    for (int l = 1; l <= hb.nlevels; ++l) {
-      f->par_off[l] = int64_t(flat.size());
-      flat.insert(flat.end(), f->parents_at[l].begin(), f->parents_at[l].end());
-    }
-    f->par_off[hb.nlevels + 1] = int64_t(flat.size());
-    up(f->parents, flat.data(), flat.size() * 4);
-  }
-  // M2L operators for levels 2..nlevels
-  constexpr int NC = Cheb<N>::NC;
-  const int nlev = std::max(0, hb.nlevels - 1);
+  const int nlev = std::max(0, t.levels - 1);
   if (nlev > 0) {
     f->M.ensure(size_t(nlev) * kOffsets * NC * NC * 8);
     k_m2l_ops<N><<<4096, 256, 0, s>>>(f->M.as<double>(), 2, nlev, b, f->kern, f->consts);
@@ -1338,14 +1114,14 @@ template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_lea
   f->G.ensure(size_t(nn) * NC * kRhs * 8);
   f->xs.ensure(size_t(n + 1) * kRhs * 8);
   f->ys.ensure(size_t(n + 1) * kRhs * 8);
-  ABH_HIP(hipStreamSynchronize(s));  // host vectors above go out of scope
+  ABH_HIP(hipStreamSynchronize(s));
   f->info.order = N;
-  f->info.levels = hb.nlevels;
+  f->info.levels = t.levels;
   f->info.n_nodes = nn;
-  f->info.n_leaves = nleaf;
-  f->info.m2l_pairs = L.m2l;
-  f->info.p2p_pairs = int64_t(psrc.size());
-  f->info.p2p_interactions = inter;
+  f->info.n_leaves = f->nleaf;
+  f->info.m2l_pairs = f->lists.m2l;
+  f->info.p2p_pairs = f->lists.p2p_pairs;
+  f->info.p2p_interactions = f->lists.interactions;
 }
 
 // doubles of the A fragments of an R x K operator (k_pack_frag layout)
@@ -1491,27 +1267,27 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
   hipStream_t s = ctx->stream;
   const int64_t n = f->n;
   const int64_t nn = f->n_nodes;
-  const int32_t *par = f->parents.as<int32_t>();
-  auto npar = [&](int l) { return int(f->par_off[l + 1] - f->par_off[l]); };
+  const int32_t *par = f->tree.internal.as<int32_t>();
+  auto npar = [&](int l) { return int(f->tree.par_off[l + 1] - f->tree.par_off[l]); };
   for (int r0 = 0; r0 < nrhs; r0 += kRhs) {
     const int nr = std::min(kRhs, nrhs - r0);
     k_gather_rhs<<<unsigned(abh::ceil_div(n * kRhs, 256)), 256, 0, s>>>(
-        f->order_buf.as<int32_t>(), n, nr, x + int64_t(r0) * ldx, ldx, f->xs.as<double>());
+        f->tree.order.as<int32_t>(), n, nr, x + int64_t(r0) * ldx, ldx, f->xs.as<double>());
     abh::check_launch();
     ABH_HIP(hipMemsetAsync(f->W.p, 0, size_t(nn) * NC * kRhs * 8, s));
     ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
     if (f->nleaf > 0) {
-      k_p2m<N><<<f->nleaf, 256, 0, s>>>(f->leaf_nodes.as<int32_t>(), f->nleaf,
-                                        f->leaf_range.as<int2>(), f->box.as<double>(),
-                                        f->pos.as<double>(), f->xs.as<double>(),
+      k_p2m<N><<<f->nleaf, 256, 0, s>>>(f->tree.leaf_node.as<int32_t>(), f->nleaf,
+                                        f->tree.leaves.as<int2>(), f->box.as<double>(),
+                                        f->tree.pos.as<double>(), f->xs.as<double>(),
                                         f->W.as<double>(), f->consts);
       abh::check_launch();
     }
     // M2M: deepest parents first
     for (int l = f->nlevels - 1; l >= 1; --l) {
       if (npar(l) == 0) continue;
-      k_transfer<N, true><<<npar(l), 256, 0, s>>>(par + f->par_off[l], npar(l),
-                                                  f->child0.as<int32_t>(), f->W.as<double>(),
+      k_transfer<N, true><<<npar(l), 256, 0, s>>>(par + f->tree.par_off[l], npar(l),
+                                                  f->tree.nodes.as<int32_t>(), f->W.as<double>(),
                                                   f->consts);
       abh::check_launch();
     }
@@ -1539,7 +1315,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
       for (int l = 2; l <= f->nlevels; ++l) {
         if (npar(l - 1) == 0) continue;
         k_box_op64<<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
-            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), KP, NC,
+            par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(), KP, NC,
             f->UTf.as<double>() + int64_t(l - 2) * frag_size(KP, NC), f->W.as<double>(),
             f->Wc.as<double>(), 0);
         abh::check_launch();
@@ -1549,24 +1325,24 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     // downward: level l receives L2L from level l-1 and its M2L
     for (int l = 2; l <= f->nlevels; ++l) {
       if (npar(l - 1) == 0) continue;
-      k_transfer<N, false><<<npar(l - 1), 256, 0, s>>>(par + f->par_off[l - 1], npar(l - 1),
-                                                       f->child0.as<int32_t>(),
+      k_transfer<N, false><<<npar(l - 1), 256, 0, s>>>(par + f->tree.par_off[l - 1], npar(l - 1),
+                                                       f->tree.nodes.as<int32_t>(),
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
       if (KP > 0) {
         // C_o W^ into G^, then G += U G^
-        m2l_compressed(KP, par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(),
+        m2l_compressed(KP, par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(),
                        f->coord.as<int32_t>(), f->maps.as<int32_t>() + f->map_off[l], 1 << l,
                        f->Cf.as<double>() + int64_t(l - 2) * kOffsets * frag_size(KP, KP),
                        f->Wc.as<double>(), f->Gc.as<double>(), s);
         abh::check_launch();
         k_box_op64<<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
-            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), NC, KP,
+            par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(), NC, KP,
             f->Uf.as<double>() + int64_t(l - 2) * frag_size(NC, KP), f->Gc.as<double>(),
             f->G.as<double>(), 1);
       } else if (mixed && NC <= kPad) {
         k_m2l_mfma<N><<<xcd_grid((npar(l - 1) + 1) / 2), 512, 0, s>>>(
-            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
+            par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mh.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset,
             f->Ml.as<uint16_t>() + int64_t(l - 2) * kOffsets * kBfPerOffset, f->Wh.as<uint16_t>(),
@@ -1582,7 +1358,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
           f->pack_ready = true;
         }
         k_m2l_mfma64<NC><<<xcd_grid(npar(l - 1)), 128 * m2l_row_groups(NC), 0, s>>>(
-            par + f->par_off[l - 1], npar(l - 1), f->child0.as<int32_t>(), f->coord.as<int32_t>(),
+            par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(), f->coord.as<int32_t>(),
             f->maps.as<int32_t>() + f->map_off[l], 1 << l,
             f->Mf.as<double>() + int64_t(l - 2) * kOffsets * F::PER_OFFSET, f->W.as<double>(),
             f->G.as<double>());
@@ -1592,19 +1368,19 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     if (f->nleaf > 0) {
       if (prec == ABH_FP64)
         k_leaf<N, false><<<f->nleaf, 256, 0, s>>>(
-            f->leaf_nodes.as<int32_t>(), f->nleaf, f->leaf_range.as<int2>(),
-            f->p2p_ptr.as<int32_t>(), f->p2p_src.as<int32_t>(), f->box.as<double>(),
-            f->pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
+            f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
+            f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
+            f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
             f->kern, f->consts);
       else
         k_leaf<N, true><<<f->nleaf, 256, 0, s>>>(
-            f->leaf_nodes.as<int32_t>(), f->nleaf, f->leaf_range.as<int2>(),
-            f->p2p_ptr.as<int32_t>(), f->p2p_src.as<int32_t>(), f->box.as<double>(),
-            f->pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
+            f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
+            f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
+            f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
             f->kern, f->consts);
       abh::check_launch();
       k_scatter_rhs<<<unsigned(abh::ceil_div(n * nr, 256)), 256, 0, s>>>(
-          f->order_buf.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
+          f->tree.order.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
       abh::check_launch();
     }
   }
@@ -1692,7 +1468,7 @@ abh_status abh_fmm_get_info(const abh_fmm *f, abh_fmm_info *info) {
   return ABH_OK;
 }
 
-const int32_t *abh_fmm_order(const abh_fmm *f) { return f ? f->order_buf.as<int32_t>() : nullptr; }
+const int32_t *abh_fmm_order(const abh_fmm *f) { return f ? f->tree.order.as<int32_t>() : nullptr; }
 
 abh_status abh_fmm_apply(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
                          int64_t ldy, abh_precision precision) {

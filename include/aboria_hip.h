@@ -370,6 +370,95 @@ abh_status abh_sparse_block_op_assemble(abh_sparse_block_op *op, int64_t *row_pt
                                         int32_t *col, double *val, int64_t capacity);
 
 /* ------------------------------------------------------------------ */
+/* HyperOctree                                                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct abh_octree abh_octree;
+
+/* Replaces HyperOctree construction + set_domain (src/OctTree.h:129-144,
+ * src/NeighbourSearchBase.h:252-268; init_neighbour_search,
+ * src/Particles.h:445-455): domain [low, high) per dimension, periodic
+ * (dim bytes, may be NULL = none), a leaf holds at most
+ * int(n_particles_in_leaf) particles unless it lies on the deepest level
+ * 32 / dim - 2.  dim 1..3.  The new tree is empty: one level of 2^dim empty
+ * boxes, as the reference's constructor builds. */
+abh_status abh_octree_create(abh_context *ctx, int dim, const double *low,
+                             const double *high, const uint8_t *periodic,
+                             double n_particles_in_leaf, abh_octree **out);
+abh_status abh_octree_destroy(abh_octree *tree);
+
+/* Replaces neighbour_search_base::update_positions + HyperOctree::
+ * update_positions_impl / build_tree (src/NeighbourSearchBase.h:350-495,
+ * src/OctTree.h:168-225, 328-475).  pos / alive exactly as for
+ * abh_cells_update (periodic dimensions wrapped in place, non-finite and
+ * out-of-domain particles dropped and marked dead).  On return the handle
+ * holds the tags of the kept particles sorted stably (tree order), the
+ * positions in tree order, the child codes and the leaves.  *n_alive_out
+ * (host, may be NULL) gets the kept count; synchronises. */
+abh_status abh_octree_update(abh_octree *tree, int64_t n, double *pos,
+                             uint8_t *alive, int64_t *n_alive_out);
+
+typedef struct abh_octree_info {
+  int32_t dim;
+  int32_t levels;    /* m_number_of_levels                              */
+  int64_t n_alive;   /* particles in the tree                           */
+  int64_t n_nodes;   /* boxes below the root, empty ones included       */
+  int64_t n_leaves;  /* non-empty leaves (m_leaves)                     */
+} abh_octree_info;
+abh_status abh_octree_get_info(const abh_octree *tree, abh_octree_info *info);
+
+/* Device arrays owned by the handle, valid until the next update/destroy:
+ * tree order -> input index (n_alive), positions in tree order
+ * (n_alive x dim), sorted tags (n_alive), child codes (m_nodes, n_nodes:
+ * index of the first of the 2^dim children, 0x80000000 | leaf for a leaf,
+ * 0xffffffff for an empty box; the root's children are nodes 0..2^dim-1)
+ * and leaves (m_leaves: n_leaves x 2, the [begin, end) particle range). */
+const int32_t *abh_octree_order(const abh_octree *tree);
+const double *abh_octree_positions(const abh_octree *tree);
+const uint32_t *abh_octree_tags(const abh_octree *tree);
+const int32_t *abh_octree_nodes(const abh_octree *tree);
+const int32_t *abh_octree_leaves(const abh_octree *tree);
+
+/* The query as a plain struct of device pointers, the counterpart of
+ * HyperOctreeQuery (src/OctTree.h:699-911), passed by value into HIP
+ * kernels; include/aboria_hip_device.hpp walks it (get_bucket,
+ * get_buckets_near_point and the euclidean_search visit).  Valid until the
+ * next update/destroy. */
+typedef struct abh_octree_view {
+  int32_t dim;
+  int32_t levels;
+  int32_t periodic[3];
+  int32_t reserved;
+  double low[3], high[3];  /* m_bounds                                   */
+  const int32_t *nodes;    /* m_nodes                                    */
+  const int32_t *leaves;   /* m_leaves, 2 per leaf                       */
+  const double *pos;       /* n x dim, tree order                        */
+  const int32_t *order;    /* tree order -> input index                  */
+  int64_t n;
+  int64_t n_nodes;
+  int64_t n_leaves;
+} abh_octree_view;
+abh_status abh_octree_query_view(const abh_octree *tree, abh_octree_view *view);
+
+/* The interaction lists of the dual tree traversal over one 3D tree (the
+ * strong / weak connectivity of H2Matrix::generate_matrices,
+ * src/H2Matrix.h:998-1090, the same walk as
+ * FastMultipoleMethod's calculate_dive_M2L_and_L2L,
+ * src/FastMultipoleMethod.h:442-548, with theta_condition,
+ * src/detail/FastMultipoleMethod.h:1249-1279), built on the device.
+ *   p2p_ptr (n_leaves + 1) / p2p_src: per non-empty leaf the non-empty
+ *     source leaves of its P2P products, in the traversal's order (strong
+ *     connections in order, each expanded depth first, children in order);
+ *   m2l_ptr (n_nodes + 1) / m2l_src: per node the source nodes of its M2L
+ *     products (empty boxes included, as the reference lists them).
+ * All four are device int32 arrays and may be NULL (a first call with NULL
+ * arrays sizes them).  counts (host, 3): P2P pairs, M2L pairs, particle
+ * pairs the P2P lists cover.  Synchronises. */
+abh_status abh_octree_interaction_lists(const abh_octree *tree, int32_t *p2p_ptr,
+                                        int32_t *p2p_src, int32_t *m2l_ptr,
+                                        int32_t *m2l_src, int64_t *counts);
+
+/* ------------------------------------------------------------------ */
 /* black-box Chebyshev FMM over a HyperOctree                          */
 /* ------------------------------------------------------------------ */
 

@@ -1,6 +1,6 @@
 /*
  * aboria_hip_device.hpp -- device-side neighbour queries for HIP kernels
- * written against libaboria_hip.so's cell lists.
+ * written against libaboria_hip.so's cell lists and octrees.
  *
  * abh_query_view (aboria_hip.h, filled by abh_cells_query_view) is a plain
  * struct: pass it to a kernel by value, as the reference copies its
@@ -29,6 +29,12 @@
  *                             [&](uint32_t j, const double *dx, double d2) { ++c; });
  *     count[i] = c;
  *   }
+ *
+ * abh_octree_view (abh_octree_query_view) is the HyperOctreeQuery
+ * counterpart: abh::octree_get_bucket, abh::octree_visit_buckets (the
+ * tree_query_iterator walk) and abh::octree_for_each_neighbour (the same
+ * euclidean_search visit over the tree's leaf buckets), with the same
+ * explicitly rounded FP64 operations.
  */
 #ifndef ABORIA_HIP_DEVICE_HPP_
 #define ABORIA_HIP_DEVICE_HPP_
@@ -172,6 +178,178 @@ __device__ void for_each_neighbour(const abh_query_view &q, const double *c, dou
   if (q.dim == 1) for_each_neighbour_d<1>(q, c, r, fn);
   else if (q.dim == 2) for_each_neighbour_d<2>(q, c, r, fn);
   else for_each_neighbour_d<3>(q, c, r, fn);
+}
+
+// ---------------------------------------------------------------------------
+// HyperOctree queries on an abh_octree_view (abh_octree_query_view)
+// ---------------------------------------------------------------------------
+
+// detail::is_leaf / is_empty / get_leaf_offset (src/detail/SpatialUtil.h:
+// 228-240): a negative child code is a leaf node (is_leaf_node), 0xffffffff
+// an empty one
+__device__ __forceinline__ bool octree_is_leaf_node(int32_t code) { return code < 0; }
+__device__ __forceinline__ bool octree_is_empty(int32_t code) {
+  return code == int32_t(0xffffffffu);
+}
+__device__ __forceinline__ int32_t octree_leaf_index(int32_t code) {
+  return int32_t(0x80000000u ^ uint32_t(code));
+}
+
+// octree_child_iterator::get_bounds (src/OctTree.h:633-644): child c halves
+// its parent's box, bit D-1-d of c selecting the high half of dimension d
+template <int D>
+__device__ __forceinline__ void octree_child_box(const double *plo, const double *phi, int c,
+                                                 double *lo, double *hi) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double mid = __dmul_rn(0.5, __dadd_rn(phi[d], plo[d]));
+    const bool high = (c >> (D - 1 - d)) & 1;
+    lo[d] = high ? mid : plo[d];
+    hi[d] = high ? phi[d] : mid;
+  }
+}
+
+// HyperOctreeQuery::get_bucket (src/OctTree.h:810-822, go_to :598-612):
+// the node index of the leaf (possibly empty) whose box holds p, which must
+// lie inside the domain
+template <int D>
+__device__ int64_t octree_get_bucket(const abh_octree_view &q, const double *p) {
+  double lo[D], hi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = q.low[d];
+    hi[d] = q.high[d];
+  }
+  int64_t base = 0;
+  for (;;) {
+    int which = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double mid = __dmul_rn(0.5, __dadd_rn(hi[d], lo[d]));
+      const bool high = !(p[d] < mid);
+      which = (which << 1) | int(high);
+      if (high) lo[d] = mid;
+      else hi[d] = mid;
+    }
+    const int64_t k = base + which;
+    const int32_t code = q.nodes[k];
+    if (code < 0) return k;
+    base = code;
+  }
+}
+
+// tree_query_iterator (src/NeighbourSearchBase.h:1439-1716) from
+// get_buckets_near_point (src/OctTree.h:836-848): depth first from the
+// root's children, children in order, entering a box whose squared
+// distance to p (get_dist_to_bucket: per dimension max(|centre - p| - half
+// side, 0)) is below r2, and calling fn(node, code) for every leaf node
+// reached, empty ones included
+template <int D, class F>
+__device__ void octree_visit_buckets(const abh_octree_view &q, const double *p, double r2,
+                                     F &&fn) {
+  constexpr int kDepth = 32 / D - 2;
+  int64_t base[kDepth];
+  int cur[kDepth];
+  double blo[kDepth][D], bhi[kDepth][D];
+  int top = 0;
+  base[0] = 0;
+  cur[0] = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    blo[0][d] = q.low[d];
+    bhi[0][d] = q.high[d];
+  }
+  while (top >= 0) {
+    if (cur[top] == (1 << D)) {
+      --top;
+      continue;
+    }
+    const int c = cur[top]++;
+    const int64_t k = base[top] + c;
+    double lo[D], hi[D];
+    octree_child_box<D>(blo[top], bhi[top], c, lo, hi);
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double half = __dmul_rn(0.5, __dsub_rn(hi[d], lo[d]));
+      const double centre = __dmul_rn(0.5, __dadd_rn(lo[d], hi[d]));
+      double x = __dsub_rn(fabs(__dsub_rn(centre, p[d])), half);
+      x = x > 0.0 ? x : 0.0;
+      acc = __dadd_rn(acc, __dmul_rn(x, x));
+    }
+    if (!(acc < r2)) continue;
+    const int32_t code = q.nodes[k];
+    if (code < 0) {
+      fn(k, code);
+      continue;
+    }
+    ++top;  // a node below the deepest level has no children: top < kDepth
+    base[top] = code;
+    cur[top] = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      blo[top][d] = lo[d];
+      bhi[top][d] = hi[d];
+    }
+  }
+}
+
+// euclidean_search over a HyperOctree (src/Search.h:64-496, :839-845):
+// every periodic image of the centre in lattice order, for each the leaf
+// buckets octree_visit_buckets reaches, and in each the particles with
+// sum_d (x_j - image)_d^2 <= r^2.  fn(j, dx, d2) with j the tree-order
+// index (view.order[j] is the input index).  A non-finite centre has no
+// neighbours.
+template <int D, class F>
+__device__ void octree_for_each_neighbour_d(const abh_octree_view &q, const double *c, double r,
+                                            F &&fn) {
+  const double r2 = __dmul_rn(r, r);
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (!isfinite(c[d])) return;
+  int lo[D], hi[D], img[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    lo[d] = q.periodic[d] ? -1 : 0;
+    hi[d] = q.periodic[d] ? 2 : 1;
+    img[d] = lo[d];
+  }
+  for (;;) {
+    double p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      p[d] = __dadd_rn(c[d], __dmul_rn(double(img[d]), __dsub_rn(q.high[d], q.low[d])));
+    octree_visit_buckets<D>(q, p, r2, [&](int64_t, int32_t code) {
+      if (octree_is_empty(code)) return;
+      const int32_t li = octree_leaf_index(code);
+      const int32_t b = q.leaves[2 * li], e = q.leaves[2 * li + 1];
+      for (int32_t j = b; j < e; ++j) {
+        double dx[D];
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dx[d] = __dsub_rn(q.pos[size_t(j) * D + d], p[d]);
+          acc = __dadd_rn(acc, __dmul_rn(dx[d], dx[d]));
+        }
+        if (!(acc > r2)) fn(uint32_t(j), dx, acc);
+      }
+    });
+    int d = D - 1;
+    for (; d >= 0; --d) {
+      if (++img[d] < hi[d]) break;
+      img[d] = lo[d];
+    }
+    if (d < 0) break;
+  }
+}
+
+// octree_for_each_neighbour_d for the view's dimension (1, 2 or 3)
+template <class F>
+__device__ void octree_for_each_neighbour(const abh_octree_view &q, const double *c, double r,
+                                          F &&fn) {
+  if (q.dim == 1) octree_for_each_neighbour_d<1>(q, c, r, fn);
+  else if (q.dim == 2) octree_for_each_neighbour_d<2>(q, c, r, fn);
+  else octree_for_each_neighbour_d<3>(q, c, r, fn);
 }
 
 }  // namespace abh

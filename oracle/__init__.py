@@ -64,7 +64,10 @@ def lib():
                                        ctypes.c_int64, ctypes.c_int]
         l.orc_max_threads.restype = ctypes.c_int
         l.orc_octree_create.restype = vp
-        l.orc_octree_create.argtypes = [_dp, _dp, ctypes.c_int64, _dp, ctypes.c_double]
+        l.orc_octree_create.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int64, _dp,
+                                        ctypes.c_double]
+        l.orc_octree_lists.restype = ctypes.c_int
+        l.orc_octree_lists.argtypes = [vp, _i64p, vp, vp, vp, vp]
         l.orc_octree_destroy.argtypes = [vp]
         l.orc_octree_info.argtypes = [vp, _i64p]
         l.orc_octree_get.argtypes = [vp, _i32p, _i32p, _i32p, _i32p]
@@ -282,13 +285,14 @@ def dense_matvec(a, b, kernel, x, nthreads=None):
 
 
 class Octree:
-    """HyperOctree restated on the CPU (3D)."""
+    """HyperOctree restated on the CPU (dimension 1, 2 or 3, from pos)."""
 
     def __init__(self, low, high, pos, n_leaf):
         pos = np.ascontiguousarray(pos, np.float64)
-        self.n = pos.shape[0]
-        self.h = lib().orc_octree_create(np.asarray(low, np.float64), np.asarray(high, np.float64),
-                                         self.n, pos.reshape(-1), float(n_leaf))
+        self.n, self.dim = pos.shape
+        self.h = lib().orc_octree_create(self.dim, np.asarray(low, np.float64),
+                                         np.asarray(high, np.float64), self.n,
+                                         pos.reshape(-1), float(n_leaf))
         info = np.zeros(3, np.int64)
         lib().orc_octree_info(self.h, info)
         self.n_nodes, self.n_leaves, self.levels = (int(v) for v in info)
@@ -298,6 +302,23 @@ class Octree:
         self.leaves = np.zeros((self.n_leaves, 2), np.int32)
         lib().orc_octree_get(self.h, self.order, self.tags, self.nodes, self.leaves.reshape(-1))
         self.positions = pos[self.order]
+
+    def interaction_lists(self):
+        """The dual traversal's lists (3D): dict with p2p_ptr, p2p_src (per
+        leaf, source leaves), m2l_ptr, m2l_src (per node, source nodes) and
+        the P2P particle-pair count."""
+        counts = np.zeros(3, np.int64)
+        if lib().orc_octree_lists(self.h, counts, None, None, None, None) != 0:
+            raise ValueError("interaction lists are restated for dim 3")
+        out = {"p2p_ptr": np.zeros(self.n_leaves + 1, np.int32),
+               "p2p_src": np.zeros(int(counts[0]), np.int32),
+               "m2l_ptr": np.zeros(self.n_nodes + 1, np.int32),
+               "m2l_src": np.zeros(int(counts[1]), np.int32)}
+        p = [out[k].ctypes.data_as(ctypes.c_void_p) for k in
+             ("p2p_ptr", "p2p_src", "m2l_ptr", "m2l_src")]
+        lib().orc_octree_lists(self.h, counts, *p)
+        out["interactions"] = int(counts[2])
+        return out
 
     def fmm_matvec(self, order, kernel, q):
         """y = K q (q in tree order), one right-hand side."""

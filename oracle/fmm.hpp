@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <numeric>
 #include <vector>
 
@@ -183,6 +184,64 @@ template <int D> struct Theta {
     return md * md > 1.0 * sq;
   }
 };
+
+// The dual traversal's interaction lists (H2Matrix::generate_matrices,
+// src/H2Matrix.h:998-1090; the same walk as the FMM's
+// calculate_dive_M2L_and_L2L, src/FastMultipoleMethod.h:442-548): per node
+// its weak (M2L) source nodes; per non-empty leaf its P2P source leaves --
+// the strong connections in order, each expanded depth first (children in
+// order) to its leaves, empty leaves dropped.
+template <int D> struct Lists {
+  std::vector<std::vector<int>> weak;  // per node
+  std::vector<std::vector<int>> p2p;   // per leaf index
+};
+
+template <int D> void interaction_lists(const Octree<D> &t, Lists<D> &L) {
+  constexpr int nchild = 1 << D;
+  L.weak.assign(t.nodes.size(), {});
+  L.p2p.assign(t.leaves.size(), {});
+  struct R {
+    int k;
+    OBox<D> box;
+  };
+  std::function<void(int, std::vector<int> &)> expand = [&](int k, std::vector<int> &dst) {
+    const int id = t.nodes[k];
+    if (is_leaf_id(id)) {
+      if (!is_empty_id(id)) dst.push_back(leaf_offset(id));
+      return;
+    }
+    for (int c = 0; c < nchild; ++c) expand(id + c, dst);
+  };
+  std::function<void(const R &, const std::vector<R> &)> gen = [&](const R &tg,
+                                                                   const std::vector<R> &ps) {
+    Theta<D> theta(tg.box);
+    std::vector<R> strong;
+    auto consider = [&](const R &s) {
+      if (theta.check(s.box)) strong.push_back(s);
+      else L.weak[tg.k].push_back(s.k);
+    };
+    if (ps.empty()) {
+      for (int c = 0; c < nchild; ++c) consider(R{c, Octree<D>::child_box(t.bounds, c)});
+    } else {
+      for (const R &src : ps) {
+        const int id = t.nodes[src.k];
+        if (is_leaf_id(id)) strong.push_back(src);
+        else
+          for (int c = 0; c < nchild; ++c) consider(R{id + c, Octree<D>::child_box(src.box, c)});
+      }
+    }
+    const int tid = t.nodes[tg.k];
+    if (!is_leaf_id(tid)) {
+      for (int c = 0; c < nchild; ++c) gen(R{tid + c, Octree<D>::child_box(tg.box, c)}, strong);
+      return;
+    }
+    if (is_empty_id(tid)) return;
+    std::vector<int> &dst = L.p2p[leaf_offset(tid)];
+    for (const R &s : strong) expand(s.k, dst);
+  };
+  const std::vector<R> none;
+  for (int c = 0; c < nchild; ++c) gen(R{c, Octree<D>::child_box(t.bounds, c)}, none);
+}
 
 // FastMultipoleMethod::matrix_vector_multiply with BlackBoxExpansions
 // (src/FastMultipoleMethod.h:110-548, src/detail/FastMultipoleMethod.h:96-368,

@@ -64,6 +64,21 @@ template <class F> auto dispatch(int dim, F &&f) {
 
 }  // namespace
 
+// one tree of dimension 1, 2 or 3 behind an opaque handle
+struct OctH {
+  int dim = 3;
+  Octree<1> t1;
+  Octree<2> t2;
+  Octree<3> t3;
+  Lists<3> lists;
+  bool have_lists = false;
+  template <class F> void with(F &&f) {
+    if (dim == 1) f(t1);
+    else if (dim == 2) f(t2);
+    else f(t3);
+  }
+};
+
 extern "C" {
 
 void *orc_cells_create(int dim, const double *lo, const double *hi, const uint8_t *periodic,
@@ -273,39 +288,85 @@ int orc_max_threads() { return omp_get_max_threads(); }
 // HyperOctree + black-box Chebyshev FMM (3D, one RHS per call)
 // ---------------------------------------------------------------------------
 
-void *orc_octree_create(const double *lo, const double *hi, int64_t n, const double *pos,
-                        double n_leaf) {
-  auto *t = new Octree<3>();
-  for (int d = 0; d < 3; ++d) {
-    t->bounds.lo[d] = lo[d];
-    t->bounds.hi[d] = hi[d];
-  }
-  t->build(pos, size_t(n), n_leaf);
-  return t;
+void *orc_octree_create(int dim, const double *lo, const double *hi, int64_t n,
+                        const double *pos, double n_leaf) {
+  auto *h = new OctH();
+  h->dim = dim;
+  h->with([&](auto &t) {
+    constexpr int D = sizeof(t.bounds.lo) / sizeof(double);
+    for (int d = 0; d < D; ++d) {
+      t.bounds.lo[d] = lo[d];
+      t.bounds.hi[d] = hi[d];
+    }
+    t.build(pos, size_t(n), n_leaf);
+  });
+  return h;
 }
-void orc_octree_destroy(void *p) { delete static_cast<Octree<3> *>(p); }
+void orc_octree_destroy(void *p) { delete static_cast<OctH *>(p); }
 // counts[0] = nodes, [1] = leaves, [2] = levels
 void orc_octree_info(void *p, int64_t *counts) {
-  auto *t = static_cast<Octree<3> *>(p);
-  counts[0] = int64_t(t->nodes.size());
-  counts[1] = int64_t(t->leaves.size());
-  counts[2] = t->levels;
+  static_cast<OctH *>(p)->with([&](auto &t) {
+    counts[0] = int64_t(t.nodes.size());
+    counts[1] = int64_t(t.leaves.size());
+    counts[2] = t.levels;
+  });
 }
 void orc_octree_get(void *p, int32_t *order, int32_t *tags, int32_t *nodes, int32_t *leaves) {
-  auto *t = static_cast<Octree<3> *>(p);
-  if (order) std::memcpy(order, t->order.data(), t->order.size() * 4);
-  if (tags) std::memcpy(tags, t->tags.data(), t->tags.size() * 4);
-  if (nodes) std::memcpy(nodes, t->nodes.data(), t->nodes.size() * 4);
-  if (leaves)
-    for (size_t i = 0; i < t->leaves.size(); ++i) {
-      leaves[2 * i] = t->leaves[i].first;
-      leaves[2 * i + 1] = t->leaves[i].second;
+  static_cast<OctH *>(p)->with([&](auto &t) {
+    if (order) std::memcpy(order, t.order.data(), t.order.size() * 4);
+    if (tags) std::memcpy(tags, t.tags.data(), t.tags.size() * 4);
+    if (nodes) std::memcpy(nodes, t.nodes.data(), t.nodes.size() * 4);
+    if (leaves)
+      for (size_t i = 0; i < t.leaves.size(); ++i) {
+        leaves[2 * i] = t.leaves[i].first;
+        leaves[2 * i + 1] = t.leaves[i].second;
+      }
+  });
+}
+
+// Interaction lists of a 3D tree.  counts: [0] P2P entries, [1] M2L
+// entries, [2] particle pairs of the P2P lists; arrays (may be NULL):
+// p2p_ptr (leaves + 1), p2p_src, m2l_ptr (nodes + 1), m2l_src.
+int orc_octree_lists(void *p, int64_t *counts, int32_t *p2p_ptr, int32_t *p2p_src,
+                     int32_t *m2l_ptr, int32_t *m2l_src) {
+  auto *h = static_cast<OctH *>(p);
+  if (h->dim != 3) return -1;
+  if (!h->have_lists) {
+    interaction_lists<3>(h->t3, h->lists);
+    h->have_lists = true;
+  }
+  const Octree<3> &t = h->t3;
+  const Lists<3> &L = h->lists;
+  int64_t np = 0, nm = 0, inter = 0;
+  for (size_t l = 0; l < L.p2p.size(); ++l) {
+    if (p2p_ptr) p2p_ptr[l] = int32_t(np);
+    for (int s : L.p2p[l]) {
+      if (p2p_src) p2p_src[np] = s;
+      ++np;
+      inter += int64_t(t.leaves[l].second - t.leaves[l].first) *
+               (t.leaves[size_t(s)].second - t.leaves[size_t(s)].first);
     }
+  }
+  if (p2p_ptr) p2p_ptr[L.p2p.size()] = int32_t(np);
+  for (size_t k = 0; k < L.weak.size(); ++k) {
+    if (m2l_ptr) m2l_ptr[k] = int32_t(nm);
+    for (int s : L.weak[k]) {
+      if (m2l_src) m2l_src[nm] = s;
+      ++nm;
+    }
+  }
+  if (m2l_ptr) m2l_ptr[L.weak.size()] = int32_t(nm);
+  counts[0] = np;
+  counts[1] = nm;
+  counts[2] = inter;
+  return 0;
 }
 
 // y += K q with Chebyshev order n; q, y in tree (sorted) order.
 int orc_fmm_matvec(void *p, int order, const Kernel *k, const double *q, double *y) {
-  auto *t = static_cast<Octree<3> *>(p);
+  auto *h = static_cast<OctH *>(p);
+  if (h->dim != 3) return -1;
+  Octree<3> *t = &h->t3;
   switch (order) {
     case 2: fmm_n<2>(t, k, q, y); return 0;
     case 3: fmm_n<3>(t, k, q, y); return 0;

@@ -27,9 +27,10 @@ void orc_sparse_matvec(void *, int64_t, const double *, double, const KernelC *,
                        const double *, int, const double *, int64_t, double *, int64_t, int);
 void orc_dense_matvec(int, int64_t, const double *, int64_t, const double *, const KernelC *, int,
                       const double *, int64_t, double *, int64_t, int);
-void *orc_octree_create(const double *, const double *, int64_t, const double *, double);
+void *orc_octree_create(int, const double *, const double *, int64_t, const double *, double);
 void orc_octree_destroy(void *);
 void orc_octree_info(void *, int64_t *);
+int orc_octree_lists(void *, int64_t *, int32_t *, int32_t *, int32_t *, int32_t *);
 int orc_fmm_matvec(void *, int, const KernelC *, const double *, double *);
 void *orc_schwarz_create(void *, const KernelC *, double, double, double);
 void orc_schwarz_destroy(void *);
@@ -94,7 +95,20 @@ int main() {
       }
       orc_cells_destroy(c);
     }
-  // octree + FMM at orders 3 and 6
+  // octrees in 1 and 2 dimensions
+  for (int dim = 1; dim <= 2; ++dim) {
+    std::uniform_real_distribution<double> v(0.0, 1.0);
+    const int n = 2000;
+    std::vector<double> pos(static_cast<size_t>(n) * dim);
+    for (double &p : pos) p = v(rng);
+    const double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    void *t = orc_octree_create(dim, lo, hi, n, pos.data(), 8.0);
+    int64_t info[3];
+    orc_octree_info(t, info);
+    if (info[1] <= 0) return fail("octree leaves");
+    orc_octree_destroy(t);
+  }
+  // octree + interaction lists + FMM at orders 3 and 6
   {
     std::uniform_real_distribution<double> v(0.0, 1.0);
     const int n = 1500;
@@ -102,9 +116,15 @@ int main() {
     for (double &p : pos) p = v(rng);
     for (double &p : q) p = v(rng) - 0.5;
     const double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
-    void *t = orc_octree_create(lo, hi, n, pos.data(), 64.0);
+    void *t = orc_octree_create(3, lo, hi, n, pos.data(), 64.0);
     int64_t info[3];
     orc_octree_info(t, info);
+    int64_t counts[3];
+    if (orc_octree_lists(t, counts, nullptr, nullptr, nullptr, nullptr) != 0) return fail("lists");
+    std::vector<int32_t> pp(static_cast<size_t>(info[1]) + 1), ps(static_cast<size_t>(counts[0])),
+        mp(static_cast<size_t>(info[0]) + 1), ms(static_cast<size_t>(counts[1]));
+    orc_octree_lists(t, counts, pp.data(), ps.data(), mp.data(), ms.data());
+    if (pp.back() != counts[0] || mp.back() != counts[1]) return fail("list sizes");
     KernelC k{4, 0, 1e-3, 0, 0, 0};
     if (orc_fmm_matvec(t, 3, &k, q.data(), y.data()) != 0) return fail("fmm order 3");
     if (orc_fmm_matvec(t, 6, &k, q.data(), y.data()) != 0) return fail("fmm order 6");
