@@ -310,6 +310,18 @@ def bench_fmm(args, torch, ab, wl, ctx, timer, extra):
                        device=ctx.device)
     ref = ab.dense_apply(ctx, pos[idx].contiguous(), pos, kern, x)  # FP64 direct sum
     out = {"n_check": c["n_check"]}
+    # the device HyperOctree alone: build (tags, sort, levels) and the dual
+    # traversal's interaction lists on the same points
+    tree = ab.HyperOctree(ctx, [0] * 3, [1] * 3, None, c["n_leaf"])
+    tree.update_positions(pos)
+    build_ms = timer(lambda: tree.update_positions(pos), reps=5)
+    lists_ms = timer(lambda: tree.interaction_lists(), reps=3)
+    ti = tree.info
+    out["octree"] = {"build_ms": build_ms, "lists_ms": lists_ms, "levels": ti.levels,
+                     "n_nodes": ti.n_nodes, "n_leaves": ti.n_leaves}
+    log("octree: build %.2f ms, interaction lists %.2f ms (%d levels, %d nodes, %d leaves)" % (
+        build_ms, lists_ms, ti.levels, ti.n_nodes, ti.n_leaves))
+    tree.close()
     best = None
     orders = [int(v) for v in args.fmm_orders.split(",") if v]
     tols = [None if v == "none" else float(v) for v in args.fmm_tols.split(",") if v]
