@@ -1097,6 +1097,35 @@ class Mlp:
                                     losses.ctypes.data), self.ctx.handle)
         return losses[:int(epochs)]
 
+    def _check_mats(self, x, mats):
+        _require_cuda(x, torch.float32, "x")
+        _require_cuda(mats, torch.float32, "mats")
+        k = self.dims[-1]
+        if tuple(mats.shape) != (x.shape[0], k, k):
+            raise ValueError("mats must be (n, %d, %d)" % (k, k))
+
+    def loss_grad_residual(self, x, mats):
+        """(mean patch residual |M y - e_1|^2, gradient of num_params FP32)."""
+        self._check_mats(x, mats)
+        g = torch.empty(self.num_params, dtype=torch.float32, device=x.device)
+        loss = ctypes.c_double()
+        check(C.lib().abh_mlp_loss_grad_residual(self.handle, x.shape[0], _ptr(x), _ptr(mats),
+                                                 ctypes.byref(loss), _ptr(g)), self.ctx.handle)
+        return loss.value, g
+
+    def train_residual(self, x, mats, epochs=20, batch=1024, lr=1e-3, beta1=0.9, beta2=0.999,
+                       eps=1e-8, lr_decay=1.0, seed=0):
+        """Adam on the patch-residual loss; returns the per-epoch mean loss."""
+        import numpy as np
+        self._check_mats(x, mats)
+        o = C.MlpTrainOptions(int(epochs), int(batch), float(lr), float(beta1), float(beta2),
+                              float(eps), float(lr_decay), 0, int(seed))
+        losses = np.zeros(max(int(epochs), 1), np.float32)
+        check(C.lib().abh_mlp_train_residual(self.handle, x.shape[0], _ptr(x), _ptr(mats),
+                                             ctypes.byref(o), losses.ctypes.data),
+              self.ctx.handle)
+        return losses[:int(epochs)]
+
     def close(self):
         if getattr(self, "handle", None):
             C.lib().abh_mlp_destroy(self.handle)
@@ -1127,6 +1156,23 @@ class ApproxInverse:
                                           float(radius), float(diag_shift), int(k),
                                           ctypes.byref(h)), ctx.handle)
         return cls(ctx, h, cells.n_alive, k, cells.info.dim)
+
+    @classmethod
+    def patch(cls, ctx, cells, kernel, radius, diag_shift, k=32):
+        """Exact local solves of every particle's k-nearest-neighbour patch."""
+        kd = kernel.to_c()
+        h = ctypes.c_void_p()
+        check(C.lib().abh_ainv_create_patch(ctx.handle, cells.handle, ctypes.byref(kd),
+                                            float(radius), float(diag_shift), int(k),
+                                            ctypes.byref(h)), ctx.handle)
+        return cls(ctx, h, cells.n_alive, k, cells.info.dim)
+
+    def patch_matrices(self, scale=1.0):
+        """The (n, k, k) FP32 patch matrices times scale."""
+        out = torch.empty((self.n, self.k, self.k), dtype=torch.float32, device=self.ctx.device)
+        check(C.lib().abh_ainv_patch_matrices(self.handle, float(scale), _ptr(out)),
+              self.ctx.handle)
+        return out
 
     @classmethod
     def learned(cls, ctx, cells, mlp, k, feature_scale, out_scale, bf16=True):

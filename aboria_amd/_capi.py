@@ -193,6 +193,13 @@ SIGNATURES = {
     "abh_octree_query_view": (c_int, [c_void_p, P(OctreeView)]),
     "abh_octree_interaction_lists": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                              P(c_int64)]),
+    "abh_ainv_create_patch": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double, c_int,
+                                      P(c_void_p)]),
+    "abh_ainv_patch_matrices": (c_int, [c_void_p, c_double, c_void_p]),
+    "abh_mlp_loss_grad_residual": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, P(c_double),
+                                           c_void_p]),
+    "abh_mlp_train_residual": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, P(MlpTrainOptions),
+                                       c_void_p]),
     "abh_fmm_create": (c_int, [c_void_p, c_int, P(c_double), P(c_double), c_int64, c_void_p,
                                c_double, c_int, P(Kernel), P(c_void_p)]),
     "abh_fmm_destroy": (c_int, [c_void_p]),

@@ -5,6 +5,11 @@
 //   k_lsq        least-squares approximate-inverse row of every particle on
 //                its neighbour pattern: normal equations accumulated in FP64
 //                over every particle within reach, solved in LDS
+//   k_patch      the local solve of every particle's patch (its k nearest
+//                neighbours): the row of (K + shift I)^{-1} of the patch
+//                matrix that belongs to the particle, Cholesky in LDS
+//   k_resid      the patch-residual loss || M y - e_1 ||^2 of the network's
+//                training and its gradient, one wavefront per row
 //   k_gemm       FP32 tiled GEMM with fused bias / SiLU, the forward and
 //                backward passes of the network's training
 //   k_mlp_bf16   whole-network inference on MFMA (BF16 operands, FP32
@@ -276,6 +281,82 @@ __global__ __launch_bounds__(256) void k_lsq(const double *__restrict__ pos,
   }
 }
 
+// ---------------------------------------------------------------------------
+// patch local solves: A_P = K(x_a, x_b) (cut at the radius) + shift I over a
+// particle's k nearest neighbours P (itself first); w = A_P^{-1} e_1, the
+// particle's row of the patch's local inverse.  Missing neighbours (fewer
+// than k particles) pad A_P with identity rows and get weight 0.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ double patch_entry(const double *__restrict__ pos,
+                                              const int32_t *__restrict__ nb, int a, int b,
+                                              const KernelDesc &kern, double r2, double shift) {
+  const int32_t ia = nb[a], ib = nb[b];
+  if (ia < 0 || ib < 0) return a == b ? 1.0 : 0.0;
+  return coeff<D>(pos + int64_t(ia) * D, pos + int64_t(ib) * D, a == b, kern, r2, shift);
+}
+
+template <int D>
+__global__ __launch_bounds__(64) void k_patch(const double *__restrict__ pos, int64_t n, int k,
+                                              const int32_t *__restrict__ nbr, KernelDesc kern,
+                                              double r2, double shift, double *__restrict__ w,
+                                              uint8_t *__restrict__ fail) {
+  __shared__ double G[kMaxK][kMaxK + 1];
+  __shared__ int bad;
+  const int64_t i = blockIdx.x;
+  const int t = threadIdx.x;
+  const int32_t *nb = nbr + i * k;
+  for (int e = t; e < k * k; e += 64) {
+    const int a = e / k, b = e - a * k;
+    G[a][b] = patch_entry<D>(pos, nb, a, b, kern, r2, shift);
+  }
+  if (t == 0) bad = 0;
+  // Cholesky, column by column (row t of the lower factor kept by lane t)
+  for (int j = 0; j < k; ++j) {
+    __syncthreads();
+    if (t == 0) {
+      const double d = G[j][j];
+      if (!(d > 0.0)) bad = 1;
+      G[j][j] = sqrt(d > 0.0 ? d : 1.0);
+    }
+    __syncthreads();
+    if (t > j && t < k) G[t][j] = G[t][j] / G[j][j];
+    __syncthreads();
+    if (t > j && t < k)
+      for (int c = j + 1; c <= t; ++c) G[t][c] = G[t][c] - G[t][j] * G[c][j];
+  }
+  __syncthreads();
+  if (t == 0) {
+    double y[kMaxK];
+    for (int a = 0; a < k; ++a) {  // L y = e_1
+      double v = a == 0 ? 1.0 : 0.0;
+      for (int q = 0; q < a; ++q) v -= G[a][q] * y[q];
+      y[a] = v / G[a][a];
+    }
+    for (int a = k - 1; a >= 0; --a) {  // L^T w = y
+      double v = y[a];
+      for (int q = a + 1; q < k; ++q) v -= G[q][a] * y[q];
+      y[a] = v / G[a][a];
+    }
+    for (int a = 0; a < k; ++a) w[i * k + a] = (bad || nb[a] < 0) ? 0.0 : y[a];
+    if (bad) fail[0] = 1;
+  }
+}
+
+// the patch matrices in FP32, times scale: mats[(i k + a) k + b]
+template <int D>
+__global__ __launch_bounds__(256) void k_patch_mats(const double *__restrict__ pos, int64_t n,
+                                                    int k, const int32_t *__restrict__ nbr,
+                                                    KernelDesc kern, double r2, double shift,
+                                                    double scale, float *__restrict__ mats) {
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t kk = int64_t(k) * k;
+  if (e >= n * kk) return;
+  const int64_t i = e / kk;
+  const int r = int(e - i * kk), a = r / k, b = r - a * k;
+  mats[e] = float(scale * patch_entry<D>(pos, nbr + i * k, a, b, kern, r2, shift));
+}
+
 __global__ void k_features(const double *__restrict__ pos, int64_t n, int k, int D,
                            const int32_t *__restrict__ nbr, double inv_scale,
                            float *__restrict__ f) {
@@ -452,6 +533,41 @@ __global__ __launch_bounds__(256) void k_mse(const float *__restrict__ Y, const 
   if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// Patch-residual loss: per row r (one wavefront, k <= 64 outputs)
+// res = M_r y_r - e_1, loss_r = |res|^2, dY_r = 2 M_r^T res / rows;
+// partial[block] = the block's sum of loss_r in FP64 (4 rows per block).
+__global__ __launch_bounds__(256) void k_resid(const float *__restrict__ Y,
+                                               const float *__restrict__ M, int64_t rows, int k,
+                                               float inv_rows, float *__restrict__ dY,
+                                               double *__restrict__ partial) {
+  __shared__ float ys[4][kMaxK], rs[4][kMaxK];
+  __shared__ double red[4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = int64_t(blockIdx.x) * 4 + wv;
+  const bool live = r < rows;
+  if (live && lane < k) ys[wv][lane] = Y[r * k + lane];
+  __syncthreads();
+  double l2 = 0.0;
+  if (live && lane < k) {
+    const float *Mr = M + r * k * k;
+    float v = lane == 0 ? -1.0f : 0.0f;
+    for (int b = 0; b < k; ++b) v += Mr[int64_t(lane) * k + b] * ys[wv][b];
+    rs[wv][lane] = v;
+    l2 = double(v) * double(v);
+  }
+  __syncthreads();
+  if (live && lane < k) {
+    const float *Mr = M + r * k * k;
+    float g = 0.0f;
+    for (int a = 0; a < k; ++a) g += Mr[int64_t(a) * k + lane] * rs[wv][a];
+    dY[r * k + lane] = 2.0f * g * inv_rows;
+  }
+  l2 = abh::wave_sum(l2);
+  if (lane == 0) red[wv] = l2;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void k_sum_partials(const double *__restrict__ partial, int nb, double scale,
                                double *__restrict__ out) {
   __shared__ double red[4];
@@ -604,6 +720,10 @@ struct abh_ainv {
   int dim = 2;
   const double *pos = nullptr;  // the cell list's positions (gather order)
   DevBuf nbr, w;
+  // the operator A = K + shift I the rows were built from (lsq, patch)
+  bool has_op = false;
+  KernelDesc kern{};
+  double r2 = 0.0, shift = 0.0;
 };
 
 namespace {
@@ -671,16 +791,19 @@ Fwd forward_f32(abh_mlp *m, int64_t rows, const float *x, float *y) {
   return f;
 }
 
-// loss (device double) and gradient (device, nparams) of the mean squared
-// error over rows; scratch holds dY and the backward activations
-void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, float *grad,
-               double *loss_dev, DevBuf &scratch) {
+// loss (device double) and gradient (device, nparams) over rows: the mean
+// squared error against t, or with mats (k x k per row, k = n_out) the mean
+// patch residual |M_r y_r - e_1|^2; scratch holds dY and the backward
+// activations
+void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, const float *mats,
+               float *grad, double *loss_dev, DevBuf &scratch) {
   hipStream_t s = m->ctx->stream;
   const int nl = m->layers();
   const int nout = m->dims.back();
   int64_t maxw = 0;
   for (int d : m->dims) maxw = std::max<int64_t>(maxw, d);
-  const int64_t nblk = abh::ceil_div(rows * nout, 256);
+  const int64_t nmse = abh::ceil_div(rows * nout, 256), nres = abh::ceil_div(rows, 4);
+  const int64_t nblk = std::max(nmse, nres);  // partial sums of either loss
   scratch.ensure(size_t(rows * nout + 2 * rows * maxw + 1) * 4 + size_t(nblk) * 8 + 256);
   float *Y = scratch.as<float>();
   float *dA = Y + rows * nout;          // gradient w.r.t. a layer output
@@ -689,9 +812,16 @@ void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, float *
   poff += poff & 1;  // 8-byte alignment of the FP64 partial sums
   double *partial = reinterpret_cast<double *>(scratch.as<float>() + poff);
   Fwd f = forward_f32(m, rows, x, Y);
-  k_mse<<<unsigned(nblk), 256, 0, s>>>(Y, t, rows * nout, 1.0f / float(rows * nout), dA, partial);
-  abh::check_launch();
-  k_sum_partials<<<1, 256, 0, s>>>(partial, int(nblk), 1.0 / double(rows * nout), loss_dev);
+  if (mats) {
+    k_resid<<<unsigned(nres), 256, 0, s>>>(Y, mats, rows, nout, 1.0f / float(rows), dA, partial);
+    abh::check_launch();
+    k_sum_partials<<<1, 256, 0, s>>>(partial, int(nres), 1.0 / double(rows), loss_dev);
+  } else {
+    k_mse<<<unsigned(nmse), 256, 0, s>>>(Y, t, rows * nout, 1.0f / float(rows * nout), dA,
+                                         partial);
+    abh::check_launch();
+    k_sum_partials<<<1, 256, 0, s>>>(partial, int(nmse), 1.0 / double(rows * nout), loss_dev);
+  }
   abh::check_launch();
   const float *P = m->params.as<float>();
   for (int l = nl - 1; l >= 0; --l) {
@@ -734,6 +864,62 @@ void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, float *
       abh::check_launch();
       std::swap(dA, dB);
     }
+  }
+}
+
+// Adam over shuffled full batches: targets t (n x n_out) for the mean
+// squared error, or mats (n x n_out x n_out) for the patch residual
+void train_impl(abh_mlp *m, int64_t n, const float *x, const float *t, const float *mats,
+                const abh_mlp_train_options *opts, float *loss_per_epoch) {
+  ABH_REQUIRE(opts->epochs >= 0 && opts->batch >= 1 && opts->batch <= (1 << 20),
+              ABH_ERR_INVALID_ARG, "invalid training options");
+  hipStream_t s = m->ctx->stream;
+  const int nin = m->dims.front(), nout = m->dims.back();
+  const int64_t tw = mats ? int64_t(nout) * nout : nout;  // floats per target row
+  const int64_t B = std::min<int64_t>(opts->batch, n);
+  const int64_t steps = n / B;  // full batches per epoch
+  DevBuf perm_d, xb, tb, scratch, losses;
+  perm_d.ensure(size_t(n) * 4 + 64);
+  xb.ensure(size_t(B) * nin * 4 + 64);
+  tb.ensure(size_t(B) * tw * 4 + 64);
+  losses.ensure(size_t(std::max<int64_t>(steps, 1)) * 8 + 64);
+  std::vector<int32_t> perm(static_cast<size_t>(n));
+  std::mt19937_64 rng(opts->seed);
+  float lr = opts->lr;
+  std::vector<double> lh(size_t(std::max<int64_t>(steps, 1)));
+  for (int ep = 0; ep < opts->epochs; ++ep) {
+    std::iota(perm.begin(), perm.end(), 0);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    ABH_HIP(hipMemcpyAsync(perm_d.p, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
+    for (int64_t st = 0; st < steps; ++st) {
+      const int32_t *rows = perm_d.as<int32_t>() + st * B;
+      k_gather_rows<<<unsigned(abh::ceil_div(B * nin, 256)), 256, 0, s>>>(x, rows, int(B), nin,
+                                                                         xb.as<float>());
+      k_gather_rows<<<unsigned(abh::ceil_div(B * tw, 256)), 256, 0, s>>>(
+          mats ? mats : t, rows, int(B), int(tw), tb.as<float>());
+      abh::check_launch();
+      loss_grad(m, B, xb.as<float>(), mats ? nullptr : tb.as<float>(),
+                mats ? tb.as<float>() : nullptr, m->grads.as<float>(),
+                losses.as<double>() + st, scratch);
+      ++m->adam_t;
+      const double b1 = opts->beta1, b2 = opts->beta2;
+      const double lr_t = lr * std::sqrt(1.0 - std::pow(b2, double(m->adam_t))) /
+                          (1.0 - std::pow(b1, double(m->adam_t)));
+      k_adam<<<unsigned(abh::ceil_div(m->nparams, 256)), 256, 0, s>>>(
+          m->params.as<float>(), m->grads.as<float>(), m->am.as<float>(), m->av.as<float>(),
+          m->nparams, opts->beta1, opts->beta2, float(lr_t), opts->eps);
+      abh::check_launch();
+    }
+    // the perm upload of the next epoch reuses the host vector: wait here
+    ABH_HIP(hipMemcpyAsync(lh.data(), losses.p, size_t(std::max<int64_t>(steps, 1)) * 8,
+                           hipMemcpyDeviceToHost, s));
+    ABH_HIP(hipStreamSynchronize(s));
+    if (loss_per_epoch) {
+      double sum = 0.0;
+      for (int64_t st = 0; st < steps; ++st) sum += lh[size_t(st)];
+      loss_per_epoch[ep] = steps > 0 ? float(sum / double(steps)) : 0.0f;
+    }
+    lr *= opts->lr_decay > 0.0f ? opts->lr_decay : 1.0f;
   }
 }
 
@@ -912,7 +1098,7 @@ abh_status abh_mlp_loss_grad(abh_mlp *m, int64_t n, const float *x, const float 
     DevBuf scratch;
     m->loss.ensure(64);
     float *g = grad ? grad : m->grads.as<float>();
-    loss_grad(m, n, x, t, g, m->loss.as<double>(), scratch);
+    loss_grad(m, n, x, t, nullptr, g, m->loss.as<double>(), scratch);
     double h = 0.0;
     ABH_HIP(hipMemcpyAsync(&h, m->loss.p, 8, hipMemcpyDeviceToHost, m->ctx->stream));
     ABH_HIP(hipStreamSynchronize(m->ctx->stream));
@@ -925,54 +1111,37 @@ abh_status abh_mlp_train(abh_mlp *m, int64_t n, const float *x, const float *t,
   if (!m || !opts) return ABH_ERR_INVALID_ARG;
   return abh::guard(m->ctx, [&] {
     ABH_REQUIRE(n >= 1 && x && t, ABH_ERR_INVALID_ARG, "empty training set");
-    ABH_REQUIRE(opts->epochs >= 0 && opts->batch >= 1 && opts->batch <= (1 << 20),
-                ABH_ERR_INVALID_ARG, "invalid training options");
-    hipStream_t s = m->ctx->stream;
-    const int nin = m->dims.front(), nout = m->dims.back();
-    const int64_t B = std::min<int64_t>(opts->batch, n);
-    const int64_t steps = n / B;  // full batches per epoch
-    DevBuf perm_d, xb, tb, scratch, losses;
-    perm_d.ensure(size_t(n) * 4 + 64);
-    xb.ensure(size_t(B) * nin * 4 + 64);
-    tb.ensure(size_t(B) * nout * 4 + 64);
-    losses.ensure(size_t(std::max<int64_t>(steps, 1)) * 8 + 64);
-    std::vector<int32_t> perm(static_cast<size_t>(n));
-    std::mt19937_64 rng(opts->seed);
-    float lr = opts->lr;
-    std::vector<double> lh(size_t(std::max<int64_t>(steps, 1)));
-    for (int ep = 0; ep < opts->epochs; ++ep) {
-      std::iota(perm.begin(), perm.end(), 0);
-      std::shuffle(perm.begin(), perm.end(), rng);
-      ABH_HIP(hipMemcpyAsync(perm_d.p, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
-      for (int64_t st = 0; st < steps; ++st) {
-        const int32_t *rows = perm_d.as<int32_t>() + st * B;
-        k_gather_rows<<<unsigned(abh::ceil_div(B * nin, 256)), 256, 0, s>>>(x, rows, int(B), nin,
-                                                                           xb.as<float>());
-        k_gather_rows<<<unsigned(abh::ceil_div(B * nout, 256)), 256, 0, s>>>(t, rows, int(B), nout,
-                                                                            tb.as<float>());
-        abh::check_launch();
-        loss_grad(m, B, xb.as<float>(), tb.as<float>(), m->grads.as<float>(),
-                  losses.as<double>() + st, scratch);
-        ++m->adam_t;
-        const double b1 = opts->beta1, b2 = opts->beta2;
-        const double lr_t = lr * std::sqrt(1.0 - std::pow(b2, double(m->adam_t))) /
-                            (1.0 - std::pow(b1, double(m->adam_t)));
-        k_adam<<<unsigned(abh::ceil_div(m->nparams, 256)), 256, 0, s>>>(
-            m->params.as<float>(), m->grads.as<float>(), m->am.as<float>(), m->av.as<float>(),
-            m->nparams, opts->beta1, opts->beta2, float(lr_t), opts->eps);
-        abh::check_launch();
-      }
-      // the perm upload of the next epoch reuses the host vector: wait here
-      ABH_HIP(hipMemcpyAsync(lh.data(), losses.p, size_t(std::max<int64_t>(steps, 1)) * 8,
-                             hipMemcpyDeviceToHost, s));
-      ABH_HIP(hipStreamSynchronize(s));
-      if (loss_per_epoch) {
-        double sum = 0.0;
-        for (int64_t st = 0; st < steps; ++st) sum += lh[size_t(st)];
-        loss_per_epoch[ep] = steps > 0 ? float(sum / double(steps)) : 0.0f;
-      }
-      lr *= opts->lr_decay > 0.0f ? opts->lr_decay : 1.0f;
-    }
+    train_impl(m, n, x, t, nullptr, opts, loss_per_epoch);
+  });
+}
+
+abh_status abh_mlp_loss_grad_residual(abh_mlp *m, int64_t n, const float *x, const float *mats,
+                                      double *loss, float *grad) {
+  if (!m) return ABH_ERR_INVALID_ARG;
+  return abh::guard(m->ctx, [&] {
+    ABH_REQUIRE(n >= 1 && n <= (int64_t(1) << 24), ABH_ERR_INVALID_ARG, "n out of range");
+    ABH_REQUIRE(x && mats, ABH_ERR_INVALID_ARG, "x or mats is null");
+    ABH_REQUIRE(m->dims.back() <= kMaxK, ABH_ERR_UNSUPPORTED,
+                "the patch-residual loss needs n_out <= 64");
+    DevBuf scratch;
+    m->loss.ensure(64);
+    float *g = grad ? grad : m->grads.as<float>();
+    loss_grad(m, n, x, nullptr, mats, g, m->loss.as<double>(), scratch);
+    double h = 0.0;
+    ABH_HIP(hipMemcpyAsync(&h, m->loss.p, 8, hipMemcpyDeviceToHost, m->ctx->stream));
+    ABH_HIP(hipStreamSynchronize(m->ctx->stream));
+    if (loss) *loss = h;
+  });
+}
+
+abh_status abh_mlp_train_residual(abh_mlp *m, int64_t n, const float *x, const float *mats,
+                                  const abh_mlp_train_options *opts, float *loss_per_epoch) {
+  if (!m || !opts) return ABH_ERR_INVALID_ARG;
+  return abh::guard(m->ctx, [&] {
+    ABH_REQUIRE(n >= 1 && x && mats, ABH_ERR_INVALID_ARG, "empty training set");
+    ABH_REQUIRE(m->dims.back() <= kMaxK, ABH_ERR_UNSUPPORTED,
+                "the patch-residual loss needs n_out <= 64");
+    train_impl(m, n, x, nullptr, mats, opts, loss_per_epoch);
   });
 }
 
@@ -1003,6 +1172,10 @@ abh_status abh_ainv_create_lsq(abh_context *ctx, const abh_cells *cells, const a
     kd.param = kernel->param;
     kd.param2 = kernel->param * kernel->param;
     abh::set_inverses(kd);
+    a->kern = kd;
+    a->r2 = radius * radius;
+    a->shift = diag_shift;
+    a->has_op = true;
     DevBuf fail;
     fail.ensure(64);
     hipStream_t s = ctx->stream;
@@ -1032,6 +1205,84 @@ abh_status abh_ainv_create_lsq(abh_context *ctx, const abh_cells *cells, const a
   abh::ctx_retain(ctx);
   *out = a;
   return ABH_OK;
+}
+
+abh_status abh_ainv_create_patch(abh_context *ctx, const abh_cells *cells,
+                                 const abh_kernel *kernel, double radius, double diag_shift,
+                                 int k, abh_ainv **out) {
+  if (!ctx || !kernel || !out) return ABH_ERR_INVALID_ARG;
+  *out = nullptr;
+  abh_ainv *a = new (std::nothrow) abh_ainv();
+  if (!a) return ABH_ERR_OUT_OF_MEMORY;
+  abh_status st = abh::guard(ctx, [&] {
+    ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
+    check_cells(cells);
+    ABH_REQUIRE(k >= 1 && k <= kMaxK, ABH_ERR_INVALID_ARG, "k must be in 1..64");
+    ABH_REQUIRE(radius > 0.0, ABH_ERR_UNSUPPORTED, "patch rows need a finite radius");
+    ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
+                ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    a->ctx = ctx;
+    a->n = cells->n_alive;
+    a->k = k;
+    a->dim = cells->dim;
+    a->pos = cells->pos.as<double>();
+    a->kern.type = kernel->type;
+    a->kern.param = kernel->param;
+    a->kern.param2 = kernel->param * kernel->param;
+    abh::set_inverses(a->kern);
+    a->r2 = radius * radius;
+    a->shift = diag_shift;
+    a->has_op = true;
+    a->nbr.ensure(size_t(a->n) * k * 4 + 64);
+    a->w.ensure(size_t(a->n) * k * 8 + 64);
+    if (a->n == 0) return;
+    hipStream_t s = ctx->stream;
+    knn_run(ctx, cells, k, a->nbr.as<int32_t>(), nullptr);
+    DevBuf fail;
+    fail.ensure(64);
+    ABH_HIP(hipMemsetAsync(fail.p, 0, 1, s));
+    const unsigned grid = unsigned(a->n);
+#define ABH_PATCH(DD)                                                                     \
+  k_patch<DD><<<grid, 64, 0, s>>>(a->pos, a->n, k, a->nbr.as<int32_t>(), a->kern, a->r2, \
+                                  a->shift, a->w.as<double>(), fail.as<uint8_t>())
+    if (a->dim == 1) ABH_PATCH(1);
+    else if (a->dim == 2) ABH_PATCH(2);
+    else ABH_PATCH(3);
+#undef ABH_PATCH
+    abh::check_launch();
+    uint8_t f = 0;
+    ABH_HIP(hipMemcpyAsync(&f, fail.p, 1, hipMemcpyDeviceToHost, s));
+    ABH_HIP(hipStreamSynchronize(s));
+    ABH_REQUIRE(!f, ABH_ERR_NUMERICAL, "a patch matrix is not positive definite");
+  });
+  if (st != ABH_OK) {
+    delete a;
+    return st;
+  }
+  abh::ctx_retain(ctx);
+  *out = a;
+  return ABH_OK;
+}
+
+abh_status abh_ainv_patch_matrices(const abh_ainv *a, double scale, float *mats) {
+  if (!a) return ABH_ERR_INVALID_ARG;
+  return abh::guard(a->ctx, [&] {
+    ABH_REQUIRE(a->has_op, ABH_ERR_INVALID_ARG,
+                "patch matrices need rows built from the operator (lsq or patch)");
+    if (a->n == 0) return;
+    ABH_REQUIRE(mats, ABH_ERR_INVALID_ARG, "mats is null");
+    const int64_t total = a->n * int64_t(a->k) * a->k;
+    const unsigned g = unsigned(abh::ceil_div(total, 256));
+    hipStream_t s = a->ctx->stream;
+#define ABH_PM(DD)                                                                           \
+  k_patch_mats<DD><<<g, 256, 0, s>>>(a->pos, a->n, a->k, a->nbr.as<int32_t>(), a->kern, a->r2, \
+                                     a->shift, scale, mats)
+    if (a->dim == 1) ABH_PM(1);
+    else if (a->dim == 2) ABH_PM(2);
+    else ABH_PM(3);
+#undef ABH_PM
+    abh::check_launch();
+  });
 }
 
 abh_status abh_ainv_create_learned(abh_context *ctx, const abh_cells *cells, abh_mlp *net, int k,

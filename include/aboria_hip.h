@@ -745,6 +745,19 @@ typedef struct abh_mlp_train_options {
  * array of epochs entries (mean training loss of each epoch) or NULL. */
 abh_status abh_mlp_train(abh_mlp *m, int64_t n, const float *x, const float *t,
                          const abh_mlp_train_options *opts, float *loss_per_epoch);
+/* No counterpart in the reference: the patch-residual loss of the learned
+ * local solves, loss = mean over rows of |M_r y_r - e_1|^2 with y_r =
+ * net(x_r) and M_r the row's n_out x n_out patch matrix (mats: device,
+ * n x n_out x n_out FP32, row-major, from abh_ainv_patch_matrices), and
+ * its gradient; n_out <= 64.  The network is trained to be a patch's local
+ * inverse where the operator weighs it, not to copy the inverse's
+ * entries. */
+abh_status abh_mlp_loss_grad_residual(abh_mlp *m, int64_t n, const float *x,
+                                      const float *mats, double *loss, float *grad);
+/* Adam on the patch-residual loss (options as abh_mlp_train). */
+abh_status abh_mlp_train_residual(abh_mlp *m, int64_t n, const float *x,
+                                  const float *mats, const abh_mlp_train_options *opts,
+                                  float *loss_per_epoch);
 
 typedef struct abh_ainv abh_ainv;
 
@@ -759,6 +772,19 @@ typedef struct abh_ainv abh_ainv;
 abh_status abh_ainv_create_lsq(abh_context *ctx, const abh_cells *cells,
                                const abh_kernel *kernel, double radius,
                                double diag_shift, int k, abh_ainv **out);
+/* _patch: the local solves of every particle's patch (its k nearest
+ * neighbours, itself first; k <= 64): w_i = the particle's row of
+ * (K_P + diag_shift I)^{-1}, the patch matrix of the same kernel / radius
+ * convention, factorised in FP64 -- the local inverses the network is
+ * trained on (the patch form of the Schwarz local solves,
+ * src/Preconditioners.h:1476-1522, one overlapping domain per particle). */
+abh_status abh_ainv_create_patch(abh_context *ctx, const abh_cells *cells,
+                                 const abh_kernel *kernel, double radius,
+                                 double diag_shift, int k, abh_ainv **out);
+/* The patch matrices of rows built from the operator (_lsq / _patch), times
+ * scale, in FP32: mats (device, n x k x k, row-major) -- the training input
+ * of abh_mlp_train_residual. */
+abh_status abh_ainv_patch_matrices(const abh_ainv *a, double scale, float *mats);
 abh_status abh_ainv_create_learned(abh_context *ctx, const abh_cells *cells,
                                    abh_mlp *net, int k, double feature_scale,
                                    double out_scale, int bf16, abh_ainv **out);

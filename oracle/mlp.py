@@ -68,3 +68,35 @@ def loss_grad(dims, params, x, t):
 def loss(dims, params, x, t):
     y = forward(dims, params, x)
     return float(np.mean((y - np.asarray(t, np.float64)) ** 2))
+
+
+def loss_grad_residual(dims, params, x, mats):
+    """(loss, gradient) of the patch-residual loss: mean over rows of
+    |M_r y_r - e_1|^2 with y = net(x) and M_r (n_out x n_out) per row."""
+    lay, npar = layout(dims)
+    p = np.asarray(params, np.float64)
+    y, acts, zs = forward(dims, p, x, keep=True)
+    M = np.asarray(mats, np.float64)
+    rows = y.shape[0]
+    res = np.einsum("rab,rb->ra", M, y)
+    res[:, 0] -= 1.0
+    loss = float(np.sum(res ** 2) / rows)
+    g = np.zeros(npar)
+    d = 2.0 * np.einsum("rab,ra->rb", M, res) / rows
+    for li in range(len(lay) - 1, -1, -1):
+        wo, bo, ni, no = lay[li]
+        if li < len(lay) - 1:
+            d = d * silu_grad(zs[li])
+        g[wo:wo + ni * no] = (d.T @ acts[li]).reshape(-1)
+        g[bo:bo + no] = d.sum(axis=0)
+        if li > 0:
+            W = p[wo:wo + ni * no].reshape(no, ni)
+            d = d @ W
+    return loss, g
+
+
+def loss_residual(dims, params, x, mats):
+    y = forward(dims, params, x)
+    res = np.einsum("rab,rb->ra", np.asarray(mats, np.float64), y)
+    res[:, 0] -= 1.0
+    return float(np.sum(res ** 2) / y.shape[0])

@@ -48,6 +48,35 @@ def test_mlp_oracle_gradient_finite_differences():
     assert np.max(np.abs(g)) > 1e-3
 
 
+def test_mlp_oracle_residual_gradient_finite_differences():
+    """The patch-residual loss |M y - e_1|^2: its hand-written gradient
+    against central differences, and its value at a known answer (y solving
+    M y = e_1 exactly gives 0)."""
+    rng = np.random.default_rng(4)
+    dims = [6, 8, 8, 3]
+    _, npar = omlp.layout(dims)
+    p = rng.normal(0, 0.5, npar)
+    x = rng.normal(0, 1, (5, 6))
+    M = rng.normal(0, 1, (5, 3, 3))
+    loss, g = omlp.loss_grad_residual(dims, p, x, M)
+    assert abs(loss - omlp.loss_residual(dims, p, x, M)) < 1e-14
+    h = 1e-6
+    fd = np.zeros(npar)
+    for i in range(npar):
+        e = np.zeros(npar)
+        e[i] = h
+        fd[i] = (omlp.loss_residual(dims, p + e, x, M) -
+                 omlp.loss_residual(dims, p - e, x, M)) / (2 * h)
+    assert np.max(np.abs(fd - g)) < 1e-7 * max(1.0, np.max(np.abs(g)))
+    # a transposed M in the gradient would not pass (M is not symmetric here)
+    assert np.max(np.abs(g)) > 1e-3
+    # zero residual: M_r = diag(1 / y_r0, 0, 0) makes M_r y_r = e_1
+    y = omlp.forward(dims, p, x)
+    Mz = np.zeros((5, 3, 3))
+    Mz[:, 0, 0] = 1.0 / y[:, 0]
+    assert omlp.loss_residual(dims, p, x, Mz) < 1e-24
+
+
 def test_schwarz_single_domain_is_the_exact_inverse():
     rng = np.random.default_rng(5)
     pos = rng.random((300, 2))
