@@ -792,6 +792,218 @@ __global__ __launch_bounds__(512) void k_m2l_mfma(const int32_t *__restrict__ ti
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compressed M2L in mixed precision: FP16 MFMA with a two-term split
+// (a = a_hi + a_lo, 22 significant bits; ab ~ a_hi b_hi + a_hi b_lo + a_lo b_hi)
+// and power-of-two scaling into FP16's range -- one scale per level for the
+// rank-k operators (set at compression) and one per level and apply for the
+// compressed source expansions (from their largest magnitude, on the
+// device) -- with FP32 accumulation, descaled into the FP64 locals.
+// ---------------------------------------------------------------------------
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split_f16(double v, _Float16 &hi, _Float16 &lo) {
+  const float f = float(v);
+  hi = _Float16(f);
+  lo = _Float16(f - float(hi));
+}
+
+// 2^(13 - e) for a largest magnitude in [2^e, 2^(e+1)): scaled values stay
+// below 2^14 (FP16 holds 65504)
+__host__ __device__ inline double pow2_scale(double mx) {
+  return mx > 0.0 ? ldexp(1.0, 13 - ilogb(mx)) : 1.0;
+}
+
+// *out = max(*out, max |x[0..n)|); x >= 0 doubles order like their bits
+__global__ __launch_bounds__(256) void k_absmax(const double *__restrict__ x, int64_t n,
+                                                unsigned long long *__restrict__ out) {
+  double m = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * 256)
+    m = fmax(m, fabs(x[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// C (column-major KP x KP per offset) * scale -> Ch, Cl (FP16
+// [offset][ks][row tile][lane][8]): the A fragment of every (k step, row
+// tile) of the 32x32x16 FP16 MFMA, rows past KP zero
+template <int KP>
+__global__ __launch_bounds__(256) void k_split_c16(const double *__restrict__ C, double scale,
+                                                   _Float16 *__restrict__ Ch,
+                                                   _Float16 *__restrict__ Cl) {
+  constexpr int RT = (KP + 31) / 32, KS = KP / 16;
+  const int64_t total = int64_t(kOffsets) * KS * RT * 512;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    const int j = int(t % 8);
+    const int lane = int((t / 8) % 64);
+    const int rt = int((t / 512) % RT);
+    const int ks = int((t / (512 * RT)) % KS);
+    const int64_t o = t / (int64_t(512) * RT * KS);
+    const int i = rt * 32 + lane % 32, k = ks * 16 + 8 * (lane / 32) + j;
+    _Float16 h = 0, l = 0;
+    if (i < KP) split_f16(C[o * KP * KP + i + int64_t(KP) * k] * scale, h, l);
+    Ch[t] = h;
+    Cl[t] = l;
+  }
+}
+
+// compressed expansions Wc ([node][KP][16]) of the nodes [b, b + cnt),
+// scaled by pow2_scale(*wmax) -> Wh, Wl (FP16 [node][ks][rhs][half][8]): the
+// B fragment of one source and k step is 512 contiguous bytes
+template <int KP>
+__global__ __launch_bounds__(256) void k_split_w16(const double *__restrict__ Wc, int64_t b,
+                                                   int64_t cnt,
+                                                   const unsigned long long *__restrict__ wmax,
+                                                   _Float16 *__restrict__ Wh,
+                                                   _Float16 *__restrict__ Wl) {
+  constexpr int KS = KP / 16;
+  const double scale = pow2_scale(__longlong_as_double((long long)*wmax));
+  const int64_t total = cnt * KS * 256;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    const int j = int(t % 8);
+    const int hf = int((t / 8) % 2);
+    const int r = int((t / 16) % kRhs);
+    const int ks = int((t / 256) % KS);
+    const int64_t node = b + t / (int64_t(256) * KS);
+    const int k = ks * 16 + 8 * hf + j;
+    _Float16 h, l;
+    split_f16(Wc[(node * KP + k) * kRhs + r] * scale, h, l);
+    const int64_t e = node * KS * 256 + (t % (int64_t(256) * KS));
+    Wh[e] = h;
+    Wl[e] = l;
+  }
+}
+
+// One block of 2 x RT wavefronts per two sibling tiles: wave w owns row tile
+// w % RT of the compressed output and target pairs 2 (w / RT), 2 (w / RT) + 1
+// of both tiles (a pair of targets x 16 right-hand sides = one 32-column
+// MFMA tile).  Per box offset both operands are read straight from L2 in the
+// fragment layout; pairs with no source at the offset are skipped.
+template <int KP>
+__global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
+    const int32_t *__restrict__ tile_parent, int ntiles, const int32_t *__restrict__ child0,
+    const int32_t *__restrict__ coord, const int32_t *__restrict__ map, int side,
+    const _Float16 *__restrict__ Ch, const _Float16 *__restrict__ Cl,
+    const _Float16 *__restrict__ Wh, const _Float16 *__restrict__ Wl,
+    const unsigned long long *__restrict__ wmax, double c_inv, double *__restrict__ Gc) {
+  constexpr int RT = (KP + 31) / 32, KS = KP / 16;
+  __shared__ int src[2][8];
+  const int tile0 = 2 * xcd_block((ntiles + 1) / 2);
+  if (tile0 >= ntiles) return;
+  const int ntl = ntiles - tile0 < 2 ? ntiles - tile0 : 2;
+  int first[2], pc[2][3];
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt) {
+    const int parent = tile_parent[tile0 + (tt < ntl ? tt : 0)];
+    first[tt] = child0[parent];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pc[tt][d] = coord[parent * 3 + d];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rt = wave % RT, p0 = 2 * (wave / RT);
+  const int half = lane >> 5, col = lane & 31;
+  const int cq = col >> 4, r = col & 15;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[tt][q][v] = 0.0f;
+  const f16x8 *ch = reinterpret_cast<const f16x8 *>(Ch);
+  const f16x8 *cl = reinterpret_cast<const f16x8 *>(Cl);
+  const f16x8 *wh = reinterpret_cast<const f16x8 *>(Wh);
+  const f16x8 *wl = reinterpret_cast<const f16x8 *>(Wl);
+  for (int o = 0; o < kOffsets; ++o) {
+    const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
+    if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      const int tt = threadIdx.x >> 3, chd = threadIdx.x & 7;
+      const int qx = ((chd >> 2) & 1) + ox, qy = ((chd >> 1) & 1) + oy, qz = (chd & 1) + oz;
+      int sidx = -1;
+      if (tt < ntl && qx >= -2 && qx <= 3 && qy >= -2 && qy <= 3 && qz >= -2 && qz <= 3) {
+        const int sx = 2 * pc[tt][0] + qx, sy = 2 * pc[tt][1] + qy, sz = 2 * pc[tt][2] + qz;
+        if (sx >= 0 && sx < side && sy >= 0 && sy < side && sz >= 0 && sz < side)
+          sidx = map[(int64_t(sx) * side + sy) * side + sz];
+      }
+      src[tt][chd] = sidx;
+    }
+    __syncthreads();
+    bool act[2][2];
+    int sl[2][2];
+    bool any = false;
+    int svalid = 0;
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int p = p0 + q;
+        act[tt][q] = src[tt][2 * p] >= 0 || src[tt][2 * p + 1] >= 0;
+        sl[tt][q] = src[tt][2 * p + cq];
+        any |= act[tt][q];
+        svalid = src[tt][2 * p] >= 0 ? src[tt][2 * p] : svalid;
+        svalid = src[tt][2 * p + 1] >= 0 ? src[tt][2 * p + 1] : svalid;
+      }
+    if (!any) continue;
+    const int64_t obase = int64_t(o) * KS * RT * 64;
+    for (int ks = 0; ks < KS; ++ks) {
+      const f16x8 ah = ch[obase + (int64_t(ks) * RT + rt) * 64 + lane];
+      const f16x8 al = cl[obase + (int64_t(ks) * RT + rt) * 64 + lane];
+      f16x8 bh[2][2], bl[2][2];
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int sv = sl[tt][q] >= 0 ? sl[tt][q] : svalid;
+          const int64_t e = ((int64_t(sv) * KS + ks) * kRhs + r) * 2 + half;
+          bh[tt][q] = wh[e];
+          bl[tt][q] = wl[e];
+        }
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          if (!act[tt][q]) continue;
+          if (sl[tt][q] < 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bh[tt][q][j] = bl[tt][q][j] = _Float16(0);
+          }
+          acc[tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[tt][q], acc[tt][q], 0, 0, 0);
+          acc[tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[tt][q], acc[tt][q], 0, 0, 0);
+          acc[tt][q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[tt][q], acc[tt][q], 0, 0, 0);
+        }
+    }
+  }
+  // D layout: register v of lane l holds row 8 (v / 4) + 4 (l / 32) + v % 4,
+  // column l % 32 of the 32 x 32 tile
+  const double inv = c_inv / pow2_scale(__longlong_as_double((long long)*wmax));
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt) {
+    if (tt >= ntl) continue;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int t = first[tt] + 2 * (p0 + q) + cq;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int row = rt * 32 + 8 * (v / 4) + 4 * half + (v % 4);
+        if (row < KP) Gc[(int64_t(t) * KP + row) * kRhs + r] += double(acc[tt][q][v]) * inv;
+      }
+    }
+  }
+}
+
 // L2P + P2P for one target leaf, 256 threads: thread t serves target particle
 // t % 64 of the current round; for L2P its quarter t / 64 of the 16
 // right-hand sides, for P2P every fourth source (partial sums of the four
@@ -1022,6 +1234,12 @@ struct abh_fmm {
   int comp_kp = 0;                 // padded rank (0: uncompressed)
   std::vector<int> comp_rank;      // rank per M2L level (2..nlevels)
   DevBuf Uf, UTf, Cf, Wc, Gc;
+  // the same operators split into FP16 hi / lo fragments for the MIXED
+  // apply (padded ranks up to 192): per level the operators' descale
+  // 1 / 2^e, the compressed expansions' split and their per-level maxima
+  bool c16 = false;
+  std::vector<double> c_inv;
+  DevBuf Ch16, Cl16, Wh16, Wl16, wmax;
 };
 
 namespace {
@@ -1155,6 +1373,21 @@ void m2l_compressed(int KP, const int32_t *par, int npar, const int32_t *child0,
   }
 }
 
+// the padded ranks the FP16 compressed M2L is instantiated for (up to 192:
+// above, a block of 2 x 7+ row-tile waves leaves too few registers)
+template <class F> bool dispatch_kp16(int KP, F &&fn) {
+  switch (KP) {
+#define ABH_KP16(V)                             \
+  case V:                                       \
+    fn(std::integral_constant<int, V>());       \
+    return true;
+    ABH_KP16(32) ABH_KP16(48) ABH_KP16(64) ABH_KP16(80) ABH_KP16(96) ABH_KP16(112)
+    ABH_KP16(128) ABH_KP16(144) ABH_KP16(160) ABH_KP16(176) ABH_KP16(192)
+#undef ABH_KP16
+    default: return false;
+  }
+}
+
 // Compression of the M2L operators of every level (the black-box FMM's SVD
 // compression, Fong & Darve 2009, in the form KernelH2::compress(tol) offers,
 // src/Kernels.h:523): the operators of one level satisfy K_{-o} = K_o^T (a
@@ -1169,6 +1402,8 @@ template <int N> void compress_impl(abh_fmm *f, double eps) {
   hipStream_t s = ctx->stream;
   f->comp_kp = 0;
   f->comp_rank.clear();
+  f->c16 = false;
+  f->c_inv.clear();
   if (eps <= 0.0) return;  // off
   const int nlev = f->nlevels - 1;
   if (nlev <= 0) return;
@@ -1233,6 +1468,19 @@ template <int N> void compress_impl(abh_fmm *f, double eps) {
   f->Cf.ensure(size_t(nlev) * kOffsets * frag_size(KP, KP) * 8);
   f->UTf.ensure(size_t(nlev) * frag_size(KP, NC) * 8);
   f->Uf.ensure(size_t(nlev) * frag_size(NC, KP) * 8);
+  const int64_t per16 = int64_t(kOffsets) * (KP / 16) * ((KP + 31) / 32) * 512;  // halfs
+  const bool c16 = dispatch_kp16(KP, [](auto) {});
+  DevBuf cmax;
+  if (c16) {
+    f->Ch16.ensure(size_t(nlev) * per16 * 2);
+    f->Cl16.ensure(size_t(nlev) * per16 * 2);
+    f->Wh16.ensure(size_t(f->n_nodes) * KP * kRhs * 2 + 64);
+    f->Wl16.ensure(size_t(f->n_nodes) * KP * kRhs * 2 + 64);
+    f->wmax.ensure(size_t(nlev) * 8 + 64);
+    cmax.ensure(size_t(nlev) * 8 + 64);
+    ABH_HIP(hipMemsetAsync(cmax.p, 0, size_t(nlev) * 8, s));
+  }
+  std::vector<unsigned long long> cmax_h(static_cast<size_t>(nlev), 0);
   for (int li = 0; li < nlev; ++li) {
     const double *Ul = U.as<double>() + int64_t(li) * NC * KP;
     // T_o = K_o U (K_o = X_o^T), C_o = U^T T_o, both column-major
@@ -1253,7 +1501,25 @@ template <int N> void compress_impl(abh_fmm *f, double eps) {
     k_pack_frag<<<1024, 256, 0, s>>>(Ul, 1, NC, KP, 0, 1, NC,
                                      f->Uf.as<double>() + int64_t(li) * frag_size(NC, KP));
     abh::check_launch();
+    if (c16) {  // the level's FP16 fragments, scaled by its largest entry
+      unsigned long long *cm = cmax.as<unsigned long long>() + li;
+      k_absmax<<<1024, 256, 0, s>>>(Cm.as<double>(), int64_t(kOffsets) * KP * KP, cm);
+      abh::check_launch();
+      ABH_HIP(hipMemcpyAsync(&cmax_h[size_t(li)], cm, 8, hipMemcpyDeviceToHost, s));
+      ABH_HIP(hipStreamSynchronize(s));
+      double mx;
+      std::memcpy(&mx, &cmax_h[size_t(li)], 8);
+      const double sc = pow2_scale(mx);
+      f->c_inv.push_back(1.0 / sc);
+      dispatch_kp16(KP, [&](auto V) {
+        k_split_c16<decltype(V)::value><<<8192, 256, 0, s>>>(
+            Cm.as<double>(), sc, f->Ch16.as<_Float16>() + int64_t(li) * per16,
+            f->Cl16.as<_Float16>() + int64_t(li) * per16);
+      });
+      abh::check_launch();
+    }
   }
+  f->c16 = c16;
   f->Wc.ensure(size_t(f->n_nodes) * KP * kRhs * 8);
   f->Gc.ensure(size_t(f->n_nodes) * KP * kRhs * 8);
   ABH_HIP(hipStreamSynchronize(s));  // U, T, Cm, gram are freed below
@@ -1321,6 +1587,26 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
         abh::check_launch();
       }
       ABH_HIP(hipMemsetAsync(f->Gc.p, 0, size_t(nn) * KP * kRhs * 8, s));
+      if (mixed && f->c16) {
+        // per level the largest compressed expansion entry, then the FP16
+        // split of every level's expansions at that scale
+        const int nlev = f->nlevels - 1;
+        unsigned long long *wm = f->wmax.as<unsigned long long>();
+        ABH_HIP(hipMemsetAsync(wm, 0, size_t(nlev) * 8, s));
+        for (int l = 2; l <= f->nlevels; ++l) {
+          const int64_t b0 = f->tree.level_begin[size_t(l) - 1];
+          const int64_t cnt = f->tree.level_begin[size_t(l)] - b0;
+          k_absmax<<<1024, 256, 0, s>>>(f->Wc.as<double>() + b0 * KP * kRhs, cnt * KP * kRhs,
+                                        wm + (l - 2));
+          abh::check_launch();
+          dispatch_kp16(KP, [&](auto V) {
+            k_split_w16<decltype(V)::value><<<4096, 256, 0, s>>>(
+                f->Wc.as<double>(), b0, cnt, wm + (l - 2), f->Wh16.as<_Float16>(),
+                f->Wl16.as<_Float16>());
+          });
+          abh::check_launch();
+        }
+      }
     }
     // downward: level l receives L2L from level l-1 and its M2L
     for (int l = 2; l <= f->nlevels; ++l) {
@@ -1331,10 +1617,26 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
       abh::check_launch();
       if (KP > 0) {
         // C_o W^ into G^, then G += U G^
-        m2l_compressed(KP, par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(),
-                       f->coord.as<int32_t>(), f->maps.as<int32_t>() + f->map_off[l], 1 << l,
-                       f->Cf.as<double>() + int64_t(l - 2) * kOffsets * frag_size(KP, KP),
-                       f->Wc.as<double>(), f->Gc.as<double>(), s);
+        if (mixed && f->c16) {
+          const int64_t per16 = int64_t(kOffsets) * (KP / 16) * ((KP + 31) / 32) * 512;
+          dispatch_kp16(KP, [&](auto V) {
+            constexpr int KPV = decltype(V)::value;
+            k_m2l_mfma16c<KPV><<<xcd_grid((npar(l - 1) + 1) / 2), 128 * ((KPV + 31) / 32), 0,
+                                 s>>>(
+                par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(),
+                f->coord.as<int32_t>(), f->maps.as<int32_t>() + f->map_off[l], 1 << l,
+                f->Ch16.as<_Float16>() + int64_t(l - 2) * per16,
+                f->Cl16.as<_Float16>() + int64_t(l - 2) * per16, f->Wh16.as<_Float16>(),
+                f->Wl16.as<_Float16>(), f->wmax.as<unsigned long long>() + (l - 2),
+                f->c_inv[size_t(l) - 2], f->Gc.as<double>());
+          });
+        } else {
+          m2l_compressed(KP, par + f->tree.par_off[l - 1], npar(l - 1),
+                         f->tree.nodes.as<int32_t>(), f->coord.as<int32_t>(),
+                         f->maps.as<int32_t>() + f->map_off[l], 1 << l,
+                         f->Cf.as<double>() + int64_t(l - 2) * kOffsets * frag_size(KP, KP),
+                         f->Wc.as<double>(), f->Gc.as<double>(), s);
+        }
         abh::check_launch();
         k_box_op64<<<xcd_grid(npar(l - 1)), 512, 0, s>>>(
             par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(), NC, KP,
