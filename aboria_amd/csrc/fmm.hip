@@ -1199,6 +1199,237 @@ __global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_n
   }
 }
 
+// L2P + P2P for one target leaf on the FP64 matrix cores.  Per chunk of up
+// to 64 targets (four 16-row blocks): Y = S G_leaf, with S the targets'
+// Chebyshev interpolation values formed in the A fragments from their
+// per-dimension values in LDS, plus K X over the concatenated sources of the
+// leaf's P2P list, with one kernel value K(x_t, y_s) formed per lane and k
+// step.  The four waves take every fourth k step of both products; their
+// partial sums are added in wave order.
+// P2M for one leaf on the FP64 matrix cores: W = S^T X, A = S^T (nodes x
+// particles, the Chebyshev interpolation values formed in the fragments from
+// per-dimension values in LDS), B = X (particles x right-hand sides); wave w
+// owns the node row blocks w, w + 4, ...
+template <int N>
+__global__ __launch_bounds__(256) void k_p2m_mfma(const int32_t *__restrict__ leaf_nodes,
+                                                  int nleaf, const int2 *__restrict__ leaf_range,
+                                                  const double *__restrict__ box,
+                                                  const double *__restrict__ pos,
+                                                  const double *__restrict__ xs,
+                                                  double *__restrict__ W, FmmConsts c) {
+  constexpr int NC = Cheb<N>::NC;
+  constexpr int RB = (NC + 15) / 16, RPW = (RB + 3) / 4;
+  __shared__ double S[64][3][N];
+  __shared__ double Q[64][kRhs];
+  const int li = blockIdx.x;
+  if (li >= nleaf) return;
+  const int node = leaf_nodes[li];
+  const int2 rg = leaf_range[li];
+  double blo[3], bhi[3];
+  for (int d = 0; d < 3; ++d) {
+    blo[d] = box[int64_t(node) * 6 + d];
+    bhi[d] = box[int64_t(node) * 6 + 3 + d];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kl = lane >> 4;
+  f64x4 acc[RPW];
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
+    const int np = min(64, rg.y - p0);
+    __syncthreads();
+    if (int(threadIdx.x) < np) {
+      const int p = p0 + threadIdx.x;
+      for (int d = 0; d < 3; ++d) {
+        const double span = bhi[d] - blo[d];
+        const double x = span > 0.0 ? (2 * pos[int64_t(p) * 3 + d] - blo[d] - bhi[d]) / span : 0.0;
+        for (int m = 0; m < N; ++m) S[threadIdx.x][d][m] = sn_dev<N>(c, x, m);
+      }
+    }
+    for (int t = threadIdx.x; t < np * kRhs; t += 256)
+      Q[t / kRhs][t % kRhs] = xs[int64_t(p0 + t / kRhs) * kRhs + t % kRhs];
+    __syncthreads();
+    const int ksn = (np + 3) / 4;
+    for (int ks = 0; ks < ksn; ++ks) {
+      const int p = ks * 4 + kl;
+      const bool pok = p < np;
+      const int pc = pok ? p : 0;
+      const double b = pok ? Q[pc][row] : 0.0;
+#pragma unroll
+      for (int i = 0; i < RPW; ++i) {
+        const int rb = wave + 4 * i;
+        if (rb >= RB) break;
+        const int j = rb * 16 + row;
+        const int jx = j / (N * N), jy = (j / N) % N, jz = j % N;
+        const double a = (pok && j < NC) ? (S[pc][0][jx] * S[pc][1][jy]) * S[pc][2][jz] : 0.0;
+        acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[i], 0, 0, 0);
+      }
+    }
+  }
+  // D layout: register v of lane l holds row l / 16 + 4 v, column l % 16
+#pragma unroll
+  for (int i = 0; i < RPW; ++i) {
+    const int rb = wave + 4 * i;
+    if (rb >= RB) break;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int j = rb * 16 + kl + 4 * v;
+      if (j < NC) W[(int64_t(node) * NC + j) * kRhs + row] = acc[i][v];
+    }
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_leaf_mfma(const int32_t *__restrict__ leaf_nodes,
+                                                   int nleaf, const int2 *__restrict__ leaf_range,
+                                                   const int32_t *__restrict__ p2p_ptr,
+                                                   const int32_t *__restrict__ p2p_src,
+                                                   const double *__restrict__ box,
+                                                   const double *__restrict__ pos,
+                                                   const double *__restrict__ xs,
+                                                   const double *__restrict__ G,
+                                                   double *__restrict__ ys, KernelDesc k,
+                                                   FmmConsts c) {
+  constexpr int NC = Cheb<N>::NC;
+  constexpr int KSL = (NC + 3) / 4;  // k steps of the L2P product
+  constexpr int SRC = 256;           // sources staged per round
+  constexpr int ECH = 64;            // P2P list entries per chunk
+  // one LDS region: the targets' per-dimension Chebyshev values, the staged
+  // sources' positions and indices, then the four waves' partial sums
+  constexpr int kSv = 64 * 3 * N, kPs = SRC * 3;
+  static_assert(kSv + kPs + SRC / 2 <= 4 * 64 * kRhs, "staging fits the reduction buffer");
+  __shared__ __align__(16) double smem[4 * 64 * kRhs];
+  __shared__ int ebeg[ECH], epre[ECH + 1];
+  double(*Sv)[3][N] = reinterpret_cast<double(*)[3][N]>(smem);
+  double(*Ps)[3] = reinterpret_cast<double(*)[3]>(smem + kSv);
+  int *Pi = reinterpret_cast<int *>(smem + kSv + kPs);
+  double(*red)[64][kRhs] = reinterpret_cast<double(*)[64][kRhs]>(smem);
+  const int li = blockIdx.x;
+  if (li >= nleaf) return;
+  const int node = leaf_nodes[li];
+  const int2 rg = leaf_range[li];
+  double blo[3], bhi[3];
+  for (int d = 0; d < 3; ++d) {
+    blo[d] = box[int64_t(node) * 6 + d];
+    bhi[d] = box[int64_t(node) * 6 + 3 + d];
+  }
+  const int e0 = p2p_ptr[li], e1 = p2p_ptr[li + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kl = lane >> 4;
+  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
+    const int T = rg.y - p0 < 64 ? rg.y - p0 : 64;
+    const int RBk = (T + 15) / 16;
+    __syncthreads();
+    if (int(threadIdx.x) < T) {
+      const int t = threadIdx.x;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double x = pos[int64_t(p0 + t) * 3 + d];
+        const double span = bhi[d] - blo[d];
+        const double u = span > 0.0 ? (2 * x - blo[d] - bhi[d]) / span : 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) Sv[t][d][m] = sn_dev<N>(c, u, m);
+      }
+    }
+    // this lane's target rows (one per row block) in registers
+    double tx[4][3];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const int t = rb * 16 + row;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) tx[rb][d] = t < T ? pos[int64_t(p0 + t) * 3 + d] : 0.0;
+    }
+    __syncthreads();
+    f64x4 acc[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) acc[rb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    // L2P: A = S (targets x nodes), B = G (nodes x right-hand sides)
+    for (int ks = wave; ks < KSL; ks += 4) {
+      const int j = ks * 4 + kl;
+      const bool jok = j < NC;
+      const int jx = j / (N * N), jy = (j / N) % N, jz = j % N;
+      const double b = jok ? G[(int64_t(node) * NC + j) * kRhs + row] : 0.0;
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb) {
+        if (rb >= RBk) break;
+        const int t = rb * 16 + row;
+        const double a = (jok && t < T) ? (Sv[t][0][jx] * Sv[t][1][jy]) * Sv[t][2][jz] : 0.0;
+        acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rb], 0, 0, 0);
+      }
+    }
+    // P2P: A = K (targets x staged sources), B = X (sources x right-hand sides)
+    for (int ec = e0; ec < e1; ec += ECH) {
+      const int ne = e1 - ec < ECH ? e1 - ec : ECH;
+      __syncthreads();
+      if (int(threadIdx.x) < ne) {
+        const int2 sr = leaf_range[p2p_src[ec + threadIdx.x]];
+        ebeg[threadIdx.x] = sr.x;
+        epre[threadIdx.x + 1] = sr.y - sr.x;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        epre[0] = 0;
+        for (int i = 0; i < ne; ++i) epre[i + 1] += epre[i];
+      }
+      __syncthreads();
+      const int total = epre[ne];
+      for (int b0 = 0; b0 < total; b0 += SRC) {
+        const int ns = total - b0 < SRC ? total - b0 : SRC;
+        __syncthreads();
+        if (int(threadIdx.x) < ns) {
+          const int gsrc = b0 + threadIdx.x;
+          int lo = 0, hi = ne - 1;  // entry i with epre[i] <= gsrc < epre[i + 1]
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (epre[mid] <= gsrc) lo = mid;
+            else hi = mid - 1;
+          }
+          const int q = ebeg[lo] + (gsrc - epre[lo]);
+          Pi[threadIdx.x] = q;
+          for (int d = 0; d < 3; ++d) Ps[threadIdx.x][d] = pos[int64_t(q) * 3 + d];
+        }
+        __syncthreads();
+        const int nsteps = (ns + 3) / 4;
+        for (int st = wave; st < nsteps; st += 4) {
+          const int sj = st * 4 + kl;
+          const bool sok = sj < ns;
+          const int sjc = sok ? sj : 0;
+          const double b = sok ? xs[int64_t(Pi[sjc]) * kRhs + row] : 0.0;
+          const double y0 = Ps[sjc][0], y1 = Ps[sjc][1], y2 = Ps[sjc][2];
+#pragma unroll
+          for (int rb = 0; rb < 4; ++rb) {
+            if (rb >= RBk) break;
+            const int t = rb * 16 + row;
+            double a = 0.0;
+            if (sok && t < T) {
+              const double d0 = y0 - tx[rb][0], d1 = y1 - tx[rb][1], d2 = y2 - tx[rb][2];
+              double s2 = 0.0;
+              s2 = s2 + d0 * d0;
+              s2 = s2 + d1 * d1;
+              s2 = s2 + d2 * d2;
+              a = abh::phi64(k, s2);
+            }
+            acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rb], 0, 0, 0);
+          }
+        }
+      }
+    }
+    // D layout: register v of lane l holds row l / 16 + 4 v, column l % 16
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      if (rb >= RBk) break;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[wave][rb * 16 + kl + 4 * v][row] = acc[rb][v];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < T * kRhs; e += 256) {
+      const int t = e / kRhs, r = e % kRhs;
+      ys[int64_t(p0 + t) * kRhs + r] = ((red[0][t][r] + red[1][t][r]) + red[2][t][r]) + red[3][t][r];
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1526,6 +1757,8 @@ template <int N> void compress_impl(abh_fmm *f, double eps) {
   f->comp_kp = KP;
 }
 
+constexpr bool kMixedLeafMfma = true;
+
 template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
                                  int64_t ldy, abh_precision prec) {
   constexpr int NC = Cheb<N>::NC;
@@ -1543,7 +1776,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     ABH_HIP(hipMemsetAsync(f->W.p, 0, size_t(nn) * NC * kRhs * 8, s));
     ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
     if (f->nleaf > 0) {
-      k_p2m<N><<<f->nleaf, 256, 0, s>>>(f->tree.leaf_node.as<int32_t>(), f->nleaf,
+      k_p2m_mfma<N><<<f->nleaf, 256, 0, s>>>(f->tree.leaf_node.as<int32_t>(), f->nleaf,
                                         f->tree.leaves.as<int2>(), f->box.as<double>(),
                                         f->tree.pos.as<double>(), f->xs.as<double>(),
                                         f->W.as<double>(), f->consts);
@@ -1669,7 +1902,13 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     }
     if (f->nleaf > 0) {
       if (prec == ABH_FP64)
-        k_leaf<N, false><<<f->nleaf, 256, 0, s>>>(
+        k_leaf_mfma<N><<<f->nleaf, 256, 0, s>>>(
+            f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
+            f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
+            f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
+            f->kern, f->consts);
+      else if (kMixedLeafMfma)
+        k_leaf_mfma<N><<<f->nleaf, 256, 0, s>>>(
             f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
             f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
             f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
