@@ -154,169 +154,122 @@ __global__ __launch_bounds__(256) void k_scatter_rhs(const int32_t *__restrict__
   y[int64_t(r) * ldy + order[p]] += ys[p * kRhs + r];
 }
 
-// P2M: W[node][j][r] = sum_p R_j(x_p; leaf box) q_p[r]
-template <int N>
-__global__ __launch_bounds__(256) void k_p2m(const int32_t *__restrict__ leaf_nodes, int nleaf,
-                                             const int2 *__restrict__ leaf_range,
-                                             const double *__restrict__ box,  // 6 per node
-                                             const double *__restrict__ pos,
-                                             const double *__restrict__ xs,
-                                             double *__restrict__ W, FmmConsts c) {
-  constexpr int NC = Cheb<N>::NC;
-  __shared__ double S[64][3][N];
-  __shared__ double Q[64][kRhs];
-  const int li = blockIdx.x;
-  if (li >= nleaf) return;
-  const int node = leaf_nodes[li];
-  const int2 rg = leaf_range[li];
-  double acc[(NC * kRhs + 255) / 256];
-#pragma unroll
-  for (int u = 0; u < (NC * kRhs + 255) / 256; ++u) acc[u] = 0.0;
-  double blo[3], bhi[3];
-  for (int d = 0; d < 3; ++d) {
-    blo[d] = box[int64_t(node) * 6 + d];
-    bhi[d] = box[int64_t(node) * 6 + 3 + d];
-  }
-  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
-    const int np = min(64, rg.y - p0);
-    __syncthreads();
-    if (int(threadIdx.x) < np) {
-      const int p = p0 + threadIdx.x;
-      for (int d = 0; d < 3; ++d) {
-        const double span = bhi[d] - blo[d];
-        const double x = span > 0.0 ? (2 * pos[int64_t(p) * 3 + d] - blo[d] - bhi[d]) / span : 0.0;
-        for (int m = 0; m < N; ++m) S[threadIdx.x][d][m] = sn_dev<N>(c, x, m);
-      }
-    }
-    for (int t = threadIdx.x; t < np * kRhs; t += 256)
-      Q[t / kRhs][t % kRhs] = xs[int64_t(p0 + t / kRhs) * kRhs + t % kRhs];
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < (NC * kRhs + 255) / 256; ++u) {
-      const int o = u * 256 + threadIdx.x;
-      if (o < NC * kRhs) {
-        const int j = o / kRhs, r = o % kRhs;
-        const int jx = j / (N * N), jy = (j / N) % N, jz = j % N;
-        double a = acc[u];
-        for (int p = 0; p < np; ++p) a += ((S[p][0][jx] * S[p][1][jy]) * S[p][2][jz]) * Q[p][r];
-        acc[u] = a;
-      }
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < (NC * kRhs + 255) / 256; ++u) {
-    const int o = u * 256 + threadIdx.x;
-    if (o < NC * kRhs) W[int64_t(node) * NC * kRhs + o] = acc[u];
-  }
-}
 
-// Separable child <-> parent transfer.  M2M: W_p[i] += sum_j prod_d
-// s1[h_d][i_d][j_d] W_c[j]; L2L (transpose): G_c[i] += sum_j prod_d
-// s1[h_d][j_d][i_d] G_p[j].
+// Separable child <-> parent transfer with every right-hand side at once:
+// one block per parent, the three 1-D contractions of each child done on
+// lines of the N x N x N x 16 coefficient tensor held by one thread each --
+// z lines from global memory (M2M: the child; L2L: the parent, loaded once),
+// y lines in LDS in place, x lines from LDS into the thread's registers.
+// M2M: W_p[i] = sum_c sum_j prod_d s1[h_d][i_d][j_d] W_c[j], the children
+// accumulated in registers, the parent written once; L2L (transpose):
+// G_c[i] += sum_j prod_d s1[h_d][j_d][i_d] G_p[j], added into each child's
+// local expansion.  Each 1-D sum runs over k in order.
 template <int N, bool UP>
-__global__ __launch_bounds__(256) void k_transfer(const int32_t *__restrict__ parents, int npar,
-                                                  const int32_t *__restrict__ child0,
-                                                  double *__restrict__ E, FmmConsts c) {
+__global__ __launch_bounds__(256) void k_transfer_lines(const int32_t *__restrict__ parents,
+                                                        int npar,
+                                                        const int32_t *__restrict__ child0,
+                                                        double *__restrict__ E, FmmConsts c) {
   constexpr int NC = Cheb<N>::NC;
-  constexpr int R = 8;  // rhs per LDS pass
-  __shared__ double A[NC * R], B[NC * R];
+  constexpr int LINES = N * N * kRhs;         // lines per direction
+  constexpr int LPT = (LINES + 255) / 256;    // lines per thread
+  __shared__ double L[NC * kRhs];
   const int pi = blockIdx.x;
   if (pi >= npar) return;
   const int parent = parents[pi];
   const int first = child0[parent];
-  for (int r0 = 0; r0 < kRhs; r0 += R) {
-    double accp[(NC * R + 255) / 256];
+  const int tid = threadIdx.x;
+  // line q of this thread: index u = tid + 256 q; (a, b, r) = (u / (N*16),
+  // (u / 16) % N, u % 16): z lines fix (x, y) = (a, b), y lines (x, z),
+  // x lines (y, z)
+  double acc[LPT][N];  // M2M: the parent's x lines
+  double gp[LPT][N];   // L2L: the parent's z lines
 #pragma unroll
-    for (int u = 0; u < (NC * R + 255) / 256; ++u) accp[u] = 0.0;
-    if (!UP) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < NC * R; t += 256)
-        A[t] = E[int64_t(parent) * NC * kRhs + (t / R) * kRhs + r0 + t % R];
+  for (int q = 0; q < LPT; ++q)
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[q][i] = 0.0;
+  if (!UP) {
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int u = tid + 256 * q;
+      const int a = u / (N * kRhs), b = (u / kRhs) % N, r = u % kRhs;
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        gp[q][k] = u < LINES ? E[(int64_t(parent) * NC + (a * N + b) * N + k) * kRhs + r] : 0.0;
     }
-    for (int ch = 0; ch < 8; ++ch) {
-      const int child = first + ch;
-      const int h[3] = {(ch >> 2) & 1, (ch >> 1) & 1, ch & 1};
-      __syncthreads();
-      if (UP) {
-        for (int t = threadIdx.x; t < NC * R; t += 256)
-          B[t] = E[int64_t(child) * NC * kRhs + (t / R) * kRhs + r0 + t % R];
-      } else {
-        for (int t = threadIdx.x; t < NC * R; t += 256) B[t] = A[t];
-      }
-      __syncthreads();
-      // three 1-D passes over z, y, x: src -> dst alternating B -> A' ...
-      // pass z: T1[jx][jy][iz] = sum_jz m_z(iz, jz) src[jx][jy][jz]
-      double tmp[(NC * R + 255) / 256];
+  }
+  for (int ch = 0; ch < 8; ++ch) {
+    const int child = first + ch;
+    const int h0 = (ch >> 2) & 1, h1 = (ch >> 1) & 1, h2 = ch & 1;
+    __syncthreads();  // the previous child's x pass has read L
+    // z pass: T1[x][y][iz] = sum_k m_z(iz, k) src[x][y][k]
 #pragma unroll
-      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
-        const int o = u * 256 + threadIdx.x;
-        tmp[u] = 0.0;
-        if (o < NC * R) {
-          const int e = o / R, r = o % R;
-          const int x = e / (N * N), y = (e / N) % N, z = e % N;
-          double a = 0.0;
-          for (int k = 0; k < N; ++k) {
-            const double m = UP ? c.s1[h[2]][z][k] : c.s1[h[2]][k][z];
-            a += m * B[((x * N + y) * N + k) * R + r];
-          }
-          tmp[u] = a;
-        }
-      }
-      __syncthreads();
+    for (int q = 0; q < LPT; ++q) {
+      const int u = tid + 256 * q;
+      if (u >= LINES) continue;
+      const int a = u / (N * kRhs), b = (u / kRhs) % N, r = u % kRhs;
+      double in[N];
 #pragma unroll
-      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
-        const int o = u * 256 + threadIdx.x;
-        if (o < NC * R) B[o] = tmp[u];
-      }
-      __syncthreads();
+      for (int k = 0; k < N; ++k)
+        in[k] = UP ? E[(int64_t(child) * NC + (a * N + b) * N + k) * kRhs + r] : gp[q][k];
 #pragma unroll
-      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
-        const int o = u * 256 + threadIdx.x;
-        tmp[u] = 0.0;
-        if (o < NC * R) {
-          const int e = o / R, r = o % R;
-          const int x = e / (N * N), y = (e / N) % N, z = e % N;
-          double a = 0.0;
-          for (int k = 0; k < N; ++k) {
-            const double m = UP ? c.s1[h[1]][y][k] : c.s1[h[1]][k][y];
-            a += m * B[((x * N + k) * N + z) * R + r];
-          }
-          tmp[u] = a;
-        }
-      }
-      __syncthreads();
+      for (int iz = 0; iz < N; ++iz) {
+        double s = 0.0;
 #pragma unroll
-      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
-        const int o = u * 256 + threadIdx.x;
-        if (o < NC * R) B[o] = tmp[u];
+        for (int k = 0; k < N; ++k) s += (UP ? c.s1[h2][iz][k] : c.s1[h2][k][iz]) * in[k];
+        L[((a * N + b) * N + iz) * kRhs + r] = s;
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    // y pass in place: T2[x][iy][z] = sum_k m_y(iy, k) T1[x][k][z]
 #pragma unroll
-      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
-        const int o = u * 256 + threadIdx.x;
-        if (o < NC * R) {
-          const int e = o / R, r = o % R;
-          const int x = e / (N * N), y = (e / N) % N, z = e % N;
-          double a = 0.0;
-          for (int k = 0; k < N; ++k) {
-            const double m = UP ? c.s1[h[0]][x][k] : c.s1[h[0]][k][x];
-            a += m * B[((k * N + y) * N + z) * R + r];
-          }
-          if (UP) {
-            accp[u] += a;
-          } else {
-            double *g = &E[int64_t(child) * NC * kRhs + e * kRhs + r0 + r];
-            *g += a;
-          }
+    for (int q = 0; q < LPT; ++q) {
+      const int u = tid + 256 * q;
+      if (u >= LINES) continue;
+      const int a = u / (N * kRhs), b = (u / kRhs) % N, r = u % kRhs;  // (x, z)
+      double in[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) in[k] = L[((a * N + k) * N + b) * kRhs + r];
+#pragma unroll
+      for (int iy = 0; iy < N; ++iy) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s += (UP ? c.s1[h1][iy][k] : c.s1[h1][k][iy]) * in[k];
+        L[((a * N + iy) * N + b) * kRhs + r] = s;
+      }
+    }
+    __syncthreads();
+    // x pass: out[ix][y][z] = sum_k m_x(ix, k) T2[k][y][z]
+#pragma unroll
+    for (int q = 0; q < LPT; ++q) {
+      const int u = tid + 256 * q;
+      if (u >= LINES) continue;
+      const int a = u / (N * kRhs), b = (u / kRhs) % N, r = u % kRhs;  // (y, z)
+      double in[N];
+#pragma unroll
+      for (int k = 0; k < N; ++k) in[k] = L[((k * N + a) * N + b) * kRhs + r];
+#pragma unroll
+      for (int ix = 0; ix < N; ++ix) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s += (UP ? c.s1[h0][ix][k] : c.s1[h0][k][ix]) * in[k];
+        if (UP) {
+          acc[q][ix] += s;
+        } else {
+          double *g = E + (int64_t(child) * NC + (ix * N + a) * N + b) * kRhs + r;
+          *g += s;
         }
       }
     }
-    if (UP) {
+  }
+  if (UP) {
 #pragma unroll
-      for (int u = 0; u < (NC * R + 255) / 256; ++u) {
-        const int o = u * 256 + threadIdx.x;
-        if (o < NC * R) E[int64_t(parent) * NC * kRhs + (o / R) * kRhs + r0 + o % R] = accp[u];
-      }
+    for (int q = 0; q < LPT; ++q) {
+      const int u = tid + 256 * q;
+      if (u >= LINES) continue;
+      const int a = u / (N * kRhs), b = (u / kRhs) % N, r = u % kRhs;
+#pragma unroll
+      for (int ix = 0; ix < N; ++ix)
+        E[(int64_t(parent) * NC + (ix * N + a) * N + b) * kRhs + r] = acc[q][ix];
     }
   }
 }
@@ -1004,208 +957,6 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
   }
 }
 
-// L2P + P2P for one target leaf, 256 threads: thread t serves target particle
-// t % 64 of the current round; for L2P its quarter t / 64 of the 16
-// right-hand sides, for P2P every fourth source (partial sums of the four
-// quarters are combined through LDS).
-template <int N, bool FP32>
-__global__ __launch_bounds__(256) void k_leaf(const int32_t *__restrict__ leaf_nodes, int nleaf,
-                                              const int2 *__restrict__ leaf_range,
-                                              const int32_t *__restrict__ p2p_ptr,
-                                              const int32_t *__restrict__ p2p_src,
-                                              const double *__restrict__ box,
-                                              const double *__restrict__ pos,
-                                              const double *__restrict__ xs,
-                                              const double *__restrict__ G,
-                                              double *__restrict__ ys, KernelDesc k, FmmConsts c) {
-  constexpr int NC = Cheb<N>::NC;
-  constexpr int SRC = 256;   // sources staged per round
-  constexpr int ECH = 64;    // P2P list entries per chunk
-  // the leaf's local expansion is staged in LDS whole up to 256 nodes
-  // (order 6); larger ones (64 KB at order 8) would leave one block per CU,
-  // so they pass through the same 256-node buffer in slabs of JXC planes of
-  // constant first index
-  constexpr bool kStageG = NC <= 256;
-  constexpr int JXC = kStageG ? N : 256 / (N * N);
-  // leaf local expansion (staged orders), then the [parts][slots][kRhs]
-  // reduction of the P2P parts
-  static_assert(!kStageG || NC <= 256, "staged expansions fit the reduction buffer");
-  __shared__ double Gs[256 * kRhs];
-  __shared__ double Ps[SRC][3];
-  __shared__ __align__(16) double Qs[SRC][kRhs];
-  __shared__ int ebeg[ECH], epre[ECH + 1];
-  const int li = blockIdx.x;
-  if (li >= nleaf) return;
-  const int node = leaf_nodes[li];
-  const int2 rg = leaf_range[li];
-  double blo[3], bhi[3];
-  for (int d = 0; d < 3; ++d) {
-    blo[d] = box[int64_t(node) * 6 + d];
-    bhi[d] = box[int64_t(node) * 6 + 3 + d];
-  }
-  const int e0 = p2p_ptr[li], e1 = p2p_ptr[li + 1];
-  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
-    // targets of this chunk on TS lane slots, the block's 256 threads split
-    // into 256 / TS parts that share out the sources (and, for the local
-    // expansion, the right-hand sides)
-    const int T = rg.y - p0 < 64 ? rg.y - p0 : 64;
-    const int TS = T <= 16 ? 16 : (T <= 32 ? 32 : 64);
-    const int parts = 256 / TS, rp = kRhs / parts;
-    const int tg = threadIdx.x % TS, part = threadIdx.x / TS;
-    const int p = p0 + tg;
-    const bool act = tg < T;
-    double x[3] = {0.0, 0.0, 0.0};
-    if (act)
-      for (int d = 0; d < 3; ++d) x[d] = pos[int64_t(p) * 3 + d];
-    // L2P for right-hand sides rp * part .. rp * part + rp - 1
-    double l2p[4] = {0.0, 0.0, 0.0, 0.0};
-    double S[3][N], u0 = 0.0;
-    if (act) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double span = bhi[d] - blo[d];
-        const double u = span > 0.0 ? (2 * x[d] - blo[d] - bhi[d]) / span : 0.0;
-        if (d == 0) u0 = u;
-#pragma unroll
-        for (int m = 0; m < N; ++m) S[d][m] = sn_dev<N>(c, u, m);
-      }
-    }
-    if (kStageG) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < NC * kRhs; t += 256) Gs[t] = G[int64_t(node) * NC * kRhs + t];
-      __syncthreads();
-    }
-    // S[1], S[2] are indexed by the unrolled jy, jz only; the first
-    // dimension's value is recomputed per jx (one Clenshaw sum), so the jx
-    // loop need not unroll for S to stay in registers
-    for (int jx = 0; jx < N; ++jx) {
-      if (!kStageG && jx % JXC == 0) {
-        const int nj = (jx + JXC < N ? JXC : N - jx) * N * N;
-        __syncthreads();
-        for (int t = threadIdx.x; t < nj * kRhs; t += 256)
-          Gs[t] = G[(int64_t(node) * NC + int64_t(jx) * N * N) * kRhs + t];
-        __syncthreads();
-      }
-      if (act) {
-        const double sx = sn_dev<N>(c, u0, jx);
-#pragma unroll
-        for (int jy = 0; jy < N; ++jy) {
-          const double sxy = sx * S[1][jy];
-#pragma unroll
-          for (int jz = 0; jz < N; ++jz) {
-            const double R = sxy * S[2][jz];
-            const int j = ((jx % JXC) * N + jy) * N + jz;
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (r < rp) l2p[r] += R * Gs[j * kRhs + rp * part + r];
-          }
-        }
-      }
-    }
-    // P2P over the concatenated sources of the leaf's P2P list: rounds of
-    // 256 staged sources, this thread taking those congruent to its part
-    double acc[kRhs];
-    float a32[kRhs], c32[kRhs];
-#pragma unroll
-    for (int r = 0; r < kRhs; ++r) {
-      acc[r] = 0.0;
-      a32[r] = 0.0f;
-      c32[r] = 0.0f;
-    }
-    for (int ec = e0; ec < e1; ec += ECH) {
-      const int ne = e1 - ec < ECH ? e1 - ec : ECH;
-      __syncthreads();
-      if (int(threadIdx.x) < ne) {
-        const int2 sr = leaf_range[p2p_src[ec + threadIdx.x]];
-        ebeg[threadIdx.x] = sr.x;
-        epre[threadIdx.x + 1] = sr.y - sr.x;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        epre[0] = 0;
-        for (int i = 0; i < ne; ++i) epre[i + 1] += epre[i];
-      }
-      __syncthreads();
-      const int total = epre[ne];
-      for (int b0 = 0; b0 < total; b0 += SRC) {
-        const int ns = total - b0 < SRC ? total - b0 : SRC;
-        __syncthreads();
-        if (int(threadIdx.x) < ns) {
-          const int gsrc = b0 + threadIdx.x;
-          int lo = 0, hi = ne - 1;  // entry i with epre[i] <= gsrc < epre[i + 1]
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (epre[mid] <= gsrc) lo = mid;
-            else hi = mid - 1;
-          }
-          const int q = ebeg[lo] + (gsrc - epre[lo]);
-          for (int d = 0; d < 3; ++d) Ps[threadIdx.x][d] = pos[int64_t(q) * 3 + d];
-          const double2 *qx = reinterpret_cast<const double2 *>(xs + int64_t(q) * kRhs);
-          if (FP32) {
-            // FP32 P2P: the right-hand sides staged as FP32, rounded once
-            float2 *qf = reinterpret_cast<float2 *>(&Qs[0][0]) + threadIdx.x * (kRhs / 2);
-#pragma unroll
-            for (int r = 0; r < kRhs / 2; ++r) {
-              const double2 v = qx[r];
-              qf[r] = make_float2(float(v.x), float(v.y));
-            }
-          } else {
-            double2 *qd = reinterpret_cast<double2 *>(&Qs[threadIdx.x][0]);
-#pragma unroll
-            for (int r = 0; r < kRhs / 2; ++r) qd[r] = qx[r];
-          }
-        }
-        __syncthreads();
-        if (act) {
-          for (int j = part; j < ns; j += parts) {
-            double s2 = 0.0;
-            for (int d = 0; d < 3; ++d) {
-              const double dx = Ps[j][d] - x[d];
-              s2 = s2 + dx * dx;
-            }
-            if (FP32) {
-              const float f = abh::phi32(k, float(s2));
-#pragma unroll
-              for (int r = 0; r < kRhs; ++r) {
-                const float qv = reinterpret_cast<const float *>(&Qs[0][0])[j * kRhs + r];
-                abh::kb_add(a32[r], c32[r], f * qv);
-              }
-            } else {
-              const double f = abh::phi64(k, s2);
-#pragma unroll
-              for (int r = 0; r < kRhs; ++r) acc[r] += f * Qs[j][r];
-            }
-          }
-        }
-      }
-    }
-    // combine the partial P2P sums of the parts (Gs is free now)
-    __syncthreads();
-    double *red = Gs;  // [parts][TS][kRhs]
-#pragma unroll
-    for (int r = 0; r < kRhs; ++r)
-      red[(part * TS + tg) * kRhs + r] = acc[r] + (FP32 ? double(a32[r]) + double(c32[r]) : 0.0);
-    __syncthreads();
-    if (act) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (r >= rp) break;
-        const int rr = rp * part + r;
-        double v = l2p[r];
-        for (int q = 0; q < parts; ++q) v += red[(q * TS + tg) * kRhs + rr];
-        ys[int64_t(p) * kRhs + rr] = v;
-      }
-    }
-  }
-}
-
-// L2P + P2P for one target leaf on the FP64 matrix cores.  Per chunk of up
-// to 64 targets (four 16-row blocks): Y = S G_leaf, with S the targets'
-// Chebyshev interpolation values formed in the A fragments from their
-// per-dimension values in LDS, plus K X over the concatenated sources of the
-// leaf's P2P list, with one kernel value K(x_t, y_s) formed per lane and k
-// step.  The four waves take every fourth k step of both products; their
-// partial sums are added in wave order.
 // P2M for one leaf on the FP64 matrix cores: W = S^T X, A = S^T (nodes x
 // particles, the Chebyshev interpolation values formed in the fragments from
 // per-dimension values in LDS), B = X (particles x right-hand sides); wave w
@@ -1757,8 +1508,6 @@ template <int N> void compress_impl(abh_fmm *f, double eps) {
   f->comp_kp = KP;
 }
 
-constexpr bool kMixedLeafMfma = true;
-
 template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
                                  int64_t ldy, abh_precision prec) {
   constexpr int NC = Cheb<N>::NC;
@@ -1785,7 +1534,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     // M2M: deepest parents first
     for (int l = f->nlevels - 1; l >= 1; --l) {
       if (npar(l) == 0) continue;
-      k_transfer<N, true><<<npar(l), 256, 0, s>>>(par + f->tree.par_off[l], npar(l),
+      k_transfer_lines<N, true><<<npar(l), 256, 0, s>>>(par + f->tree.par_off[l], npar(l),
                                                   f->tree.nodes.as<int32_t>(), f->W.as<double>(),
                                                   f->consts);
       abh::check_launch();
@@ -1844,7 +1593,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     // downward: level l receives L2L from level l-1 and its M2L
     for (int l = 2; l <= f->nlevels; ++l) {
       if (npar(l - 1) == 0) continue;
-      k_transfer<N, false><<<npar(l - 1), 256, 0, s>>>(par + f->tree.par_off[l - 1], npar(l - 1),
+      k_transfer_lines<N, false><<<npar(l - 1), 256, 0, s>>>(par + f->tree.par_off[l - 1], npar(l - 1),
                                                        f->tree.nodes.as<int32_t>(),
                                                        f->G.as<double>(), f->consts);
       abh::check_launch();
@@ -1901,24 +1650,11 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
       abh::check_launch();
     }
     if (f->nleaf > 0) {
-      if (prec == ABH_FP64)
-        k_leaf_mfma<N><<<f->nleaf, 256, 0, s>>>(
-            f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
-            f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
-            f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
-            f->kern, f->consts);
-      else if (kMixedLeafMfma)
-        k_leaf_mfma<N><<<f->nleaf, 256, 0, s>>>(
-            f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
-            f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
-            f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
-            f->kern, f->consts);
-      else
-        k_leaf<N, true><<<f->nleaf, 256, 0, s>>>(
-            f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
-            f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
-            f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
-            f->kern, f->consts);
+      k_leaf_mfma<N><<<f->nleaf, 256, 0, s>>>(
+          f->tree.leaf_node.as<int32_t>(), f->nleaf, f->tree.leaves.as<int2>(),
+          f->lists.p2p_ptr.as<int32_t>(), f->lists.p2p_src.as<int32_t>(), f->box.as<double>(),
+          f->tree.pos.as<double>(), f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(),
+          f->kern, f->consts);
       abh::check_launch();
       k_scatter_rhs<<<unsigned(abh::ceil_div(n * nr, 256)), 256, 0, s>>>(
           f->tree.order.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);

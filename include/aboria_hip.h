@@ -277,9 +277,10 @@ typedef enum abh_precision {
   ABH_FP64 = 0,               /* FP64 kernel and accumulation               */
   ABH_FP32_COMPENSATED = 1,   /* FP64 radius test, FP32 kernel, FP32 sum
                                  with Kahan-Babuska compensation            */
-  ABH_MIXED = 2               /* FMM: M2L on BF16 MFMA with a three-term
-                                 split and FP32 accumulation, P2P as
-                                 ABH_FP32_COMPENSATED, other stages FP64    */
+  ABH_MIXED = 2               /* FMM: M2L on BF16 (uncompressed) or FP16
+                                 (compressed, power-of-two scaled) MFMA with
+                                 a three-term split and FP32 accumulation,
+                                 other stages FP64                          */
 } abh_precision;
 
 typedef struct abh_sparse_op abh_sparse_op;
@@ -492,9 +493,12 @@ const int32_t *abh_fmm_order(const abh_fmm *fmm);
 /* y += K x (FastMultipoleMethod::matrix_vector_multiply,
  * src/FastMultipoleMethod.h:110-167) for nrhs right-hand sides; x, y are
  * indexed like the positions given at creation (column-major, leading
- * dimensions ldx / ldy).  precision: ABH_FP64 or ABH_MIXED (for orders 7
- * and 8 MIXED keeps the M2L in FP64 -- its BF16 split would floor the error
- * near 1e-6 -- and runs P2P in compensated FP32). */
+ * dimensions ldx / ldy).  precision: ABH_FP64 or ABH_MIXED.  MIXED runs the
+ * uncompressed M2L of orders up to 6 as a BF16 split and, after
+ * abh_fmm_compress with a padded rank up to 192, the compressed M2L as an
+ * FP16 split with power-of-two scaling per level; otherwise (uncompressed
+ * orders 7 and 8, whose BF16 split would floor the error near 1e-6) its M2L
+ * is FP64.  P2M, L2P and P2P run in FP64 on the matrix cores in both. */
 abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
                          double *y, int64_t ldy, abh_precision precision);
 

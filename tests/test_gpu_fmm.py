@@ -3,8 +3,9 @@ traversal) and against an FP64 direct sum.
 
 Bar: the FP64 GPU FMM equals the oracle's FMM to 1e-11 relative (2-norm;
 summation order and the translated M2L operators differ only by rounding);
-the mixed form (BF16x3 MFMA M2L with FP32 accumulation, FP32-compensated
-P2P) is within 5e-5 relative of the FP64 form; the order-3 multiquadric case meets the reference's own accuracy test
+the mixed form (BF16x3 MFMA M2L with FP32 accumulation; P2M, L2P and P2P
+in FP64 on the matrix cores as in the FP64 form) is within 5e-5 relative of
+the FP64 form; the order-3 multiquadric case meets the reference's own accuracy test
 (tests/fmm.h:107-109: sum (y - y_direct)^2 / sum y_direct^2 < 1e-2).
 """
 import numpy as np
@@ -88,7 +89,7 @@ def test_fmm_high_order_accuracy(ctx, order):
     points (the 1/(r + 1e-3) kernel, leaves of at most 64): it falls with the
     order, and order 8 is below 1e-6 -- the accuracy the FMM throughput is
     quoted at.  MIXED at orders above 6 keeps the M2L in FP64 (its BF16 split
-    would floor the error near 1e-6) and runs P2P in compensated FP32."""
+    would floor the error near 1e-6)."""
     n = 40_000
     rng = np.random.default_rng(11)
     pos = rng.random((n, 3))
@@ -171,8 +172,8 @@ def test_fmm_compressed_accuracy(ctx, order, eps, bound):
 @pytest.mark.parametrize("order,eps", [(6, 1e-7), (8, 2e-7)])
 def test_fmm_compressed_mixed_fp16(ctx, order, eps):
     """MIXED on compressed operators: the rank-k M2L on FP16 matrix cores
-    (two-term split, power-of-two scaling per level, FP32 accumulation) and
-    the FP32-compensated P2P, 16 right-hand sides on 40,000 uniform points:
+    (two-term split, power-of-two scaling per level, FP32 accumulation), 16
+    right-hand sides on 40,000 uniform points:
     within 5e-7 (relative, 2-norm) of the FP64 compressed apply and within
     1e-6 of the FP64 direct sum at order 8."""
     print("fmm compressed mixed order %d eps %g" % (order, eps), flush=True)
