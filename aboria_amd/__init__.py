@@ -929,6 +929,32 @@ class DenseBatchOperator:
         return x, [{"iterations": s.iterations, "converged": bool(s.converged),
                     "residual": s.residual} for s in stats]
 
+    def residual_dd(self, b, x, r=None):
+        """r = b - K x in double-double arithmetic, rounded once to FP64."""
+        if r is None:
+            r = torch.empty_like(b)
+        check(C.lib().abh_dense_batch_residual_dd(self.handle, _ptr(b), _ptr(x), _ptr(r)),
+              self.ctx.handle)
+        return r
+
+    def refine(self, b, x=None, tol=1e-8, max_outer=8, inner_tol=1e-3, restart=50,
+               inner_iters=100, precision=FP64, preconditioner=None, stall_restarts=0):
+        """Iterative refinement: the residual in double-double arithmetic,
+        corrections from preconditioned GMRES (inner_tol, inner_iters), at
+        most max_outer residual evaluations.  Returns (x, stats) with the
+        inner iterations summed and the double-double relative residual."""
+        if x is None:
+            x = torch.zeros_like(b)
+        opts = C.GmresOptions(int(restart), int(inner_iters), float(inner_tol), int(precision),
+                              int(stall_restarts))
+        stats = (C.GmresStats * self.nprob)()
+        ph = preconditioner.handle if preconditioner is not None else None
+        check(C.lib().abh_dense_batch_refine(self.handle, ph, _ptr(b), _ptr(x),
+                                             ctypes.byref(opts), float(tol), int(max_outer),
+                                             stats), self.ctx.handle)
+        return x, [{"iterations": s.iterations, "converged": bool(s.converged),
+                    "residual": s.residual} for s in stats]
+
     def close(self):
         if getattr(self, "handle", None):
             C.lib().abh_dense_batch_destroy(self.handle)

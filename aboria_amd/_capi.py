@@ -211,6 +211,9 @@ SIGNATURES = {
                                        P(c_void_p)]),
     "abh_dense_batch_destroy": (c_int, [c_void_p]),
     "abh_dense_batch_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    "abh_dense_batch_residual_dd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "abh_dense_batch_refine": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, P(GmresOptions),
+                                       c_double, c_int, c_void_p]),
     "abh_dense_batch_gmres": (c_int, [c_void_p, c_void_p, c_void_p, P(GmresOptions),
                                       P(GmresStats)]),
     "abh_dense_apply": (c_int, [c_void_p, c_int, c_int64, c_void_p, c_int64, c_void_p, P(Kernel),

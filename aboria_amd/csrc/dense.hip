@@ -127,6 +127,90 @@ __global__ __launch_bounds__(kRows) void k_dense(const int2 *__restrict__ tiles,
   }
 }
 
+// r = b - K x for every row of the batch in double-double arithmetic: each
+// term K_ij x_j split exactly into its rounded product and the product's
+// error (fma), the terms summed with Knuth's two-sum into a (hi, lo) pair,
+// and b - (hi + lo) formed the same way before one final rounding.  K_ij is
+// the FP64 value the operator's own FP64 apply uses (the multiquadric form
+// with fused multiply-adds, or the generic profile), so r is the residual of
+// the FP64 matrix, free of the summation's rounding.  One row per thread,
+// sources staged in LDS as in k_dense.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+  s = a + b;
+  const double bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+
+template <int D, bool MQ>
+__global__ __launch_bounds__(kRows) void k_dense_resid_dd(const int2 *__restrict__ tiles,
+                                                          int ntiles,
+                                                          const int64_t *__restrict__ off,
+                                                          const double *__restrict__ pos,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ b,
+                                                          double *__restrict__ r,
+                                                          const uint8_t *__restrict__ active,
+                                                          KernelDesc k) {
+  __shared__ double sp[kRows][D];
+  __shared__ double sx[kRows];
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int p = tiles[t].x;
+  if (active && !active[p]) return;
+  const int64_t base = off[p];
+  const int64_t n = off[p + 1] - base;
+  const int64_t i = int64_t(tiles[t].y) + threadIdx.x;
+  const bool act = i < n;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = act ? pos[(base + i) * D + d] : 0.0;
+  double hi = 0.0, lo = 0.0;
+  for (int64_t c0 = 0; c0 < n; c0 += kRows) {
+    const int m = int(n - c0 < kRows ? n - c0 : kRows);
+    __syncthreads();
+    if (int(threadIdx.x) < m) {
+      const int64_t j = base + c0 + threadIdx.x;
+#pragma unroll
+      for (int d = 0; d < D; ++d) sp[threadIdx.x][d] = pos[j * D + d];
+      sx[threadIdx.x] = x[j];
+    }
+    __syncthreads();
+    if (!act) continue;
+    for (int j = 0; j < m; ++j) {
+      double kij;
+      if (MQ) {
+        double s = k.param2;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+          const double dx = sp[j][d] - xi[d];
+          s = fma(dx, dx, s);
+        }
+        kij = abh::sqrt_pos(s);
+      } else {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const double dx = sp[j][d] - xi[d];
+          s = s + dx * dx;
+        }
+        kij = abh::phi64(k, s);
+      }
+      const double xj = sx[j];
+      const double pr = kij * xj;
+      const double pe = fma(kij, xj, -pr);
+      double sum, err;
+      two_sum(hi, pr, sum, err);
+      hi = sum;
+      lo += err + pe;
+    }
+  }
+  if (act) {
+    double s, e;
+    two_sum(b[base + i], -hi, s, e);
+    r[base + i] = s + (e - lo);
+  }
+}
+
 // Rectangular operator y += K x over separate row / column point sets,
 // up to kRectRhs right-hand sides per pass (column-major x, y).  Block
 // (bx, by) takes rows bx * kRows .. and columns by * cspan ..; with several
@@ -272,6 +356,25 @@ struct abh_dense_batch {
     else { ABH_DENSE_LAUNCH(3) }
 #undef ABH_DENSE_LAUNCH
 #undef ABH_DENSE_LAUNCH2
+    abh::check_launch();
+  }
+
+  // r = b - K x in double-double (k_dense_resid_dd) over the one-row tiles
+  void residual_dd(const double *x, const double *b, double *r, const uint8_t *act) {
+    if (ntiles[0] == 0) return;
+    hipStream_t s = ctx->stream;
+    const int2 *tl = tiles[0].as<int2>();
+    const int64_t *o = off.as<int64_t>();
+    const int nt = ntiles[0];
+    const bool mq =
+        k.type == ABH_KERNEL_MULTIQUADRIC && !k.row_s && !k.col_s && k.param2 >= 1e-280;
+#define ABH_RES_DD(DD)                                                                      \
+  if (mq) k_dense_resid_dd<DD, true><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, b, r, act, k);  \
+  else k_dense_resid_dd<DD, false><<<nt, kRows, 0, s>>>(tl, nt, o, pos, x, b, r, act, k);
+    if (dim == 1) { ABH_RES_DD(1) }
+    else if (dim == 2) { ABH_RES_DD(2) }
+    else { ABH_RES_DD(3) }
+#undef ABH_RES_DD
     abh::check_launch();
   }
 
@@ -540,6 +643,106 @@ abh_status abh_dense_batch_gmres_precond(abh_dense_batch *op, const abh_schwarz 
     abh::batch_gmres(ctx, op->L, f_op, f_true, prec ? &f_prec : nullptr, f_act, b, x, *opts, res);
     if (stats)
       for (int p = 0; p < op->L.nprob; ++p) stats[p] = res.stats[p];
+  });
+}
+
+abh_status abh_dense_batch_residual_dd(abh_dense_batch *op, const double *b, const double *x,
+                                       double *r) {
+  if (!op) return ABH_ERR_INVALID_ARG;
+  return abh::guard(op->ctx, [&] {
+    ABH_REQUIRE(op->L.total() == 0 || (b && x && r), ABH_ERR_INVALID_ARG, "b, x or r is null");
+    op->residual_dd(x, b, r, nullptr);
+  });
+}
+
+abh_status abh_dense_batch_refine(abh_dense_batch *op, const abh_schwarz *prec,
+                                  const double *b, double *x, const abh_gmres_options *inner,
+                                  double tol, int max_outer, abh_gmres_stats *stats) {
+  if (!op || !inner) return ABH_ERR_INVALID_ARG;
+  abh_context *ctx = op->ctx;
+  return abh::guard(ctx, [&] {
+    ABH_REQUIRE(!prec || abh::schwarz_size(prec) == op->L.total(), ABH_ERR_INVALID_ARG,
+                "preconditioner size differs from the batch");
+    ABH_REQUIRE(inner->restart >= 1 && inner->max_iters >= 1 && inner->tol > 0,
+                ABH_ERR_INVALID_ARG, "invalid inner GMRES options");
+    ABH_REQUIRE(inner->precision == ABH_FP64 || inner->precision == ABH_FP32_COMPENSATED,
+                ABH_ERR_UNSUPPORTED, "GMRES operator precision must be FP64 or FP32_COMPENSATED");
+    ABH_REQUIRE(tol > 0 && max_outer >= 1, ABH_ERR_INVALID_ARG, "invalid refinement options");
+    const abh::BatchLayout &L = op->L;
+    const int np = L.nprob;
+    const int64_t n = L.total();
+    ABH_REQUIRE(n == 0 || (b && x), ABH_ERR_INVALID_ARG, "b or x is null");
+    hipStream_t s = ctx->stream;
+    const bool fp32 = inner->precision == ABH_FP32_COMPENSATED;
+    const uint8_t *act = op->active.as<uint8_t>();
+    std::vector<uint8_t> active(np, 1);
+    auto set_act = [&](const uint8_t *a) {
+      ABH_HIP(hipMemcpyAsync(op->active.p, a, size_t(np), hipMemcpyHostToDevice, s));
+      ABH_HIP(hipStreamSynchronize(s));
+    };
+    std::function<void(const double *, double *)> f_op = [&](const double *in, double *out) {
+      op->apply(in, out, fp32, 1.0, true, act);
+    };
+    std::function<void(const double *, double *)> f_true = [&](const double *in, double *out) {
+      op->apply(in, out, false, 1.0, true, act);
+    };
+    std::function<void(const uint8_t *)> f_act = set_act;
+    std::function<void(const double *, double *)> f_prec = [&](const double *in, double *out) {
+      abh::schwarz_apply(prec, in, out);
+    };
+    const int ld = 2;
+    DevBuf r, d, partial, hdev, sdev;
+    r.ensure(size_t(n) * 8 + 64);
+    d.ensure(size_t(n) * 8 + 64);
+    partial.ensure(size_t(std::max(L.nchunks, 1)) * ld * 8 + 64);
+    hdev.ensure(size_t(np) * ld * 8 + 64);
+    sdev.ensure(size_t(np) * 8 + 64);
+    std::vector<double> hh(size_t(np) * ld), bn(np), rn(np);
+    auto norms = [&](const double *v, std::vector<double> &out) {
+      abh::batch_dots(ctx, L, v, n, 1, v, partial.as<double>(), hdev.as<double>(), ld);
+      ABH_HIP(hipMemcpyAsync(hh.data(), hdev.p, hh.size() * 8, hipMemcpyDeviceToHost, s));
+      ABH_HIP(hipStreamSynchronize(s));
+      for (int p = 0; p < np; ++p) out[p] = std::sqrt(std::max(0.0, hh[size_t(p) * ld]));
+    };
+    std::vector<abh_gmres_stats> st(np, abh_gmres_stats{0, 0, 0.0});
+    norms(b, bn);
+    for (int outer = 0; outer < max_outer; ++outer) {
+      // the accurate residual of the problems still refining
+      set_act(active.data());
+      ABH_HIP(hipMemsetAsync(r.p, 0, size_t(n) * 8, s));  // finished problems' rows stay 0
+      op->residual_dd(x, b, r.as<double>(), act);
+      norms(r.as<double>(), rn);
+      bool any = false;
+      std::vector<double> keep(np, 0.0);
+      for (int p = 0; p < np; ++p) {
+        if (!active[p]) continue;
+        st[p].residual = bn[p] > 0 ? rn[p] / bn[p] : 0.0;
+        if (bn[p] == 0.0 || rn[p] <= tol * bn[p]) {
+          st[p].converged = 1;
+          active[p] = 0;
+          continue;
+        }
+        keep[p] = 1.0;
+        any = true;
+      }
+      if (!any || outer == max_outer - 1) break;
+      // K d = r for the active problems (the others' right-hand sides are
+      // zeroed, so the inner solve leaves them at once), then x += d
+      ABH_HIP(hipMemcpyAsync(sdev.p, keep.data(), size_t(np) * 8, hipMemcpyHostToDevice, s));
+      abh::batch_scale(ctx, L, r.as<double>(), sdev.as<double>(), r.as<double>());
+      ABH_HIP(hipMemsetAsync(d.p, 0, size_t(n) * 8, s));
+      abh::GmresResult res;
+      abh::batch_gmres(ctx, L, f_op, f_true, prec ? &f_prec : nullptr, f_act, r.as<double>(),
+                       d.as<double>(), *inner, res);
+      for (int p = 0; p < np; ++p) st[p].iterations += res.stats[p].iterations;
+      std::vector<double> one(size_t(np) * ld, 0.0);
+      for (int p = 0; p < np; ++p) one[size_t(p) * ld] = 1.0;
+      ABH_HIP(hipMemcpyAsync(hdev.p, one.data(), one.size() * 8, hipMemcpyHostToDevice, s));
+      abh::batch_combine(ctx, L, d.as<double>(), 0, 1, hdev.as<double>(), ld, 1.0, x);
+      ABH_HIP(hipStreamSynchronize(s));  // keep, one are host sources of the copies above
+    }
+    if (stats)
+      for (int p = 0; p < np; ++p) stats[p] = st[p];
   });
 }
 

@@ -23,9 +23,10 @@ workload):
   rbf_batch   512 uneven 2D multiquadric (c=0.05) RBF systems, batched
               GMRES(50) on a bounded iteration budget, FP64 and FP32
               compensated operator (FP64 residual / orthogonalisation);
-              Schwarz-preconditioned FP64 GMRES over the whole batch, and the
-              time to 1e-8 (set-up + solve) of the problems with N < 2,000,
-              the ones whose FP64 residual can reach 1e-8
+              Schwarz-preconditioned FP64 GMRES over the whole batch, the
+              same with iterative refinement (double-double residual), and
+              the time to 1e-8 (set-up + solve) of the problems with
+              N < 2,000 as a batch of their own
   schwarz     2D jittered grid N=400,056, Gaussian sigma=0.01 (+1e-5 I) and a
               narrow variant sigma=0.002, GMRES to 1e-8 on a 300-iteration
               budget: plain Schwarz vs the learned patch solves (the network
@@ -443,6 +444,23 @@ def bench_rbf(args, torch, ab, wl, ctx, timer, extra):
         log("rbf %s: finished in %.1f s (max %d iters), converged %d/%d, residual median %.2e "
             "max %.2e, %s" % (name, solve_s, int(its.max()), conv, c["nprob"], np.median(res),
                               res.max(), out[name]["converged_by_size"]))
+    # iterative refinement: the residual in double-double, Schwarz-GMRES(50)
+    # corrections to 1e-3 (one cycle each), four residual evaluations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, stats = op.refine(b, tol=c["tol"], max_outer=4, inner_tol=1e-3, inner_iters=50,
+                         preconditioner=sw)
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t0
+    res = np.array([s["residual"] for s in stats])
+    conv = int(sum(s["converged"] for s in stats))
+    its = np.array([s["iterations"] for s in stats])
+    out["fp64_schwarz_refine"] = {
+        "solve_s": solve_s, "converged": conv, "max_outer": 4, "inner_tol": 1e-3,
+        "inner_iters": 50, "residual_median": float(np.median(res)),
+        "iterations_max": int(its.max()), "converged_by_size": by_size(sizes, stats)}
+    log("rbf refine (double-double residual, Schwarz-GMRES corrections): %.1f s, converged "
+        "%d/%d, %s" % (solve_s, conv, c["nprob"], out["fp64_schwarz_refine"]["converged_by_size"]))
     sw.close()
     op.close()
     # time to 1e-8 on the problems FP64 can solve to 1e-8 (N < 2,000: the

@@ -695,6 +695,26 @@ abh_status abh_dense_batch_schwarz_create(abh_dense_batch *op, const double *low
  * abh_dense_batch_schwarz_create; NULL = none): Eigen::GMRES<...,
  * SchwartzPreconditioner> (tests/rbf_interpolation.h:358-369) for every
  * problem.  Convergence is judged on the true residual ||b - K x||. */
+/* No counterpart in the reference: r = b - K x for every problem in
+ * double-double arithmetic (exact products split by fma, two-sum
+ * accumulation, one final rounding), K the FP64 matrix abh_dense_batch_apply
+ * evaluates.  b, x, r: device, total points. */
+abh_status abh_dense_batch_residual_dd(abh_dense_batch *op, const double *b,
+                                       const double *x, double *r);
+/* No counterpart in the reference (its GMRES stops at the FP64 residual):
+ * iterative refinement of every problem of the batch.  Each outer step
+ * forms r = b - K x in double-double arithmetic (the residual of the FP64
+ * matrix without the summation's rounding), stops the problems with
+ * ||r|| <= tol ||b||, and adds to the others the solution d of K d = r from
+ * GMRES (inner options: restart, iteration budget, relative tolerance,
+ * operator precision), right-preconditioned by prec (NULL = none).
+ * max_outer bounds the residual evaluations.  stats (host, one per problem,
+ * may be NULL): the inner iterations summed, converged, and the last
+ * double-double relative residual. */
+abh_status abh_dense_batch_refine(abh_dense_batch *op, const abh_schwarz *prec,
+                                  const double *b, double *x,
+                                  const abh_gmres_options *inner, double tol,
+                                  int max_outer, abh_gmres_stats *stats);
 abh_status abh_dense_batch_gmres_precond(abh_dense_batch *op,
                                          const abh_schwarz *prec,
                                          const double *b, double *x,
