@@ -285,7 +285,7 @@ class HyperOctree:
         return v
 
     def interaction_lists(self):
-        """The dual traversal's lists (3D): dict of device int32 tensors
+        """The dual traversal's lists: dict of device int32 tensors
         p2p_ptr, p2p_src (per non-empty leaf, its source leaves), m2l_ptr,
         m2l_src (per node, its M2L source nodes) and the P2P particle-pair
         count."""
@@ -786,18 +786,21 @@ def create_sparse_operator(row_particles, col_particles, radius, kernel, row_sca
 
 
 class FmmOperator:
-    """Black-box Chebyshev FMM operator over a HyperOctree (3D)."""
+    """Black-box Chebyshev FMM operator over a HyperOctree in 1-3
+    dimensions (positions (n, dim) float64; the M2L compression and the
+    mixed-precision apply are 3D, other dimensions apply in FP64)."""
 
     def __init__(self, ctx, positions, low, high, n_particles_in_leaf, order, kernel):
         _require_cuda(positions, torch.float64, "positions")
         self.ctx = ctx
         self.n = positions.shape[0]
+        self.dim = positions.shape[1] if positions.dim() == 2 else 1
         self._keep = positions
         lo = (ctypes.c_double * 3)(*[float(v) for v in low])
         hi = (ctypes.c_double * 3)(*[float(v) for v in high])
         kd = kernel.to_c()
         h = ctypes.c_void_p()
-        check(C.lib().abh_fmm_create(ctx.handle, 3, lo, hi, self.n, _ptr(positions),
+        check(C.lib().abh_fmm_create(ctx.handle, self.dim, lo, hi, self.n, _ptr(positions),
                                      float(n_particles_in_leaf), int(order), ctypes.byref(kd),
                                      ctypes.byref(h)), ctx.handle)
         self.handle = h

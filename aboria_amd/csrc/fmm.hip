@@ -1181,6 +1181,331 @@ __global__ __launch_bounds__(256) void k_leaf_mfma(const int32_t *__restrict__ l
   }
 }
 
+// ---------------------------------------------------------------------------
+// one and two dimensions: the same black-box expansions on the N^D
+// Chebyshev nodes (lattice order, last dimension fastest), FP64 on the
+// vector units, the M2L over the dual traversal's weak lists
+// ---------------------------------------------------------------------------
+
+template <int D, int N> struct ChebD {
+  static constexpr int NC = D == 1 ? N : N * N;
+  static constexpr int OFFS = D == 1 ? 7 : 49;  // [-3,3]^D
+};
+
+template <int D, int N> __device__ __forceinline__ void lattice(int j, int (&jd)[D]) {
+  if (D == 1) {
+    jd[0] = j;
+  } else {
+    jd[0] = j / N;
+    jd[D - 1] = j % N;
+  }
+}
+
+// M2L operators: M[l][o][i][j] = K(target node i, source node j) for boxes
+// of level l at offset o in [-3,3]^D (box units, first dimension slowest)
+template <int D, int N>
+__global__ __launch_bounds__(256) void k_m2l_ops_gen(double *__restrict__ M, int level0, int nlev,
+                                                     Bounds b, KernelDesc k, FmmConsts c) {
+  constexpr int NC = ChebD<D, N>::NC, OFFS = ChebD<D, N>::OFFS;
+  const int64_t total = int64_t(nlev) * OFFS * NC * NC;
+  for (int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x; t < total;
+       t += int64_t(gridDim.x) * 256) {
+    int64_t r = t;
+    const int j = int(r % NC);
+    r /= NC;
+    const int i = int(r % NC);
+    r /= NC;
+    const int o = int(r % OFFS);
+    const int l = level0 + int(r / OFFS);
+    int off[D], ii[D], jj[D];
+    if (D == 1) off[0] = o - 3;
+    else {
+      off[0] = o / 7 - 3;
+      off[D - 1] = o % 7 - 3;
+    }
+    lattice<D, N>(i, ii);
+    lattice<D, N>(j, jj);
+    double s = 0.0;
+    for (int d = 0; d < D; ++d) {
+      const double h = (b.hi[d] - b.lo[d]) / double(int64_t(1) << l);
+      const double ti = 0.5 * (c.node[ii[d]] + 1) * h;
+      const double sj = 0.5 * (c.node[jj[d]] + 1) * h + double(off[d]) * h;
+      const double dx = sj - ti;
+      s = s + dx * dx;
+    }
+    M[t] = abh::phi64(k, s);
+  }
+}
+
+// P2M: one block per leaf; W[j][r] = sum_i prod_d S_{j_d}(x_i) q_i[r], the
+// particles' Chebyshev values staged in LDS 64 at a time
+template <int D, int N>
+__global__ __launch_bounds__(256) void k_p2m_gen(const int32_t *__restrict__ leaf_nodes, int nleaf,
+                                                 const int2 *__restrict__ leaf_range,
+                                                 const double *__restrict__ box,
+                                                 const double *__restrict__ pos,
+                                                 const double *__restrict__ xs,
+                                                 double *__restrict__ W, FmmConsts c) {
+  constexpr int NC = ChebD<D, N>::NC;
+  constexpr int OPT = (NC * kRhs + 255) / 256;
+  __shared__ double S[64][D][N];
+  __shared__ double Q[64][kRhs];
+  const int li = blockIdx.x;
+  if (li >= nleaf) return;
+  const int node = leaf_nodes[li];
+  const int2 rg = leaf_range[li];
+  double blo[D], bhi[D];
+  for (int d = 0; d < D; ++d) {
+    blo[d] = box[int64_t(node) * 2 * D + d];
+    bhi[d] = box[int64_t(node) * 2 * D + D + d];
+  }
+  double acc[OPT];
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) acc[q] = 0.0;
+  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
+    const int np = min(64, rg.y - p0);
+    __syncthreads();
+    if (int(threadIdx.x) < np) {
+      const int p = p0 + threadIdx.x;
+      for (int d = 0; d < D; ++d) {
+        const double span = bhi[d] - blo[d];
+        const double x = span > 0.0 ? (2 * pos[int64_t(p) * D + d] - blo[d] - bhi[d]) / span : 0.0;
+        for (int m = 0; m < N; ++m) S[threadIdx.x][d][m] = sn_dev<N>(c, x, m);
+      }
+    }
+    for (int t = threadIdx.x; t < np * kRhs; t += 256) Q[t / kRhs][t % kRhs] = xs[int64_t(p0) * kRhs + t];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int u = threadIdx.x + 256 * q;
+      if (u >= NC * kRhs) break;
+      const int r = u % kRhs;
+      int jd[D];
+      lattice<D, N>(u / kRhs, jd);
+      for (int i = 0; i < np; ++i) {
+        double w = S[i][0][jd[0]];
+        if (D == 2) w *= S[i][D - 1][jd[D - 1]];
+        acc[q] += w * Q[i][r];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int u = threadIdx.x + 256 * q;
+    if (u < NC * kRhs) W[int64_t(node) * NC * kRhs + u] = acc[q];
+  }
+}
+
+// M2M (UP): W_p[i] = sum_c sum_j prod_d s1[h_d][i_d][j_d] W_c[j]; L2L:
+// G_c[i] += sum_j prod_d s1[h_d][j_d][i_d] G_p[j].  One block per parent,
+// child c's half along dimension d is bit D-1-d of c.
+template <int D, int N, bool UP>
+__global__ __launch_bounds__(256) void k_transfer_gen(const int32_t *__restrict__ parents, int npar,
+                                                      const int32_t *__restrict__ child0,
+                                                      double *__restrict__ E, FmmConsts c) {
+  constexpr int NC = ChebD<D, N>::NC;
+  constexpr int OPT = (NC * kRhs + 255) / 256;
+  __shared__ double P[NC * kRhs];
+  const int pi = blockIdx.x;
+  if (pi >= npar) return;
+  const int parent = parents[pi];
+  const int first = child0[parent];
+  if (!UP)
+    for (int u = threadIdx.x; u < NC * kRhs; u += 256) P[u] = E[int64_t(parent) * NC * kRhs + u];
+  __syncthreads();
+  double acc[OPT];
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) acc[q] = 0.0;
+  for (int ch = 0; ch < (1 << D); ++ch) {
+    const int cn = first + ch;
+    if (child0[cn] == int32_t(0xffffffffu)) continue;  // empty: no expansion
+    int h[D];
+    for (int d = 0; d < D; ++d) h[d] = (ch >> (D - 1 - d)) & 1;
+    const double *ec = E + int64_t(cn) * NC * kRhs;
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int u = threadIdx.x + 256 * q;
+      if (u >= NC * kRhs) break;
+      const int r = u % kRhs;
+      int id[D];
+      lattice<D, N>(u / kRhs, id);
+      double sum = 0.0;
+      for (int j = 0; j < NC; ++j) {
+        int jd[D];
+        lattice<D, N>(j, jd);
+        double w = UP ? c.s1[h[0]][id[0]][jd[0]] : c.s1[h[0]][jd[0]][id[0]];
+        if (D == 2) w *= UP ? c.s1[h[D - 1]][id[D - 1]][jd[D - 1]] : c.s1[h[D - 1]][jd[D - 1]][id[D - 1]];
+        sum += w * (UP ? ec[j * kRhs + r] : P[j * kRhs + r]);
+      }
+      if (UP) acc[q] += sum;
+      else E[int64_t(cn) * NC * kRhs + u] += sum;
+    }
+  }
+  if (UP) {
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int u = threadIdx.x + 256 * q;
+      if (u < NC * kRhs) E[int64_t(parent) * NC * kRhs + u] = acc[q];
+    }
+  }
+}
+
+// M2L: one block per node of one level; G_t[i] += sum_s sum_j M_o[i][j]
+// W_s[j] over the node's weak list, o the source's offset in box units (a
+// pair outside [-3,3]^D would evaluate the kernel between the nodes)
+template <int D, int N>
+__global__ __launch_bounds__(256) void k_m2l_gen(int64_t b0, int64_t cnt,
+                                                 const uint32_t *__restrict__ weak_ptr,
+                                                 const int32_t *__restrict__ weak_src,
+                                                 const int32_t *__restrict__ coord,
+                                                 const double *__restrict__ box,
+                                                 const double *__restrict__ M,
+                                                 const double *__restrict__ W,
+                                                 double *__restrict__ G, KernelDesc k,
+                                                 FmmConsts c) {
+  constexpr int NC = ChebD<D, N>::NC;
+  constexpr int OPT = (NC * kRhs + 255) / 256;
+  __shared__ double Ws[NC * kRhs];
+  const int64_t t = b0 + blockIdx.x;
+  if (int64_t(blockIdx.x) >= cnt) return;
+  const uint32_t e0 = weak_ptr[t], e1 = weak_ptr[t + 1];
+  if (e0 == e1) return;
+  double acc[OPT];
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) acc[q] = 0.0;
+  for (uint32_t e = e0; e < e1; ++e) {
+    const int sn = weak_src[e];
+    __syncthreads();
+    for (int u = threadIdx.x; u < NC * kRhs; u += 256) Ws[u] = W[int64_t(sn) * NC * kRhs + u];
+    __syncthreads();
+    int o = 0;
+    bool near = true;
+    for (int d = 0; d < D; ++d) {
+      const int off = coord[int64_t(sn) * D + d] - coord[t * D + d];
+      near = near && off >= -3 && off <= 3;
+      o = o * 7 + (off + 3);
+    }
+    const double *Mo = M + int64_t(o) * NC * NC;
+#pragma unroll
+    for (int q = 0; q < OPT; ++q) {
+      const int u = threadIdx.x + 256 * q;
+      if (u >= NC * kRhs) break;
+      const int i = u / kRhs, r = u % kRhs;
+      double sum = 0.0;
+      if (near) {
+        for (int j = 0; j < NC; ++j) sum += Mo[i * NC + j] * Ws[j * kRhs + r];
+      } else {
+        int id[D];
+        lattice<D, N>(i, id);
+        double xt[D];
+        for (int d = 0; d < D; ++d) {
+          const double lo = box[t * 2 * D + d], hi = box[t * 2 * D + D + d];
+          xt[d] = 0.5 * (c.node[id[d]] + 1) * (hi - lo) + lo;
+        }
+        for (int j = 0; j < NC; ++j) {
+          int jd[D];
+          lattice<D, N>(j, jd);
+          double s2 = 0.0;
+          for (int d = 0; d < D; ++d) {
+            const double lo = box[int64_t(sn) * 2 * D + d], hi = box[int64_t(sn) * 2 * D + D + d];
+            const double dx = 0.5 * (c.node[jd[d]] + 1) * (hi - lo) + lo - xt[d];
+            s2 = s2 + dx * dx;
+          }
+          sum += abh::phi64(k, s2) * Ws[j * kRhs + r];
+        }
+      }
+      acc[q] += sum;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < OPT; ++q) {
+    const int u = threadIdx.x + 256 * q;
+    if (u < NC * kRhs) G[t * NC * kRhs + u] += acc[q];
+  }
+}
+
+// L2P + P2P: one block of one wave per leaf, one target per lane with all
+// right-hand sides in registers; the local expansion and the P2P sources
+// (positions and charges, 64 at a time) staged in LDS
+template <int D, int N>
+__global__ __launch_bounds__(64) void k_leaf_gen(const int32_t *__restrict__ leaf_nodes, int nleaf,
+                                                 const int2 *__restrict__ leaf_range,
+                                                 const int32_t *__restrict__ p2p_ptr,
+                                                 const int32_t *__restrict__ p2p_src,
+                                                 const double *__restrict__ box,
+                                                 const double *__restrict__ pos,
+                                                 const double *__restrict__ xs,
+                                                 const double *__restrict__ G,
+                                                 double *__restrict__ ys, KernelDesc k,
+                                                 FmmConsts c) {
+  constexpr int NC = ChebD<D, N>::NC;
+  __shared__ double Gs[NC * kRhs];
+  __shared__ double Ps[64][D];
+  __shared__ double Qs[64][kRhs];
+  const int li = blockIdx.x;
+  if (li >= nleaf) return;
+  const int node = leaf_nodes[li];
+  const int2 rg = leaf_range[li];
+  const int lane = threadIdx.x;
+  for (int u = lane; u < NC * kRhs; u += 64) Gs[u] = G[int64_t(node) * NC * kRhs + u];
+  double blo[D], bhi[D];
+  for (int d = 0; d < D; ++d) {
+    blo[d] = box[int64_t(node) * 2 * D + d];
+    bhi[d] = box[int64_t(node) * 2 * D + D + d];
+  }
+  const int e0 = p2p_ptr[li], e1 = p2p_ptr[li + 1];
+  for (int p0 = rg.x; p0 < rg.y; p0 += 64) {
+    const int p = p0 + lane;
+    const bool act = p < rg.y;
+    double xp[D], sv[D][N];
+    double acc[kRhs];
+#pragma unroll
+    for (int r = 0; r < kRhs; ++r) acc[r] = 0.0;
+    for (int d = 0; d < D; ++d) {
+      xp[d] = act ? pos[int64_t(p) * D + d] : 0.0;
+      const double span = bhi[d] - blo[d];
+      const double x = span > 0.0 ? (2 * xp[d] - blo[d] - bhi[d]) / span : 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) sv[d][m] = sn_dev<N>(c, x, m);
+    }
+    __syncthreads();
+    // L2P
+    for (int j = 0; j < NC; ++j) {
+      int jd[D];
+      lattice<D, N>(j, jd);
+      double w = sv[0][jd[0]];
+      if (D == 2) w *= sv[D - 1][jd[D - 1]];
+#pragma unroll
+      for (int r = 0; r < kRhs; ++r) acc[r] += w * Gs[j * kRhs + r];
+    }
+    // P2P over the source leaves' particles
+    for (int e = e0; e < e1; ++e) {
+      const int2 sr = leaf_range[p2p_src[e]];
+      for (int q0 = sr.x; q0 < sr.y; q0 += 64) {
+        const int nq = min(64, sr.y - q0);
+        __syncthreads();
+        if (lane < nq)
+          for (int d = 0; d < D; ++d) Ps[lane][d] = pos[int64_t(q0 + lane) * D + d];
+        for (int u = lane; u < nq * kRhs; u += 64) Qs[u / kRhs][u % kRhs] = xs[int64_t(q0) * kRhs + u];
+        __syncthreads();
+        for (int m = 0; m < nq; ++m) {
+          double s2 = 0.0;
+          for (int d = 0; d < D; ++d) {
+            const double dx = Ps[m][d] - xp[d];
+            s2 = s2 + dx * dx;
+          }
+          const double kv = abh::phi64(k, s2);
+#pragma unroll
+          for (int r = 0; r < kRhs; ++r) acc[r] += kv * Qs[m][r];
+        }
+      }
+    }
+    if (act)
+#pragma unroll
+      for (int r = 0; r < kRhs; ++r) ys[int64_t(p) * kRhs + r] = acc[r];
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1189,6 +1514,7 @@ __global__ __launch_bounds__(256) void k_leaf_mfma(const int32_t *__restrict__ l
 
 struct abh_fmm {
   abh_context *ctx = nullptr;
+  int dim = 3;
   int order = 6;
   int64_t n = 0;
   Box bounds{};
@@ -1663,6 +1989,122 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
   }
 }
 
+// ---- one and two dimensions ----
+
+template <int D, int N> void create_gen(abh_fmm *f, const double *pos_in, double n_leaf) {
+  abh_context *ctx = f->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t n = f->n;
+  make_consts<N>(f->consts);
+  abh::Octree &t = f->tree;
+  t.dim = D;
+  t.max_level = 32 / D - 2;  // src/OctTree.h:131
+  t.n_leaf = n_leaf;
+  for (int d = 0; d < D; ++d) {
+    t.lo[d] = f->bounds.lo[d];
+    t.hi[d] = f->bounds.hi[d];
+    t.periodic[d] = false;
+  }
+  abh::octree_update(ctx, t, n, const_cast<double *>(pos_in), nullptr);
+  ABH_REQUIRE(t.n_alive == n, ABH_ERR_INVALID_ARG,
+              "every position must be finite and inside [low, high)");
+  f->nlevels = t.levels;
+  const int64_t nn = t.n_nodes;
+  f->n_nodes = nn;
+  f->nleaf = int(t.n_leaves);
+  f->box.ensure(size_t(nn + 1) * 16 * D);
+  f->coord.ensure(size_t(nn + 1) * 4 * D);
+  abh::octree_geometry(ctx, t, f->box.as<double>(), f->coord.as<int32_t>());
+  abh::octree_lists(ctx, t, f->box.as<double>(), f->lists, true);
+  constexpr int NC = ChebD<D, N>::NC, OFFS = ChebD<D, N>::OFFS;
+  Bounds b{};
+  for (int d = 0; d < D; ++d) {
+    b.lo[d] = f->bounds.lo[d];
+    b.hi[d] = f->bounds.hi[d];
+  }
+  const int nlev = std::max(0, t.levels - 1);
+  if (nlev > 0) {
+    f->M.ensure(size_t(nlev) * OFFS * NC * NC * 8);
+    k_m2l_ops_gen<D, N><<<1024, 256, 0, s>>>(f->M.as<double>(), 2, nlev, b, f->kern, f->consts);
+    abh::check_launch();
+  }
+  f->W.ensure(size_t(nn) * NC * kRhs * 8);
+  f->G.ensure(size_t(nn) * NC * kRhs * 8);
+  f->xs.ensure(size_t(n + 1) * kRhs * 8);
+  f->ys.ensure(size_t(n + 1) * kRhs * 8);
+  ABH_HIP(hipStreamSynchronize(s));
+  f->info.order = N;
+  f->info.levels = t.levels;
+  f->info.n_nodes = nn;
+  f->info.n_leaves = f->nleaf;
+  f->info.m2l_pairs = f->lists.m2l;
+  f->info.p2p_pairs = f->lists.p2p_pairs;
+  f->info.p2p_interactions = f->lists.interactions;
+}
+
+template <int D, int N> void apply_gen(abh_fmm *f, int nrhs, const double *x, int64_t ldx,
+                                       double *y, int64_t ldy) {
+  constexpr int NC = ChebD<D, N>::NC, OFFS = ChebD<D, N>::OFFS;
+  abh_context *ctx = f->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t n = f->n;
+  const int64_t nn = f->n_nodes;
+  const abh::Octree &t = f->tree;
+  const int32_t *par = t.internal.as<int32_t>();
+  auto npar = [&](int l) { return int(t.par_off[l + 1] - t.par_off[l]); };
+  for (int r0 = 0; r0 < nrhs; r0 += kRhs) {
+    const int nr = std::min(kRhs, nrhs - r0);
+    k_gather_rhs<<<unsigned(abh::ceil_div(n * kRhs, 256)), 256, 0, s>>>(
+        t.order.as<int32_t>(), n, nr, x + int64_t(r0) * ldx, ldx, f->xs.as<double>());
+    abh::check_launch();
+    ABH_HIP(hipMemsetAsync(f->W.p, 0, size_t(nn) * NC * kRhs * 8, s));
+    ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
+    if (f->nleaf > 0) {
+      k_p2m_gen<D, N><<<f->nleaf, 256, 0, s>>>(t.leaf_node.as<int32_t>(), f->nleaf,
+                                              t.leaves.as<int2>(), f->box.as<double>(),
+                                              t.pos.as<double>(), f->xs.as<double>(),
+                                              f->W.as<double>(), f->consts);
+      abh::check_launch();
+    }
+    for (int l = f->nlevels - 1; l >= 1; --l) {  // M2M, deepest parents first
+      if (npar(l) == 0) continue;
+      k_transfer_gen<D, N, true><<<npar(l), 256, 0, s>>>(par + t.par_off[l], npar(l),
+                                                       t.nodes.as<int32_t>(), f->W.as<double>(),
+                                                       f->consts);
+      abh::check_launch();
+    }
+    // every level's M2L first (G of each node holds its own far field),
+    // then the L2L top-down adds each parent's complete local expansion
+    for (int l = 2; l <= f->nlevels; ++l) {
+      const int64_t b0 = t.level_begin[size_t(l) - 1], cnt = t.level_begin[size_t(l)] - b0;
+      if (cnt == 0) continue;
+      k_m2l_gen<D, N><<<unsigned(cnt), 256, 0, s>>>(
+          b0, cnt, f->lists.weak_ptr.as<uint32_t>(), f->lists.weak_src.as<int32_t>(),
+          f->coord.as<int32_t>(), f->box.as<double>(),
+          f->M.as<double>() + int64_t(l - 2) * OFFS * NC * NC, f->W.as<double>(),
+          f->G.as<double>(), f->kern, f->consts);
+      abh::check_launch();
+    }
+    for (int l = 1; l < f->nlevels; ++l) {
+      if (npar(l) == 0) continue;
+      k_transfer_gen<D, N, false><<<npar(l), 256, 0, s>>>(par + t.par_off[l], npar(l),
+                                                        t.nodes.as<int32_t>(), f->G.as<double>(),
+                                                        f->consts);
+      abh::check_launch();
+    }
+    if (f->nleaf > 0) {
+      k_leaf_gen<D, N><<<f->nleaf, 64, 0, s>>>(
+          t.leaf_node.as<int32_t>(), f->nleaf, t.leaves.as<int2>(), f->lists.p2p_ptr.as<int32_t>(),
+          f->lists.p2p_src.as<int32_t>(), f->box.as<double>(), t.pos.as<double>(),
+          f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(), f->kern, f->consts);
+      abh::check_launch();
+      k_scatter_rhs<<<unsigned(abh::ceil_div(n * nr, 256)), 256, 0, s>>>(
+          t.order.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
+      abh::check_launch();
+    }
+  }
+}
+
 template <class F> void dispatch_order(int order, F &&fn) {
   switch (order) {
     case 3: fn(std::integral_constant<int, 3>()); break;
@@ -1688,16 +2130,17 @@ abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low, const do
   if (!f) return ABH_ERR_OUT_OF_MEMORY;
   abh_status st = abh::guard(ctx, [&] {
     ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
-    ABH_REQUIRE(dim == 3, ABH_ERR_UNSUPPORTED, "the FMM is implemented for dim == 3");
+    ABH_REQUIRE(dim >= 1 && dim <= 3, ABH_ERR_UNSUPPORTED, "dim must be 1, 2 or 3");
     ABH_REQUIRE(n >= 0 && n < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "n out of range");
     ABH_REQUIRE(n == 0 || pos, ABH_ERR_INVALID_ARG, "pos is null");
     ABH_REQUIRE(n_leaf > 0, ABH_ERR_INVALID_ARG, "n_particles_in_leaf must be positive");
     ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
                 ABH_ERR_UNSUPPORTED, "unknown kernel type");
     f->ctx = ctx;
+    f->dim = dim;
     f->n = n;
     f->order = order;
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < dim; ++d) {
       ABH_REQUIRE(high[d] > low[d], ABH_ERR_INVALID_ARG, "domain high must exceed low");
       f->bounds.lo[d] = low[d];
       f->bounds.hi[d] = high[d];
@@ -1706,7 +2149,12 @@ abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low, const do
     f->kern.param = kernel->param;
     f->kern.param2 = kernel->param * kernel->param;
     abh::set_inverses(f->kern);
-    dispatch_order(order, [&](auto NN) { create_impl<decltype(NN)::value>(f, pos, n_leaf); });
+    dispatch_order(order, [&](auto NN) {
+      constexpr int NV = decltype(NN)::value;
+      if (dim == 3) create_impl<NV>(f, pos, n_leaf);
+      else if (dim == 2) create_gen<2, NV>(f, pos, n_leaf);
+      else create_gen<1, NV>(f, pos, n_leaf);
+    });
   });
   if (st != ABH_OK) {
     delete f;
@@ -1722,6 +2170,7 @@ abh_status abh_fmm_compress(abh_fmm *f, double eps, int *padded_rank, int *ranks
   return abh::guard(f->ctx, [&] {
     ABH_REQUIRE(std::isfinite(eps) && eps < 1.0, ABH_ERR_INVALID_ARG,
                 "compression tolerance must be finite and below 1");
+    ABH_REQUIRE(f->dim == 3, ABH_ERR_UNSUPPORTED, "M2L compression is implemented for dim == 3");
     dispatch_order(f->order, [&](auto NN) { compress_impl<decltype(NN)::value>(f, eps); });
     if (padded_rank) *padded_rank = f->comp_kp;
     if (ranks)
@@ -1758,7 +2207,10 @@ abh_status abh_fmm_apply(abh_fmm *f, int nrhs, const double *x, int64_t ldx, dou
     ABH_REQUIRE(x && y, ABH_ERR_INVALID_ARG, "x or y is null");
     ABH_REQUIRE(ldx >= f->n && ldy >= f->n, ABH_ERR_INVALID_ARG, "leading dimension too small");
     dispatch_order(f->order, [&](auto NN) {
-      apply_impl<decltype(NN)::value>(f, nrhs, x, ldx, y, ldy, precision);
+      constexpr int NV = decltype(NN)::value;
+      if (f->dim == 3) apply_impl<NV>(f, nrhs, x, ldx, y, ldy, precision);
+      else if (f->dim == 2) apply_gen<2, NV>(f, nrhs, x, ldx, y, ldy);
+      else apply_gen<1, NV>(f, nrhs, x, ldx, y, ldy);
     });
   });
 }

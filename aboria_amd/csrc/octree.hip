@@ -17,7 +17,7 @@
 //      level's nodes; k_oct_assign writes the child codes, the leaves and
 //      the next active list.  The host reads one count pair per level.
 //
-// Traversal (octree_lists, 3D): the dual traversal of the FMM / H2 matrix
+// Traversal (octree_lists): the dual traversal of the FMM / H2 matrix
 // evaluated level by level instead of by recursion.  The strong (too close
 // for M2L) connections of every node are derived from its parent's, in the
 // parent's order, children in child order; a count pass, a scan and a fill
@@ -194,21 +194,22 @@ __global__ __launch_bounds__(kThreads) void k_oct_geometry(int64_t b, int64_t cn
 
 // theta_condition::check (src/detail/FastMultipoleMethod.h:1249-1279): the
 // source box is too close for M2L when the largest side of either box
-// exceeds their distance.  t, s: lo[3] then hi[3].
+// exceeds their distance.  t, s: lo[D] then hi[D].
+template <int D>
 __device__ __forceinline__ bool theta_close(const double *t, const double *s) {
   double md = 2.2250738585072014e-308;  // numeric_limits<double>::min()
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const double w = __dsub_rn(t[3 + i], t[i]);
+  for (int i = 0; i < D; ++i) {
+    const double w = __dsub_rn(t[D + i], t[i]);
     if (w > md) md = w;
   }
   double sq = 0.0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < D; ++i) {
     double dist = 0.0;
-    if (t[3 + i] < s[i]) dist = __dsub_rn(s[i], t[3 + i]);
-    else if (s[3 + i] < t[i]) dist = __dsub_rn(t[i], s[3 + i]);
-    const double w = __dsub_rn(s[3 + i], s[i]);
+    if (t[D + i] < s[i]) dist = __dsub_rn(s[i], t[D + i]);
+    else if (s[D + i] < t[i]) dist = __dsub_rn(t[i], s[D + i]);
+    const double w = __dsub_rn(s[D + i], s[i]);
     if (w > md) md = w;
     sq = __dadd_rn(sq, __dmul_rn(dist, dist));
   }
@@ -218,7 +219,7 @@ __device__ __forceinline__ bool theta_close(const double *t, const double *s) {
 // Strong and weak connections of the nodes of one level.  Count pass:
 // strong and weak counts per node (into ptr, scanned afterwards).  Fill
 // pass: entries at base + the scanned offset; ptr made absolute.
-template <bool FILL>
+template <int D, bool FILL>
 __global__ __launch_bounds__(kThreads) void k_oct_conn(
     int level, int64_t b, int64_t cnt, const int32_t *__restrict__ nodes,
     const int32_t *__restrict__ parent, const double *__restrict__ box,
@@ -228,9 +229,10 @@ __global__ __launch_bounds__(kThreads) void k_oct_conn(
   const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= cnt) return;
   const int64_t k = b + i;
-  double tb[6];
+  constexpr int NCH = 1 << D;
+  double tb[2 * D];
 #pragma unroll
-  for (int j = 0; j < 6; ++j) tb[j] = box[k * 6 + j];
+  for (int j = 0; j < 2 * D; ++j) tb[j] = box[k * 2 * D + j];
   uint32_t ns = 0, nw = 0;
   int32_t *so = nullptr, *wo = nullptr;
   if (FILL) {
@@ -244,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_oct_conn(
     }
   }
   auto test = [&](int32_t src) {
-    if (theta_close(tb, box + int64_t(src) * 6)) {
+    if (theta_close<D>(tb, box + int64_t(src) * 2 * D)) {
       if (FILL) so[ns] = src;
       ++ns;
     } else {
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_oct_conn(
     }
   };
   if (level == 1) {
-    for (int c = 0; c < 8; ++c) test(c);
+    for (int c = 0; c < NCH; ++c) test(c);
   } else {
     const int32_t p = parent[k];
     const uint32_t st = s_ptr[p];
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_oct_conn(
         if (FILL) so[ns] = src;
         ++ns;
       } else {
-        for (int c = 0; c < 8; ++c) test(code + c);
+        for (int c = 0; c < NCH; ++c) test(code + c);
       }
     }
   }
@@ -278,6 +280,7 @@ __global__ __launch_bounds__(kThreads) void k_oct_conn(
 }
 
 // non-empty leaves under every node of one level (children first)
+template <int D>
 __global__ __launch_bounds__(kThreads) void k_oct_sub(int64_t b, int64_t cnt,
                                                       const int32_t *__restrict__ nodes,
                                                       int32_t *__restrict__ sub) {
@@ -290,14 +293,14 @@ __global__ __launch_bounds__(kThreads) void k_oct_sub(int64_t b, int64_t cnt,
   else if (code < 0) v = 1;
   else {
     v = 0;
-    for (int c = 0; c < 8; ++c) v += sub[code + c];
+    for (int c = 0; c < (1 << D); ++c) v += sub[code + c];
   }
   sub[k] = v;
 }
 
 // P2P lists: the strong connections of every non-empty leaf expanded depth
 // first (children in order) to their non-empty leaves
-template <bool FILL>
+template <int D, bool FILL>
 __global__ __launch_bounds__(kThreads) void k_oct_p2p(
     int64_t nleaf, const int32_t *__restrict__ leaf_node, const int2 *__restrict__ leaves,
     const int32_t *__restrict__ nodes, const int32_t *__restrict__ sub,
@@ -326,7 +329,8 @@ __global__ __launch_bounds__(kThreads) void k_oct_p2p(
     const int2 sr = leaves[sl];
     acc += int64_t(tr.y - tr.x) * (sr.y - sr.x);
   };
-  int32_t base[9], nxt[9];
+  constexpr int ML = max_level_of<D>();
+  int32_t base[ML + 1], nxt[ML + 1];
   for (int32_t j = 0; j < ct; ++j) {
     const int32_t code0 = nodes[s_idx[st + j]];
     if (code0 < 0) {
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(kThreads) void k_oct_p2p(
     base[0] = code0;
     nxt[0] = 0;
     while (top >= 0) {
-      if (nxt[top] == 8) {
+      if (nxt[top] == (1 << D)) {
         --top;
         continue;
       }
@@ -345,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void k_oct_p2p(
       if (code < 0) {
         emit(code);
       } else {
-        ++top;  // the source is on level 1 or deeper: top stays below 8
+        ++top;  // the source is on level 1 or deeper: top stays below ML
         base[top] = code;
         nxt[top] = 0;
       }
@@ -485,27 +489,9 @@ template <int D> void geometry_impl(abh_context *ctx, const abh::Octree &t, doub
   }
 }
 
-}  // namespace
-
-namespace abh {
-
-void octree_update(abh_context *ctx, Octree &t, int64_t n, double *pos, uint8_t *alive) {
-  ABH_REQUIRE(n >= 0 && n < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "n out of range");
-  ABH_REQUIRE(n == 0 || pos, ABH_ERR_INVALID_ARG, "pos is null");
-  if (t.dim == 1) update_impl<1>(ctx, t, n, pos, alive);
-  else if (t.dim == 2) update_impl<2>(ctx, t, n, pos, alive);
-  else update_impl<3>(ctx, t, n, pos, alive);
-}
-
-void octree_geometry(abh_context *ctx, const Octree &t, double *box, int32_t *coord) {
-  if (t.dim == 1) geometry_impl<1>(ctx, t, box, coord);
-  else if (t.dim == 2) geometry_impl<2>(ctx, t, box, coord);
-  else geometry_impl<3>(ctx, t, box, coord);
-}
-
-void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLists &L,
-                  bool weak_lists) {
-  ABH_REQUIRE(t.dim == 3, ABH_ERR_UNSUPPORTED, "interaction lists are built for dim == 3");
+template <int D>
+void lists_impl(abh_context *ctx, const abh::Octree &t, const double *box, abh::OctreeLists &L,
+                bool weak_lists) {
   hipStream_t s = ctx->stream;
   const int64_t nn = t.n_nodes, nleaf = t.n_leaves;
   DevBuf s_cnt, s_ptr, s_idx, sub, ctr;
@@ -526,7 +512,7 @@ void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLi
   for (int l = 1; l <= t.levels; ++l) {
     const int64_t b = t.level_begin[size_t(l) - 1], cnt = t.level_begin[size_t(l)] - b;
     const unsigned g = unsigned(abh::ceil_div(cnt, kThreads));
-    k_oct_conn<false><<<g, kThreads, 0, s>>>(l, b, cnt, t.nodes.as<int32_t>(),
+    k_oct_conn<D, false><<<g, kThreads, 0, s>>>(l, b, cnt, t.nodes.as<int32_t>(),
                                              t.parent.as<int32_t>(), box, s_cnt.as<int32_t>(),
                                              s_ptr.as<uint32_t>(), s_idx.as<int32_t>(), 0, wptr,
                                              nullptr, 0, m2l);
@@ -553,7 +539,7 @@ void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLi
     };
     grow(s_idx, s_total, s_total + h[0] + 1);
     if (wptr) grow(L.weak_src, w_total, w_total + h[1] + 1);
-    k_oct_conn<true><<<g, kThreads, 0, s>>>(l, b, cnt, t.nodes.as<int32_t>(),
+    k_oct_conn<D, true><<<g, kThreads, 0, s>>>(l, b, cnt, t.nodes.as<int32_t>(),
                                             t.parent.as<int32_t>(), box, s_cnt.as<int32_t>(),
                                             s_ptr.as<uint32_t>(), s_idx.as<int32_t>(),
                                             uint32_t(s_total), wptr, L.weak_src.as<int32_t>(),
@@ -569,7 +555,7 @@ void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLi
   // non-empty leaves per subtree, deepest level first
   for (int l = t.levels; l >= 1; --l) {
     const int64_t b = t.level_begin[size_t(l) - 1], cnt = t.level_begin[size_t(l)] - b;
-    k_oct_sub<<<unsigned(abh::ceil_div(cnt, kThreads)), kThreads, 0, s>>>(
+    k_oct_sub<D><<<unsigned(abh::ceil_div(cnt, kThreads)), kThreads, 0, s>>>(
         b, cnt, t.nodes.as<int32_t>(), sub.as<int32_t>());
     abh::check_launch();
   }
@@ -579,7 +565,7 @@ void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLi
   uint32_t ptot = 0;
   if (nleaf > 0) {
     const unsigned g = unsigned(abh::ceil_div(nleaf, kThreads));
-    k_oct_p2p<false><<<g, kThreads, 0, s>>>(nleaf, t.leaf_node.as<int32_t>(), t.leaves.as<int2>(),
+    k_oct_p2p<D, false><<<g, kThreads, 0, s>>>(nleaf, t.leaf_node.as<int32_t>(), t.leaves.as<int2>(),
                                             t.nodes.as<int32_t>(), sub.as<int32_t>(),
                                             s_cnt.as<int32_t>(), s_ptr.as<uint32_t>(),
                                             s_idx.as<int32_t>(), pptr, nullptr, inter);
@@ -589,7 +575,7 @@ void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLi
     ABH_HIP(hipStreamSynchronize(s));
     ABH_REQUIRE(ptot < (uint32_t(1) << 31), ABH_ERR_CAPACITY, "P2P list too large");
     L.p2p_src.ensure(size_t(ptot + 1) * 4);
-    k_oct_p2p<true><<<g, kThreads, 0, s>>>(nleaf, t.leaf_node.as<int32_t>(), t.leaves.as<int2>(),
+    k_oct_p2p<D, true><<<g, kThreads, 0, s>>>(nleaf, t.leaf_node.as<int32_t>(), t.leaves.as<int2>(),
                                            t.nodes.as<int32_t>(), sub.as<int32_t>(),
                                            s_cnt.as<int32_t>(), s_ptr.as<uint32_t>(),
                                            s_idx.as<int32_t>(), pptr, L.p2p_src.as<int32_t>(),
@@ -605,6 +591,31 @@ void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLi
   L.m2l = int64_t(hc[0]);
   L.interactions = int64_t(hc[1]);
   L.p2p_pairs = int64_t(ptot);
+}
+
+}  // namespace
+
+namespace abh {
+
+void octree_update(abh_context *ctx, Octree &t, int64_t n, double *pos, uint8_t *alive) {
+  ABH_REQUIRE(n >= 0 && n < (int64_t(1) << 31), ABH_ERR_INVALID_ARG, "n out of range");
+  ABH_REQUIRE(n == 0 || pos, ABH_ERR_INVALID_ARG, "pos is null");
+  if (t.dim == 1) update_impl<1>(ctx, t, n, pos, alive);
+  else if (t.dim == 2) update_impl<2>(ctx, t, n, pos, alive);
+  else update_impl<3>(ctx, t, n, pos, alive);
+}
+
+void octree_geometry(abh_context *ctx, const Octree &t, double *box, int32_t *coord) {
+  if (t.dim == 1) geometry_impl<1>(ctx, t, box, coord);
+  else if (t.dim == 2) geometry_impl<2>(ctx, t, box, coord);
+  else geometry_impl<3>(ctx, t, box, coord);
+}
+
+void octree_lists(abh_context *ctx, const Octree &t, const double *box, OctreeLists &L,
+                  bool weak_lists) {
+  if (t.dim == 1) lists_impl<1>(ctx, t, box, L, weak_lists);
+  else if (t.dim == 2) lists_impl<2>(ctx, t, box, L, weak_lists);
+  else lists_impl<3>(ctx, t, box, L, weak_lists);
 }
 
 }  // namespace abh
@@ -718,11 +729,10 @@ abh_status abh_octree_interaction_lists(const abh_octree *o, int32_t *p2p_ptr, i
   if (!o || !counts) return ABH_ERR_INVALID_ARG;
   return abh::guard(o->ctx, [&] {
     const abh::Octree &t = o->t;
-    ABH_REQUIRE(t.dim == 3, ABH_ERR_UNSUPPORTED, "interaction lists are built for dim == 3");
     hipStream_t s = o->ctx->stream;
     DevBuf box, coord;
-    box.ensure(size_t(t.n_nodes + 1) * 48);
-    coord.ensure(size_t(t.n_nodes + 1) * 12);
+    box.ensure(size_t(t.n_nodes + 1) * 16 * t.dim);
+    coord.ensure(size_t(t.n_nodes + 1) * 4 * t.dim);
     abh::octree_geometry(o->ctx, t, box.as<double>(), coord.as<int32_t>());
     abh::OctreeLists L;
     const bool weak = m2l_ptr || m2l_src;

@@ -46,7 +46,7 @@ void octree_update(abh_context *ctx, Octree &t, int64_t n, double *pos, uint8_t 
 // (src/OctTree.h:633-644).  box and coord are device arrays of n_nodes rows.
 void octree_geometry(abh_context *ctx, const Octree &t, double *box, int32_t *coord);
 
-// The dual traversal's interaction lists (3D; src/H2Matrix.h:998-1090,
+// The dual traversal's interaction lists (src/H2Matrix.h:998-1090,
 // src/FastMultipoleMethod.h:442-548 with theta_condition,
 // src/detail/FastMultipoleMethod.h:1249-1279).
 struct OctreeLists {

@@ -441,7 +441,7 @@ typedef struct abh_octree_view {
 } abh_octree_view;
 abh_status abh_octree_query_view(const abh_octree *tree, abh_octree_view *view);
 
-/* The interaction lists of the dual tree traversal over one 3D tree (the
+/* The interaction lists of the dual tree traversal over the tree (the
  * strong / weak connectivity of H2Matrix::generate_matrices,
  * src/H2Matrix.h:998-1090, the same walk as
  * FastMultipoleMethod's calculate_dive_M2L_and_L2L,
@@ -478,10 +478,12 @@ typedef struct abh_fmm_info {
 /* Replaces create_fmm_operator<N>(particles, particles, f_pos, f)
  * (src/Operators.h:396-407, KernelFMM src/Kernels.h:533-576) for a particle
  * set searched by a HyperOctree (src/OctTree.h:102-475) built here with
- * n_particles_in_leaf.  dim must be 3; order (Chebyshev nodes per dimension)
- * in 3..8.  pos: device, n x 3.  Builds the tree, the interaction lists and
- * the M2L operators (synchronises; the M2L operators take
- * levels x 343 x order^6 x 8 bytes, 3.6 GB for order 8 and 5 levels). */
+ * n_particles_in_leaf.  dim in 1..3; order (Chebyshev nodes per dimension)
+ * in 3..8.  pos: device, n x dim.  Builds the tree, the interaction lists
+ * and the M2L operators (synchronises; the M2L operators take
+ * levels x 7^dim x order^(2 dim) x 8 bytes, 3.6 GB for order 8, dim 3 and
+ * 5 levels).  In 1D and 2D the applies run the M2L over the dual
+ * traversal's weak lists in FP64 on the vector units. */
 abh_status abh_fmm_create(abh_context *ctx, int dim, const double *low,
                           const double *high, int64_t n, const double *pos,
                           double n_particles_in_leaf, int order,
@@ -498,7 +500,8 @@ const int32_t *abh_fmm_order(const abh_fmm *fmm);
  * abh_fmm_compress with a padded rank up to 192, the compressed M2L as an
  * FP16 split with power-of-two scaling per level; otherwise (uncompressed
  * orders 7 and 8, whose BF16 split would floor the error near 1e-6) its M2L
- * is FP64.  P2M, L2P and P2P run in FP64 on the matrix cores in both. */
+ * is FP64.  P2M, L2P and P2P run in FP64 on the matrix cores in both.  In
+ * 1D and 2D both precisions compute in FP64. */
 abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
                          double *y, int64_t ldy, abh_precision precision);
 
@@ -506,7 +509,8 @@ abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
  * low-rank M2L blocks): per level, the M2L operators K_o become
  * U (U^T K_o U) U^T with U the left singular vectors of the level's stacked
  * far-offset operators whose singular values exceed eps times the largest
- * (eps in (0, 1); eps <= 0 restores the uncompressed operators).  Later
+ * (eps in (0, 1); eps <= 0 restores the uncompressed operators; dim 3
+ * only, ABH_ERR_UNSUPPORTED otherwise).  Later
  * applies (either precision) contract the rank-k operators in FP64 on the
  * matrix cores.  padded_rank (host, may be NULL): the rank the kernels run
  * at (the largest level rank rounded up to a multiple of 16 up to 192, then

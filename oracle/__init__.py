@@ -304,12 +304,12 @@ class Octree:
         self.positions = pos[self.order]
 
     def interaction_lists(self):
-        """The dual traversal's lists (3D): dict with p2p_ptr, p2p_src (per
+        """The dual traversal's lists: dict with p2p_ptr, p2p_src (per
         leaf, source leaves), m2l_ptr, m2l_src (per node, source nodes) and
         the P2P particle-pair count."""
         counts = np.zeros(3, np.int64)
         if lib().orc_octree_lists(self.h, counts, None, None, None, None) != 0:
-            raise ValueError("interaction lists are restated for dim 3")
+            raise ValueError("interaction lists failed")
         out = {"p2p_ptr": np.zeros(self.n_leaves + 1, np.int32),
                "p2p_src": np.zeros(int(counts[0]), np.int32),
                "m2l_ptr": np.zeros(self.n_nodes + 1, np.int32),

@@ -51,8 +51,8 @@ inline uint64_t digest_mix(uint64_t x) {
   return x ^ (x >> 31);
 }
 
-template <int N> void fmm_n(Octree<3> *t, const Kernel *k, const double *q, double *y) {
-  FMM<3, N> f(*t, *k);
+template <int D, int N> void fmm_n(const Octree<D> &t, const Kernel *k, const double *q, double *y) {
+  FMM<D, N> f(t, *k);
   f.matvec(q, y);
 }
 
@@ -70,7 +70,7 @@ struct OctH {
   Octree<1> t1;
   Octree<2> t2;
   Octree<3> t3;
-  Lists<3> lists;
+  std::vector<std::vector<int>> weak, p2p;  // interaction lists, filled on demand
   bool have_lists = false;
   template <class F> void with(F &&f) {
     if (dim == 1) f(t1);
@@ -324,38 +324,43 @@ void orc_octree_get(void *p, int32_t *order, int32_t *tags, int32_t *nodes, int3
   });
 }
 
-// Interaction lists of a 3D tree.  counts: [0] P2P entries, [1] M2L
+// Interaction lists of the tree.  counts: [0] P2P entries, [1] M2L
 // entries, [2] particle pairs of the P2P lists; arrays (may be NULL):
 // p2p_ptr (leaves + 1), p2p_src, m2l_ptr (nodes + 1), m2l_src.
 int orc_octree_lists(void *p, int64_t *counts, int32_t *p2p_ptr, int32_t *p2p_src,
                      int32_t *m2l_ptr, int32_t *m2l_src) {
   auto *h = static_cast<OctH *>(p);
-  if (h->dim != 3) return -1;
   if (!h->have_lists) {
-    interaction_lists<3>(h->t3, h->lists);
+    h->with([&](auto &t) {
+      constexpr int D = sizeof(t.bounds.lo) / sizeof(double);
+      Lists<D> L;
+      interaction_lists<D>(t, L);
+      h->weak = std::move(L.weak);
+      h->p2p = std::move(L.p2p);
+    });
     h->have_lists = true;
   }
-  const Octree<3> &t = h->t3;
-  const Lists<3> &L = h->lists;
   int64_t np = 0, nm = 0, inter = 0;
-  for (size_t l = 0; l < L.p2p.size(); ++l) {
-    if (p2p_ptr) p2p_ptr[l] = int32_t(np);
-    for (int s : L.p2p[l]) {
-      if (p2p_src) p2p_src[np] = s;
-      ++np;
-      inter += int64_t(t.leaves[l].second - t.leaves[l].first) *
-               (t.leaves[size_t(s)].second - t.leaves[size_t(s)].first);
+  h->with([&](auto &t) {
+    for (size_t l = 0; l < h->p2p.size(); ++l) {
+      if (p2p_ptr) p2p_ptr[l] = int32_t(np);
+      for (int s : h->p2p[l]) {
+        if (p2p_src) p2p_src[np] = s;
+        ++np;
+        inter += int64_t(t.leaves[l].second - t.leaves[l].first) *
+                 (t.leaves[size_t(s)].second - t.leaves[size_t(s)].first);
+      }
     }
-  }
-  if (p2p_ptr) p2p_ptr[L.p2p.size()] = int32_t(np);
-  for (size_t k = 0; k < L.weak.size(); ++k) {
+    if (p2p_ptr) p2p_ptr[h->p2p.size()] = int32_t(np);
+  });
+  for (size_t k = 0; k < h->weak.size(); ++k) {
     if (m2l_ptr) m2l_ptr[k] = int32_t(nm);
-    for (int s : L.weak[k]) {
+    for (int s : h->weak[k]) {
       if (m2l_src) m2l_src[nm] = s;
       ++nm;
     }
   }
-  if (m2l_ptr) m2l_ptr[L.weak.size()] = int32_t(nm);
+  if (m2l_ptr) m2l_ptr[h->weak.size()] = int32_t(nm);
   counts[0] = np;
   counts[1] = nm;
   counts[2] = inter;
@@ -365,18 +370,21 @@ int orc_octree_lists(void *p, int64_t *counts, int32_t *p2p_ptr, int32_t *p2p_sr
 // y += K q with Chebyshev order n; q, y in tree (sorted) order.
 int orc_fmm_matvec(void *p, int order, const Kernel *k, const double *q, double *y) {
   auto *h = static_cast<OctH *>(p);
-  if (h->dim != 3) return -1;
-  Octree<3> *t = &h->t3;
-  switch (order) {
-    case 2: fmm_n<2>(t, k, q, y); return 0;
-    case 3: fmm_n<3>(t, k, q, y); return 0;
-    case 4: fmm_n<4>(t, k, q, y); return 0;
-    case 5: fmm_n<5>(t, k, q, y); return 0;
-    case 6: fmm_n<6>(t, k, q, y); return 0;
-    case 7: fmm_n<7>(t, k, q, y); return 0;
-    case 8: fmm_n<8>(t, k, q, y); return 0;
-    default: return -1;
-  }
+  int rc = 0;
+  h->with([&](auto &t) {
+    constexpr int D = sizeof(t.bounds.lo) / sizeof(double);
+    switch (order) {
+      case 2: fmm_n<D, 2>(t, k, q, y); break;
+      case 3: fmm_n<D, 3>(t, k, q, y); break;
+      case 4: fmm_n<D, 4>(t, k, q, y); break;
+      case 5: fmm_n<D, 5>(t, k, q, y); break;
+      case 6: fmm_n<D, 6>(t, k, q, y); break;
+      case 7: fmm_n<D, 7>(t, k, q, y); break;
+      case 8: fmm_n<D, 8>(t, k, q, y); break;
+      default: rc = -1;
+    }
+  });
+  return rc;
 }
 
 // Chebyshev helpers for the closed-form checks

@@ -1,7 +1,8 @@
 """GPU black-box Chebyshev FMM against the CPU oracle (same tree, same
 traversal) and against an FP64 direct sum.
 
-Bar: the FP64 GPU FMM equals the oracle's FMM to 1e-11 relative (2-norm;
+Bar: the FP64 GPU FMM equals the oracle's FMM to 1e-11 relative in 1-3
+dimensions (2-norm;
 summation order and the translated M2L operators differ only by rounding);
 the mixed form (BF16x3 MFMA M2L with FP32 accumulation; P2M, L2P and P2P
 in FP64 on the matrix cores as in the FP64 form) is within 5e-5 relative of
@@ -218,6 +219,62 @@ def test_fmm_tree_leaf_rule(ctx):
     q = np.linspace(-1.0, 1.0, len(pos))
     y = (f @ torch.tensor(q, device="cuda")).cpu().numpy()
     assert rel(y, oracle.dense_matvec(pos, pos, (KI, 1e-3), q)) < 1e-11
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("order", [3, 5, 8])
+def test_fmm_lower_dims_match_oracle(ctx, dim, order):
+    """1D and 2D (tests/fmm.h runs helper_fast_methods<D> for D = 1, 2, 3):
+    the device tree's order and pair counts are the oracle's, the FP64 FMM
+    equals the oracle's FMM to 1e-11 on two columns of a 20-column apply
+    (crossing the 16-column pass), MIXED equals FP64 bit for bit (both
+    compute in FP64 here) and the M2L compression is refused.  In 1D a
+    cluster of 200 particles within 1e-6 drives leaves to about level 24."""
+    print("fmm lower dims", dim, order, flush=True)
+    n, leaf = (4000, 12) if dim == 1 else (3000, 16)
+    rng = np.random.default_rng(60 + 10 * dim + order)
+    pos = rng.random((n, dim))
+    if dim == 1:
+        pos[:200, 0] = 0.3 + 1e-6 * rng.random(200)
+    X = rng.uniform(-1, 1, (20, n))
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0.0] * dim, [1.0] * dim, leaf,
+                       order, ab.Kernel.smooth_inv_r(1e-3))
+    xt = torch.tensor(X, device="cuda")
+    y = f.apply_colmajor(xt, torch.zeros_like(xt)).cpu().numpy()
+    ym = f.apply_colmajor(xt, torch.zeros_like(xt), ab.MIXED).cpu().numpy()
+    assert np.array_equal(y, ym)
+    t = oracle.Octree([0.0] * dim, [1.0] * dim, pos, leaf)
+    assert np.array_equal(f.order().cpu().numpy(), t.order)
+    L = t.interaction_lists()
+    info = f.info
+    assert info["levels"] == t.levels
+    assert info["m2l_pairs"] == len(L["m2l_src"]) and info["p2p_pairs"] == len(L["p2p_src"])
+    if dim == 1:
+        assert t.levels >= 20
+    for r in (0, 17):
+        yo = np.empty(n)
+        yo[t.order] = t.fmm_matvec(order, (KI, 1e-3), X[r][t.order])
+        assert rel(y[r], yo) < 1e-11, (r, rel(y[r], yo))
+    with pytest.raises(Exception):
+        f.compress(1e-6)
+    f.close()
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_fmm_lower_dims_reference_accuracy(ctx, dim):
+    """tests/fmm.h:165-215 in 1D and 2D: N = 1000, 10 per leaf, cosine
+    source, sqrt(|dx|^2 + 0.01), order 3: sum (y - y_direct)^2 /
+    sum y_direct^2 < 1e-2 against the FP64 direct sum."""
+    n = 1000
+    rng = np.random.default_rng(70 + dim)
+    pos = rng.random((n, dim))
+    q = np.prod(np.cos(2 * np.pi * pos), axis=1) / n
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda"), [0.0] * dim, [1.0] * dim, 10, 3,
+                       ab.Kernel.multiquadric(0.1))
+    y = (f @ torch.tensor(q, device="cuda")).cpu().numpy()
+    ref = oracle.dense_matvec(pos, pos, (KM, 0.1), q)
+    assert np.sum((y - ref) ** 2) / np.sum(ref ** 2) < 1e-2
+    f.close()
 
 
 def test_fmm_c3_scale(ctx):

@@ -6,7 +6,7 @@ and leaves in dimensions 1-3, for uniform, clustered (the deepest level
 reached), tiny and empty inputs; the domain enforcement keeps the cell
 list's rule (out-of-domain and non-finite particles dropped, periodic ones
 wrapped); the device dual traversal gives the oracle's P2P and M2L lists
-entry for entry; and the device query header (tests/cpp, built with
+entry for entry in dimensions 1-3; and the device query header (tests/cpp, built with
 contraction on) finds exactly the brute-force FP64 neighbours of every
 centre through the tree, periodic images included.
 """
@@ -113,21 +113,24 @@ def test_octree_domain_enforcement(ctx):
     assert np.array_equal(t2.nodes().cpu().numpy(), o2.nodes)
 
 
-@pytest.mark.parametrize("case", [("uniform", 3000, 16), ("clustered", 8000, 8),
-                                  ("large", 200000, 64)])
+@pytest.mark.parametrize("case", [("uniform", 3000, 16, 3), ("clustered", 8000, 8, 3),
+                                  ("large", 200000, 64, 3), ("uniform", 5000, 10, 2),
+                                  ("clustered", 6000, 6, 2), ("uniform", 4000, 8, 1),
+                                  ("clustered", 6000, 5, 1)])
 def test_octree_interaction_lists_match_oracle(ctx, case):
-    kind, n, leaf = case
-    print("octree lists", kind, n, flush=True)
+    kind, n, leaf, dim = case
+    print("octree lists", kind, n, dim, flush=True)
     rng = np.random.default_rng(n)
     if kind == "clustered":
-        c = rng.random((8, 3))
-        pos = np.clip(c[rng.integers(0, 8, n)] + 0.04 * rng.standard_normal((n, 3)), 0,
+        c = rng.random((8, dim))
+        w = 0.04 if dim > 1 else 1e-4
+        pos = np.clip(c[rng.integers(0, 8, n)] + w * rng.standard_normal((n, dim)), 0,
                       np.nextafter(1.0, 0))
     else:
-        pos = rng.random((n, 3))
+        pos = rng.random((n, dim))
     t, _, _ = _build(ctx, pos, leaf)
     g = t.interaction_lists()
-    o = oracle.Octree([0.0] * 3, [1.0] * 3, pos, leaf).interaction_lists()
+    o = oracle.Octree([0.0] * dim, [1.0] * dim, pos, leaf).interaction_lists()
     for k in ("p2p_ptr", "p2p_src", "m2l_ptr", "m2l_src"):
         assert np.array_equal(g[k].cpu().numpy(), o[k]), k
     assert g["interactions"] == o["interactions"]

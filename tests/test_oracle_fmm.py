@@ -127,3 +127,24 @@ def test_octree_leaf_rule_is_inclusive():
     leaves7 = {tuple(l) for l in t.leaves for i in in7 if l[0] <= i < l[1]}
     assert len(leaves7) >= 2
     assert t.levels >= 2
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_fmm_lower_dimensions_converge(dim):
+    """tests/fmm.h runs the FMM in 1-3 dimensions (helper_fast_methods<D>):
+    in 1D and 2D the black-box FMM against the direct sum, with the error
+    falling as the order rises (errors of a Chebyshev interpolant of the
+    smooth kernel over well-separated boxes)."""
+    N = 3000
+    rng = np.random.default_rng(20 + dim)
+    pos = rng.random((N, dim))
+    t = oracle.Octree([0.0] * dim, [1.0] * dim, pos, 12)
+    q = rng.uniform(-1, 1, N)
+    kern = (KI, 1e-3)
+    ref = direct(t.positions, q, kern)
+    errs = []
+    for order in (3, 5, 8):
+        y = t.fmm_matvec(order, kern, q)
+        errs.append(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+    assert errs[0] > errs[1] > errs[2], errs
+    assert errs[2] < 1e-6, errs

@@ -8,16 +8,17 @@ and the partition invariants (tests/data_structures.h:327-393: every
 particle in exactly one leaf, inside the leaf's box).
 
 Lists (src/H2Matrix.h:998-1090, src/FastMultipoleMethod.h:442-548):
-  * a complete tree (one particle per finest box, n_leaf = 1) against the
-    closed form: same-level boxes are strongly connected exactly when they
-    touch, so the M2L pairs of level l >= 2 number (sum_x a(x))^3 -
-    (sum_x b(x))^3 with a(x) the children of the parent's neighbours and
-    b(x) the neighbours along one axis, and every leaf's P2P sources are its
-    touching boxes;
-  * on irregular trees, for every non-empty target leaf the M2L sources of
-    the leaf and its ancestors and its P2P sources cover every particle
-    exactly once, and every M2L pair is well separated (theta_condition
-    false, recomputed here from the boxes) and on one level.
+  * a complete tree (one particle per finest box, n_leaf = 1) in 1-3
+    dimensions against the closed form: same-level boxes are strongly
+    connected exactly when they touch, so the M2L pairs of level l >= 2
+    number (sum_x a(x))^D - (sum_x b(x))^D with a(x) the children of the
+    parent's neighbours and b(x) the neighbours along one axis, and every
+    leaf's P2P sources are its touching boxes;
+  * on irregular trees in 1-3 dimensions, for every non-empty target leaf
+    the M2L sources of the leaf and its ancestors and its P2P sources cover
+    every particle exactly once, and every M2L pair is well separated
+    (theta_condition false, recomputed here from the boxes) and on one
+    level.
 """
 import numpy as np
 import pytest
@@ -82,11 +83,12 @@ def subtree_range(t):
 
 
 def theta_close(tb, sb):
+    dim = len(tb[0])
     md = np.finfo(np.float64).tiny
-    for i in range(3):
+    for i in range(dim):
         md = max(md, tb[1][i] - tb[0][i])
     sq = 0.0
-    for i in range(3):
+    for i in range(dim):
         dist = 0.0
         if tb[1][i] < sb[0][i]:
             dist = sb[0][i] - tb[1][i]
@@ -130,13 +132,13 @@ def test_octree_partition_any_dim(dim):
         assert np.all(p >= np.array(box[k][0])) and np.all(p < np.array(box[k][1]))
 
 
-def test_lists_complete_tree_closed_form():
-    L = 3
+@pytest.mark.parametrize("dim,L", [(1, 6), (2, 4), (3, 3)])
+def test_lists_complete_tree_closed_form(dim, L):
     m = 1 << L
     g = (np.arange(m) + 0.5) / m
-    pos = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T.copy()
-    t = oracle.Octree([0.0] * 3, [1.0] * 3, pos, 1)
-    assert t.levels == L and t.n_leaves == m ** 3
+    pos = np.array(np.meshgrid(*([g] * dim), indexing="ij")).reshape(dim, -1).T.copy()
+    t = oracle.Octree([0.0] * dim, [1.0] * dim, pos, 1)
+    assert t.levels == L and t.n_leaves == m ** dim
     lists = t.interaction_lists()
     want_m2l = 0
     for lev in range(2, L + 1):
@@ -145,29 +147,30 @@ def test_lists_complete_tree_closed_form():
         px = x // 2
         a = 2 * (np.minimum(px + 1, side // 2 - 1) - np.maximum(px - 1, 0) + 1)
         b = np.minimum(x + 1, side - 1) - np.maximum(x - 1, 0) + 1
-        want_m2l += int(a.sum()) ** 3 - int(b.sum()) ** 3
+        want_m2l += int(a.sum()) ** dim - int(b.sum()) ** dim
     assert len(lists["m2l_src"]) == want_m2l
     x = np.arange(m)
     b = np.minimum(x + 1, m - 1) - np.maximum(x - 1, 0) + 1
-    assert lists["p2p_ptr"][-1] == int(b.sum()) ** 3
-    assert lists["interactions"] == int(b.sum()) ** 3  # one particle per leaf
+    assert lists["p2p_ptr"][-1] == int(b.sum()) ** dim
+    assert lists["interactions"] == int(b.sum()) ** dim  # one particle per leaf
 
 
+@pytest.mark.parametrize("dim", [1, 2, 3])
 @pytest.mark.parametrize("case", ["uniform", "clustered", "tiny"])
-def test_lists_partition_and_separation(case):
-    rng = np.random.default_rng({"uniform": 1, "clustered": 2, "tiny": 3}[case])
+def test_lists_partition_and_separation(case, dim):
+    rng = np.random.default_rng({"uniform": 1, "clustered": 2, "tiny": 3}[case] + 10 * dim)
     if case == "uniform":
-        pos, leaf = rng.random((3000, 3)), 16
+        pos, leaf = rng.random((3000, dim)), 16
     elif case == "clustered":
-        c = rng.random((6, 3))
-        pos = np.clip(c[rng.integers(0, 6, 4000)] + 0.03 * rng.standard_normal((4000, 3)), 0,
-                      np.nextafter(1.0, 0))
+        c = rng.random((6, dim))
+        pos = np.clip(c[rng.integers(0, 6, 4000)] + 0.03 * rng.standard_normal((4000, dim)),
+                      0, np.nextafter(1.0, 0))
         leaf = 8
     else:
-        pos, leaf = rng.random((20, 3)), 3
-    t = oracle.Octree([0.0] * 3, [1.0] * 3, pos, leaf)
+        pos, leaf = rng.random((20, dim)), 3
+    t = oracle.Octree([0.0] * dim, [1.0] * dim, pos, leaf)
     L = t.interaction_lists()
-    box, level, parent = tree_geometry(t, [0.0] * 3, [1.0] * 3)
+    box, level, parent = tree_geometry(t, [0.0] * dim, [1.0] * dim)
     lo, hi = subtree_range(t)
     n = len(pos)
     mp, ms = L["m2l_ptr"], L["m2l_src"]
