@@ -975,16 +975,20 @@ class SchwartzPreconditioner:
     a cell list (SchwartzPreconditioner<Eigen::LLT<MatrixXd>>,
     src/Preconditioners.h:1121-1537): analyzePattern + factorize happen at
     construction, solve(b) is _solve_impl.  The kernel is restricted to
-    |dx| <= radius (radius <= 0: every pair)."""
+    |dx| <= radius (radius <= 0: every pair).  coarse_grid_n > 0 adds the
+    coarse-grid domain of set_coarse_grid_n (particles sampled near an
+    (n+1)^dim lattice with a seeded shuffle, solved last)."""
 
-    def __init__(self, ctx, cells, kernel, radius, diag_shift=0.0):
+    def __init__(self, ctx, cells, kernel, radius, diag_shift=0.0, coarse_grid_n=0, seed=0):
         self.ctx = ctx
         self.cells = cells
         self.n = cells.n_alive
         kd = kernel.to_c()
         h = ctypes.c_void_p()
-        check(C.lib().abh_schwarz_create(ctx.handle, cells.handle, ctypes.byref(kd),
-                                         float(radius), float(diag_shift), ctypes.byref(h)),
+        check(C.lib().abh_schwarz_create_coarse(ctx.handle, cells.handle, ctypes.byref(kd),
+                                                float(radius), float(diag_shift),
+                                                int(coarse_grid_n), ctypes.c_uint64(seed),
+                                                ctypes.byref(h)),
               ctx.handle)
         self.handle = h
 

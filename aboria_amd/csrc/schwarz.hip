@@ -269,6 +269,11 @@ struct abh_schwarz {
   int nchunks = 0;
   size_t factor_bytes = 0;
   bool batch = false;  // built per problem of a dense batch (signed factors)
+  // the coarse-grid domain (set_coarse_grid_n): one domain of sampled
+  // particles, all interior, factorised like the others and applied after
+  // them, its rows overwriting theirs
+  abh_schwarz *coarse = nullptr;
+  ~abh_schwarz() { delete coarse; }
 };
 
 namespace {
@@ -279,19 +284,45 @@ void take(abh_schwarz *s, abh::SchwarzDomains &dm) {
   s->nint.swap(dm.nint);
 }
 
-void analyse_into(abh_schwarz *s, const abh_cells *c, const std::vector<double> &pos,
-             const std::vector<uint32_t> &beg, const std::vector<uint32_t> &end) {
+abh::GridView grid_view(const abh_cells *c) {
   abh::GridView g;
   g.dim = c->dim;
   for (int d = 0; d < c->dim; ++d) {
     g.size[d] = c->size[d];
     g.side[d] = c->side[d];
     g.lo[d] = c->lo[d];
+    g.hi[d] = c->hi[d];
+    g.inv[d] = c->inv[d];
   }
+  return g;
+}
+
+void analyse_into(abh_schwarz *s, const abh_cells *c, const std::vector<double> &pos,
+             const std::vector<uint32_t> &beg, const std::vector<uint32_t> &end) {
+  const abh::GridView g = grid_view(c);
   abh::SchwarzDomains dm;
   const char *err = abh::analyse_cells(g, pos, beg, end, dm);
   ABH_REQUIRE(!err, ABH_ERR_INVALID_ARG, err ? err : "");
   take(s, dm);
+}
+
+// the coarse-grid domain of n x ... x n lattice cells as a one-domain
+// preconditioner over the same particles
+void coarse_into(abh_schwarz *s, const abh_cells *c, const std::vector<uint32_t> &beg,
+                 const std::vector<uint32_t> &end, int n, uint64_t seed) {
+  std::vector<int32_t> idx;
+  const char *err = abh::coarse_grid(grid_view(c), beg, end, n, seed, idx);
+  ABH_REQUIRE(!err, ABH_ERR_INVALID_ARG, err ? err : "");
+  std::unique_ptr<abh_schwarz> q(new abh_schwarz());
+  q->ctx = s->ctx;
+  q->dim = s->dim;
+  q->n = s->n;
+  q->off = {0, int64_t(idx.size())};
+  q->nint = {int32_t(idx.size())};
+  q->max_m = int(idx.size());
+  q->idx.swap(idx);
+  delete s->coarse;
+  s->coarse = q.release();
 }
 
 void analyse_batch_into(abh_schwarz *s, int D, const std::vector<int64_t> &boff,
@@ -516,10 +547,9 @@ void factorise(abh_schwarz *s, const double *pos, const KernelDesc &k, double r2
 
 namespace abh {
 
-void schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
+// every domain's interior rows x[i] = (A_dom^-1 b_dom)_i
+void apply_domains(const abh_schwarz *s, const double *b, double *x) {
   hipStream_t st = s->ctx->stream;
-  if (s->n == 0) return;
-  ABH_HIP(hipMemcpyAsync(x, b, size_t(s->n) * 8, hipMemcpyDeviceToDevice, st));
   if (s->nchunks == 0) return;
   const ApplyChunk *ch = s->d_chunks.as<ApplyChunk>();
   const DomDev *dm = s->d_doms.as<DomDev>();
@@ -530,6 +560,14 @@ void schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
   else
     k_ras_apply<false><<<s->nchunks, 256, 0, st>>>(ch, dm, id, R, b, x);
   check_launch();
+}
+
+void schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
+  hipStream_t st = s->ctx->stream;
+  if (s->n == 0) return;
+  ABH_HIP(hipMemcpyAsync(x, b, size_t(s->n) * 8, hipMemcpyDeviceToDevice, st));
+  apply_domains(s, b, x);
+  if (s->coarse) apply_domains(s->coarse, b, x);  // solved last (src/Preconditioners.h:1489-1520)
 }
 
 int64_t schwarz_size(const abh_schwarz *s) { return s->n; }
@@ -559,8 +597,9 @@ abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<i
 
 extern "C" {
 
-abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const abh_kernel *kernel,
-                              double radius, double diag_shift, abh_schwarz **out) {
+abh_status abh_schwarz_create_coarse(abh_context *ctx, const abh_cells *cells,
+                                     const abh_kernel *kernel, double radius, double diag_shift,
+                                     int coarse_grid_n, uint64_t seed, abh_schwarz **out) {
   if (!ctx || !cells || !kernel || !out) return ABH_ERR_INVALID_ARG;
   *out = nullptr;
   abh_schwarz *s = new (std::nothrow) abh_schwarz();
@@ -569,6 +608,8 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
     ABH_REQUIRE(!ctx->closed, ABH_ERR_INVALID_ARG, "context already destroyed");
     ABH_REQUIRE(kernel->type >= ABH_KERNEL_CONSTANT && kernel->type <= ABH_KERNEL_SMOOTH_INV_R,
                 ABH_ERR_UNSUPPORTED, "unknown kernel type");
+    ABH_REQUIRE(coarse_grid_n >= 0 && coarse_grid_n <= 4096, ABH_ERR_INVALID_ARG,
+                "coarse grid size must be in [0, 4096]");
     for (int d = 0; d < cells->dim; ++d)
       ABH_REQUIRE(!cells->periodic[d], ABH_ERR_UNSUPPORTED,
                   "Schwarz preconditioner needs a non-periodic cell list");
@@ -586,7 +627,10 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
       ABH_HIP(hipMemcpyAsync(end.data(), cells->end.p, end.size() * 4, hipMemcpyDeviceToHost, hs));
     }
     ABH_HIP(hipStreamSynchronize(hs));
-    if (s->n > 0) analyse_into(s, cells, pos, beg, end);
+    if (s->n > 0) {
+      analyse_into(s, cells, pos, beg, end);
+      if (coarse_grid_n > 0) coarse_into(s, cells, beg, end, coarse_grid_n, seed);
+    }
     for (size_t d = 0; d + 1 < s->off.size(); ++d)
       s->max_m = std::max(s->max_m, int(s->off[d + 1] - s->off[d]));
     KernelDesc k{};
@@ -597,6 +641,7 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
     k.w0 = 1.0;
     const double r2 = radius > 0.0 ? radius * radius : -1.0;
     factorise(s, cells->pos.as<double>(), k, r2, diag_shift, false);
+    if (s->coarse) factorise(s->coarse, cells->pos.as<double>(), k, r2, diag_shift, false);
   });
   if (st != ABH_OK) {
     delete s;
@@ -605,6 +650,11 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const ab
   abh::ctx_retain(ctx);
   *out = s;
   return ABH_OK;
+}
+
+abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells, const abh_kernel *kernel,
+                              double radius, double diag_shift, abh_schwarz **out) {
+  return abh_schwarz_create_coarse(ctx, cells, kernel, radius, diag_shift, 0, 0, out);
 }
 
 abh_status abh_schwarz_factorize(abh_schwarz *s, const abh_cells *cells,
@@ -626,6 +676,10 @@ abh_status abh_schwarz_factorize(abh_schwarz *s, const abh_cells *cells,
     const double r2 = radius > 0.0 ? radius * radius : -1.0;
     s->failed = 0;
     factorise(s, cells->pos.as<double>(), k, r2, diag_shift, false);
+    if (s->coarse) {
+      s->coarse->failed = 0;
+      factorise(s->coarse, cells->pos.as<double>(), k, r2, diag_shift, false);
+    }
   });
 }
 
@@ -643,18 +697,23 @@ abh_status abh_schwarz_get_info(const abh_schwarz *s, abh_schwarz_info *info) {
   if (!s || !info) return ABH_ERR_INVALID_ARG;
   abh_schwarz_info r{};
   r.n = s->n;
-  r.n_domains = int64_t(s->nint.size());
+  r.n_domains = int64_t(s->nint.size()) + (s->coarse ? 1 : 0);
   r.min_size = r.n_domains ? INT64_MAX : 0;
   r.min_interior = r.n_domains ? INT64_MAX : 0;
-  for (int64_t d = 0; d < r.n_domains; ++d) {
-    const int64_t m = s->off[d + 1] - s->off[d];
+  auto add = [&](int64_t m, int64_t ni) {
     r.min_size = std::min(r.min_size, m);
     r.max_size = std::max(r.max_size, m);
-    r.min_interior = std::min<int64_t>(r.min_interior, s->nint[d]);
-    r.max_interior = std::max<int64_t>(r.max_interior, s->nint[d]);
-  }
+    r.min_interior = std::min<int64_t>(r.min_interior, ni);
+    r.max_interior = std::max<int64_t>(r.max_interior, ni);
+  };
+  for (size_t d = 0; d < s->nint.size(); ++d) add(s->off[d + 1] - s->off[d], s->nint[d]);
   r.factor_bytes = int64_t(s->factor_bytes);
   r.failed_domains = s->failed;
+  if (s->coarse) {
+    add(int64_t(s->coarse->idx.size()), s->coarse->nint[0]);
+    r.factor_bytes += int64_t(s->coarse->factor_bytes);
+    r.failed_domains += s->coarse->failed;
+  }
   *info = r;
   return ABH_OK;
 }
@@ -662,9 +721,16 @@ abh_status abh_schwarz_get_info(const abh_schwarz *s, abh_schwarz_info *info) {
 abh_status abh_schwarz_get_domains(const abh_schwarz *s, int64_t *offsets, int32_t *indices,
                                    int32_t *n_interior) {
   if (!s) return ABH_ERR_INVALID_ARG;
+  const size_t nd = s->nint.size();
   if (offsets) std::copy(s->off.begin(), s->off.end(), offsets);
   if (indices) std::copy(s->idx.begin(), s->idx.end(), indices);
   if (n_interior) std::copy(s->nint.begin(), s->nint.end(), n_interior);
+  if (s->coarse) {  // the coarse domain last, all interior
+    const abh_schwarz *q = s->coarse;
+    if (offsets) offsets[nd + 1] = s->off[nd] + int64_t(q->idx.size());
+    if (indices) std::copy(q->idx.begin(), q->idx.end(), indices + s->off[nd]);
+    if (n_interior) n_interior[nd] = q->nint[0];
+  }
   return ABH_OK;
 }
 

@@ -35,6 +35,8 @@ struct GridView {
   int64_t size[3] = {1, 1, 1};
   double side[3] = {1, 1, 1};
   double lo[3] = {0, 0, 0};
+  double hi[3] = {1, 1, 1};   // domain bounds given to the cell list
+  double inv[3] = {1, 1, 1};  // 1 / side, as the bucket index uses it
   int64_t nb() const { return size[0] * size[1] * size[2]; }
 };
 
@@ -194,6 +196,74 @@ inline const char *analyse_batch(int D, const std::vector<int64_t> &boff,
       out.off.push_back(int64_t(out.idx.size()));
     }
   }
+  return nullptr;
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014): the sampler's random stream.
+inline uint64_t splitmix64(uint64_t &state) {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// The coarse-grid domain of SchwartzPreconditioner::set_coarse_grid_n
+// (src/Preconditioners.h:1230-1283): the (n+1)^dim lattice points
+// lo + i dx, dx = (hi - lo) / n - DBL_EPSILON, are counted per bucket
+// (floor((p - lo) * inv), clamped to the grid), and from every bucket
+// holding c of them the first min(c, particles) particles of a shuffle of
+// its particles are taken.  The reference shuffles with the unseeded
+// std::random_shuffle and walks its buckets in std::unordered_map order;
+// here buckets are walked in ascending index and each bucket's particles
+// (local indices 0..m-1) are shuffled by a forward Fisher-Yates pass,
+// a[i] <-> a[u mod (i+1)] for i = 1..m-1, with u drawn from SplitMix64
+// seeded with seed ^ (bucket * 0xD1B54A32D192ED03).  out: indices in
+// gather order.  Returns nullptr or a message.
+inline const char *coarse_grid(const GridView &g, const std::vector<uint32_t> &beg,
+                               const std::vector<uint32_t> &end, int n, uint64_t seed,
+                               std::vector<int32_t> &out) {
+  out.clear();
+  const int D = g.dim;
+  if (D < 1 || D > 3) return "dimension must be 1, 2 or 3";
+  if (n <= 0) return "coarse grid size must be positive";
+  const int64_t nb = g.nb();
+  if (int64_t(beg.size()) != nb || int64_t(end.size()) != nb) return "bucket arrays do not match the grid";
+  double dx[3] = {0, 0, 0};
+  for (int d = 0; d < D; ++d) dx[d] = (g.hi[d] - g.lo[d]) / double(n) - 2.220446049250313e-16;
+  std::vector<int64_t> count(size_t(nb), 0);
+  int64_t npts = 1;
+  for (int d = 0; d < D; ++d) npts *= int64_t(n) + 1;
+  for (int64_t t = 0; t < npts; ++t) {
+    int64_t r = t, b = 0;
+    int64_t iv[3] = {0, 0, 0};
+    for (int d = D - 1; d >= 0; --d) {
+      iv[d] = r % (int64_t(n) + 1);
+      r /= int64_t(n) + 1;
+    }
+    for (int d = 0; d < D; ++d) {
+      const double p = double(iv[d]) * dx[d] + g.lo[d];
+      int64_t v = int64_t(std::floor((p - g.lo[d]) * g.inv[d]));
+      v = std::min<int64_t>(std::max<int64_t>(v, 0), g.size[d] - 1);
+      b = b * g.size[d] + v;
+    }
+    ++count[size_t(b)];
+  }
+  std::vector<int32_t> loc;
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t c = count[size_t(b)];
+    const int64_t m = int64_t(end[size_t(b)]) - int64_t(beg[size_t(b)]);
+    if (c == 0 || m <= 0) continue;
+    loc.resize(size_t(m));
+    for (int64_t i = 0; i < m; ++i) loc[size_t(i)] = int32_t(i);
+    uint64_t st = seed ^ (uint64_t(b) * 0xD1B54A32D192ED03ull);
+    for (int64_t i = 1; i < m; ++i) {
+      const int64_t j = int64_t(splitmix64(st) % uint64_t(i + 1));
+      std::swap(loc[size_t(i)], loc[size_t(j)]);
+    }
+    const int64_t take = std::min(c, m);
+    for (int64_t i = 0; i < take; ++i) out.push_back(int32_t(beg[size_t(b)]) + loc[size_t(i)]);
+  }
+  if (out.empty()) return "coarse grid holds no particles";
   return nullptr;
 }
 

@@ -509,7 +509,8 @@ def by_size(sizes, stats):
 def bench_schwarz(torch, ab, wl, ctx, extra):
     """Schwarz-preconditioned GMRES on the jittered grid (N=400,056): the
     configured kernel width sigma=0.01 and a narrow variant sigma=0.002 (same
-    points), each with the plain Schwarz preconditioner, the learned patch
+    points), each with the plain Schwarz preconditioner, the same with the
+    coarse-grid domain (set_coarse_grid_n(32)), the learned patch
     solves (the network trained on the patch residual; Schwarz fallback), the
     exact patch solves, the exact least-squares approximate inverse and no
     preconditioner, on bounded iteration budgets."""
@@ -541,6 +542,14 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
         (_, st), t = timed(lambda: op.gmres(b, diag_shift=shift, preconditioner=sw, restart=50,
                                             max_iters=300))
         r["schwarz"] = {"setup_s": t_sw, "solve_s": t, "info": sw.info, **st}
+        # with the coarse-grid domain of set_coarse_grid_n(32) (1,089
+        # lattice points; seeded sample)
+        swc, t_swc = timed(lambda: ab.SchwartzPreconditioner(ctx, cl, kern, radius, shift,
+                                                             coarse_grid_n=32, seed=c["seed"]))
+        (_, st), t = timed(lambda: op.gmres(b, diag_shift=shift, preconditioner=swc, restart=50,
+                                            max_iters=300))
+        r["schwarz_coarse32"] = {"setup_s": t_swc, "solve_s": t, "info": swc.info, **st}
+        swc.close()
         # the network standing in for the patch local solves, trained on the
         # patch residual over a second seeded grid
         (X, M, _, oscale), t_data = timed(lambda: learned.training_set_patch(
@@ -567,10 +576,13 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
         r["lsq"] = {"setup_s": t_ls, "solve_s": t, **st}
         (_, st), t = timed(lambda: op.gmres(b, diag_shift=shift, restart=50, max_iters=300))
         r["none"] = {"solve_s": t, **st}
-        log("schwarz %s: plain %d its (%s) setup %.2f s solve %.2f s | learned %s fallback %s "
+        log("schwarz %s: plain %d its (%s) setup %.2f s solve %.2f s | coarse32 %d its (%s) "
+            "res %.2e | learned %s fallback %s "
             "(setup %.3f s, solve %.2f s) | patch %d its (%s) | lsq %d its (%s) | none %d its (%s)"
             % (name, r["schwarz"]["iterations"], r["schwarz"]["converged"], t_sw,
-               r["schwarz"]["solve_s"], rep["learned"], rep["fallback"], t_ai,
+               r["schwarz"]["solve_s"], r["schwarz_coarse32"]["iterations"],
+               r["schwarz_coarse32"]["converged"], r["schwarz_coarse32"]["residual"],
+               rep["learned"], rep["fallback"], t_ai,
                r["learned"]["solve_s"], r["patch"]["iterations"], r["patch"]["converged"],
                r["lsq"]["iterations"], r["lsq"]["converged"], r["none"]["iterations"],
                r["none"]["converged"]))

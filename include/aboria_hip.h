@@ -651,6 +651,23 @@ typedef struct abh_schwarz_info {
 abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells,
                               const abh_kernel *kernel, double radius,
                               double diag_shift, abh_schwarz **out);
+/* abh_schwarz_create after SchwartzPreconditioner::set_coarse_grid_n(n)
+ * (src/Preconditioners.h:1167, the coarse domain :1230-1283): one more
+ * domain, all interior and no buffer, of particles sampled near the
+ * (n+1)^dim lattice lo + i dx, dx = (high - low) / n - DBL_EPSILON: each
+ * bucket holding c lattice points gives min(c, its particles) of them.  It
+ * is factorised with the others and solved last, its rows overwriting the
+ * bucket domains' (the reference's order without OpenMP).  The reference
+ * samples with the unseeded std::random_shuffle in std::unordered_map
+ * bucket order; here buckets are taken in ascending index and shuffled by
+ * a Fisher-Yates pass on SplitMix64 seeded with seed ^ (bucket *
+ * 0xD1B54A32D192ED03).  coarse_grid_n in [0, 4096] (0: abh_schwarz_create).
+ * The coarse domain is the last in abh_schwarz_get_info / get_domains;
+ * abh_schwarz_factorize refactorises it too. */
+abh_status abh_schwarz_create_coarse(abh_context *ctx, const abh_cells *cells,
+                                     const abh_kernel *kernel, double radius,
+                                     double diag_shift, int coarse_grid_n,
+                                     uint64_t seed, abh_schwarz **out);
 /* Replaces SchwartzPreconditioner::factorize alone (src/Preconditioners.h:
  * 1402-1467) on the pattern analyzePattern left: the domains of `s` are
  * kept and their local matrices rebuilt and factorised for a new kernel,
@@ -667,7 +684,7 @@ abh_status abh_schwarz_get_info(const abh_schwarz *s, abh_schwarz_info *info);
  * n_domains+1 entries; domain d is indices[offsets[d] .. offsets[d+1]), its
  * first n_interior[d] entries the interior.  Domains are in bucket order and
  * particles in bucket order, the reference's visiting order being a
- * permutation of the same sets. */
+ * permutation of the same sets; a coarse-grid domain comes last. */
 abh_status abh_schwarz_get_domains(const abh_schwarz *s, int64_t *offsets,
                                    int32_t *indices, int32_t *n_interior);
 /* x = M^-1 b (SchwartzPreconditioner::_solve_impl,

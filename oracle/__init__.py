@@ -80,6 +80,9 @@ def lib():
         l.orc_schwarz_create.restype = vp
         l.orc_schwarz_create.argtypes = [vp, P(KernelC), ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double]
+        l.orc_schwarz_create_coarse.restype = vp
+        l.orc_schwarz_create_coarse.argtypes = [vp, P(KernelC), ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_int, ctypes.c_uint64]
         l.orc_schwarz_analyse.restype = vp
         l.orc_schwarz_analyse.argtypes = [vp, ctypes.c_double]
         l.orc_schwarz_destroy.argtypes = [vp]
@@ -102,6 +105,8 @@ class Cells:
 
     def __init__(self, low, high, periodic, n_leaf=10.0):
         self.dim = len(low)
+        self.low = np.asarray(low, np.float64)
+        self.high = np.asarray(high, np.float64)
         self.h = lib().orc_cells_create(self.dim, np.asarray(low, np.float64),
                                         np.asarray(high, np.float64),
                                         np.asarray(periodic, np.uint8), float(n_leaf))
@@ -194,9 +199,11 @@ class Cells:
         out = yt.T.copy()
         return out[:, 0] if np.asarray(x).ndim == 1 else out
 
-    def schwarz(self, kernel, radius, shift, buffer=1e5 * np.finfo(float).eps):
-        """Restricted additive Schwarz preconditioner of A = K + shift I."""
-        return Schwarz(self, kernel, radius, shift, buffer)
+    def schwarz(self, kernel, radius, shift, buffer=1e5 * np.finfo(float).eps,
+                coarse_grid_n=0, seed=0):
+        """Restricted additive Schwarz preconditioner of A = K + shift I
+        (coarse_grid_n > 0: plus the coarse-grid domain, solved last)."""
+        return Schwarz(self, kernel, radius, shift, buffer, coarse_grid_n, seed)
 
     def schwarz_domains(self, buffer=1e5 * np.finfo(float).eps):
         """(offsets, indices, n_interior) of the Schwarz domains
@@ -240,11 +247,12 @@ class Cells:
 class Schwarz:
     """SchwartzPreconditioner<LLT> restated on the CPU over a Cells object."""
 
-    def __init__(self, cells, kernel, radius, shift, buffer):
+    def __init__(self, cells, kernel, radius, shift, buffer, coarse_grid_n=0, seed=0):
         self.cells = cells
         kd = KernelC(kernel[0], 0, kernel[1], 0.0, 0.0, 0.0)
-        self.h = lib().orc_schwarz_create(cells.h, ctypes.byref(kd), float(radius), float(shift),
-                                          float(buffer))
+        self.h = lib().orc_schwarz_create_coarse(cells.h, ctypes.byref(kd), float(radius),
+                                                 float(shift), float(buffer),
+                                                 int(coarse_grid_n), ctypes.c_uint64(seed))
         info = np.zeros(3, np.int64)
         lib().orc_schwarz_info(self.h, info)
         self.n_domains, total, self.failed = (int(v) for v in info)

@@ -411,6 +411,22 @@ void *orc_schwarz_create(void *p, const Kernel *k, double radius, double shift, 
   return sh;
 }
 
+// with the coarse-grid domain of set_coarse_grid_n(n) (seeded sampler)
+void *orc_schwarz_create_coarse(void *p, const Kernel *k, double radius, double shift,
+                                double buffer, int n, uint64_t seed) {
+  auto *h = static_cast<AnyCells *>(p);
+  auto *sh = new SchwarzH();
+  const double r2 = radius > 0.0 ? radius * radius : -1.0;
+  dispatch(h->dim, [&](auto DD) {
+    constexpr int D = decltype(DD)::value;
+    schwarz_analyse<D>(get<D>(h), buffer, sh->s);
+    if (n > 0) schwarz_coarse<D>(get<D>(h), n, seed, sh->s);
+    schwarz_factorise<D>(get<D>(h), *k, r2, shift, sh->s);
+    return 0;
+  });
+  return sh;
+}
+
 // analyzePattern only (src/Preconditioners.h:1169-1316): the domain lists,
 // no factorisation (orc_schwarz_apply must not be called on the handle)
 void *orc_schwarz_analyse(void *p, double buffer) {

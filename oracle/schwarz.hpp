@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <numeric>
 #include <vector>
 
 #include "cells_search.hpp"
@@ -21,6 +22,7 @@ struct Schwarz {
   std::vector<std::vector<double>> rows;  // interior rows of each local inverse
   int64_t n = 0;
   int64_t failed = 0;
+  bool coarse = false;  // the last domain is the coarse grid, solved last
 };
 
 // analyze_impl_block (src/Preconditioners.h:1169-1230): for every non-empty
@@ -183,13 +185,13 @@ void schwarz_factorise(const Cells<D> &c, const Kernel &k, double r2, double shi
 }
 
 // _solve_impl (src/Preconditioners.h:1476-1522): x = b, then each domain's
-// local solution overwrites its interior rows.
+// local solution overwrites its interior rows; the coarse-grid domain, the
+// last one, after all others (the serial loop's order).
 inline void schwarz_apply(const Schwarz &s, const double *b, double *x) {
   for (int64_t i = 0; i < s.n; ++i) x[i] = b[i];
   const int64_t nd = int64_t(s.nint.size());
-#pragma omp parallel for schedule(dynamic)
-  for (int64_t d = 0; d < nd; ++d) {
-    if (s.rows[size_t(d)].empty()) continue;
+  auto solve = [&](int64_t d) {
+    if (s.rows[size_t(d)].empty()) return;
     const int32_t *id = &s.idx[size_t(s.off[d])];
     const int m = int(s.off[d + 1] - s.off[d]);
     for (int a = 0; a < s.nint[size_t(d)]; ++a) {
@@ -198,7 +200,65 @@ inline void schwarz_apply(const Schwarz &s, const double *b, double *x) {
       for (int j = 0; j < m; ++j) v += Ra[j] * b[id[j]];
       x[id[a]] = v;
     }
+  };
+  const int64_t nmain = s.coarse ? nd - 1 : nd;
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t d = 0; d < nmain; ++d) solve(d);
+  if (s.coarse) solve(nd - 1);
+}
+
+// The coarse-grid domain (SchwartzPreconditioner::set_coarse_grid_n,
+// src/Preconditioners.h:1230-1283) appended to the analysed domains: the
+// lattice points lo + i * ((hi - lo) / n - DBL_EPSILON), i in [0, n]^D,
+// counted per bucket (bucket index clamped to the grid), then from each
+// bucket with c points the first min(c, m) of its m particles after a
+// shuffle.  The reference's random_shuffle is unseeded and its buckets come
+// in unordered_map order; the sampler pinned here walks buckets in
+// ascending index and shuffles local indices 0..m-1 forward (swap i with
+// r mod (i+1), i = 1..m-1), r from SplitMix64 (state += golden gamma,
+// two xor-shift-multiply rounds) seeded with seed ^ bucket * 0xD1B54A32D192ED03.
+template <int D> void schwarz_coarse(const Cells<D> &c, int n, uint64_t seed, Schwarz &s) {
+  const Grid<D> &g = c.g;
+  const size_t nb = g.nbuckets();
+  std::vector<long> hits(nb, 0);
+  double step[D];
+  for (int d = 0; d < D; ++d) step[d] = (g.hi[d] - g.lo[d]) / n - 2.220446049250313e-16;
+  int it[D] = {};
+  for (;;) {
+    int v[D];
+    for (int d = 0; d < D; ++d) {
+      const double p = it[d] * step[d] + g.lo[d];
+      long q = long(std::floor((p - g.lo[d]) * g.inv[d]));
+      if (q < 0) q = 0;
+      if (q > long(g.size[d]) - 1) q = long(g.size[d]) - 1;
+      v[d] = int(q);
+    }
+    ++hits[collapse<D>(g, v)];
+    int d = D - 1;
+    while (d >= 0 && ++it[d] > n) it[d--] = 0;
+    if (d < 0) break;
   }
+  std::vector<int32_t> dom;
+  for (size_t b = 0; b < nb; ++b) {
+    const long m = long(c.end[b]) - long(c.begin[b]);
+    if (hits[b] == 0 || m <= 0) continue;
+    std::vector<int32_t> perm(static_cast<size_t>(m));
+    std::iota(perm.begin(), perm.end(), 0);
+    uint64_t state = seed ^ (uint64_t(b) * 0xD1B54A32D192ED03ull);
+    for (long i = 1; i < m; ++i) {
+      uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      std::swap(perm[size_t(i)], perm[size_t(z % uint64_t(i + 1))]);
+    }
+    for (long i = 0; i < std::min(hits[b], m); ++i)
+      dom.push_back(int32_t(c.begin[b]) + perm[size_t(i)]);
+  }
+  s.idx.insert(s.idx.end(), dom.begin(), dom.end());
+  s.nint.push_back(int32_t(dom.size()));
+  s.off.push_back(int64_t(s.idx.size()));
+  s.coarse = true;
 }
 
 // k nearest particles of every particle: itself first, then the k-1

@@ -75,6 +75,7 @@ int main(int argc, char **argv) {
   g.size[0] = 24;
   g.size[1] = 24;
   g.side[0] = g.side[1] = 1.0 / 24;
+  g.inv[0] = g.inv[1] = 24.0;
   std::vector<double> p2;
   std::vector<int64_t> key;
   for (int i = 0; i < nx; ++i)
@@ -113,6 +114,24 @@ int main(int argc, char **argv) {
     abh::SchwarzDomains d3;
     if (!abh::analyse_cells(g, sorted, beg, e2, d3)) return fail("bad bucket range accepted");
   }
-  std::printf("ok batch %zu domains, cells %zu domains\n", dm.nint.size(), dc.nint.size());
+  // the coarse-grid sample: distinct particles, each in a bucket; sizes
+  // from a single lattice point to more points than particles
+  size_t ncoarse = 0;
+  for (int cn : {1, 7, 23, 60}) {
+    std::vector<int32_t> cs;
+    if (const char *e = abh::coarse_grid(g, beg, end, cn, 11, cs)) return fail(e);
+    std::vector<uint8_t> seen(size_t(n), 0);
+    for (int32_t i : cs) {
+      if (i < 0 || i >= n || seen[size_t(i)]) return fail("coarse sample out of range or repeated");
+      seen[size_t(i)] = 1;
+    }
+    ncoarse += cs.size();
+  }
+  {
+    std::vector<int32_t> cs;
+    if (!abh::coarse_grid(g, beg, end, 0, 11, cs)) return fail("coarse size 0 accepted");
+  }
+  std::printf("ok batch %zu domains, cells %zu domains, coarse %zu\n", dm.nint.size(),
+              dc.nint.size(), ncoarse);
   return 0;
 }

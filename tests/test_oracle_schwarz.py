@@ -151,6 +151,87 @@ def test_schwarz_apply_equals_dense_local_solves():
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-10
 
 
+M64 = (1 << 64) - 1
+
+
+def splitmix64(state):
+    """SplitMix64 (Steele, Lea, Flood 2014): (next state, output)."""
+    state = (state + 0x9E3779B97F4A7C15) & M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return state, z ^ (z >> 31)
+
+
+def test_splitmix64_known_answer():
+    # the generator's published first outputs from state 0
+    st, a = splitmix64(0)
+    st, b = splitmix64(st)
+    assert (a, b) == (0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4)
+
+
+def coarse_sample(c, n, seed):
+    """The coarse-grid sample (set_coarse_grid_n, src/Preconditioners.h:
+    1230-1283) with the seeded sampler: lattice points counted per bucket
+    in numpy, buckets ascending, a forward Fisher-Yates shuffle per bucket."""
+    dim = len(c.grid)
+    lo, hi = c.low, c.high
+    dx = (hi - lo) / n - np.finfo(float).eps
+    pts = np.array(np.meshgrid(*[np.arange(n + 1)] * dim, indexing="ij")).reshape(dim, -1).T
+    p = pts * dx + lo
+    v = np.floor((p - lo) * (1.0 / np.asarray(c.side))).astype(np.int64)
+    v = np.clip(v, 0, np.asarray(c.grid) - 1)
+    counts = np.bincount(np.ravel_multi_index(v.T, c.grid), minlength=int(np.prod(c.grid)))
+    out = []
+    for b in range(len(counts)):
+        m = int(c.end[b]) - int(c.begin[b])
+        if counts[b] == 0 or m <= 0:
+            continue
+        perm = list(range(m))
+        st = seed ^ ((b * 0xD1B54A32D192ED03) & M64)
+        for i in range(1, m):
+            st, u = splitmix64(st)
+            j = u % (i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        out += [int(c.begin[b]) + q for q in perm[:min(int(counts[b]), m)]]
+    return np.array(out, np.int32), counts
+
+
+@pytest.mark.parametrize("dim,n", [(2, 8), (2, 40), (3, 5), (1, 30)])
+def test_schwarz_coarse_grid(dim, n):
+    """The coarse-grid domain: the last domain, all interior; its particles
+    the seeded sample (distinct, min(points, particles) from every bucket
+    the lattice reaches); the apply equals the bucket domains' apply except
+    on the coarse particles, where it is the dense solve of their block,
+    written last."""
+    rng = np.random.default_rng(40 + dim + n)
+    npts = {1: 300, 2: 900, 3: 1200}[dim]
+    pos = rng.random((npts, dim))
+    c = oracle.Cells([0] * dim, [1] * dim, [0] * dim, 12.0)
+    c.update(pos)
+    sigma, radius, shift = 0.08, 0.3, 1e-3
+    s0 = c.schwarz((GAUSS, sigma), radius, shift)
+    s = c.schwarz((GAUSS, sigma), radius, shift, coarse_grid_n=n, seed=7)
+    assert s.n_domains == s0.n_domains + 1
+    assert np.array_equal(s.offsets[:-1], s0.offsets)
+    ids = s.indices[s.offsets[-2]:s.offsets[-1]]
+    assert s.n_interior[-1] == len(ids)
+    want, counts = coarse_sample(c, n, 7)
+    assert np.array_equal(ids, want)
+    assert len(np.unique(ids)) == len(ids)
+    sizes = np.array(c.end, np.int64) - np.array(c.begin, np.int64)
+    assert len(ids) == int(np.minimum(counts, sizes).sum())
+    other = c.schwarz((GAUSS, sigma), radius, shift, coarse_grid_n=n, seed=8)
+    if len(ids) < c.n_alive:
+        assert not np.array_equal(other.indices[other.offsets[-2]:], ids)
+    A = gaussian_matrix(c.positions, sigma, radius, shift)
+    b = rng.normal(size=c.n_alive)
+    want_x = s0.apply(b)
+    want_x[ids] = np.linalg.solve(A[np.ix_(ids, ids)], b[ids])
+    x = s.apply(b)
+    assert np.linalg.norm(x - want_x) / np.linalg.norm(want_x) < 1e-10
+
+
 @pytest.mark.parametrize("dim,k", [(2, 16), (3, 32), (2, 1)])
 def test_knn_oracle_brute_force(dim, k):
     rng = np.random.default_rng(10 + k)
