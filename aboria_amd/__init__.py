@@ -916,15 +916,16 @@ class DenseBatchOperator:
         return rpt.value, {1: ms[0], 2: ms[1], 4: ms[2]}
 
     def gmres(self, b, x=None, restart=50, max_iters=1000, tol=1e-8, precision=FP64,
-              preconditioner=None, stall_restarts=0):
+              preconditioner=None, stall_restarts=0, graph=False):
         """Solves K x = b for every problem, right-preconditioned by
         `preconditioner` (SchwartzPreconditioner.for_dense_batch) if given; a
         problem stops after stall_restarts (> 0) restarts in a row that fail
-        to cut its residual by 10%.  Returns (x, stats)."""
+        to cut its residual by 10%.  graph: each restart cycle's Arnoldi
+        steps run as one replayed HIP graph.  Returns (x, stats)."""
         if x is None:
             x = torch.zeros_like(b)
         opts = C.GmresOptions(int(restart), int(max_iters), float(tol), int(precision),
-                              int(stall_restarts))
+                              int(stall_restarts), int(bool(graph)))
         stats = (C.GmresStats * self.nprob)()
         ph = preconditioner.handle if preconditioner is not None else None
         check(C.lib().abh_dense_batch_gmres_precond(self.handle, ph, _ptr(b), _ptr(x),
@@ -941,7 +942,8 @@ class DenseBatchOperator:
         return r
 
     def refine(self, b, x=None, tol=1e-8, max_outer=8, inner_tol=1e-3, restart=50,
-               inner_iters=100, precision=FP64, preconditioner=None, stall_restarts=0):
+               inner_iters=100, precision=FP64, preconditioner=None, stall_restarts=0,
+               graph=False):
         """Iterative refinement: the residual in double-double arithmetic,
         corrections from preconditioned GMRES (inner_tol, inner_iters), at
         most max_outer residual evaluations.  Returns (x, stats) with the
@@ -949,7 +951,7 @@ class DenseBatchOperator:
         if x is None:
             x = torch.zeros_like(b)
         opts = C.GmresOptions(int(restart), int(inner_iters), float(inner_tol), int(precision),
-                              int(stall_restarts))
+                              int(stall_restarts), int(bool(graph)))
         stats = (C.GmresStats * self.nprob)()
         ph = preconditioner.handle if preconditioner is not None else None
         check(C.lib().abh_dense_batch_refine(self.handle, ph, _ptr(b), _ptr(x),

@@ -113,7 +113,8 @@ MIXED = 2
 
 class GmresOptions(ctypes.Structure):
     _fields_ = [("restart", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("tol", ctypes.c_double),
-                ("precision", ctypes.c_int32), ("stall_restarts", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("stall_restarts", ctypes.c_int32),
+                ("graph", ctypes.c_int32)]
 
 
 class GmresStats(ctypes.Structure):

@@ -10,6 +10,7 @@ namespace {
 
 constexpr int kChunk = 4096;
 constexpr int kThreads = 256;
+constexpr int kSegment = 5;  // Arnoldi steps per captured graph
 
 __global__ __launch_bounds__(kThreads) void k_multidot(const Chunk *__restrict__ ch,
                                                        const double *__restrict__ V,
@@ -69,6 +70,161 @@ __global__ __launch_bounds__(kThreads) void k_scale(const Chunk *__restrict__ ch
     out[c.start + t] = in[c.start + t] * f;
 }
 
+// zero the coefficients of the problems outside the cycle and accumulate
+// the two CGS passes: htot = (first ? 0 : htot) + h
+__global__ void k_mask_acc(int np, int nv, int ld, const uint8_t *__restrict__ cyc,
+                           double *__restrict__ h, double *__restrict__ htot, int first) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np * nv) return;
+  const int p = t / nv, i = t % nv;
+  const int64_t q = int64_t(p) * ld + i;
+  const double v = cyc[p] ? h[q] : 0.0;
+  h[q] = v;
+  htot[q] = first ? v : htot[q] + v;
+}
+
+// hk1 = ||w||, sc = 1 / hk1 for the problems in the cycle (0 otherwise)
+__global__ void k_norm_scale(int np, int ld, const uint8_t *__restrict__ cyc,
+                             const double *__restrict__ h, double *__restrict__ hk1,
+                             double *__restrict__ sc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double v = sqrt(fmax(0.0, h[int64_t(p) * ld]));
+  hk1[p] = v;
+  sc[p] = (cyc[p] && v > 0.0) ? 1.0 / v : 0.0;
+}
+
+// a restart cycle begins for the active problems with iterations left:
+// v_0 = r / beta, g = beta e_1
+__global__ void k_cycle_start(int np, int m, const uint8_t *__restrict__ active,
+                              const int32_t *__restrict__ iters, int max_iters,
+                              const double *__restrict__ beta, uint8_t *__restrict__ cyc,
+                              double *__restrict__ sc, double *__restrict__ g,
+                              int32_t *__restrict__ kp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const bool on = active[p] && iters[p] < max_iters;
+  cyc[p] = on ? 1 : 0;
+  sc[p] = on ? 1.0 / beta[p] : 0.0;
+  kp[p] = 0;
+  double *gp = g + int64_t(p) * (m + 1);
+  for (int i = 0; i <= m; ++i) gp[i] = 0.0;
+  gp[0] = beta[p];
+}
+
+// Arnoldi step k of every problem in the cycle: column k of the Hessenberg
+// matrix (H column-major, (m+1) x m per problem) from the CGS2 coefficients
+// and ||w||, the previous Givens rotations applied to it, a new rotation
+// zeroing H[k+1][k], the rotated right-hand side g; the problem leaves the
+// cycle when |g[k+1]| <= tol ||b||, at a breakdown, or out of iterations.
+__global__ void k_givens(int np, int k, int m, int ld, uint8_t *__restrict__ cyc,
+                         const double *__restrict__ htot, const double *__restrict__ hk1,
+                         double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
+                         double *__restrict__ g, int32_t *__restrict__ kp,
+                         int32_t *__restrict__ iters, const double *__restrict__ bnorm, double tol,
+                         int max_iters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np || !cyc[p]) return;
+  double *Hc = H + (int64_t(p) * m + k) * (m + 1);
+  for (int i = 0; i <= k; ++i) Hc[i] = htot[int64_t(p) * ld + i];
+  Hc[k + 1] = hk1[p];
+  double *c = cs + int64_t(p) * m, *s = sn + int64_t(p) * m;
+  for (int i = 0; i < k; ++i) {
+    const double t = c[i] * Hc[i] + s[i] * Hc[i + 1];
+    Hc[i + 1] = -s[i] * Hc[i] + c[i] * Hc[i + 1];
+    Hc[i] = t;
+  }
+  const double den = hypot(Hc[k], Hc[k + 1]);
+  c[k] = den > 0.0 ? Hc[k] / den : 1.0;
+  s[k] = den > 0.0 ? Hc[k + 1] / den : 0.0;
+  Hc[k] = den;
+  Hc[k + 1] = 0.0;
+  double *gp = g + int64_t(p) * (m + 1);
+  gp[k + 1] = -s[k] * gp[k];
+  gp[k] = c[k] * gp[k];
+  kp[p] = k + 1;
+  const int it = iters[p] + 1;
+  iters[p] = it;
+  if (fabs(gp[k + 1]) <= tol * bnorm[p] || hk1[p] == 0.0 || it >= max_iters) cyc[p] = 0;
+}
+
+// y = H^-1 g over the problem's kp steps (upper triangular), zero beyond
+__global__ void k_hsolve(int np, int m, int ld, const int32_t *__restrict__ kp,
+                         const double *__restrict__ H, const double *__restrict__ g,
+                         double *__restrict__ y) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int kk = kp[p];
+  double *yp = y + int64_t(p) * ld;
+  for (int i = kk; i < ld; ++i) yp[i] = 0.0;
+  for (int i = kk - 1; i >= 0; --i) {
+    double t = g[int64_t(p) * (m + 1) + i];
+    for (int j = i + 1; j < kk; ++j) t -= H[(int64_t(p) * m + j) * (m + 1) + i] * yp[j];
+    const double d = H[(int64_t(p) * m + i) * (m + 1) + i];
+    yp[i] = d != 0.0 ? t / d : 0.0;
+  }
+}
+
+// r = b - r
+__global__ void k_rsub(int64_t n, const double *__restrict__ b, double *__restrict__ r) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (j < n) r[j] = b[j] - r[j];
+}
+
+// x += z
+__global__ void k_add(int64_t n, const double *__restrict__ z, double *__restrict__ x) {
+  const int64_t j = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (j < n) x[j] += z[j];
+}
+
+// x += back_p (xprev - x): a problem whose residual grew returns to xprev
+__global__ void k_restore(const Chunk *__restrict__ ch, const double *__restrict__ back,
+                          const double *__restrict__ xprev, double *__restrict__ x) {
+  const Chunk c = ch[blockIdx.x];
+  if (back[c.prob] == 0.0) return;
+  for (int64_t t = threadIdx.x; t < c.len; t += kThreads) x[c.start + t] = xprev[c.start + t];
+}
+
+// Graph capture needs a stream other than the null stream: the context's
+// work moves to a stream of its own for the solve (ordered after the
+// caller's stream by an event, and the caller's stream after it on exit).
+struct StreamSwap {
+  abh_context *ctx = nullptr;
+  hipStream_t orig = nullptr, mine = nullptr;
+  hipEvent_t ev = nullptr;
+  void begin(abh_context *c) {
+    ctx = c;
+    orig = c->stream;
+    ABH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    ABH_HIP(hipStreamCreateWithFlags(&mine, hipStreamNonBlocking));
+    ABH_HIP(hipEventRecord(ev, orig));
+    ABH_HIP(hipStreamWaitEvent(mine, ev, 0));
+    c->stream = mine;
+  }
+  ~StreamSwap() {
+    if (!mine) return;
+    (void)hipEventRecord(ev, mine);
+    (void)hipStreamWaitEvent(orig, ev, 0);
+    (void)hipStreamSynchronize(mine);
+    ctx->stream = orig;
+    (void)hipStreamDestroy(mine);
+    (void)hipEventDestroy(ev);
+  }
+};
+
+// ends a stream capture left open by an exception
+struct CaptureGuard {
+  hipStream_t s;
+  bool open = false;
+  ~CaptureGuard() {
+    if (open) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  }
+};
+
 }  // namespace
 
 void BatchLayout::build(abh_context *ctx, const int64_t *offsets, int np) {
@@ -120,41 +276,52 @@ void batch_scale(abh_context *ctx, const BatchLayout &L, const double *in, const
   check_launch();
 }
 
-void batch_gmres(abh_context *ctx, const BatchLayout &L,
-                 const std::function<void(const double *, double *)> &op,
-                 const std::function<void(const double *, double *)> &op_true,
-                 const std::function<void(const double *, double *)> *prec,
-                 const std::function<void(const uint8_t *)> &set_active, const double *b,
-                 double *x, const abh_gmres_options &opt, GmresResult &res) {
+void batch_gmres(abh_context *ctx, const BatchLayout &L, const BatchOp &op,
+                 const BatchOp &op_true, const BatchPrec *prec, const double *b, double *x,
+                 const abh_gmres_options &opt, bool graph, GmresResult &res) {
+  StreamSwap swap;
+  if (graph) swap.begin(ctx);
   hipStream_t s = ctx->stream;
-  // small host -> device copies wait for completion: the host vectors they
-  // read are reused right after
-  auto up = [&](void *dst, const void *src, size_t bytes) {
-    ABH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
-    ABH_HIP(hipStreamSynchronize(s));
-  };
   const int np = L.nprob;
   const int64_t n = L.total();
   const int m = std::max(1, opt.restart);
   const int ld = m + 2;
-  DevBuf V, w, z, partial, hdev, sdev, xprev;
+  const int max_iters = std::max(1, opt.max_iters);
+  const unsigned gp = unsigned((np + 255) / 256), gn = unsigned(std::max<int64_t>(1, (n + 255) / 256));
+  DevBuf V, w, z, partial, hdev, htot, xprev, Hd, csd, snd, gd, kpd, itd, hk1, sc, bnd, betad,
+      cyc, act, back;
   V.ensure(size_t(m + 1) * size_t(n) * 8 + 64);
   xprev.ensure(size_t(n) * 8 + 64);
   w.ensure(size_t(n) * 8 + 64);
   z.ensure(size_t(n) * 8 + 64);
   partial.ensure(size_t(std::max(L.nchunks, 1)) * ld * 8);
   hdev.ensure(size_t(np) * ld * 8 + 64);
-  sdev.ensure(size_t(np) * 8 + 64);
+  htot.ensure(size_t(np) * ld * 8 + 64);
+  Hd.ensure(size_t(np) * m * (m + 1) * 8 + 64);
+  csd.ensure(size_t(np) * m * 8 + 64);
+  snd.ensure(size_t(np) * m * 8 + 64);
+  gd.ensure(size_t(np) * (m + 1) * 8 + 64);
+  kpd.ensure(size_t(np) * 4 + 64);
+  itd.ensure(size_t(np) * 4 + 64);
+  hk1.ensure(size_t(np) * 8 + 64);
+  sc.ensure(size_t(np) * 8 + 64);
+  bnd.ensure(size_t(np) * 8 + 64);
+  betad.ensure(size_t(np) * 8 + 64);
+  cyc.ensure(size_t(np) + 64);
+  act.ensure(size_t(np) + 64);
+  back.ensure(size_t(np) * 8 + 64);
   double *Vp = V.as<double>();
   auto vec = [&](int i) { return Vp + int64_t(i) * n; };
   std::vector<double> hh(size_t(np) * ld), bnorm(np), beta(np);
   res.stats.assign(np, abh_gmres_stats{0, 0, 0.0});
-  std::vector<uint8_t> active(np, 1), cyc(np, 0);
-  // per problem Hessenberg (column-major (m+1) x m), Givens, rhs g
-  std::vector<double> H(size_t(np) * (m + 1) * m), cs(size_t(np) * m), sn(size_t(np) * m),
-      g(size_t(np) * (m + 1));
-  std::vector<int> kp(np, 0);
-
+  std::vector<uint8_t> active(np, 1), cyc_h(np, 0);
+  std::vector<int32_t> iters(np, 0), kp(np, 0);
+  // host -> device copies of host vectors reused right after: wait for them
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    ABH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    ABH_HIP(hipStreamSynchronize(s));
+  };
   auto norms = [&](const double *v, std::vector<double> &out) {
     batch_dots(ctx, L, v, n, 1, v, partial.as<double>(), hdev.as<double>(), ld);
     ABH_HIP(hipMemcpyAsync(hh.data(), hdev.p, size_t(np) * ld * 8, hipMemcpyDeviceToHost, s));
@@ -164,18 +331,50 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
   // r = b - A x (FP64) for the problems still iterating (the rows of the
   // others are left unspecified)
   auto residual = [&](double *r) {
-    set_active(active.data());
-    op_true(x, r);
-    std::vector<double> neg(np, -1.0);
-    up(sdev.p, neg.data(), size_t(np) * 8);
-    batch_scale(ctx, L, r, sdev.as<double>(), r);
-    std::vector<double> one(size_t(np) * ld, 0.0);
-    for (int p = 0; p < np; ++p) one[size_t(p) * ld] = 1.0;
-    up(hdev.p, one.data(), one.size() * 8);
-    batch_combine(ctx, L, b, 0, 1, hdev.as<double>(), ld, 1.0, r);
+    up(act.p, active.data(), size_t(np));
+    op_true(x, r, act.as<uint8_t>());
+    if (n > 0) {
+      k_rsub<<<gn, 256, 0, s>>>(n, b, r);
+      check_launch();
+    }
+  };
+  // Arnoldi step k on the device
+  auto step = [&](int k) {
+    const double *zin = vec(k);
+    if (prec) {
+      (*prec)(vec(k), z.as<double>());
+      zin = z.as<double>();
+    }
+    op(zin, w.as<double>(), cyc.as<uint8_t>());
+    for (int pass = 0; pass < 2; ++pass) {  // CGS2 against v_0..v_k
+      batch_dots(ctx, L, Vp, n, k + 1, w.as<double>(), partial.as<double>(), hdev.as<double>(), ld);
+      if (np > 0) {
+        k_mask_acc<<<unsigned((np * (k + 1) + 255) / 256), 256, 0, s>>>(
+            np, k + 1, ld, cyc.as<uint8_t>(), hdev.as<double>(), htot.as<double>(), pass == 0);
+        check_launch();
+      }
+      batch_combine(ctx, L, Vp, n, k + 1, hdev.as<double>(), ld, -1.0, w.as<double>());
+    }
+    batch_dots(ctx, L, w.as<double>(), n, 1, w.as<double>(), partial.as<double>(),
+               hdev.as<double>(), ld);
+    if (np > 0) {
+      k_norm_scale<<<gp, 256, 0, s>>>(np, ld, cyc.as<uint8_t>(), hdev.as<double>(),
+                                      hk1.as<double>(), sc.as<double>());
+      check_launch();
+    }
+    batch_scale(ctx, L, w.as<double>(), sc.as<double>(), vec(k + 1));
+    if (np > 0) {
+      k_givens<<<gp, 256, 0, s>>>(np, k, m, ld, cyc.as<uint8_t>(), htot.as<double>(),
+                                  hk1.as<double>(), Hd.as<double>(), csd.as<double>(),
+                                  snd.as<double>(), gd.as<double>(), kpd.as<int32_t>(),
+                                  itd.as<int32_t>(), bnd.as<double>(), opt.tol, max_iters);
+      check_launch();
+    }
   };
 
   norms(b, bnorm);
+  up(bnd.p, bnorm.data(), size_t(np) * 8);
+  up(itd.p, iters.data(), size_t(np) * 4);
   residual(w.as<double>());
   norms(w.as<double>(), beta);
   std::vector<int> stalls(np, 0);
@@ -187,102 +386,81 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
       res.stats[p].converged = 1;
     }
   }
-  const int max_iters = std::max(1, opt.max_iters);
+  // graph mode: the cycle's steps in segments of kSegment, each captured on
+  // first use and replayed, the host checking between segments whether a
+  // problem is still iterating
+  const int nseg = (m + kSegment - 1) / kSegment;
+  std::vector<hipGraph_t> gr(size_t(nseg), nullptr);
+  std::vector<hipGraphExec_t> ge(size_t(nseg), nullptr);
+  struct GraphFree {
+    std::vector<hipGraph_t> &g;
+    std::vector<hipGraphExec_t> &e;
+    ~GraphFree() {
+      for (hipGraphExec_t x : e)
+        if (x) (void)hipGraphExecDestroy(x);
+      for (hipGraph_t x : g)
+        if (x) (void)hipGraphDestroy(x);
+    }
+  } graph_free{gr, ge};
+  auto any_cycling = [&]() {
+    ABH_HIP(hipMemcpyAsync(cyc_h.data(), cyc.p, size_t(np), hipMemcpyDeviceToHost, s));
+    ABH_HIP(hipStreamSynchronize(s));
+    bool more = false;
+    for (int p = 0; p < np; ++p) more |= cyc_h[p] != 0;
+    return more;
+  };
   for (;;) {
     bool any = false;
-    for (int p = 0; p < np; ++p) any |= active[p] && res.stats[p].iterations < max_iters;
+    for (int p = 0; p < np; ++p) any |= active[p] && iters[p] < max_iters;
     if (!any) break;
-    // v0 = r / beta
-    std::vector<double> sc(np, 0.0);
-    for (int p = 0; p < np; ++p) {
-      cyc[p] = active[p] && res.stats[p].iterations < max_iters;
-      sc[p] = cyc[p] ? 1.0 / beta[p] : 0.0;
-      kp[p] = 0;
-      std::fill(g.begin() + size_t(p) * (m + 1), g.begin() + size_t(p + 1) * (m + 1), 0.0);
-      g[size_t(p) * (m + 1)] = beta[p];
-    }
-    up(sdev.p, sc.data(), size_t(np) * 8);
-    batch_scale(ctx, L, w.as<double>(), sdev.as<double>(), vec(0));
-    for (int k = 0; k < m; ++k) {
-      set_active(cyc.data());
-      const double *zin = vec(k);
-      if (prec) {
-        (*prec)(vec(k), z.as<double>());
-        zin = z.as<double>();
-      }
-      op(zin, w.as<double>());
-      // CGS2 against v_0..v_k
-      std::vector<double> htot(size_t(np) * (k + 1), 0.0);
-      for (int pass = 0; pass < 2; ++pass) {
-        batch_dots(ctx, L, Vp, n, k + 1, w.as<double>(), partial.as<double>(), hdev.as<double>(),
-                   ld);
-        ABH_HIP(hipMemcpyAsync(hh.data(), hdev.p, size_t(np) * ld * 8, hipMemcpyDeviceToHost, s));
-        ABH_HIP(hipStreamSynchronize(s));
-        for (int p = 0; p < np; ++p)
-          for (int i = 0; i <= k; ++i) {
-            double &v = hh[size_t(p) * ld + i];
-            if (!cyc[p]) v = 0.0;
-            htot[size_t(p) * (k + 1) + i] += v;
-          }
-        up(hdev.p, hh.data(), size_t(np) * ld * 8);
-        batch_combine(ctx, L, Vp, n, k + 1, hdev.as<double>(), ld, -1.0, w.as<double>());
-      }
-      std::vector<double> hk1(np);
-      norms(w.as<double>(), hk1);
-      for (int p = 0; p < np; ++p) sc[p] = (cyc[p] && hk1[p] > 0.0) ? 1.0 / hk1[p] : 0.0;
-      up(sdev.p, sc.data(), size_t(np) * 8);
-      batch_scale(ctx, L, w.as<double>(), sdev.as<double>(), vec(k + 1));
-      bool more = false;
-      for (int p = 0; p < np; ++p) {
-        if (!cyc[p]) continue;
-        double *Hc = &H[(size_t(p) * m + k) * (m + 1)];
-        for (int i = 0; i <= k; ++i) Hc[i] = htot[size_t(p) * (k + 1) + i];
-        Hc[k + 1] = hk1[p];
-        double *c = &cs[size_t(p) * m], *sg = &sn[size_t(p) * m];
-        for (int i = 0; i < k; ++i) {
-          const double t = c[i] * Hc[i] + sg[i] * Hc[i + 1];
-          Hc[i + 1] = -sg[i] * Hc[i] + c[i] * Hc[i + 1];
-          Hc[i] = t;
+    up(act.p, active.data(), size_t(np));
+    up(betad.p, beta.data(), size_t(np) * 8);
+    k_cycle_start<<<gp, 256, 0, s>>>(np, m, act.as<uint8_t>(), itd.as<int32_t>(), max_iters,
+                                     betad.as<double>(), cyc.as<uint8_t>(), sc.as<double>(),
+                                     gd.as<double>(), kpd.as<int32_t>());
+    check_launch();
+    batch_scale(ctx, L, w.as<double>(), sc.as<double>(), vec(0));
+    if (graph) {
+      for (int q = 0; q < nseg; ++q) {
+        const int k0 = q * kSegment, k1 = std::min(m, k0 + kSegment);
+        if (!ge[size_t(q)]) {
+          CaptureGuard cg{s};
+          ABH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          cg.open = true;
+          for (int k = k0; k < k1; ++k) step(k);
+          cg.open = false;
+          ABH_HIP(hipStreamEndCapture(s, &gr[size_t(q)]));
+          ABH_HIP(hipGraphInstantiate(&ge[size_t(q)], gr[size_t(q)], nullptr, nullptr, 0));
         }
-        const double den = std::hypot(Hc[k], Hc[k + 1]);
-        c[k] = den > 0 ? Hc[k] / den : 1.0;
-        sg[k] = den > 0 ? Hc[k + 1] / den : 0.0;
-        Hc[k] = den;
-        Hc[k + 1] = 0.0;
-        double *gp = &g[size_t(p) * (m + 1)];
-        gp[k + 1] = -sg[k] * gp[k];
-        gp[k] = c[k] * gp[k];
-        kp[p] = k + 1;
-        res.stats[p].iterations += 1;
-        const double est = std::fabs(gp[k + 1]);
-        if (est <= opt.tol * bnorm[p] || hk1[p] == 0.0 || res.stats[p].iterations >= max_iters)
-          cyc[p] = 0;
-        more |= cyc[p] != 0;
+        ABH_HIP(hipGraphLaunch(ge[size_t(q)], s));
+        if (!any_cycling()) break;
       }
-      if (!more) break;
+    } else {
+      for (int k = 0; k < m; ++k) {
+        step(k);
+        if (!any_cycling()) break;
+      }
     }
     // x += M^-1 V y with H y = g (upper triangular, size kp)
-    std::vector<double> y(size_t(np) * ld, 0.0);
+    k_hsolve<<<gp, 256, 0, s>>>(np, m, ld, kpd.as<int32_t>(), Hd.as<double>(), gd.as<double>(),
+                                hdev.as<double>());
+    check_launch();
+    ABH_HIP(hipMemcpyAsync(kp.data(), kpd.p, size_t(np) * 4, hipMemcpyDeviceToHost, s));
+    ABH_HIP(hipMemcpyAsync(iters.data(), itd.p, size_t(np) * 4, hipMemcpyDeviceToHost, s));
+    ABH_HIP(hipStreamSynchronize(s));
     int kmax = 0;
     for (int p = 0; p < np; ++p) {
-      const int kk = kp[p];
-      kmax = std::max(kmax, kk);
-      for (int i = kk - 1; i >= 0; --i) {
-        double t = g[size_t(p) * (m + 1) + i];
-        for (int j = i + 1; j < kk; ++j) t -= H[(size_t(p) * m + j) * (m + 1) + i] * y[size_t(p) * ld + j];
-        const double d = H[(size_t(p) * m + i) * (m + 1) + i];
-        y[size_t(p) * ld + i] = d != 0.0 ? t / d : 0.0;
-      }
+      kmax = std::max(kmax, kp[p]);
+      res.stats[p].iterations = iters[p];
     }
-    up(hdev.p, y.data(), y.size() * 8);
     if (prec) {
       ABH_HIP(hipMemsetAsync(w.p, 0, size_t(n) * 8, s));
       batch_combine(ctx, L, Vp, n, kmax, hdev.as<double>(), ld, 1.0, w.as<double>());
       (*prec)(w.as<double>(), z.as<double>());
-      std::vector<double> one(size_t(np) * ld, 0.0);
-      for (int p = 0; p < np; ++p) one[size_t(p) * ld] = 1.0;
-      up(hdev.p, one.data(), one.size() * 8);
-      batch_combine(ctx, L, z.as<double>(), 0, 1, hdev.as<double>(), ld, 1.0, x);
+      if (n > 0) {
+        k_add<<<gn, 256, 0, s>>>(n, z.as<double>(), x);
+        check_launch();
+      }
     } else {
       batch_combine(ctx, L, Vp, n, kmax, hdev.as<double>(), ld, 1.0, x);
     }
@@ -292,7 +470,7 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
     norms(w.as<double>(), beta);
     // a problem whose true residual grew over the cycle (rounding has taken
     // over) goes back to its previous iterate and stops
-    std::vector<double> back(np, 0.0);
+    std::vector<double> bk(np, 0.0);
     bool any_back = false;
     for (int p = 0; p < np; ++p) {
       if (!active[p]) {
@@ -300,7 +478,7 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
         continue;
       }
       if (beta[p] > beta_prev[p]) {
-        back[p] = 1.0;
+        bk[p] = 1.0;
         any_back = true;
         beta[p] = beta_prev[p];
         active[p] = 0;
@@ -315,15 +493,11 @@ void batch_gmres(abh_context *ctx, const BatchLayout &L,
         if (stalls[p] >= opt.stall_restarts) active[p] = 0;  // stagnated
       }
     }
-    if (any_back) {  // x += back_p (xprev - x); z is free here
-      ABH_HIP(hipMemcpyAsync(z.p, xprev.p, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
-      std::vector<double> one(size_t(np) * ld, 0.0);
-      for (int p = 0; p < np; ++p) one[size_t(p) * ld] = 1.0;
-      up(hdev.p, one.data(), one.size() * 8);
-      batch_combine(ctx, L, x, 0, 1, hdev.as<double>(), ld, -1.0, z.as<double>());
-      up(sdev.p, back.data(), size_t(np) * 8);
-      batch_scale(ctx, L, z.as<double>(), sdev.as<double>(), z.as<double>());
-      batch_combine(ctx, L, z.as<double>(), 0, 1, hdev.as<double>(), ld, 1.0, x);
+    if (any_back && L.nchunks > 0) {
+      up(back.p, bk.data(), size_t(np) * 8);
+      k_restore<<<L.nchunks, kThreads, 0, s>>>(L.chunks.as<Chunk>(), back.as<double>(),
+                                               xprev.as<double>(), x);
+      check_launch();
     }
     if (n > 0) ABH_HIP(hipMemcpyAsync(xprev.p, x, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
   }

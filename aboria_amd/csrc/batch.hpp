@@ -40,23 +40,31 @@ void batch_scale(abh_context *ctx, const BatchLayout &L, const double *in, const
                  double *out);
 
 // Restarted GMRES(m), right preconditioned, every problem of the batch in
-// lockstep.  op(in, out): out = A in; prec(in, out): out = M^-1 in (or copy).
-// The Arnoldi basis is orthogonalised by classical Gram-Schmidt applied twice
-// (CGS2) in FP64; the small Hessenberg systems are reduced with Givens
-// rotations on the host.  Each restart recomputes the true residual with
-// op_true (FP64 operator), for the problems still iterating; a problem whose
-// true residual grew over a cycle is put back to its previous iterate and
-// stopped, and with opt.stall_restarts > 0 one whose residual fell by less
-// than 10% on that many restarts in a row stops too.
+// lockstep.  op(in, out, mask): out = A in for the problems whose mask byte
+// (device, one per problem) is set, other rows unspecified; prec(in, out):
+// out = M^-1 in.  The Arnoldi basis is orthogonalised by classical
+// Gram-Schmidt applied twice (CGS2) in FP64.  The steps of a restart cycle
+// run on the device without host round trips: the CGS2 coefficients, the
+// norms, the Givens reduction of each problem's Hessenberg column and its
+// convergence test are kernels over the batch, and a problem that leaves
+// the cycle is masked out of the operator and the updates.  With graph the
+// steps of a cycle run as HIP graphs of a few steps each, captured on first
+// use and replayed every cycle, the host checking between them whether any
+// problem is still iterating (op and prec must then be capturable: kernels
+// and device copies only); without it the host checks after every step.  Each restart recomputes the true
+// residual with op_true (FP64 operator), for the problems still iterating;
+// a problem whose true residual grew over a cycle is put back to its
+// previous iterate and stopped, and with opt.stall_restarts > 0 one whose
+// residual fell by less than 10% on that many restarts in a row stops too.
 struct GmresResult {
   std::vector<abh_gmres_stats> stats;
 };
 
-void batch_gmres(abh_context *ctx, const BatchLayout &L,
-                 const std::function<void(const double *, double *)> &op,
-                 const std::function<void(const double *, double *)> &op_true,
-                 const std::function<void(const double *, double *)> *prec,
-                 const std::function<void(const uint8_t *)> &set_active, const double *b,
-                 double *x, const abh_gmres_options &opt, GmresResult &res);
+using BatchOp = std::function<void(const double *, double *, const uint8_t *)>;
+using BatchPrec = std::function<void(const double *, double *)>;
+
+void batch_gmres(abh_context *ctx, const BatchLayout &L, const BatchOp &op,
+                 const BatchOp &op_true, const BatchPrec *prec, const double *b, double *x,
+                 const abh_gmres_options &opt, bool graph, GmresResult &res);
 
 }  // namespace abh

@@ -623,24 +623,24 @@ abh_status abh_dense_batch_gmres_precond(abh_dense_batch *op, const abh_schwarz 
     ABH_REQUIRE(opts->precision == ABH_FP64 || opts->precision == ABH_FP32_COMPENSATED,
                 ABH_ERR_UNSUPPORTED, "GMRES operator precision must be FP64 or FP32_COMPENSATED");
     ABH_REQUIRE(op->L.total() == 0 || (b && x), ABH_ERR_INVALID_ARG, "b or x is null");
+    ABH_REQUIRE(!opts->graph || !prec || abh::schwarz_context(prec) == ctx, ABH_ERR_INVALID_ARG,
+                "a graph-captured solve needs the preconditioner on the operator's context");
     const bool fp32 = opts->precision == ABH_FP32_COMPENSATED;
-    const uint8_t *act = op->active.as<uint8_t>();
-    std::function<void(const double *, double *)> f_op = [&](const double *in, double *out) {
-      op->apply(in, out, fp32, 1.0, true, act);
+    abh::BatchOp f_op = [&](const double *in, double *out, const uint8_t *mask) {
+      op->apply(in, out, fp32, 1.0, true, mask);
     };
-    std::function<void(const double *, double *)> f_true = [&](const double *in, double *out) {
-      op->apply(in, out, false, 1.0, true, act);
+    abh::BatchOp f_true = [&](const double *in, double *out, const uint8_t *mask) {
+      op->apply(in, out, false, 1.0, true, mask);
     };
-    std::function<void(const uint8_t *)> f_act = [&](const uint8_t *a) {
-      ABH_HIP(hipMemcpyAsync(op->active.p, a, size_t(op->L.nprob), hipMemcpyHostToDevice,
-                             ctx->stream));
-      ABH_HIP(hipStreamSynchronize(ctx->stream));
-    };
-    std::function<void(const double *, double *)> f_prec = [&](const double *in, double *out) {
+    abh::BatchPrec f_prec = [&](const double *in, double *out) {
       abh::schwarz_apply(prec, in, out);
     };
+    // tiles tuned before any graph capture (the tuner allocates and times)
+    op->tune(fp32);
+    op->tune(false);
     abh::GmresResult res;
-    abh::batch_gmres(ctx, op->L, f_op, f_true, prec ? &f_prec : nullptr, f_act, b, x, *opts, res);
+    abh::batch_gmres(ctx, op->L, f_op, f_true, prec ? &f_prec : nullptr, b, x, *opts,
+                     opts->graph != 0, res);
     if (stats)
       for (int p = 0; p < op->L.nprob; ++p) stats[p] = res.stats[p];
   });
@@ -673,6 +673,9 @@ abh_status abh_dense_batch_refine(abh_dense_batch *op, const abh_schwarz *prec,
     const int64_t n = L.total();
     ABH_REQUIRE(n == 0 || (b && x), ABH_ERR_INVALID_ARG, "b or x is null");
     hipStream_t s = ctx->stream;
+    ABH_REQUIRE(!inner->graph || !prec || abh::schwarz_context(prec) == ctx,
+                ABH_ERR_INVALID_ARG,
+                "a graph-captured solve needs the preconditioner on the operator's context");
     const bool fp32 = inner->precision == ABH_FP32_COMPENSATED;
     const uint8_t *act = op->active.as<uint8_t>();
     std::vector<uint8_t> active(np, 1);
@@ -680,16 +683,17 @@ abh_status abh_dense_batch_refine(abh_dense_batch *op, const abh_schwarz *prec,
       ABH_HIP(hipMemcpyAsync(op->active.p, a, size_t(np), hipMemcpyHostToDevice, s));
       ABH_HIP(hipStreamSynchronize(s));
     };
-    std::function<void(const double *, double *)> f_op = [&](const double *in, double *out) {
-      op->apply(in, out, fp32, 1.0, true, act);
+    abh::BatchOp f_op = [&](const double *in, double *out, const uint8_t *mask) {
+      op->apply(in, out, fp32, 1.0, true, mask);
     };
-    std::function<void(const double *, double *)> f_true = [&](const double *in, double *out) {
-      op->apply(in, out, false, 1.0, true, act);
+    abh::BatchOp f_true = [&](const double *in, double *out, const uint8_t *mask) {
+      op->apply(in, out, false, 1.0, true, mask);
     };
-    std::function<void(const uint8_t *)> f_act = set_act;
-    std::function<void(const double *, double *)> f_prec = [&](const double *in, double *out) {
+    abh::BatchPrec f_prec = [&](const double *in, double *out) {
       abh::schwarz_apply(prec, in, out);
     };
+    op->tune(fp32);
+    op->tune(false);
     const int ld = 2;
     DevBuf r, d, partial, hdev, sdev;
     r.ensure(size_t(n) * 8 + 64);
@@ -732,8 +736,8 @@ abh_status abh_dense_batch_refine(abh_dense_batch *op, const abh_schwarz *prec,
       abh::batch_scale(ctx, L, r.as<double>(), sdev.as<double>(), r.as<double>());
       ABH_HIP(hipMemsetAsync(d.p, 0, size_t(n) * 8, s));
       abh::GmresResult res;
-      abh::batch_gmres(ctx, L, f_op, f_true, prec ? &f_prec : nullptr, f_act, r.as<double>(),
-                       d.as<double>(), *inner, res);
+      abh::batch_gmres(ctx, L, f_op, f_true, prec ? &f_prec : nullptr, r.as<double>(),
+                       d.as<double>(), *inner, inner->graph != 0, res);
       for (int p = 0; p < np; ++p) st[p].iterations += res.stats[p].iterations;
       std::vector<double> one(size_t(np) * ld, 0.0);
       for (int p = 0; p < np; ++p) one[size_t(p) * ld] = 1.0;

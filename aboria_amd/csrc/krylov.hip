@@ -38,6 +38,8 @@ abh_status abh_sparse_op_gmres(abh_sparse_op *op, double diag_shift, const abh_s
     ABH_REQUIRE(opts->precision == ABH_FP64 || opts->precision == ABH_FP32_COMPENSATED,
                 ABH_ERR_UNSUPPORTED, "GMRES operator precision must be FP64 or FP32_COMPENSATED");
     ABH_REQUIRE(n == 0 || (b && x), ABH_ERR_INVALID_ARG, "b or x is null");
+    ABH_REQUIRE(opts->graph == 0, ABH_ERR_UNSUPPORTED,
+                "graph-captured GMRES cycles are for dense batches");
     const int64_t off[2] = {0, n};
     abh::BatchLayout L;
     L.build(ctx, off, 1);
@@ -51,20 +53,19 @@ abh_status abh_sparse_op_gmres(abh_sparse_op *op, double diag_shift, const abh_s
         abh::check_launch();
       }
     };
-    std::function<void(const double *, double *)> f_op = [&](const double *in, double *out) {
+    abh::BatchOp f_op = [&](const double *in, double *out, const uint8_t *) {
       apply(in, out, fp32);
     };
-    std::function<void(const double *, double *)> f_true = [&](const double *in, double *out) {
+    abh::BatchOp f_true = [&](const double *in, double *out, const uint8_t *) {
       apply(in, out, false);
     };
-    std::function<void(const double *, double *)> f_prec = [&](const double *in, double *out) {
+    abh::BatchPrec f_prec = [&](const double *in, double *out) {
       if (schwarz) abh::schwarz_apply(schwarz, in, out);
       else abh::ainv_apply(ainv, in, out);
     };
-    std::function<void(const uint8_t *)> f_act = [](const uint8_t *) {};
     abh::GmresResult res;
-    abh::batch_gmres(ctx, L, f_op, f_true, (schwarz || ainv) ? &f_prec : nullptr, f_act, b, x,
-                     *opts, res);
+    abh::batch_gmres(ctx, L, f_op, f_true, (schwarz || ainv) ? &f_prec : nullptr, b, x, *opts,
+                     false, res);
     if (stats) stats[0] = res.stats[0];
   });
 }

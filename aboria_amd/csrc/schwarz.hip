@@ -571,6 +571,7 @@ void schwarz_apply(const abh_schwarz *s, const double *b, double *x) {
 }
 
 int64_t schwarz_size(const abh_schwarz *s) { return s->n; }
+abh_context *schwarz_context(const abh_schwarz *s) { return s->ctx; }
 
 abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<int64_t> &off,
                                   const double *pos_dev, const double *lo, const double *hi,

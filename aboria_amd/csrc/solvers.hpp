@@ -22,6 +22,7 @@ double *dense_scratch_address();
 // x = M^-1 b, restricted additive Schwarz (schwarz.hip).
 void schwarz_apply(const abh_schwarz *s, const double *b, double *x);
 int64_t schwarz_size(const abh_schwarz *s);
+abh_context *schwarz_context(const abh_schwarz *s);
 // Batched restricted additive Schwarz over independent particle sets
 // (schwarz.hip): signed factorisation, no kernel truncation.  Throws abh::Error.
 abh_schwarz *schwarz_create_batch(abh_context *ctx, int dim, const std::vector<int64_t> &off,

@@ -600,6 +600,13 @@ typedef struct abh_gmres_options {
                             true residual by 10% (converged stays 0), so a
                             batch ends once every problem has converged or
                             stagnated, e.g. at its FP64 residual floor.     */
+  int32_t graph;          /* 0: the host checks after every Arnoldi step
+                            whether a problem is still iterating.  1 (dense
+                            batches only; ABH_ERR_UNSUPPORTED for the sparse
+                            operator): the steps of a restart cycle run as
+                            HIP graphs of 5 steps each, captured once per
+                            solve and replayed every cycle, the host checking
+                            between graphs.                                  */
 } abh_gmres_options;
 
 typedef struct abh_gmres_stats {

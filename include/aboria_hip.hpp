@@ -192,7 +192,7 @@ class SparseOperator {
   abh_gmres_stats gmres(const double *b, double *x, double diag_shift = 0.0,
                         const abh_schwarz *prec = nullptr, int restart = 50,
                         int max_iters = 1000, double tol = 1e-8) const {
-    abh_gmres_options o{restart, max_iters, tol, ABH_FP64, 0};
+    abh_gmres_options o{restart, max_iters, tol, ABH_FP64, 0, 0};
     abh_gmres_stats st{};
     check(abh_sparse_op_gmres(h_, diag_shift, prec, nullptr, b, x, &o, &st), ctx_->get());
     return st;

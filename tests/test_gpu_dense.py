@@ -200,6 +200,43 @@ def test_batch_gmres_with_schwarz_converges(ctx):
         assert st[p]["iterations"] < st0[p]["iterations"] or not st0[p]["converged"]
 
 
+@pytest.mark.parametrize("precision", [ab.FP64, ab.FP32_COMPENSATED])
+def test_batch_gmres_graph_equals_eager(ctx, precision):
+    """graph=True captures a restart cycle's Arnoldi steps (operator,
+    Schwarz apply with its device copy, CGS2, norms, Givens kernels) into
+    HIP graphs of 5 steps replayed every cycle, with the host's checks and
+    synchronisations between replays.  Restart 4 gives several replays; every
+    problem's iterations, flags and residuals and the solution are those of
+    the eager run bit for bit (the same kernels in the same order; steps
+    after every problem has left a cycle are masked no-ops), and in FP64 the
+    true residual recomputed by the oracle meets the tolerance."""
+    sizes = [300, 900, 1400, 60]
+    pos, off = make_batch(sizes, 31)
+    allp = np.concatenate(pos)
+    f = np.sin(2 * np.pi * allp[:, 0]) * np.cos(2 * np.pi * allp[:, 1])
+    A = ab.DenseBatchOperator(ctx, off, torch.tensor(allp, device="cuda"),
+                              ab.Kernel.multiquadric(0.05))
+    sw = ab.SchwartzPreconditioner.for_dense_batch(A, [0, 0], [1, 1], 64.0)
+    bt = torch.tensor(f, device="cuda")
+    xe, se = A.gmres(bt, restart=4, max_iters=400, preconditioner=sw, precision=precision)
+    xg, sg = A.gmres(bt, restart=4, max_iters=400, preconditioner=sw, precision=precision,
+                     graph=True)
+    print("graph", sg, "eager", se, flush=True)
+    assert se == sg
+    assert torch.equal(xe, xg)
+    assert max(s["iterations"] for s in sg) > 8  # three replays or more
+    x = xg.cpu().numpy()
+    for p in range(len(sizes) if precision == ab.FP64 else 0):
+        a, e = off[p], off[p + 1]
+        assert sg[p]["converged"], (p, sg[p])
+        r = oracle.dense_matvec(pos[p], pos[p], (KM, 0.05), x[a:e]) - f[a:e]
+        assert np.linalg.norm(r) / np.linalg.norm(f[a:e]) <= 2e-8
+    # unpreconditioned too (no device copy in the graph)
+    xe, se = A.gmres(bt, restart=10, max_iters=60, precision=precision)
+    xg, sg = A.gmres(bt, restart=10, max_iters=60, precision=precision, graph=True)
+    assert se == sg and torch.equal(xe, xg)
+
+
 def test_batch_gmres_stops_stagnated_problems(ctx):
     """stall_restarts: a problem whose residual cannot reach the tolerance
     (multiquadric at N = 6000: FP64 rounding of K x sits far above 1e-8 ||b||)
