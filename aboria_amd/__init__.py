@@ -826,6 +826,19 @@ class FmmOperator:
               self.ctx.handle)
         return kp.value, [ranks[i] for i in range(max(0, levels - 1))] if kp.value else []
 
+    def m2l_tuning(self):
+        """(tiles per block chosen for the FP16 compressed M2L, [ms with 1,
+        ms with 2]); 0 before the first MIXED apply after compress."""
+        tt = ctypes.c_int()
+        ms = (ctypes.c_float * 2)()
+        check(C.lib().abh_fmm_m2l_tuning(self.handle, ctypes.byref(tt), ms), self.ctx.handle)
+        return tt.value, [ms[0], ms[1]]
+
+    def set_m2l_tuning(self, tiles_per_block):
+        """Fix the FP16 compressed M2L's tiles per block (1, 2; 0 = retune)."""
+        check(C.lib().abh_fmm_set_m2l_tuning(self.handle, int(tiles_per_block)),
+              self.ctx.handle)
+
     def apply_colmajor(self, xt, yt, precision=FP64):
         """yt += (K x)^T for (nrhs, n) contiguous xt, yt."""
         check(C.lib().abh_fmm_apply(self.handle, xt.shape[0], _ptr(xt), xt.shape[1], _ptr(yt),

@@ -243,6 +243,8 @@ SIGNATURES = {
     "abh_dense_batch_tune": (c_int, [c_void_p, c_int, P(c_int), P(ctypes.c_float)]),
     "abh_schwarz_create": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double,
                                    P(c_void_p)]),
+    "abh_fmm_m2l_tuning": (c_int, [c_void_p, P(c_int), P(ctypes.c_float)]),
+    "abh_fmm_set_m2l_tuning": (c_int, [c_void_p, c_int]),
     "abh_schwarz_create_coarse": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double,
                                           c_int, ctypes.c_uint64, P(c_void_p)]),
     "abh_schwarz_factorize": (c_int, [c_void_p, c_void_p, P(Kernel), c_double, c_double]),

@@ -838,12 +838,14 @@ __global__ __launch_bounds__(256) void k_split_w16(const double *__restrict__ Wc
   }
 }
 
-// One block of 2 x RT wavefronts per two sibling tiles: wave w owns row tile
-// w % RT of the compressed output and target pairs 2 (w / RT), 2 (w / RT) + 1
-// of both tiles (a pair of targets x 16 right-hand sides = one 32-column
-// MFMA tile).  Per box offset both operands are read straight from L2 in the
-// fragment layout; pairs with no source at the offset are skipped.
-template <int KP>
+// One block of 2 x RT wavefronts per TT sibling tiles (TT = 1 or 2, picked
+// per FMM by timing both): wave w owns row tile w % RT of the compressed
+// output and target pairs 2 (w / RT), 2 (w / RT) + 1 of every tile (a pair
+// of targets x 16 right-hand sides = one 32-column MFMA tile), each A
+// fragment feeding 2 TT products.  Per box offset both operands are read
+// straight from L2 in the fragment layout; pairs with no source at the
+// offset are skipped.
+template <int KP, int TT>
 __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
     const int32_t *__restrict__ tile_parent, int ntiles, const int32_t *__restrict__ child0,
     const int32_t *__restrict__ coord, const int32_t *__restrict__ map, int side,
@@ -851,13 +853,13 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
     const _Float16 *__restrict__ Wh, const _Float16 *__restrict__ Wl,
     const unsigned long long *__restrict__ wmax, double c_inv, double *__restrict__ Gc) {
   constexpr int RT = (KP + 31) / 32, KS = KP / 16;
-  __shared__ int src[2][8];
-  const int tile0 = 2 * xcd_block((ntiles + 1) / 2);
+  __shared__ int src[TT][8];
+  const int tile0 = TT * xcd_block((ntiles + TT - 1) / TT);
   if (tile0 >= ntiles) return;
-  const int ntl = ntiles - tile0 < 2 ? ntiles - tile0 : 2;
-  int first[2], pc[2][3];
+  const int ntl = ntiles - tile0 < TT ? ntiles - tile0 : TT;
+  int first[TT], pc[TT][3];
 #pragma unroll
-  for (int tt = 0; tt < 2; ++tt) {
+  for (int tt = 0; tt < TT; ++tt) {
     const int parent = tile_parent[tile0 + (tt < ntl ? tt : 0)];
     first[tt] = child0[parent];
 #pragma unroll
@@ -867,9 +869,9 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
   const int rt = wave % RT, p0 = 2 * (wave / RT);
   const int half = lane >> 5, col = lane & 31;
   const int cq = col >> 4, r = col & 15;
-  f32x16 acc[2][2];
+  f32x16 acc[TT][2];
 #pragma unroll
-  for (int tt = 0; tt < 2; ++tt)
+  for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
@@ -882,7 +884,7 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
     const int ox = o / 49 - 3, oy = (o / 7) % 7 - 3, oz = o % 7 - 3;
     if (ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1 && oz >= -1 && oz <= 1) continue;
     __syncthreads();
-    if (threadIdx.x < 16) {
+    if (threadIdx.x < 8 * TT) {
       const int tt = threadIdx.x >> 3, chd = threadIdx.x & 7;
       const int qx = ((chd >> 2) & 1) + ox, qy = ((chd >> 1) & 1) + oy, qz = (chd & 1) + oz;
       int sidx = -1;
@@ -894,12 +896,12 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
       src[tt][chd] = sidx;
     }
     __syncthreads();
-    bool act[2][2];
-    int sl[2][2];
+    bool act[TT][2];
+    int sl[TT][2];
     bool any = false;
     int svalid = 0;
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt)
+    for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const int p = p0 + q;
@@ -914,9 +916,9 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
     for (int ks = 0; ks < KS; ++ks) {
       const f16x8 ah = ch[obase + (int64_t(ks) * RT + rt) * 64 + lane];
       const f16x8 al = cl[obase + (int64_t(ks) * RT + rt) * 64 + lane];
-      f16x8 bh[2][2], bl[2][2];
+      f16x8 bh[TT][2], bl[TT][2];
 #pragma unroll
-      for (int tt = 0; tt < 2; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           const int sv = sl[tt][q] >= 0 ? sl[tt][q] : svalid;
@@ -925,7 +927,7 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
           bl[tt][q] = wl[e];
         }
 #pragma unroll
-      for (int tt = 0; tt < 2; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (!act[tt][q]) continue;
@@ -943,7 +945,7 @@ __global__ __launch_bounds__(128 * ((KP + 31) / 32)) void k_m2l_mfma16c(
   // column l % 32 of the 32 x 32 tile
   const double inv = c_inv / pow2_scale(__longlong_as_double((long long)*wmax));
 #pragma unroll
-  for (int tt = 0; tt < 2; ++tt) {
+  for (int tt = 0; tt < TT; ++tt) {
     if (tt >= ntl) continue;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -1548,6 +1550,10 @@ struct abh_fmm {
   bool c16 = false;
   std::vector<double> c_inv;
   DevBuf Ch16, Cl16, Wh16, Wl16, wmax;
+  // tiles per block of the FP16 compressed M2L (0: not tuned yet) and the
+  // tuner's timings of one pass over every level with 1 and 2 tiles
+  int m2l16_tt = 0;
+  float m2l16_ms[2] = {0.0f, 0.0f};
 };
 
 namespace {
@@ -1828,10 +1834,62 @@ template <int N> void compress_impl(abh_fmm *f, double eps) {
     }
   }
   f->c16 = c16;
+  f->m2l16_tt = 0;  // new operators: tuned again at the next MIXED apply
   f->Wc.ensure(size_t(f->n_nodes) * KP * kRhs * 8);
   f->Gc.ensure(size_t(f->n_nodes) * KP * kRhs * 8);
   ABH_HIP(hipStreamSynchronize(s));  // U, T, Cm, gram are freed below
   f->comp_kp = KP;
+}
+
+// the FP16 compressed M2L of level l (targets: the children of level l-1's
+// internal nodes) with tt sibling tiles per block
+void m2l16_level(abh_fmm *f, int l, int tt, hipStream_t s) {
+  const int KP = f->comp_kp;
+  const int np = int(f->tree.par_off[l] - f->tree.par_off[l - 1]);
+  if (np == 0) return;
+  const int32_t *par = f->tree.internal.as<int32_t>() + f->tree.par_off[l - 1];
+  const int64_t per16 = int64_t(kOffsets) * (KP / 16) * ((KP + 31) / 32) * 512;
+  dispatch_kp16(KP, [&](auto V) {
+    constexpr int KPV = decltype(V)::value;
+    const unsigned threads = 128 * ((KPV + 31) / 32);
+#define ABH_M2L16_ARGS                                                                        \
+  par, np, f->tree.nodes.as<int32_t>(), f->coord.as<int32_t>(),                                \
+      f->maps.as<int32_t>() + f->map_off[l], 1 << l, f->Ch16.as<_Float16>() + int64_t(l - 2) * per16, \
+      f->Cl16.as<_Float16>() + int64_t(l - 2) * per16, f->Wh16.as<_Float16>(),                 \
+      f->Wl16.as<_Float16>(), f->wmax.as<unsigned long long>() + (l - 2), f->c_inv[size_t(l) - 2], \
+      f->Gc.as<double>()
+    if (tt == 1)
+      k_m2l_mfma16c<KPV, 1><<<xcd_grid(np), threads, 0, s>>>(ABH_M2L16_ARGS);
+    else
+      k_m2l_mfma16c<KPV, 2><<<xcd_grid((np + 1) / 2), threads, 0, s>>>(ABH_M2L16_ARGS);
+#undef ABH_M2L16_ARGS
+  });
+  abh::check_launch();
+}
+
+// Autotuner of the FP16 compressed M2L: one pass over every level with 1
+// and with 2 sibling tiles per block, timed with events on the stream, the
+// faster kept for this FMM.  The passes add into the compressed locals,
+// which are zeroed again afterwards.
+void tune_m2l16(abh_fmm *f, hipStream_t s) {
+  hipEvent_t ev[3];
+  for (hipEvent_t &e : ev) ABH_HIP(hipEventCreate(&e));
+  struct Free {
+    hipEvent_t *e;
+    ~Free() {
+      for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } fr{ev};
+  ABH_HIP(hipEventRecord(ev[0], s));
+  for (int tt = 1; tt <= 2; ++tt) {
+    for (int l = 2; l <= f->nlevels; ++l) m2l16_level(f, l, tt, s);
+    ABH_HIP(hipEventRecord(ev[tt], s));
+  }
+  ABH_HIP(hipEventSynchronize(ev[2]));
+  ABH_HIP(hipEventElapsedTime(&f->m2l16_ms[0], ev[0], ev[1]));
+  ABH_HIP(hipEventElapsedTime(&f->m2l16_ms[1], ev[1], ev[2]));
+  f->m2l16_tt = f->m2l16_ms[0] < f->m2l16_ms[1] ? 1 : 2;
+  ABH_HIP(hipMemsetAsync(f->Gc.p, 0, size_t(f->n_nodes) * f->comp_kp * kRhs * 8, s));
 }
 
 template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t ldx, double *y,
@@ -1914,6 +1972,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
           });
           abh::check_launch();
         }
+        if (f->m2l16_tt == 0) tune_m2l16(f, s);
       }
     }
     // downward: level l receives L2L from level l-1 and its M2L
@@ -1927,17 +1986,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
         // C_o W^ into G^, then G += U G^
         if (mixed && f->c16) {
           const int64_t per16 = int64_t(kOffsets) * (KP / 16) * ((KP + 31) / 32) * 512;
-          dispatch_kp16(KP, [&](auto V) {
-            constexpr int KPV = decltype(V)::value;
-            k_m2l_mfma16c<KPV><<<xcd_grid((npar(l - 1) + 1) / 2), 128 * ((KPV + 31) / 32), 0,
-                                 s>>>(
-                par + f->tree.par_off[l - 1], npar(l - 1), f->tree.nodes.as<int32_t>(),
-                f->coord.as<int32_t>(), f->maps.as<int32_t>() + f->map_off[l], 1 << l,
-                f->Ch16.as<_Float16>() + int64_t(l - 2) * per16,
-                f->Cl16.as<_Float16>() + int64_t(l - 2) * per16, f->Wh16.as<_Float16>(),
-                f->Wl16.as<_Float16>(), f->wmax.as<unsigned long long>() + (l - 2),
-                f->c_inv[size_t(l) - 2], f->Gc.as<double>());
-          });
+          m2l16_level(f, l, f->m2l16_tt, s);
         } else {
           m2l_compressed(KP, par + f->tree.par_off[l - 1], npar(l - 1),
                          f->tree.nodes.as<int32_t>(), f->coord.as<int32_t>(),
@@ -2185,6 +2234,22 @@ abh_status abh_fmm_destroy(abh_fmm *f) {
   (void)hipStreamSynchronize(ctx->stream);
   delete f;
   abh::ctx_release(ctx);
+  return ABH_OK;
+}
+
+abh_status abh_fmm_m2l_tuning(const abh_fmm *f, int *tiles_per_block, float *ms) {
+  if (!f) return ABH_ERR_INVALID_ARG;
+  if (tiles_per_block) *tiles_per_block = f->m2l16_tt;
+  if (ms) {
+    ms[0] = f->m2l16_ms[0];
+    ms[1] = f->m2l16_ms[1];
+  }
+  return ABH_OK;
+}
+
+abh_status abh_fmm_set_m2l_tuning(abh_fmm *f, int tiles_per_block) {
+  if (!f || tiles_per_block < 0 || tiles_per_block > 2) return ABH_ERR_INVALID_ARG;
+  f->m2l16_tt = tiles_per_block;
   return ABH_OK;
 }
 

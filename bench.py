@@ -354,8 +354,14 @@ def bench_fmm(args, torch, ab, wl, ctx, timer, extra):
                            reps=3 if (order <= 6 or tol is not None) else 1)
                 mps = c["nrhs"] / (ms * 1e-3)
                 o[name + tag] = {"apply_ms": ms, "matvec_per_s": mps, "rel_err_vs_direct": err}
-                log("fmm order %d %s%s: %d rhs in %.1f ms (%.1f matvec/s), rel err %.2e" % (
-                    order, name, tag, c["nrhs"], ms, mps, err))
+                if prec == ab.MIXED and tol is not None:
+                    tt, tms = f.m2l_tuning()
+                    o[name + tag]["m2l_tiles_per_block"] = tt
+                    o[name + tag]["m2l_tuning_ms"] = tms
+                log("fmm order %d %s%s: %d rhs in %.1f ms (%.1f matvec/s), rel err %.2e%s" % (
+                    order, name, tag, c["nrhs"], ms, mps, err,
+                    " (M2L tuner: %d tile(s)/block, %s ms)" % (tt, tms)
+                    if prec == ab.MIXED and tol is not None else ""))
                 if err <= 1e-6 and (best is None or mps > best["matvec_per_s"]):
                     best = {"order": order, "precision": name + tag, "matvec_per_s": mps,
                             "rel_err_vs_direct": err}

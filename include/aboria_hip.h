@@ -518,6 +518,17 @@ abh_status abh_fmm_apply(abh_fmm *fmm, int nrhs, const double *x, int64_t ldx,
  * 2..levels.  Synchronises. */
 abh_status abh_fmm_compress(abh_fmm *fmm, double eps, int *padded_rank,
                             int *ranks);
+/* The M2L autotuner's choice (no counterpart in the reference, which runs
+ * its M2L on the CPU): the first MIXED apply after abh_fmm_compress times
+ * one pass of the FP16 compressed M2L over every level with 1 and with 2
+ * sibling tiles per block and keeps the faster.  tiles_per_block (host, may
+ * be NULL): 0 before tuning, else 1 or 2; ms (host, 2 floats, may be NULL):
+ * the two passes' times in milliseconds. */
+abh_status abh_fmm_m2l_tuning(const abh_fmm *fmm, int *tiles_per_block, float *ms);
+/* Fixes the FP16 compressed M2L's tiles per block (1 or 2), or 0: tune
+ * again at the next MIXED apply.  Results do not depend on it (the same
+ * products in the same order). */
+abh_status abh_fmm_set_m2l_tuning(abh_fmm *fmm, int tiles_per_block);
 
 /* ------------------------------------------------------------------ */
 /* batched dense kernel operators and GMRES                            */

@@ -176,7 +176,8 @@ def test_fmm_compressed_mixed_fp16(ctx, order, eps):
     (two-term split, power-of-two scaling per level, FP32 accumulation), 16
     right-hand sides on 40,000 uniform points:
     within 5e-7 (relative, 2-norm) of the FP64 compressed apply and within
-    1e-6 of the FP64 direct sum at order 8."""
+    1e-6 of the FP64 direct sum at order 8; the M2L autotuner's two tile
+    shapes (1 and 2 sibling tiles per block) give identical results."""
     print("fmm compressed mixed order %d eps %g" % (order, eps), flush=True)
     n = 40_000
     rng = np.random.default_rng(12)
@@ -198,6 +199,16 @@ def test_fmm_compressed_mixed_fp16(ctx, order, eps):
     assert d < 5e-7
     if order == 8:
         assert emx < 1e-6
+    # the autotuner timed both tile shapes on the first MIXED apply; either
+    # shape gives the same bits
+    tt, ms = f.m2l_tuning()
+    assert tt in (1, 2) and ms[0] > 0 and ms[1] > 0
+    outs = []
+    for shape in (1, 2):
+        f.set_m2l_tuning(shape)
+        outs.append(f.apply_colmajor(xt, torch.zeros_like(xt), ab.MIXED))
+    assert torch.equal(outs[0], outs[1])
+    assert np.array_equal(outs[tt - 1].cpu().numpy(), ymx)
 
 
 def test_fmm_tree_leaf_rule(ctx):
