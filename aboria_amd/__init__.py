@@ -1133,13 +1133,14 @@ class Mlp:
         return loss.value, g
 
     def train(self, x, t, epochs=20, batch=1024, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
-              lr_decay=1.0, seed=0):
-        """Adam on the mean squared error; returns the per-epoch mean loss."""
+              lr_decay=1.0, seed=0, graph=False):
+        """Adam on the mean squared error; returns the per-epoch mean loss.
+        graph: the steps replay one captured HIP graph."""
         import numpy as np
         _require_cuda(x, torch.float32, "x")
         _require_cuda(t, torch.float32, "t")
         o = C.MlpTrainOptions(int(epochs), int(batch), float(lr), float(beta1), float(beta2),
-                              float(eps), float(lr_decay), 0, int(seed))
+                              float(eps), float(lr_decay), int(bool(graph)), int(seed))
         losses = np.zeros(max(int(epochs), 1), np.float32)
         check(C.lib().abh_mlp_train(self.handle, x.shape[0], _ptr(x), _ptr(t), ctypes.byref(o),
                                     losses.ctypes.data), self.ctx.handle)
@@ -1162,12 +1163,13 @@ class Mlp:
         return loss.value, g
 
     def train_residual(self, x, mats, epochs=20, batch=1024, lr=1e-3, beta1=0.9, beta2=0.999,
-                       eps=1e-8, lr_decay=1.0, seed=0):
-        """Adam on the patch-residual loss; returns the per-epoch mean loss."""
+                       eps=1e-8, lr_decay=1.0, seed=0, graph=False):
+        """Adam on the patch-residual loss; returns the per-epoch mean loss.
+        graph: the steps replay one captured HIP graph."""
         import numpy as np
         self._check_mats(x, mats)
         o = C.MlpTrainOptions(int(epochs), int(batch), float(lr), float(beta1), float(beta2),
-                              float(eps), float(lr_decay), 0, int(seed))
+                              float(eps), float(lr_decay), int(bool(graph)), int(seed))
         losses = np.zeros(max(int(epochs), 1), np.float32)
         check(C.lib().abh_mlp_train_residual(self.handle, x.shape[0], _ptr(x), _ptr(mats),
                                              ctypes.byref(o), losses.ctypes.data),

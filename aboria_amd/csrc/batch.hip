@@ -185,46 +185,6 @@ __global__ void k_restore(const Chunk *__restrict__ ch, const double *__restrict
   for (int64_t t = threadIdx.x; t < c.len; t += kThreads) x[c.start + t] = xprev[c.start + t];
 }
 
-// Graph capture needs a stream other than the null stream: the context's
-// work moves to a stream of its own for the solve (ordered after the
-// caller's stream by an event, and the caller's stream after it on exit).
-struct StreamSwap {
-  abh_context *ctx = nullptr;
-  hipStream_t orig = nullptr, mine = nullptr;
-  hipEvent_t ev = nullptr;
-  void begin(abh_context *c) {
-    ctx = c;
-    orig = c->stream;
-    ABH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    ABH_HIP(hipStreamCreateWithFlags(&mine, hipStreamNonBlocking));
-    ABH_HIP(hipEventRecord(ev, orig));
-    ABH_HIP(hipStreamWaitEvent(mine, ev, 0));
-    c->stream = mine;
-  }
-  ~StreamSwap() {
-    if (!mine) return;
-    (void)hipEventRecord(ev, mine);
-    (void)hipStreamWaitEvent(orig, ev, 0);
-    (void)hipStreamSynchronize(mine);
-    ctx->stream = orig;
-    (void)hipStreamDestroy(mine);
-    (void)hipEventDestroy(ev);
-  }
-};
-
-// ends a stream capture left open by an exception
-struct CaptureGuard {
-  hipStream_t s;
-  bool open = false;
-  ~CaptureGuard() {
-    if (open) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-    }
-  }
-};
-
 }  // namespace
 
 void BatchLayout::build(abh_context *ctx, const int64_t *offsets, int np) {
@@ -279,7 +239,7 @@ void batch_scale(abh_context *ctx, const BatchLayout &L, const double *in, const
 void batch_gmres(abh_context *ctx, const BatchLayout &L, const BatchOp &op,
                  const BatchOp &op_true, const BatchPrec *prec, const double *b, double *x,
                  const abh_gmres_options &opt, bool graph, GmresResult &res) {
-  StreamSwap swap;
+  CaptureStream swap;
   if (graph) swap.begin(ctx);
   hipStream_t s = ctx->stream;
   const int np = L.nprob;

@@ -118,6 +118,53 @@ inline void ctx_release(abh_context *ctx) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Graph capture needs a stream other than the null stream: for the life of
+// this object the context's work goes to a stream of its own (default
+// priority, ordered after the caller's stream by an event, and the caller's
+// stream after it on exit).
+struct CaptureStream {
+  abh_context *ctx = nullptr;
+  hipStream_t orig = nullptr, mine = nullptr;
+  hipEvent_t ev = nullptr;
+  CaptureStream() = default;
+  CaptureStream(const CaptureStream &) = delete;
+  CaptureStream &operator=(const CaptureStream &) = delete;
+  void begin(abh_context *c) {
+    ctx = c;
+    orig = c->stream;
+    ABH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    ABH_HIP(hipStreamCreateWithFlags(&mine, hipStreamNonBlocking));
+    ABH_HIP(hipEventRecord(ev, orig));
+    ABH_HIP(hipStreamWaitEvent(mine, ev, 0));
+    c->stream = mine;
+  }
+  ~CaptureStream() {
+    if (!mine) {
+      if (ev) (void)hipEventDestroy(ev);
+      return;
+    }
+    (void)hipEventRecord(ev, mine);
+    (void)hipStreamWaitEvent(orig, ev, 0);
+    (void)hipStreamSynchronize(mine);
+    ctx->stream = orig;
+    (void)hipStreamDestroy(mine);
+    (void)hipEventDestroy(ev);
+  }
+};
+
+// Ends a stream capture left open by an exception.
+struct CaptureGuard {
+  hipStream_t s;
+  bool open = false;
+  ~CaptureGuard() {
+    if (open) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  }
+};
+
 // -------------------------------------------------------------------------
 // device helpers (wave64)
 // -------------------------------------------------------------------------

@@ -568,22 +568,26 @@ __global__ __launch_bounds__(256) void k_resid(const float *__restrict__ Y,
   if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// out[step ? *step : 0] = scale * sum of the partials
 __global__ void k_sum_partials(const double *__restrict__ partial, int nb, double scale,
-                               double *__restrict__ out) {
+                               double *__restrict__ out, const int32_t *__restrict__ step) {
   __shared__ double red[4];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += 256) s += partial[b];
   s = abh::wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) *out = ((red[0] + red[1]) + (red[2] + red[3])) * scale;
+  if (threadIdx.x == 0) out[step ? *step : 0] = ((red[0] + red[1]) + (red[2] + red[3])) * scale;
 }
 
+// Adam with the step size of training step *step (lr_steps, per epoch)
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, int64_t n, float b1, float b2, float lr_t,
+                       float *__restrict__ v, int64_t n, float b1, float b2,
+                       const float *__restrict__ lr_steps, const int32_t *__restrict__ step,
                        float eps) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n) return;
+  const float lr_t = lr_steps[*step];
   const float ge = g[e];
   const float me = b1 * m[e] + (1.0f - b1) * ge;
   const float ve = b2 * v[e] + (1.0f - b2) * ge * ge;
@@ -592,13 +596,24 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
   p[e] = p[e] - lr_t * me / (sqrtf(ve) + eps);
 }
 
-__global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__restrict__ rows,
-                              int nrows, int cols, float *__restrict__ dst) {
+// the rows of training step *step: perm[*step * nrows + r]
+__global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__restrict__ perm,
+                              const int32_t *__restrict__ step, int nrows, int cols,
+                              float *__restrict__ dst) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= int64_t(nrows) * cols) return;
   const int64_t r = e / cols;
   const int c = int(e - r * cols);
+  const int32_t *rows = perm + int64_t(*step) * nrows;
   dst[e] = src[int64_t(rows[r]) * cols + c];
+}
+
+__global__ void k_set_step(int32_t *__restrict__ step, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step = v;
+}
+
+__global__ void k_next_step(int32_t *__restrict__ step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -796,7 +811,7 @@ Fwd forward_f32(abh_mlp *m, int64_t rows, const float *x, float *y) {
 // patch residual |M_r y_r - e_1|^2; scratch holds dY and the backward
 // activations
 void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, const float *mats,
-               float *grad, double *loss_dev, DevBuf &scratch) {
+               float *grad, double *loss_dev, DevBuf &scratch, const int32_t *step = nullptr) {
   hipStream_t s = m->ctx->stream;
   const int nl = m->layers();
   const int nout = m->dims.back();
@@ -815,12 +830,13 @@ void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, const f
   if (mats) {
     k_resid<<<unsigned(nres), 256, 0, s>>>(Y, mats, rows, nout, 1.0f / float(rows), dA, partial);
     abh::check_launch();
-    k_sum_partials<<<1, 256, 0, s>>>(partial, int(nres), 1.0 / double(rows), loss_dev);
+    k_sum_partials<<<1, 256, 0, s>>>(partial, int(nres), 1.0 / double(rows), loss_dev, step);
   } else {
     k_mse<<<unsigned(nmse), 256, 0, s>>>(Y, t, rows * nout, 1.0f / float(rows * nout), dA,
                                          partial);
     abh::check_launch();
-    k_sum_partials<<<1, 256, 0, s>>>(partial, int(nmse), 1.0 / double(rows * nout), loss_dev);
+    k_sum_partials<<<1, 256, 0, s>>>(partial, int(nmse), 1.0 / double(rows * nout), loss_dev,
+                                     step);
   }
   abh::check_launch();
   const float *P = m->params.as<float>();
@@ -868,49 +884,95 @@ void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, const f
 }
 
 // Adam over shuffled full batches: targets t (n x n_out) for the mean
-// squared error, or mats (n x n_out x n_out) for the patch residual
+// squared error, or mats (n x n_out x n_out) for the patch residual.  The
+// kernels of a step read the step's rows, loss slot and step size through a
+// device step counter, so every step launches the same kernels with the
+// same arguments; with opts->graph the second step is captured into a HIP
+// graph (on a stream of the context's own) and every later step replays it.
 void train_impl(abh_mlp *m, int64_t n, const float *x, const float *t, const float *mats,
                 const abh_mlp_train_options *opts, float *loss_per_epoch) {
   ABH_REQUIRE(opts->epochs >= 0 && opts->batch >= 1 && opts->batch <= (1 << 20),
               ABH_ERR_INVALID_ARG, "invalid training options");
+  const bool graph = opts->graph != 0;
+  abh::CaptureStream cs;
+  if (graph) cs.begin(m->ctx);
   hipStream_t s = m->ctx->stream;
   const int nin = m->dims.front(), nout = m->dims.back();
   const int64_t tw = mats ? int64_t(nout) * nout : nout;  // floats per target row
   const int64_t B = std::min<int64_t>(opts->batch, n);
   const int64_t steps = n / B;  // full batches per epoch
-  DevBuf perm_d, xb, tb, scratch, losses;
+  DevBuf perm_d, xb, tb, scratch, losses, lr_d, step_d;
   perm_d.ensure(size_t(n) * 4 + 64);
   xb.ensure(size_t(B) * nin * 4 + 64);
   tb.ensure(size_t(B) * tw * 4 + 64);
   losses.ensure(size_t(std::max<int64_t>(steps, 1)) * 8 + 64);
+  lr_d.ensure(size_t(std::max<int64_t>(steps, 1)) * 4 + 64);
+  step_d.ensure(64);
   std::vector<int32_t> perm(static_cast<size_t>(n));
+  std::vector<float> lrs(size_t(std::max<int64_t>(steps, 1)));
   std::mt19937_64 rng(opts->seed);
   float lr = opts->lr;
   std::vector<double> lh(size_t(std::max<int64_t>(steps, 1)));
+  const int32_t *stp = step_d.as<int32_t>();
+  auto one_step = [&]() {
+    k_gather_rows<<<unsigned(abh::ceil_div(B * nin, 256)), 256, 0, s>>>(
+        x, perm_d.as<int32_t>(), stp, int(B), nin, xb.as<float>());
+    k_gather_rows<<<unsigned(abh::ceil_div(B * tw, 256)), 256, 0, s>>>(
+        mats ? mats : t, perm_d.as<int32_t>(), stp, int(B), int(tw), tb.as<float>());
+    abh::check_launch();
+    loss_grad(m, B, xb.as<float>(), mats ? nullptr : tb.as<float>(),
+              mats ? tb.as<float>() : nullptr, m->grads.as<float>(), losses.as<double>(), scratch,
+              stp);
+    k_adam<<<unsigned(abh::ceil_div(m->nparams, 256)), 256, 0, s>>>(
+        m->params.as<float>(), m->grads.as<float>(), m->am.as<float>(), m->av.as<float>(),
+        m->nparams, opts->beta1, opts->beta2, lr_d.as<float>(), stp, opts->eps);
+    k_next_step<<<1, 64, 0, s>>>(step_d.as<int32_t>());
+    abh::check_launch();
+  };
+  hipGraph_t gr = nullptr;
+  hipGraphExec_t ge = nullptr;
+  struct GraphFree {
+    hipGraph_t &g;
+    hipGraphExec_t &e;
+    ~GraphFree() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } graph_free{gr, ge};
+  bool sized = false;  // one eager step has sized every work buffer
   for (int ep = 0; ep < opts->epochs; ++ep) {
     std::iota(perm.begin(), perm.end(), 0);
     std::shuffle(perm.begin(), perm.end(), rng);
-    ABH_HIP(hipMemcpyAsync(perm_d.p, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
     for (int64_t st = 0; st < steps; ++st) {
-      const int32_t *rows = perm_d.as<int32_t>() + st * B;
-      k_gather_rows<<<unsigned(abh::ceil_div(B * nin, 256)), 256, 0, s>>>(x, rows, int(B), nin,
-                                                                         xb.as<float>());
-      k_gather_rows<<<unsigned(abh::ceil_div(B * tw, 256)), 256, 0, s>>>(
-          mats ? mats : t, rows, int(B), int(tw), tb.as<float>());
-      abh::check_launch();
-      loss_grad(m, B, xb.as<float>(), mats ? nullptr : tb.as<float>(),
-                mats ? tb.as<float>() : nullptr, m->grads.as<float>(),
-                losses.as<double>() + st, scratch);
       ++m->adam_t;
       const double b1 = opts->beta1, b2 = opts->beta2;
-      const double lr_t = lr * std::sqrt(1.0 - std::pow(b2, double(m->adam_t))) /
-                          (1.0 - std::pow(b1, double(m->adam_t)));
-      k_adam<<<unsigned(abh::ceil_div(m->nparams, 256)), 256, 0, s>>>(
-          m->params.as<float>(), m->grads.as<float>(), m->am.as<float>(), m->av.as<float>(),
-          m->nparams, opts->beta1, opts->beta2, float(lr_t), opts->eps);
-      abh::check_launch();
+      lrs[size_t(st)] = float(lr * std::sqrt(1.0 - std::pow(b2, double(m->adam_t))) /
+                              (1.0 - std::pow(b1, double(m->adam_t))));
     }
-    // the perm upload of the next epoch reuses the host vector: wait here
+    ABH_HIP(hipMemcpyAsync(perm_d.p, perm.data(), size_t(n) * 4, hipMemcpyHostToDevice, s));
+    if (steps > 0)
+      ABH_HIP(hipMemcpyAsync(lr_d.p, lrs.data(), size_t(steps) * 4, hipMemcpyHostToDevice, s));
+    k_set_step<<<1, 64, 0, s>>>(step_d.as<int32_t>(), 0);
+    abh::check_launch();
+    for (int64_t st = 0; st < steps; ++st) {
+      if (!graph || !sized) {
+        one_step();
+        sized = true;
+        continue;
+      }
+      if (!ge) {
+        abh::CaptureGuard cg{s};
+        ABH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        cg.open = true;
+        one_step();
+        cg.open = false;
+        ABH_HIP(hipStreamEndCapture(s, &gr));
+        ABH_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+      }
+      ABH_HIP(hipGraphLaunch(ge, s));
+    }
+    // the perm and step-size uploads of the next epoch reuse the host
+    // vectors: wait here
     ABH_HIP(hipMemcpyAsync(lh.data(), losses.p, size_t(std::max<int64_t>(steps, 1)) * 8,
                            hipMemcpyDeviceToHost, s));
     ABH_HIP(hipStreamSynchronize(s));

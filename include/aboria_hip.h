@@ -800,7 +800,8 @@ typedef struct abh_mlp_train_options {
   int32_t batch;         /* rows per Adam step                               */
   float lr, beta1, beta2, eps;  /* Adam                                      */
   float lr_decay;        /* learning rate multiplied by this every epoch    */
-  int32_t reserved;
+  int32_t graph;         /* non-zero: training steps replayed from one HIP
+                            graph (same kernels, same results)              */
   uint64_t seed;         /* shuffling                                        */
 } abh_mlp_train_options;
 

@@ -293,6 +293,35 @@ def test_mlp_training_reduces_loss(ctx):
     assert l1 < 0.1 * l0
 
 
+def test_mlp_training_graph_equals_eager(ctx):
+    """graph=True replays every training step from one captured HIP graph
+    (the step's rows, loss slot and Adam step size read through a device
+    step counter): per-epoch losses and the trained parameters equal the
+    eager run's bit for bit, for the mean squared error and the patch
+    residual, over several epochs (with the host's shuffles, uploads and
+    synchronisations between the replays)."""
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, (3000, 24)).astype(np.float32)
+    t = np.tanh(x @ rng.normal(0, 0.3, (24, 8))).astype(np.float32)
+    mats = rng.normal(0, 1, (3000, 8, 8)).astype(np.float32)
+    xt, tt, mt = (torch.tensor(a, device="cuda") for a in (x, t, mats))
+    for kind in ("mse", "residual"):
+        runs = []
+        for graph in (False, True):
+            net = ab.Mlp(ctx, 24, 64, 2, 8, seed=3)
+            if kind == "mse":
+                losses = net.train(xt, tt, epochs=4, batch=256, lr=2e-3, lr_decay=0.9,
+                                   seed=5, graph=graph)
+            else:
+                losses = net.train_residual(xt, mt, epochs=4, batch=256, lr=2e-3, lr_decay=0.9,
+                                            seed=5, graph=graph)
+            runs.append((np.array(losses), net.get_params()))
+            net.close()
+        assert np.array_equal(runs[0][0], runs[1][0]), kind
+        assert np.array_equal(runs[0][1], runs[1][1]), kind
+        assert runs[0][0][-1] < runs[0][0][0]
+
+
 def test_learned_preconditioner_gain(ctx):
     """Trained on least-squares rows of seeded point sets, applied to a new
     one: the learned preconditioner cuts GMRES iterations well below the
