@@ -338,26 +338,48 @@ __global__ __launch_bounds__(512) void k_box_op64(const int32_t *__restrict__ ti
   const int rq = wave & 3, c0 = 4 * (wave >> 2);
   const int kl = lane >> 4, rl = lane & 15;
   const int RB = (DOUT + 15) / 16, KS = (DIN + 3) / 4;
+  constexpr int NB = 2;  // operand sets in flight: loads run one k step ahead
   for (int p0 = 0; p0 < RB; p0 += 4 * RPW) {
     f64x4 acc[RPW][4];
 #pragma unroll
     for (int i = 0; i < RPW; ++i)
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[i][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-    for (int ks = 0; ks < KS; ++ks) {
-      const int k = ks * 4 + kl;
-      double bv[4];
+    // operands of k step ks, every load unconditional from a valid address
+    // (a k past DIN reads its clamped row and is zeroed at use; a row block
+    // past RB reads the last one and is skipped)
+    double av[NB][RPW], bv[NB][4];
+    auto load = [&](int ks, double(&a)[RPW], double(&b)[4]) {
+      const int k = min(ks * 4 + kl, DIN - 1);
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        bv[c] = k < DIN ? in[(int64_t(first + c0 + c) * DIN + k) * kRhs + rl] : 0.0;
+      for (int c = 0; c < 4; ++c) b[c] = in[(int64_t(first + c0 + c) * DIN + k) * kRhs + rl];
 #pragma unroll
       for (int i = 0; i < RPW; ++i) {
-        const int rb = p0 + rq + 4 * i;
-        if (rb >= RB) continue;  // wave-uniform
-        const double av = Op[(int64_t(ks) * RB + rb) * 64 + lane];
+        const int rb = min(p0 + rq + 4 * i, RB - 1);
+        a[i] = Op[(int64_t(ks) * RB + rb) * 64 + lane];
+      }
+    };
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-          acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[i][c], 0, 0, 0);
+    for (int u = 0; u < NB - 1; ++u)
+      if (u < KS) load(u, av[u], bv[u]);
+    for (int ks0 = 0; ks0 < KS; ks0 += NB) {
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int ks = ks0 + u;
+        if (ks >= KS) break;  // uniform
+        if (ks + NB - 1 < KS) load(ks + NB - 1, av[(u + NB - 1) % NB], bv[(u + NB - 1) % NB]);
+        const bool kin = ks * 4 + kl < DIN;
+        double b[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) b[c] = kin ? bv[u][c] : 0.0;
+#pragma unroll
+        for (int i = 0; i < RPW; ++i) {
+          const int rb = p0 + rq + 4 * i;
+          if (rb >= RB) continue;  // wave-uniform
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            acc[i][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][i], b[c], acc[i][c], 0, 0, 0);
+        }
       }
     }
 #pragma unroll
