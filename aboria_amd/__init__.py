@@ -785,6 +785,10 @@ def create_sparse_operator(row_particles, col_particles, radius, kernel, row_sca
                           radius, kernel, rs, cs)
 
 
+def _train_flags(graph, fused):
+    return (1 if graph else 0) | (2 if fused else 0)
+
+
 class FmmOperator:
     """Black-box Chebyshev FMM operator over a HyperOctree in 1-3
     dimensions (positions (n, dim) float64; the M2L compression and the
@@ -1133,14 +1137,16 @@ class Mlp:
         return loss.value, g
 
     def train(self, x, t, epochs=20, batch=1024, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
-              lr_decay=1.0, seed=0, graph=False):
+              lr_decay=1.0, seed=0, graph=False, fused=False):
         """Adam on the mean squared error; returns the per-epoch mean loss.
-        graph: the steps replay one captured HIP graph."""
+        graph: the steps replay one captured HIP graph; fused: a step runs as
+        one kernel per 16 rows through the whole network plus one update."""
         import numpy as np
         _require_cuda(x, torch.float32, "x")
         _require_cuda(t, torch.float32, "t")
         o = C.MlpTrainOptions(int(epochs), int(batch), float(lr), float(beta1), float(beta2),
-                              float(eps), float(lr_decay), int(bool(graph)), int(seed))
+                              float(eps), float(lr_decay), _train_flags(graph, fused),
+                              int(seed))
         losses = np.zeros(max(int(epochs), 1), np.float32)
         check(C.lib().abh_mlp_train(self.handle, x.shape[0], _ptr(x), _ptr(t), ctypes.byref(o),
                                     losses.ctypes.data), self.ctx.handle)
@@ -1163,13 +1169,14 @@ class Mlp:
         return loss.value, g
 
     def train_residual(self, x, mats, epochs=20, batch=1024, lr=1e-3, beta1=0.9, beta2=0.999,
-                       eps=1e-8, lr_decay=1.0, seed=0, graph=False):
+                       eps=1e-8, lr_decay=1.0, seed=0, graph=False, fused=False):
         """Adam on the patch-residual loss; returns the per-epoch mean loss.
-        graph: the steps replay one captured HIP graph."""
+        graph / fused as for train()."""
         import numpy as np
         self._check_mats(x, mats)
         o = C.MlpTrainOptions(int(epochs), int(batch), float(lr), float(beta1), float(beta2),
-                              float(eps), float(lr_decay), int(bool(graph)), int(seed))
+                              float(eps), float(lr_decay), _train_flags(graph, fused),
+                              int(seed))
         losses = np.zeros(max(int(epochs), 1), np.float32)
         check(C.lib().abh_mlp_train_residual(self.handle, x.shape[0], _ptr(x), _ptr(mats),
                                              ctypes.byref(o), losses.ctypes.data),

@@ -132,7 +132,7 @@ class SchwarzInfo(ctypes.Structure):
 class MlpTrainOptions(ctypes.Structure):
     _fields_ = [("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
-                ("lr_decay", ctypes.c_float), ("graph", ctypes.c_int32),
+                ("lr_decay", ctypes.c_float), ("flags", ctypes.c_int32),
                 ("seed", ctypes.c_uint64)]
 
 

@@ -608,6 +608,250 @@ __global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__re
   dst[e] = src[int64_t(rows[r]) * cols + c];
 }
 
+// ---------------------------------------------------------------------------
+// MLP: one training step in two kernels.  k_train_chunk takes kFR rows of
+// the step's batch per block through the whole network -- forward with the
+// activations in LDS, the loss, backward -- and writes the chunk's weight
+// and bias gradients to its own slice of a partial buffer; k_train_update
+// sums the slices in chunk order and applies Adam.  For networks of at most
+// kFMaxL layers no wider than kFMaxW (the learned preconditioner's).
+// ---------------------------------------------------------------------------
+
+constexpr int kFR = 16;      // rows per block
+constexpr int kFMaxW = 128;  // widest layer
+constexpr int kFMaxL = 4;    // layers
+
+struct FusedNet {
+  int L;
+  int dims[kFMaxL + 1];
+  int64_t w_off[kFMaxL], b_off[kFMaxL];
+  int64_t nparams;
+};
+
+// floats of the block's LDS: input, per hidden layer Z and activation,
+// output, two gradient buffers, the staged transposed weights
+inline size_t fused_lds_floats(const FusedNet &f) {
+  size_t n = size_t(kFR) * f.dims[0];
+  for (int l = 0; l + 1 < f.L; ++l) n += 2 * size_t(kFR) * f.dims[l + 1];
+  n += size_t(kFR) * f.dims[f.L];
+  n += 2 * size_t(kFR) * kFMaxW;
+  n += size_t(kFMaxW) * (kFMaxW + 1);
+  return n;
+}
+
+// Ws[k * (dout + 1) + n] = W[n][k]
+__device__ __forceinline__ void stage_wt(const float *__restrict__ W, int din, int dout,
+                                         float *__restrict__ Ws) {
+  for (int e = threadIdx.x; e < din * dout; e += 256) {
+    const int n = e / din, k = e - n * din;
+    Ws[k * (dout + 1) + n] = W[e];
+  }
+}
+
+// mats != nullptr: patch-residual loss (M_r: k x k per row, k = n_out);
+// otherwise the mean squared error against t.  rows: the batch size.
+__global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__restrict__ P,
+                                                     const float *__restrict__ x,
+                                                     const float *__restrict__ t,
+                                                     const float *__restrict__ mats,
+                                                     const int32_t *__restrict__ perm,
+                                                     const int32_t *__restrict__ step, int rows,
+                                                     float *__restrict__ part,
+                                                     double *__restrict__ lossp) {
+  extern __shared__ __align__(16) float sm[];
+  const int L = f.L, nin = f.dims[0], nout = f.dims[L];
+  const int chunk = blockIdx.x;
+  const int r0 = chunk * kFR;
+  const int nr = min(kFR, rows - r0);
+  const int32_t *rid = perm + int64_t(*step) * rows + r0;
+  // LDS carve-up
+  float *act[kFMaxL + 1], *Z[kFMaxL];
+  float *q = sm;
+  act[0] = q;
+  q += kFR * nin;
+  for (int l = 0; l + 1 < L; ++l) {
+    Z[l] = q;
+    q += kFR * f.dims[l + 1];
+    act[l + 1] = q;
+    q += kFR * f.dims[l + 1];
+  }
+  float *Y = q;
+  q += kFR * nout;
+  float *dA = q;
+  q += kFR * kFMaxW;
+  float *dB = q;
+  q += kFR * kFMaxW;
+  float *Ws = q;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kFR * nin; e += 256) {
+    const int r = e / nin, c = e - r * nin;
+    act[0][e] = r < nr ? x[int64_t(rid[r]) * nin + c] : 0.0f;
+  }
+  // forward
+  for (int l = 0; l < L; ++l) {
+    const int din = f.dims[l], dout = f.dims[l + 1];
+    __syncthreads();
+    stage_wt(P + f.w_off[l], din, dout, Ws);
+    __syncthreads();
+    const int n = tid & 127, g = tid >> 7;
+    if (n < dout) {
+      float acc[kFR / 2];
+#pragma unroll
+      for (int i = 0; i < kFR / 2; ++i) acc[i] = 0.0f;
+      const float *in = act[l] + (g * (kFR / 2)) * din;
+      for (int k = 0; k < din; ++k) {
+        const float w = Ws[k * (dout + 1) + n];
+#pragma unroll
+        for (int i = 0; i < kFR / 2; ++i) acc[i] = acc[i] + in[i * din + k] * w;
+      }
+      const float b = P[f.b_off[l] + n];
+#pragma unroll
+      for (int i = 0; i < kFR / 2; ++i) {
+        const int r = g * (kFR / 2) + i;
+        const float z = acc[i] + b;
+        if (l + 1 < L) {
+          Z[l][r * dout + n] = z;
+          act[l + 1][r * dout + n] = silu(z);
+        } else {
+          Y[r * dout + n] = z;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // loss and dY (into dA), rows past nr contribute nothing
+  __shared__ double red[4];
+  double l2 = 0.0;
+  if (mats) {
+    // one wavefront per row, k = nout <= 64 outputs
+    const int wv = tid >> 6, lane = tid & 63;
+    for (int r = wv; r < kFR; r += 4) {
+      float v = 0.0f;
+      const float *Mr = mats + int64_t(r < nr ? rid[r] : 0) * nout * nout;
+      if (r < nr && lane < nout) {
+        v = lane == 0 ? -1.0f : 0.0f;
+        for (int b = 0; b < nout; ++b) v += Mr[int64_t(lane) * nout + b] * Y[r * nout + b];
+        l2 += double(v) * double(v);
+      }
+      dB[r * kFMaxW + lane] = v;  // the residual, staged for M^T res
+    }
+    __syncthreads();
+    for (int r = wv; r < kFR; r += 4) {
+      if (lane >= nout) continue;
+      float gsum = 0.0f;
+      if (r < nr) {
+        const float *Mr = mats + int64_t(rid[r]) * nout * nout;
+        for (int a = 0; a < nout; ++a) gsum += Mr[int64_t(a) * nout + lane] * dB[r * kFMaxW + a];
+      }
+      dA[r * nout + lane] = 2.0f * gsum / float(rows);
+    }
+  } else {
+    const float scale = 1.0f / float(int64_t(rows) * nout);
+    for (int e = tid; e < kFR * nout; e += 256) {
+      const int r = e / nout, c = e - r * nout;
+      float d = 0.0f;
+      if (r < nr) d = Y[e] - t[int64_t(rid[r]) * nout + c];
+      dA[e] = 2.0f * d * scale;
+      l2 += double(d) * double(d);
+    }
+  }
+  l2 = abh::wave_sum(l2);
+  if ((tid & 63) == 0) red[tid >> 6] = l2;
+  __syncthreads();
+  if (tid == 0) lossp[chunk] = (red[0] + red[1]) + (red[2] + red[3]);
+  // backward: dA holds dZ of layer l (kFR x dout)
+  float *part_c = part + int64_t(chunk) * f.nparams;
+  for (int l = L - 1; l >= 0; --l) {
+    const int din = f.dims[l], dout = f.dims[l + 1];
+    const float *ain = act[l];
+    // dW[n][k] = sum_r dZ[r][n] act[r][k]: lane k, 16 outputs n = g + 2j
+    {
+      const int k = tid & 127, g = tid >> 7;
+      if (k < din) {
+        for (int nb = g; nb < dout; nb += 32) {
+          float acc[16];
+#pragma unroll
+          for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+          for (int r = 0; r < kFR; ++r) {
+            const float a = ain[r * din + k];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+              const int n = nb + 2 * j;
+              if (n < dout) acc[j] = acc[j] + dA[r * dout + n] * a;
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < 16; ++j) {
+            const int n = nb + 2 * j;
+            if (n < dout) part_c[f.w_off[l] + int64_t(n) * din + k] = acc[j];
+          }
+        }
+      }
+      if (tid < dout) {
+        float sb = 0.0f;
+        for (int r = 0; r < kFR; ++r) sb = sb + dA[r * dout + tid];
+        part_c[f.b_off[l] + tid] = sb;
+      }
+    }
+    if (l == 0) break;
+    // dact[r][k] = sum_n dZ[r][n] W[n][k], then through the SiLU of layer l-1
+    __syncthreads();
+    stage_wt(P + f.w_off[l], din, dout, Ws);
+    __syncthreads();
+    {
+      const int k = tid & 127, g = tid >> 7;
+      if (k < din) {
+        float acc[kFR / 2];
+#pragma unroll
+        for (int i = 0; i < kFR / 2; ++i) acc[i] = 0.0f;
+        for (int n = 0; n < dout; ++n) {
+          const float w = Ws[k * (dout + 1) + n];
+#pragma unroll
+          for (int i = 0; i < kFR / 2; ++i)
+            acc[i] = acc[i] + dA[(g * (kFR / 2) + i) * dout + n] * w;
+        }
+#pragma unroll
+        for (int i = 0; i < kFR / 2; ++i) {
+          const int r = g * (kFR / 2) + i;
+          dB[r * din + k] = acc[i] * silu_grad(Z[l - 1][r * din + k]);
+        }
+      }
+    }
+    __syncthreads();
+    float *tmp = dA;
+    dA = dB;
+    dB = tmp;
+  }
+}
+
+// grad[p] = sum over chunks (in order) of part[c][p], then Adam with the
+// step size of step *step; block 0 also writes the step's loss
+__global__ __launch_bounds__(256) void k_train_update(
+    float *__restrict__ p, float *__restrict__ m, float *__restrict__ v, int64_t n,
+    const float *__restrict__ part, int nchunks, float b1, float b2,
+    const float *__restrict__ lr_steps, const int32_t *__restrict__ step, float eps,
+    const double *__restrict__ lossp, double loss_scale, double *__restrict__ losses) {
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (blockIdx.x == 0) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int c = threadIdx.x; c < nchunks; c += 256) s += lossp[c];
+    s = abh::wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) losses[*step] = ((red[0] + red[1]) + (red[2] + red[3])) * loss_scale;
+  }
+  if (e >= n) return;
+  float g = 0.0f;
+  for (int c = 0; c < nchunks; ++c) g = g + part[int64_t(c) * n + e];
+  const float lr_t = lr_steps[*step];
+  const float me = b1 * m[e] + (1.0f - b1) * g;
+  const float ve = b2 * v[e] + (1.0f - b2) * g * g;
+  m[e] = me;
+  v[e] = ve;
+  p[e] = p[e] - lr_t * me / (sqrtf(ve) + eps);
+}
+
 __global__ void k_set_step(int32_t *__restrict__ step, int32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step = v;
 }
@@ -887,13 +1131,32 @@ void loss_grad(abh_mlp *m, int64_t rows, const float *x, const float *t, const f
 // squared error, or mats (n x n_out x n_out) for the patch residual.  The
 // kernels of a step read the step's rows, loss slot and step size through a
 // device step counter, so every step launches the same kernels with the
-// same arguments; with opts->graph the second step is captured into a HIP
-// graph (on a stream of the context's own) and every later step replays it.
+// same arguments; with ABH_TRAIN_GRAPH the second step is captured into a
+// HIP graph (on a stream of the context's own) and every later step replays
+// it.  ABH_TRAIN_FUSED runs a step as k_train_chunk + k_train_update.
 void train_impl(abh_mlp *m, int64_t n, const float *x, const float *t, const float *mats,
                 const abh_mlp_train_options *opts, float *loss_per_epoch) {
   ABH_REQUIRE(opts->epochs >= 0 && opts->batch >= 1 && opts->batch <= (1 << 20),
               ABH_ERR_INVALID_ARG, "invalid training options");
-  const bool graph = opts->graph != 0;
+  ABH_REQUIRE((opts->flags & ~(ABH_TRAIN_GRAPH | ABH_TRAIN_FUSED)) == 0, ABH_ERR_INVALID_ARG,
+              "unknown training flags");
+  const bool graph = (opts->flags & ABH_TRAIN_GRAPH) != 0;
+  const bool fused = (opts->flags & ABH_TRAIN_FUSED) != 0;
+  FusedNet fn{};
+  if (fused) {
+    ABH_REQUIRE(m->layers() <= kFMaxL, ABH_ERR_UNSUPPORTED,
+                "the fused training step takes at most 4 layers");
+    for (int d : m->dims)
+      ABH_REQUIRE(d <= kFMaxW, ABH_ERR_UNSUPPORTED, "the fused training step takes widths <= 128");
+    ABH_REQUIRE(!mats || m->dims.back() <= kMaxK, ABH_ERR_UNSUPPORTED, "outputs above 64");
+    fn.L = m->layers();
+    for (int l = 0; l <= fn.L; ++l) fn.dims[l] = m->dims[size_t(l)];
+    for (int l = 0; l < fn.L; ++l) {
+      fn.w_off[l] = m->w_off[size_t(l)];
+      fn.b_off[l] = m->b_off[size_t(l)];
+    }
+    fn.nparams = m->nparams;
+  }
   abh::CaptureStream cs;
   if (graph) cs.begin(m->ctx);
   hipStream_t s = m->ctx->stream;
@@ -901,7 +1164,15 @@ void train_impl(abh_mlp *m, int64_t n, const float *x, const float *t, const flo
   const int64_t tw = mats ? int64_t(nout) * nout : nout;  // floats per target row
   const int64_t B = std::min<int64_t>(opts->batch, n);
   const int64_t steps = n / B;  // full batches per epoch
-  DevBuf perm_d, xb, tb, scratch, losses, lr_d, step_d;
+  const int nchunks = int(abh::ceil_div(B, kFR));
+  const size_t flds = fused ? fused_lds_floats(fn) * 4 : 0;
+  DevBuf perm_d, xb, tb, scratch, losses, lr_d, step_d, part, lossp;
+  if (fused) {
+    part.ensure(size_t(nchunks) * size_t(m->nparams) * 4 + 64);
+    lossp.ensure(size_t(nchunks) * 8 + 64);
+    ABH_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_train_chunk),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(flds)));
+  }
   perm_d.ensure(size_t(n) * 4 + 64);
   xb.ensure(size_t(B) * nin * 4 + 64);
   tb.ensure(size_t(B) * tw * 4 + 64);
@@ -914,7 +1185,23 @@ void train_impl(abh_mlp *m, int64_t n, const float *x, const float *t, const flo
   float lr = opts->lr;
   std::vector<double> lh(size_t(std::max<int64_t>(steps, 1)));
   const int32_t *stp = step_d.as<int32_t>();
+  const double loss_scale = mats ? 1.0 / double(B) : 1.0 / double(B * nout);
+  auto fused_step = [&]() {
+    k_train_chunk<<<unsigned(nchunks), 256, flds, s>>>(
+        fn, m->params.as<float>(), x, mats ? nullptr : t, mats, perm_d.as<int32_t>(), stp, int(B),
+        part.as<float>(), lossp.as<double>());
+    k_train_update<<<unsigned(abh::ceil_div(m->nparams, 256)), 256, 0, s>>>(
+        m->params.as<float>(), m->am.as<float>(), m->av.as<float>(), m->nparams, part.as<float>(),
+        nchunks, opts->beta1, opts->beta2, lr_d.as<float>(), stp, opts->eps, lossp.as<double>(),
+        loss_scale, losses.as<double>());
+    k_next_step<<<1, 64, 0, s>>>(step_d.as<int32_t>());
+    abh::check_launch();
+  };
   auto one_step = [&]() {
+    if (fused) {
+      fused_step();
+      return;
+    }
     k_gather_rows<<<unsigned(abh::ceil_div(B * nin, 256)), 256, 0, s>>>(
         x, perm_d.as<int32_t>(), stp, int(B), nin, xb.as<float>());
     k_gather_rows<<<unsigned(abh::ceil_div(B * tw, 256)), 256, 0, s>>>(

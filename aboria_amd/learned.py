@@ -52,14 +52,14 @@ def training_set(ctx, point_sets, kernel, radius, shift, n_leaf, k=32, feature_s
 
 
 def train(ctx, X, T, hidden=128, n_hidden=3, epochs=30, batch=1024, lr=2e-3, lr_decay=0.93,
-          seed=0, graph=False):
+          seed=0, graph=False, fused=False):
     """Trains a network on (X, T); targets are divided by their RMS (returned
     as out_scale, the factor the network's outputs are multiplied by)."""
     out_scale = float(torch.sqrt(torch.mean(T * T)).item()) or 1.0
     Tn = (T / out_scale).to(torch.float32).contiguous()
     net = Mlp(ctx, X.shape[1], hidden, n_hidden, T.shape[1], seed=seed)
     losses = net.train(X.contiguous(), Tn, epochs=epochs, batch=batch, lr=lr, lr_decay=lr_decay,
-                       seed=seed, graph=graph)
+                       seed=seed, graph=graph, fused=fused)
     return net, out_scale, losses
 
 
@@ -96,11 +96,12 @@ def training_set_patch(ctx, point_sets, kernel, radius, shift, n_leaf, k=32, fea
 
 
 def train_residual(ctx, X, mats, hidden=128, n_hidden=3, epochs=30, batch=512, lr=2e-3,
-                   lr_decay=0.93, seed=0, graph=False):
+                   lr_decay=0.93, seed=0, graph=False, fused=False):
     """Trains a network on the patch-residual loss (mats already scaled by
     out_scale); returns (net, per-epoch losses)."""
     k = mats.shape[1]
     net = Mlp(ctx, X.shape[1], hidden, n_hidden, k, seed=seed)
     losses = net.train_residual(X.contiguous(), mats.contiguous(), epochs=epochs, batch=batch,
-                                lr=lr, lr_decay=lr_decay, seed=seed, graph=graph)
+                                lr=lr, lr_decay=lr_decay, seed=seed, graph=graph,
+                                fused=fused)
     return net, losses

@@ -569,12 +569,12 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
         (X, M, _, oscale), t_data = timed(lambda: learned.training_set_patch(
             ctx, [train_pos], kern, radius, shift, 16.0, k, feature_scale=sigma,
             max_rows=400_000, seed=c["seed"]))
-        (net, losses), t_train = timed(lambda: learned.train_residual(ctx, X, M, epochs=60,
-                                                                      batch=1024, graph=True))
+        (net, losses), t_train = timed(lambda: learned.train_residual(
+            ctx, X, M, epochs=60, batch=1024, graph=True, fused=True))
         t_train_eager = None
-        if name == "sigma_0.002":  # the same training with one launch per kernel
+        if name == "sigma_0.002":  # the same training, one kernel per layer operation
             (net_e, _), t_train_eager = timed(lambda: learned.train_residual(
-                ctx, X, M, epochs=60, batch=1024))
+                ctx, X, M, epochs=60, batch=1024, graph=True))
             net_e.close()
         del X, M
         ai, t_ai = timed(lambda: ab.ApproxInverse.learned(ctx, cl, net, k, sigma, oscale,
@@ -582,8 +582,8 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
         (_, rep), t = timed(lambda: ab.gmres_with_fallback(op, b, shift, ai, sw, restart=50,
                                                            max_iters=400, learned_iters=300))
         r["learned"] = {"training_rows": 400_000, "training_set_s": t_data,
-                        "train_s": t_train, "train_graph": True,
-                        "train_s_eager_launches": t_train_eager,
+                        "train_s": t_train, "train_graph": True, "train_fused": True,
+                        "train_s_unfused": t_train_eager,
                         "loss_first": float(losses[0]),
                         "loss_last": float(losses[-1]), "setup_s": t_ai, "solve_s": t,
                         "learned": rep["learned"], "fallback": rep["fallback"]}
@@ -599,13 +599,13 @@ def bench_schwarz(torch, ab, wl, ctx, extra):
         r["none"] = {"solve_s": t, **st}
         log("schwarz %s: plain %d its (%s) setup %.2f s solve %.2f s | coarse32 %d its (%s) "
             "res %.2e | learned %s fallback %s "
-            "(train %.1f s graph%s, setup %.3f s, solve %.2f s) | patch %d its (%s) | lsq %d its "
+            "(train %.1f s fused%s, setup %.3f s, solve %.2f s) | patch %d its (%s) | lsq %d its "
             "(%s) | none %d its (%s)"
             % (name, r["schwarz"]["iterations"], r["schwarz"]["converged"], t_sw,
                r["schwarz"]["solve_s"], r["schwarz_coarse32"]["iterations"],
                r["schwarz_coarse32"]["converged"], r["schwarz_coarse32"]["residual"],
                rep["learned"], rep["fallback"], t_train,
-               "" if t_train_eager is None else " / %.1f s launches" % t_train_eager, t_ai,
+               "" if t_train_eager is None else " / %.1f s unfused" % t_train_eager, t_ai,
                r["learned"]["solve_s"], r["patch"]["iterations"], r["patch"]["converged"],
                r["lsq"]["iterations"], r["lsq"]["converged"], r["none"]["iterations"],
                r["none"]["converged"]))

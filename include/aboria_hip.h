@@ -795,13 +795,19 @@ abh_status abh_mlp_forward(abh_mlp *m, int64_t n, const float *x, float *y, int 
 abh_status abh_mlp_loss_grad(abh_mlp *m, int64_t n, const float *x, const float *t,
                              double *loss, float *grad);
 
+#define ABH_TRAIN_GRAPH 1
+#define ABH_TRAIN_FUSED 2
 typedef struct abh_mlp_train_options {
   int32_t epochs;
   int32_t batch;         /* rows per Adam step                               */
   float lr, beta1, beta2, eps;  /* Adam                                      */
   float lr_decay;        /* learning rate multiplied by this every epoch    */
-  int32_t graph;         /* non-zero: training steps replayed from one HIP
-                            graph (same kernels, same results)              */
+  int32_t flags;         /* ABH_TRAIN_GRAPH: training steps replayed from one
+                            HIP graph (same kernels, same results);
+                            ABH_TRAIN_FUSED: a step as one kernel per 16 rows
+                            through the whole network plus one reduction and
+                            Adam kernel (at most 4 layers, widths <= 128;
+                            another summation order)                        */
   uint64_t seed;         /* shuffling                                        */
 } abh_mlp_train_options;
 

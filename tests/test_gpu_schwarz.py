@@ -322,6 +322,52 @@ def test_mlp_training_graph_equals_eager(ctx):
         assert runs[0][0][-1] < runs[0][0][0]
 
 
+@pytest.mark.parametrize("kind", ["mse", "residual"])
+def test_mlp_fused_step_gradient(ctx, kind):
+    """The fused training step (k_train_chunk: 16 rows per block through the
+    whole network in LDS; k_train_update: chunk sums + Adam): one step over
+    the whole set with Adam reduced to p -= g / (|g| + 1) (beta1 = beta2 = 0,
+    eps = 1, lr = 1) gives back its gradient g = -dp / (1 - |dp|), which
+    equals the unfused gradient (abh_mlp_loss_grad, itself pinned to the
+    FP64 oracle) to 1e-4 of its largest entry, and the step's loss equals
+    the unfused loss to 1e-5.  Fused training with graph replay equals fused
+    training without it bit for bit."""
+    rng = np.random.default_rng(21)
+    n, k = 1000, 8
+    x = rng.uniform(-1, 1, (n, 24)).astype(np.float32)
+    t = np.tanh(x @ rng.normal(0, 0.3, (24, k))).astype(np.float32)
+    mats = rng.normal(0, 1, (n, k, k)).astype(np.float32)
+    xt, tt, mt = (torch.tensor(a, device="cuda") for a in (x, t, mats))
+    ref = ab.Mlp(ctx, 24, 64, 3, k, seed=4)
+    if kind == "mse":
+        loss_e, g_e = ref.loss_grad(xt, tt)
+    else:
+        loss_e, g_e = ref.loss_grad_residual(xt, mt)
+    g_e = g_e.cpu().numpy().astype(np.float64)
+    p0 = ref.get_params().astype(np.float64)
+    net = ab.Mlp(ctx, 24, 64, 3, k, seed=4)
+    kw = dict(epochs=1, batch=n, lr=1.0, beta1=0.0, beta2=0.0, eps=1.0, seed=2, fused=True)
+    losses = net.train(xt, tt, **kw) if kind == "mse" else net.train_residual(xt, mt, **kw)
+    dp = net.get_params().astype(np.float64) - p0
+    g_f = -dp / (1.0 - np.abs(dp))
+    err = np.max(np.abs(g_f - g_e)) / np.max(np.abs(g_e))
+    print("fused %s: loss %.6e vs %.6e, gradient max rel %.2e" % (kind, losses[0], loss_e, err),
+          flush=True)
+    assert abs(losses[0] - loss_e) <= 1e-5 * abs(loss_e)
+    assert err < 1e-4
+    runs = []
+    for graph in (False, True):
+        m2 = ab.Mlp(ctx, 24, 64, 3, k, seed=4)
+        kw2 = dict(epochs=3, batch=200, lr=2e-3, lr_decay=0.9, seed=5, graph=graph, fused=True)
+        ls = m2.train(xt, tt, **kw2) if kind == "mse" else m2.train_residual(xt, mt, **kw2)
+        runs.append((np.array(ls), m2.get_params()))
+        m2.close()
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][0][-1] < runs[0][0][0]
+    net.close()
+    ref.close()
+
+
 def test_learned_preconditioner_gain(ctx):
     """Trained on least-squares rows of seeded point sets, applied to a new
     one: the learned preconditioner cuts GMRES iterations well below the
