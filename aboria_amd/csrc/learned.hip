@@ -639,12 +639,25 @@ inline size_t fused_lds_floats(const FusedNet &f) {
   return n;
 }
 
-// Ws[k * (dout + 1) + n] = W[n][k]
+// Ws[k * (dout + 1) + n] = W[n][k]; eight loads per thread in flight
 __device__ __forceinline__ void stage_wt(const float *__restrict__ W, int din, int dout,
                                          float *__restrict__ Ws) {
-  for (int e = threadIdx.x; e < din * dout; e += 256) {
-    const int n = e / din, k = e - n * din;
-    Ws[k * (dout + 1) + n] = W[e];
+  const int total = din * dout;
+  for (int base = 0; base < total; base += 256 * 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * 256 + int(threadIdx.x);
+      v[u] = e < total ? W[e] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * 256 + int(threadIdx.x);
+      if (e < total) {
+        const int n = e / din, k = e - n * din;
+        Ws[k * (dout + 1) + n] = v[u];
+      }
+    }
   }
 }
 
@@ -723,27 +736,54 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
   __shared__ double red[4];
   double l2 = 0.0;
   if (mats) {
-    // one wavefront per row, k = nout <= 64 outputs
+    // the rows' k x k matrices staged in the weight buffer (row stride k+1,
+    // as many rows at a time as fit), then one wavefront per row
     const int wv = tid >> 6, lane = tid & 63;
-    for (int r = wv; r < kFR; r += 4) {
-      float v = 0.0f;
-      const float *Mr = mats + int64_t(r < nr ? rid[r] : 0) * nout * nout;
-      if (r < nr && lane < nout) {
-        v = lane == 0 ? -1.0f : 0.0f;
-        for (int b = 0; b < nout; ++b) v += Mr[int64_t(lane) * nout + b] * Y[r * nout + b];
-        l2 += double(v) * double(v);
+    const int ms = nout + 1, per = nout * ms;
+    const int grp = max(1, min(kFR, (kFMaxW * (kFMaxW + 1)) / per));
+    for (int rg = 0; rg < kFR; rg += grp) {
+      const int re = min(kFR, rg + grp);
+      __syncthreads();
+      const int tot = (re - rg) * nout * nout;
+      for (int base = 0; base < tot; base += 256 * 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + u * 256 + tid;
+          const int rr = rg + e / (nout * nout);
+          v[u] = (e < tot && rr < nr)
+                     ? mats[int64_t(rid[rr]) * nout * nout + (e % (nout * nout))]
+                     : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + u * 256 + tid;
+          if (e < tot) {
+            const int rr = rg + e / (nout * nout), j = (e / nout) % nout, b = e % nout;
+            Ws[(rr - rg) * per + j * ms + b] = v[u];
+          }
+        }
       }
-      dB[r * kFMaxW + lane] = v;  // the residual, staged for M^T res
-    }
-    __syncthreads();
-    for (int r = wv; r < kFR; r += 4) {
-      if (lane >= nout) continue;
-      float gsum = 0.0f;
-      if (r < nr) {
-        const float *Mr = mats + int64_t(rid[r]) * nout * nout;
-        for (int a = 0; a < nout; ++a) gsum += Mr[int64_t(a) * nout + lane] * dB[r * kFMaxW + a];
+      __syncthreads();
+      for (int r = rg + wv; r < re; r += 4) {
+        const float *Mr = Ws + (r - rg) * per;
+        float v = 0.0f;
+        if (r < nr && lane < nout) {
+          v = lane == 0 ? -1.0f : 0.0f;
+          for (int b = 0; b < nout; ++b) v += Mr[lane * ms + b] * Y[r * nout + b];
+          l2 += double(v) * double(v);
+        }
+        dB[r * kFMaxW + lane] = v;  // the residual, staged for M^T res
       }
-      dA[r * nout + lane] = 2.0f * gsum / float(rows);
+      __syncthreads();
+      for (int r = rg + wv; r < re; r += 4) {
+        if (lane >= nout) continue;
+        const float *Mr = Ws + (r - rg) * per;
+        float gsum = 0.0f;
+        if (r < nr)
+          for (int a = 0; a < nout; ++a) gsum += Mr[a * ms + lane] * dB[r * kFMaxW + a];
+        dA[r * nout + lane] = 2.0f * gsum / float(rows);
+      }
     }
   } else {
     const float scale = 1.0f / float(int64_t(rows) * nout);
