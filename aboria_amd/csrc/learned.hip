@@ -617,7 +617,9 @@ __global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__re
 // kFMaxL layers no wider than kFMaxW (the learned preconditioner's).
 // ---------------------------------------------------------------------------
 
-constexpr int kFR = 16;      // rows per block
+constexpr int kFR = 8;       // rows per block
+constexpr int kFT = 1024;    // threads per block
+constexpr int kFG = kFT / 128;  // thread groups of 128 (rows kFR / kFG each)
 constexpr int kFMaxW = 128;  // widest layer
 constexpr int kFMaxL = 4;    // layers
 
@@ -643,16 +645,16 @@ inline size_t fused_lds_floats(const FusedNet &f) {
 __device__ __forceinline__ void stage_wt(const float *__restrict__ W, int din, int dout,
                                          float *__restrict__ Ws) {
   const int total = din * dout;
-  for (int base = 0; base < total; base += 256 * 8) {
+  for (int base = 0; base < total; base += kFT * 8) {
     float v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int e = base + u * 256 + int(threadIdx.x);
+      const int e = base + u * kFT + int(threadIdx.x);
       v[u] = e < total ? W[e] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int e = base + u * 256 + int(threadIdx.x);
+      const int e = base + u * kFT + int(threadIdx.x);
       if (e < total) {
         const int n = e / din, k = e - n * din;
         Ws[k * (dout + 1) + n] = v[u];
@@ -663,7 +665,7 @@ __device__ __forceinline__ void stage_wt(const float *__restrict__ W, int din, i
 
 // mats != nullptr: patch-residual loss (M_r: k x k per row, k = n_out);
 // otherwise the mean squared error against t.  rows: the batch size.
-__global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__restrict__ P,
+__global__ __launch_bounds__(kFT) void k_train_chunk(FusedNet f, const float *__restrict__ P,
                                                      const float *__restrict__ x,
                                                      const float *__restrict__ t,
                                                      const float *__restrict__ mats,
@@ -696,7 +698,7 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
   q += kFR * kFMaxW;
   float *Ws = q;
   const int tid = threadIdx.x;
-  for (int e = tid; e < kFR * nin; e += 256) {
+  for (int e = tid; e < kFR * nin; e += kFT) {
     const int r = e / nin, c = e - r * nin;
     act[0][e] = r < nr ? x[int64_t(rid[r]) * nin + c] : 0.0f;
   }
@@ -708,19 +710,19 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
     __syncthreads();
     const int n = tid & 127, g = tid >> 7;
     if (n < dout) {
-      float acc[kFR / 2];
+      float acc[kFR / kFG];
 #pragma unroll
-      for (int i = 0; i < kFR / 2; ++i) acc[i] = 0.0f;
-      const float *in = act[l] + (g * (kFR / 2)) * din;
+      for (int i = 0; i < kFR / kFG; ++i) acc[i] = 0.0f;
+      const float *in = act[l] + (g * (kFR / kFG)) * din;
       for (int k = 0; k < din; ++k) {
         const float w = Ws[k * (dout + 1) + n];
 #pragma unroll
-        for (int i = 0; i < kFR / 2; ++i) acc[i] = acc[i] + in[i * din + k] * w;
+        for (int i = 0; i < kFR / kFG; ++i) acc[i] = acc[i] + in[i * din + k] * w;
       }
       const float b = P[f.b_off[l] + n];
 #pragma unroll
-      for (int i = 0; i < kFR / 2; ++i) {
-        const int r = g * (kFR / 2) + i;
+      for (int i = 0; i < kFR / kFG; ++i) {
+        const int r = g * (kFR / kFG) + i;
         const float z = acc[i] + b;
         if (l + 1 < L) {
           Z[l][r * dout + n] = z;
@@ -733,7 +735,7 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
   }
   __syncthreads();
   // loss and dY (into dA), rows past nr contribute nothing
-  __shared__ double red[4];
+  __shared__ double red[kFT / 64];
   double l2 = 0.0;
   if (mats) {
     // the rows' k x k matrices staged in the weight buffer (row stride k+1,
@@ -745,11 +747,11 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
       const int re = min(kFR, rg + grp);
       __syncthreads();
       const int tot = (re - rg) * nout * nout;
-      for (int base = 0; base < tot; base += 256 * 8) {
+      for (int base = 0; base < tot; base += kFT * 8) {
         float v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int e = base + u * 256 + tid;
+          const int e = base + u * kFT + tid;
           const int rr = rg + e / (nout * nout);
           v[u] = (e < tot && rr < nr)
                      ? mats[int64_t(rid[rr]) * nout * nout + (e % (nout * nout))]
@@ -757,7 +759,7 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int e = base + u * 256 + tid;
+          const int e = base + u * kFT + tid;
           if (e < tot) {
             const int rr = rg + e / (nout * nout), j = (e / nout) % nout, b = e % nout;
             Ws[(rr - rg) * per + j * ms + b] = v[u];
@@ -765,7 +767,7 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
         }
       }
       __syncthreads();
-      for (int r = rg + wv; r < re; r += 4) {
+      for (int r = rg + wv; r < re; r += kFT / 64) {
         const float *Mr = Ws + (r - rg) * per;
         float v = 0.0f;
         if (r < nr && lane < nout) {
@@ -776,7 +778,7 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
         dB[r * kFMaxW + lane] = v;  // the residual, staged for M^T res
       }
       __syncthreads();
-      for (int r = rg + wv; r < re; r += 4) {
+      for (int r = rg + wv; r < re; r += kFT / 64) {
         if (lane >= nout) continue;
         const float *Mr = Ws + (r - rg) * per;
         float gsum = 0.0f;
@@ -787,7 +789,7 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
     }
   } else {
     const float scale = 1.0f / float(int64_t(rows) * nout);
-    for (int e = tid; e < kFR * nout; e += 256) {
+    for (int e = tid; e < kFR * nout; e += kFT) {
       const int r = e / nout, c = e - r * nout;
       float d = 0.0f;
       if (r < nr) d = Y[e] - t[int64_t(rid[r]) * nout + c];
@@ -798,17 +800,21 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
   l2 = abh::wave_sum(l2);
   if ((tid & 63) == 0) red[tid >> 6] = l2;
   __syncthreads();
-  if (tid == 0) lossp[chunk] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (tid == 0) {
+    double t8 = 0.0;
+    for (int w = 0; w < kFT / 64; ++w) t8 += red[w];
+    lossp[chunk] = t8;
+  }
   // backward: dA holds dZ of layer l (kFR x dout)
   float *part_c = part + int64_t(chunk) * f.nparams;
   for (int l = L - 1; l >= 0; --l) {
     const int din = f.dims[l], dout = f.dims[l + 1];
     const float *ain = act[l];
-    // dW[n][k] = sum_r dZ[r][n] act[r][k]: lane k, 16 outputs n = g + 2j
+    // dW[n][k] = sum_r dZ[r][n] act[r][k]: lane k, 16 outputs n = nb + kFG j
     {
       const int k = tid & 127, g = tid >> 7;
       if (k < din) {
-        for (int nb = g; nb < dout; nb += 32) {
+        for (int nb = g; nb < dout; nb += 16 * kFG) {
           float acc[16];
 #pragma unroll
           for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
@@ -816,13 +822,13 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
             const float a = ain[r * din + k];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-              const int n = nb + 2 * j;
+              const int n = nb + kFG * j;
               if (n < dout) acc[j] = acc[j] + dA[r * dout + n] * a;
             }
           }
 #pragma unroll
           for (int j = 0; j < 16; ++j) {
-            const int n = nb + 2 * j;
+            const int n = nb + kFG * j;
             if (n < dout) part_c[f.w_off[l] + int64_t(n) * din + k] = acc[j];
           }
         }
@@ -841,18 +847,18 @@ __global__ __launch_bounds__(256) void k_train_chunk(FusedNet f, const float *__
     {
       const int k = tid & 127, g = tid >> 7;
       if (k < din) {
-        float acc[kFR / 2];
+        float acc[kFR / kFG];
 #pragma unroll
-        for (int i = 0; i < kFR / 2; ++i) acc[i] = 0.0f;
+        for (int i = 0; i < kFR / kFG; ++i) acc[i] = 0.0f;
         for (int n = 0; n < dout; ++n) {
           const float w = Ws[k * (dout + 1) + n];
 #pragma unroll
-          for (int i = 0; i < kFR / 2; ++i)
-            acc[i] = acc[i] + dA[(g * (kFR / 2) + i) * dout + n] * w;
+          for (int i = 0; i < kFR / kFG; ++i)
+            acc[i] = acc[i] + dA[(g * (kFR / kFG) + i) * dout + n] * w;
         }
 #pragma unroll
-        for (int i = 0; i < kFR / 2; ++i) {
-          const int r = g * (kFR / 2) + i;
+        for (int i = 0; i < kFR / kFG; ++i) {
+          const int r = g * (kFR / kFG) + i;
           dB[r * din + k] = acc[i] * silu_grad(Z[l - 1][r * din + k]);
         }
       }
@@ -883,7 +889,15 @@ __global__ __launch_bounds__(256) void k_train_update(
   }
   if (e >= n) return;
   float g = 0.0f;
-  for (int c = 0; c < nchunks; ++c) g = g + part[int64_t(c) * n + e];
+  int c = 0;
+  for (; c + 8 <= nchunks; c += 8) {  // eight loads in flight, added in order
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[int64_t(c + u) * n + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) g = g + v[u];
+  }
+  for (; c < nchunks; ++c) g = g + part[int64_t(c) * n + e];
   const float lr_t = lr_steps[*step];
   const float me = b1 * m[e] + (1.0f - b1) * g;
   const float ve = b2 * v[e] + (1.0f - b2) * g * g;
@@ -1227,7 +1241,7 @@ void train_impl(abh_mlp *m, int64_t n, const float *x, const float *t, const flo
   const int32_t *stp = step_d.as<int32_t>();
   const double loss_scale = mats ? 1.0 / double(B) : 1.0 / double(B * nout);
   auto fused_step = [&]() {
-    k_train_chunk<<<unsigned(nchunks), 256, flds, s>>>(
+    k_train_chunk<<<unsigned(nchunks), kFT, flds, s>>>(
         fn, m->params.as<float>(), x, mats ? nullptr : t, mats, perm_d.as<int32_t>(), stp, int(B),
         part.as<float>(), lossp.as<double>());
     k_train_update<<<unsigned(abh::ceil_div(m->nparams, 256)), 256, 0, s>>>(
