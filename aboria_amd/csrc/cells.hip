@@ -9,7 +9,7 @@
 //   3. bucket sizing on the host (set_domain_impl, src/CellListOrdered.h:132-186)
 //   4. k_compact_bucket: scatter_if + point_to_bucket_index fused
 //   5. stable LSD radix sort of (bucket, alive index) on the bits that occur
-//   6. k_bucket_bounds: lower/upper bound of every bucket from the sorted keys
+//   6. bucket offsets: counts per bucket from the sorted keys, exclusive scan
 //   7. gather of the positions into sorted order (Particles::reorder)
 // All FP64 arithmetic is compiled without contraction so bucket indices are
 // the reference's bit for bit.
@@ -109,21 +109,42 @@ __global__ __launch_bounds__(kThreads) void k_compact_bucket(const double *__res
   keys[k] = bucket_of<D>(g, p);
 }
 
-// lower_bound / upper_bound of every bucket id in the sorted keys
-// (src/CellListOrdered.h:231-239): thread t owns the bucket ids whose bound
-// is t.
-__global__ __launch_bounds__(kThreads) void k_bucket_bounds(const uint32_t *__restrict__ keys,
-                                                            int64_t n, int64_t nb,
-                                                            uint32_t *__restrict__ begin,
-                                                            uint32_t *__restrict__ end) {
+// Bucket offsets from counts (src/CellListOrdered.h:231-239 takes lower /
+// upper bounds of every bucket id in the sorted keys; with counts c_b,
+// lower_bound(b) = sum_{b' < b} c_b' and upper_bound(b) = that + c_b):
+// k_bucket_counts adds, per run of equal keys within a wavefront, the run's
+// length to count[key] once (keys >= nb lie past every bucket and are not
+// counted); an exclusive scan of the counts gives begin, end = begin + count.
+__global__ __launch_bounds__(kThreads) void k_fill_u32(uint32_t *__restrict__ p, int64_t n,
+                                                       uint32_t v) {
   const int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (t > n) return;
-  const int64_t prev = t == 0 ? -1 : int64_t(keys[t - 1]);
-  const int64_t cur = t == n ? nb - 1 : (int64_t(keys[t]) < nb - 1 ? int64_t(keys[t]) : nb - 1);
-  for (int64_t c = (prev + 1 > 0 ? prev + 1 : 0); c <= cur; ++c) begin[c] = uint32_t(t);
-  const int64_t prev2 = t == 0 ? 0 : int64_t(keys[t - 1]);
-  const int64_t cur2 = t == n ? nb : (int64_t(keys[t]) < nb ? int64_t(keys[t]) : nb);
-  for (int64_t c = prev2; c < cur2; ++c) end[c] = uint32_t(t);
+  if (t < n) p[t] = v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bucket_counts(const uint32_t *__restrict__ keys,
+                                                            int64_t n, int64_t nb,
+                                                            uint32_t *__restrict__ count) {
+  const int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < n;
+  const uint32_t k = valid ? keys[i] : 0xffffffffu;
+  const uint32_t kp = __shfl_up(k, 1, 64);
+  const bool head = valid && (lane == 0 || kp != k);
+  const uint64_t heads = __ballot(head);
+  const uint64_t vmask = __ballot(valid);
+  if (head && int64_t(k) < nb) {
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= lane
+    const uint64_t after = heads & ~upto;
+    const int next = after ? __ffsll((long long)after) - 1 : 64 - __clzll((long long)vmask);
+    atomicAdd(&count[k], uint32_t(next - lane));
+  }
+}
+
+// end[b] = begin[b] + count[b], the counts held in end
+__global__ __launch_bounds__(kThreads) void k_bucket_ends(const uint32_t *__restrict__ begin,
+                                                          int64_t nb, uint32_t *__restrict__ end) {
+  const int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (t < nb) end[t] = begin[t] + end[t];
 }
 
 template <int D> void launch_enforce(abh_context *ctx, const GridParams &g, int64_t n,
@@ -361,9 +382,20 @@ abh_status abh_cells_update(abh_cells *c, int64_t n, double *pos, uint8_t *alive
       abh::radix_sort_pairs(ctx, c->keys.as<uint32_t>(), c->order.as<int32_t>(), na,
                             bits_for(uint64_t(2 * nb - 1)));
     }
-    k_bucket_bounds<<<unsigned(abh::ceil_div(na + 1, kThreads)), kThreads, 0, s>>>(
-        c->keys.as<uint32_t>(), na, nb, c->begin.as<uint32_t>(), c->end.as<uint32_t>());
-    abh::check_launch();
+    {
+      uint32_t *cnt = c->end.as<uint32_t>();
+      k_fill_u32<<<unsigned(abh::ceil_div(nb, kThreads)), kThreads, 0, s>>>(cnt, nb, 0u);
+      abh::check_launch();
+      if (na > 0) {
+        k_bucket_counts<<<unsigned(abh::ceil_div(na, kThreads)), kThreads, 0, s>>>(
+            c->keys.as<uint32_t>(), na, nb, cnt);
+        abh::check_launch();
+      }
+      abh::scan_exclusive(ctx, cnt, c->begin.as<uint32_t>(), nb, nullptr);
+      k_bucket_ends<<<unsigned(abh::ceil_div(nb, kThreads)), kThreads, 0, s>>>(
+          c->begin.as<uint32_t>(), nb, cnt);
+      abh::check_launch();
+    }
     if (na > 0)
       abh::gather_bytes(ctx, c->order.as<int32_t>(), na, pos, c->pos.p, size_t(D) * 8);
     if (n_alive_out) *n_alive_out = na;

@@ -148,30 +148,60 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *__re
   hist[int64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter of one tile.  Keys are taken in index order, 256 per round;
-// within a round a wave finds the lanes sharing its digit with 8 ballots, and
-// per-wave digit counts in LDS order the four waves.
+// Stable scatter of one tile through LDS.  The tile's keys and values are
+// loaded once (16 per thread), counted per digit in LDS, and ranked in index
+// order, 256 per round: within a round a wave finds the lanes sharing its
+// digit with 8 ballots, and per-wave digit counts order the four waves.
+// Each key goes to its place in the tile sorted by digit (in LDS); the tile
+// is then written out in that order, so every digit's run is one
+// contiguous store instead of one scattered word per key.
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin, uint32_t *__restrict__ kout,
     int32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ offs,
     int ntiles) {
-  __shared__ uint32_t base[kBins];
+  __shared__ uint32_t gbase[kBins];  // global offset of each digit's run of this tile
+  __shared__ uint32_t lbase[kBins];  // offset of each digit's run within the tile
+  __shared__ uint32_t run[kBins];    // keys of each digit placed so far
   __shared__ uint32_t wcount[kSortThreads / 64][kBins];
+  __shared__ uint32_t wsum[kSortThreads / 64];
+  __shared__ uint32_t sk[kSortTile];
+  __shared__ int32_t sv[kSortTile];
   const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  base[tid] = offs[int64_t(tid) * ntiles + blockIdx.x];
+  const int wave = tid >> 6, lane = tid & 63;
   const int64_t tile0 = int64_t(blockIdx.x) * kSortTile;
+  const int tn = int(min<int64_t>(kSortTile, n - tile0));
+  uint32_t key[kSortRounds];
+  int32_t val[kSortRounds];
+#pragma unroll
   for (int r = 0; r < kSortRounds; ++r) {
-    const int64_t first = tile0 + int64_t(r) * kSortThreads;
-    if (first >= n) break;  // uniform across the block
+    const int e = r * kSortThreads + tid;
+    key[r] = e < tn ? kin[tile0 + e] : 0u;
+    val[r] = e < tn ? vin[tile0 + e] : 0;
+  }
+  run[tid] = 0;
+  gbase[tid] = offs[int64_t(tid) * ntiles + blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortRounds; ++r)
+    if (r * kSortThreads + tid < tn) atomicAdd(&run[(key[r] >> shift) & (kBins - 1)], 1u);
+  __syncthreads();
+  // exclusive scan of the tile's digit counts
+  const uint32_t c = run[tid];
+  const uint32_t incl = wave_incl_scan(c);
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  lbase[tid] = before + incl - c;
+  run[tid] = 0;
+  __syncthreads();
+  for (int r = 0; r < kSortRounds; ++r) {
+    if (r * kSortThreads >= tn) break;  // uniform across the block
 #pragma unroll
     for (int w = 0; w < kSortThreads / 64; ++w) wcount[w][tid] = 0;
     __syncthreads();
-    const int64_t idx = first + tid;
-    const bool valid = idx < n;
-    const uint32_t key = valid ? kin[idx] : 0u;
-    const int32_t val = valid ? vin[idx] : 0;
-    const uint32_t digit = (key >> shift) & (kBins - 1);
+    const bool valid = r * kSortThreads + tid < tn;
+    const uint32_t digit = (key[r] >> shift) & (kBins - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -183,17 +213,24 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     if (valid && rank == 0) wcount[wave][digit] = __popcll(peers);
     __syncthreads();
     if (valid) {
-      uint32_t pos = base[digit] + rank;
+      uint32_t pos = lbase[digit] + run[digit] + rank;
       for (int w = 0; w < wave; ++w) pos += wcount[w][digit];
-      kout[pos] = key;
-      vout[pos] = val;
+      sk[pos] = key[r];
+      sv[pos] = val[r];
     }
     __syncthreads();
     uint32_t add = 0;
 #pragma unroll
     for (int w = 0; w < kSortThreads / 64; ++w) add += wcount[w][tid];
-    base[tid] += add;
+    run[tid] += add;
     __syncthreads();
+  }
+  for (int e = tid; e < tn; e += kSortThreads) {
+    const uint32_t k = sk[e];
+    const uint32_t d = (k >> shift) & (kBins - 1);
+    const uint32_t pos = gbase[d] + uint32_t(e) - lbase[d];
+    kout[pos] = k;
+    vout[pos] = sv[e];
   }
 }
 
