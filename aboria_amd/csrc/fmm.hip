@@ -1165,11 +1165,19 @@ __global__ __launch_bounds__(256) void k_leaf_mfma(const int32_t *__restrict__ l
         }
         __syncthreads();
         const int nsteps = (ns + 3) / 4;
+        // the source charges of a step loaded one step (of this wave) ahead,
+        // from a valid row (the staged sources' first) past the end
+        auto charge = [&](int st2) {
+          const int sj2 = st2 * 4 + kl;
+          return xs[int64_t(Pi[sj2 < ns ? sj2 : 0]) * kRhs + row];
+        };
+        double b_next = wave < nsteps ? charge(wave) : 0.0;
         for (int st = wave; st < nsteps; st += 4) {
           const int sj = st * 4 + kl;
           const bool sok = sj < ns;
           const int sjc = sok ? sj : 0;
-          const double b = sok ? xs[int64_t(Pi[sjc]) * kRhs + row] : 0.0;
+          const double b = sok ? b_next : 0.0;
+          if (st + 4 < nsteps) b_next = charge(st + 4);
           const double y0 = Ps[sjc][0], y1 = Ps[sjc][1], y2 = Ps[sjc][2];
 #pragma unroll
           for (int rb = 0; rb < 4; ++rb) {
