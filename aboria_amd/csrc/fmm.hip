@@ -1118,12 +1118,17 @@ __global__ __launch_bounds__(256) void k_leaf_mfma(const int32_t *__restrict__ l
     f64x4 acc[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) acc[rb] = f64x4{0.0, 0.0, 0.0, 0.0};
-    // L2P: A = S (targets x nodes), B = G (nodes x right-hand sides)
+    // L2P: A = S (targets x nodes), B = G (nodes x right-hand sides); the
+    // local expansion's rows loaded one step (of this wave) ahead, from a
+    // valid row past the end
+    const double *Gn = G + int64_t(node) * NC * kRhs + row;
+    double g_next = wave < KSL ? Gn[int64_t(min(wave * 4 + kl, NC - 1)) * kRhs] : 0.0;
     for (int ks = wave; ks < KSL; ks += 4) {
       const int j = ks * 4 + kl;
       const bool jok = j < NC;
       const int jx = j / (N * N), jy = (j / N) % N, jz = j % N;
-      const double b = jok ? G[(int64_t(node) * NC + j) * kRhs + row] : 0.0;
+      const double b = jok ? g_next : 0.0;
+      if (ks + 4 < KSL) g_next = Gn[int64_t(min((ks + 4) * 4 + kl, NC - 1)) * kRhs];
 #pragma unroll
       for (int rb = 0; rb < 4; ++rb) {
         if (rb >= RBk) break;
