@@ -1140,7 +1140,7 @@ class Mlp:
               lr_decay=1.0, seed=0, graph=False, fused=False):
         """Adam on the mean squared error; returns the per-epoch mean loss.
         graph: the steps replay one captured HIP graph; fused: a step runs as
-        one kernel per 16 rows through the whole network plus one update."""
+        one kernel per 8 rows through the whole network plus one update."""
         import numpy as np
         _require_cuda(x, torch.float32, "x")
         _require_cuda(t, torch.float32, "t")

@@ -681,7 +681,9 @@ abh_status abh_schwarz_create(abh_context *ctx, const abh_cells *cells,
  * a Fisher-Yates pass on SplitMix64 seeded with seed ^ (bucket *
  * 0xD1B54A32D192ED03).  coarse_grid_n in [0, 4096] (0: abh_schwarz_create).
  * The coarse domain is the last in abh_schwarz_get_info / get_domains;
- * abh_schwarz_factorize refactorises it too. */
+ * abh_schwarz_factorize refactorises it too.  ABH_ERR_INVALID_ARG when no
+ * lattice point falls in a non-empty bucket (the reference asserts a
+ * non-empty coarse domain). */
 abh_status abh_schwarz_create_coarse(abh_context *ctx, const abh_cells *cells,
                                      const abh_kernel *kernel, double radius,
                                      double diag_shift, int coarse_grid_n,
@@ -804,7 +806,7 @@ typedef struct abh_mlp_train_options {
   float lr_decay;        /* learning rate multiplied by this every epoch    */
   int32_t flags;         /* ABH_TRAIN_GRAPH: training steps replayed from one
                             HIP graph (same kernels, same results);
-                            ABH_TRAIN_FUSED: a step as one kernel per 16 rows
+                            ABH_TRAIN_FUSED: a step as one kernel per 8 rows 
                             through the whole network plus one reduction and
                             Adam kernel (at most 4 layers, widths <= 128;
                             another summation order)                        */

@@ -324,7 +324,7 @@ def test_mlp_training_graph_equals_eager(ctx):
 
 @pytest.mark.parametrize("kind", ["mse", "residual"])
 def test_mlp_fused_step_gradient(ctx, kind):
-    """The fused training step (k_train_chunk: 16 rows per block through the
+    """The fused training step (k_train_chunk: 8 rows per block through the
     whole network in LDS; k_train_update: chunk sums + Adam): one step over
     the whole set with Adam reduced to p -= g / (|g| + 1) (beta1 = beta2 = 0,
     eps = 1, lr = 1) gives back its gradient g = -dp / (1 - |dp|), which
