@@ -2020,7 +2020,6 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
       if (KP > 0) {
         // C_o W^ into G^, then G += U G^
         if (mixed && f->c16) {
-          const int64_t per16 = int64_t(kOffsets) * (KP / 16) * ((KP + 31) / 32) * 512;
           m2l16_level(f, l, f->m2l16_tt, s);
         } else {
           m2l_compressed(KP, par + f->tree.par_off[l - 1], npar(l - 1),
