@@ -237,6 +237,31 @@ def test_batch_gmres_graph_equals_eager(ctx, precision):
     assert se == sg and torch.equal(xe, xg)
 
 
+def test_batch_gmres_graph_ragged_batch(ctx):
+    """Graph-captured GMRES on a ragged batch with an empty problem, a
+    one-point problem and a problem whose right-hand side is zero: equal to
+    the eager run bit for bit; the one-point problem is solved in one step
+    (x = b / K(0)), the zero right-hand side returns x = 0 converged."""
+    sizes = [0, 1, 400, 3, 0, 250]
+    pos, off = make_batch(sizes, 41)
+    allp = np.concatenate(pos)
+    b = np.random.default_rng(42).normal(size=int(off[-1]))
+    b[off[3]:off[4]] = 0.0
+    A = ab.DenseBatchOperator(ctx, off, torch.tensor(allp, device="cuda"),
+                              ab.Kernel.gaussian(0.02))
+    bt = torch.tensor(b, device="cuda")
+    xe, se = A.gmres(bt, restart=20, max_iters=1000, tol=1e-8)
+    xg, sg = A.gmres(bt, restart=20, max_iters=1000, tol=1e-8, graph=True)
+    assert se == sg and torch.equal(xe, xg)
+    x = xg.cpu().numpy()
+    assert sg[1]["converged"] and abs(x[off[1]] - b[off[1]]) <= 1e-15 * abs(b[off[1]])
+    assert sg[3]["converged"] and np.all(x[off[3]:off[4]] == 0.0)
+    for p in (2, 5):
+        a, e = off[p], off[p + 1]
+        r = b[a:e] - oracle.dense_matvec(pos[p], pos[p], (1, 0.02), x[a:e])
+        assert np.linalg.norm(r) / np.linalg.norm(b[a:e]) <= 1.01e-8, (p, sg[p])
+
+
 def test_batch_gmres_stops_stagnated_problems(ctx):
     """stall_restarts: a problem whose residual cannot reach the tolerance
     (multiquadric at N = 6000: FP64 rounding of K x sits far above 1e-8 ||b||)

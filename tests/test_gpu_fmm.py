@@ -288,6 +288,31 @@ def test_fmm_lower_dims_reference_accuracy(ctx, dim):
     f.close()
 
 
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("n", [0, 1, 5])
+def test_fmm_tiny_inputs(ctx, dim, n):
+    """Empty and tiny particle sets (at most 8 per leaf: every pair is near,
+    so the FMM is its P2P): y += K x leaves y untouched for n = 0 and equals
+    the FP64 direct sum (self term included) otherwise; the tree's order is
+    the oracle's."""
+    rng = np.random.default_rng(90 + 10 * dim + n)
+    pos = rng.random((n, dim))
+    f = ab.FmmOperator(ctx, torch.tensor(pos, device="cuda").reshape(n, dim), [0.0] * dim,
+                       [1.0] * dim, 8, 4, ab.Kernel.smooth_inv_r(1e-3))
+    x = rng.uniform(-1, 1, (3, n))
+    xt = torch.tensor(x, device="cuda")
+    y = f.apply_colmajor(xt, torch.full_like(xt, 2.0)).cpu().numpy()
+    if n == 0:
+        assert y.size == 0
+    else:
+        t = oracle.Octree([0.0] * dim, [1.0] * dim, pos, 8)
+        assert np.array_equal(f.order().cpu().numpy(), t.order)
+        for r in range(3):
+            ref = 2.0 + oracle.dense_matvec(pos, pos, (KI, 1e-3), x[r])
+            assert rel(y[r], ref) < 1e-12
+    f.close()
+
+
 def test_fmm_c3_scale(ctx):
     """The C3 workload itself (N = 2,000,000 uniform, leaves of at most 64:
     6 levels, 107,417 leaves, the leaf-level M2L tiles): FP64 order 6 within

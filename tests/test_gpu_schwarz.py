@@ -333,7 +333,7 @@ def test_mlp_fused_step_gradient(ctx, kind):
     the unfused loss to 1e-5.  Fused training with graph replay equals fused
     training without it bit for bit."""
     rng = np.random.default_rng(21)
-    n, k = 1000, 8
+    n, k = 1003, 8  # 126 chunks of 8 rows, the last one of 3
     x = rng.uniform(-1, 1, (n, 24)).astype(np.float32)
     t = np.tanh(x @ rng.normal(0, 0.3, (24, k))).astype(np.float32)
     mats = rng.normal(0, 1, (n, k, k)).astype(np.float32)
