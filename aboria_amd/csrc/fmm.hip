@@ -162,8 +162,9 @@ __global__ __launch_bounds__(256) void k_scatter_rhs(const int32_t *__restrict__
 // y lines in LDS in place, x lines from LDS into the thread's registers.
 // M2M: W_p[i] = sum_c sum_j prod_d s1[h_d][i_d][j_d] W_c[j], the children
 // accumulated in registers, the parent written once; L2L (transpose):
-// G_c[i] += sum_j prod_d s1[h_d][j_d][i_d] G_p[j], added into each child's
-// local expansion.  Each 1-D sum runs over k in order.
+// G_c[i] = sum_j prod_d s1[h_d][j_d][i_d] G_p[j], written into each child's
+// local expansion before its level's M2L adds to it.  Each 1-D sum runs
+// over k in order.
 template <int N, bool UP>
 __global__ __launch_bounds__(256) void k_transfer_lines(const int32_t *__restrict__ parents,
                                                         int npar,
@@ -255,8 +256,9 @@ __global__ __launch_bounds__(256) void k_transfer_lines(const int32_t *__restric
         if (UP) {
           acc[q][ix] += s;
         } else {
-          double *g = E + (int64_t(child) * NC + (ix * N + a) * N + b) * kRhs + r;
-          *g += s;
+          // the first write of the child's local expansion in an apply (its
+          // level's M2L adds to it afterwards)
+          E[(int64_t(child) * NC + (ix * N + a) * N + b) * kRhs + r] = s;
         }
       }
     }
@@ -1679,6 +1681,8 @@ template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_lea
   }
   f->W.ensure(size_t(nn) * NC * kRhs * 8);
   f->G.ensure(size_t(nn) * NC * kRhs * 8);
+  ABH_HIP(hipMemsetAsync(f->W.p, 0, size_t(nn) * NC * kRhs * 8, s));
+  ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
   f->xs.ensure(size_t(n + 1) * kRhs * 8);
   f->ys.ensure(size_t(n + 1) * kRhs * 8);
   ABH_HIP(hipStreamSynchronize(s));
@@ -1941,8 +1945,10 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
     k_gather_rhs<<<unsigned(abh::ceil_div(n * kRhs, 256)), 256, 0, s>>>(
         f->tree.order.as<int32_t>(), n, nr, x + int64_t(r0) * ldx, ldx, f->xs.as<double>());
     abh::check_launch();
-    ABH_HIP(hipMemsetAsync(f->W.p, 0, size_t(nn) * NC * kRhs * 8, s));
-    ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
+    // no zeroing of W and G per apply: P2M and M2M assign every non-empty
+    // node's W, the L2L assigns every G below level 1, and the W of empty
+    // nodes and the G of level 1 (read, never written) stay zero from
+    // creation
     if (f->nleaf > 0) {
       k_p2m_mfma<N><<<f->nleaf, 256, 0, s>>>(f->tree.leaf_node.as<int32_t>(), f->nleaf,
                                         f->tree.leaves.as<int2>(), f->box.as<double>(),
