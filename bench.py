@@ -17,22 +17,27 @@ workload):
               radius 0.04, build + search and a Gaussian sparse matvec,
               beside the CPU oracle
   fmm         black-box Chebyshev FMM, N=2,000,000 uniform, 1/(r+1e-3),
-              leaves <= 64, order 6, 16 RHS: matvec/s in FP64 and mixed
-              precision, relative error against an FP64 direct sum on 10,000
-              seeded targets
+              leaves <= 64, orders 6-8 uncompressed and with the M2L
+              compressed at several tolerances, 16 RHS: matvec/s in FP64 and
+              mixed precision (the M2L autotuner's choice reported),
+              relative error against an FP64 direct sum on 10,000 seeded
+              targets, and the fastest form at <= 1e-6
   rbf_batch   512 uneven 2D multiquadric (c=0.05) RBF systems, batched
               GMRES(50) on a bounded iteration budget, FP64 and FP32
               compensated operator (FP64 residual / orthogonalisation);
-              Schwarz-preconditioned FP64 GMRES over the whole batch, the
-              same with iterative refinement (double-double residual), and
-              the time to 1e-8 (set-up + solve) of the problems with
-              N < 2,000 as a batch of their own
+              Schwarz-preconditioned FP64 GMRES over the whole batch
+              (Arnoldi steps replayed from HIP graphs), the same with
+              iterative refinement (double-double residual), and the time to
+              1e-8 (set-up + solve) of the problems with N < 2,000 as a
+              batch of their own, with graphs and with a host check per step
   schwarz     2D jittered grid N=400,056, Gaussian sigma=0.01 (+1e-5 I) and a
               narrow variant sigma=0.002, GMRES to 1e-8 on a 300-iteration
-              budget: plain Schwarz vs the learned patch solves (the network
-              trained on the patch residual, with the Schwarz fallback), the
-              exact patch solves, the exact least-squares approximate inverse
-              and none; setup / solve times and iterations
+              budget: plain Schwarz (with and without the coarse grid of
+              set_coarse_grid_n(32)) vs the learned patch solves (the network
+              trained on the patch residual with the fused training step,
+              graph-replayed; the Schwarz fallback), the exact patch solves,
+              the exact least-squares approximate inverse and none; setup /
+              training / solve times and iterations
 and a CPU baseline of the headline -- the oracle (C++/OpenMP restatement of
 the reference's CPU search) timed on the host on a bounded sample of centres.
 """
