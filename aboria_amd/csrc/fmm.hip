@@ -144,14 +144,24 @@ __global__ __launch_bounds__(256) void k_gather_rhs(const int32_t *__restrict__ 
   xs[t] = r < nrhs ? x[int64_t(r) * ldx + order[p]] : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_scatter_rhs(const int32_t *__restrict__ order, int64_t n,
+// y (column-major, input order) += ys in tree order: thread (i, r) with i
+// fastest adds ys[inv[i]][r] to y[r][i] (coalesced updates of y, the rows of
+// ys read where they lie; inv is the inverse of the tree order)
+__global__ __launch_bounds__(256) void k_scatter_rhs(const int32_t *__restrict__ inv, int64_t n,
                                                      int nrhs, const double *__restrict__ ys,
                                                      double *__restrict__ y, int64_t ldy) {
   const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (t >= n * int64_t(nrhs)) return;
   const int r = int(t / n);
-  const int64_t p = t % n;
-  y[int64_t(r) * ldy + order[p]] += ys[p * kRhs + r];
+  const int64_t i = t % n;
+  y[int64_t(r) * ldy + i] += ys[int64_t(inv[i]) * kRhs + r];
+}
+
+// inv[order[p]] = p
+__global__ __launch_bounds__(256) void k_invert_order(const int32_t *__restrict__ order, int64_t n,
+                                                      int32_t *__restrict__ inv) {
+  const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (p < n) inv[order[p]] = int32_t(p);
 }
 
 
@@ -1567,6 +1577,7 @@ struct abh_fmm {
   abh::Octree tree;
   abh::OctreeLists lists;
   DevBuf box, coord, M, W, G, xs, ys, maps;
+  DevBuf inv;  // input index -> tree position (the inverse of tree.order)
   std::vector<int64_t> map_off;          // per level offset into maps
   int nleaf = 0;
   int64_t n_nodes = 0;
@@ -1685,6 +1696,12 @@ template <int N> void create_impl(abh_fmm *f, const double *pos_in, double n_lea
   ABH_HIP(hipMemsetAsync(f->G.p, 0, size_t(nn) * NC * kRhs * 8, s));
   f->xs.ensure(size_t(n + 1) * kRhs * 8);
   f->ys.ensure(size_t(n + 1) * kRhs * 8);
+  f->inv.ensure(size_t(n + 1) * 4);
+  if (n > 0) {
+    k_invert_order<<<unsigned(abh::ceil_div(n, 256)), 256, 0, s>>>(t.order.as<int32_t>(), n,
+                                                                  f->inv.as<int32_t>());
+    abh::check_launch();
+  }
   ABH_HIP(hipStreamSynchronize(s));
   f->info.order = N;
   f->info.levels = t.levels;
@@ -2072,7 +2089,7 @@ template <int N> void apply_impl(abh_fmm *f, int nrhs, const double *x, int64_t 
           f->kern, f->consts);
       abh::check_launch();
       k_scatter_rhs<<<unsigned(abh::ceil_div(n * nr, 256)), 256, 0, s>>>(
-          f->tree.order.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
+          f->inv.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
       abh::check_launch();
     }
   }
@@ -2121,6 +2138,12 @@ template <int D, int N> void create_gen(abh_fmm *f, const double *pos_in, double
   f->G.ensure(size_t(nn) * NC * kRhs * 8);
   f->xs.ensure(size_t(n + 1) * kRhs * 8);
   f->ys.ensure(size_t(n + 1) * kRhs * 8);
+  f->inv.ensure(size_t(n + 1) * 4);
+  if (n > 0) {
+    k_invert_order<<<unsigned(abh::ceil_div(n, 256)), 256, 0, s>>>(t.order.as<int32_t>(), n,
+                                                                  f->inv.as<int32_t>());
+    abh::check_launch();
+  }
   ABH_HIP(hipStreamSynchronize(s));
   f->info.order = N;
   f->info.levels = t.levels;
@@ -2188,7 +2211,7 @@ template <int D, int N> void apply_gen(abh_fmm *f, int nrhs, const double *x, in
           f->xs.as<double>(), f->G.as<double>(), f->ys.as<double>(), f->kern, f->consts);
       abh::check_launch();
       k_scatter_rhs<<<unsigned(abh::ceil_div(n * nr, 256)), 256, 0, s>>>(
-          t.order.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
+          f->inv.as<int32_t>(), n, nr, f->ys.as<double>(), y + int64_t(r0) * ldy, ldy);
       abh::check_launch();
     }
   }
