@@ -1,8 +1,8 @@
 // CPU AddressSanitizer / UBSan run of the oracle (oracle/oracle_capi.cpp,
-// compiled together with this driver): every entry point on small seeded
+// compiled together with this program): every entry point on small seeded
 // inputs in 1, 2 and 3 dimensions, periodic and not, with non-finite and
 // out-of-domain particles.  Any out-of-bounds access or undefined operation
-// aborts; the driver also checks that count and fill agree.
+// aborts; this program also checks that count and fill agree.
 //
 // usage: asan_oracle   (prints "ok" on success)
 #include <cmath>
