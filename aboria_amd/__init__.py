@@ -387,13 +387,14 @@ class Particles:
     @classmethod
     def from_host(cls, ctx, position, id=None, alive=None, columns=None):
         """A particle set on the context's device from host arrays (what
-        aboria_amd.io's readers return); ids are kept when given."""
-        p = cls(ctx, torch.as_tensor(np.ascontiguousarray(position), dtype=torch.float64),
-                **{k: torch.as_tensor(np.ascontiguousarray(v)) for k, v in (columns or {}).items()})
+        aboria_amd.io's readers return); ids are kept when given.  The arrays
+        are copied (readers may hand over read-only buffers)."""
+        p = cls(ctx, torch.from_numpy(np.array(position, dtype=np.float64, order="C")),
+                **{k: torch.from_numpy(np.array(v, order="C")) for k, v in (columns or {}).items()})
         if id is not None:
-            p.id = torch.as_tensor(np.asarray(id, dtype=np.int64), device=ctx.device)
+            p.id = torch.from_numpy(np.array(id, dtype=np.int64)).to(ctx.device)
         if alive is not None:
-            p.alive = torch.as_tensor(np.asarray(alive, dtype=np.uint8), device=ctx.device)
+            p.alive = torch.from_numpy(np.array(alive, dtype=np.uint8)).to(ctx.device)
         return p
 
 
