@@ -5,7 +5,8 @@ KernelDense::evaluate restatement for every problem; FP32-compensated within
 2e-6 of the same scale; GMRES reaches ||b - A x|| <= 1e-8 ||b|| with the
 residual recomputed independently by the oracle; the GMRES(m) iterate after
 a fixed number of steps (full or restarted, eager or graph-replayed) has the
-residual of the Krylov-space least-squares minimiser within 1e-7 relative.
+residual of the Krylov-space least-squares minimiser within 1e-7 relative
+(1e-6 with the batched Schwarz preconditioner on the right).
 """
 import numpy as np
 import pytest
@@ -88,8 +89,8 @@ def test_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters, graph):
     # mathematics: each cycle of k steps from x adds the d in the Krylov
     # space K_k(A, b - A x) that minimises ||b - A (x + d)||.  With a
     # tolerance no residual reaches (1e-300) every problem runs exactly
-    # max_iters steps, cycles of `restart` steps
-    # and a shorter last one; the residual after each cycle is pinned here
+    # max_iters steps, cycles of `restart` steps and a shorter last one;
+    # the residual after each cycle is pinned here
     # against a dense least-squares restatement (a dropped term, wrong sign
     # or index in the Arnoldi or Givens updates leaves a larger residual).
     sizes = [150, 300, 600]
@@ -115,6 +116,51 @@ def test_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters, graph):
         assert 1e-3 < want < 0.9, want
         assert abs(got - want) <= 1e-7 * want, (p, got, want)
         assert abs(st["residual"] - got) <= 1e-9 * got, (p, st, got)
+
+
+@pytest.mark.parametrize("restart,max_iters,graph", [
+    (50, 1, False), (50, 2, False), (2, 4, False), (2, 4, True)])
+def test_preconditioned_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters, graph):
+    # Right-preconditioned GMRES(m): per cycle x += M^-1 u with u in
+    # K_k(A M^-1, r) minimising ||r - A M^-1 u||.  M^-1 is the batched
+    # Schwarz apply (pinned against the oracle by the test above), taken
+    # column by column as a matrix.
+    sizes = [300, 600, 900]
+    pos, off = make_batch(sizes, 31)
+    allp = np.concatenate(pos)
+    A = ab.DenseBatchOperator(ctx, off, torch.tensor(allp, device="cuda"),
+                              ab.Kernel.gaussian(0.05))
+    sw = ab.SchwartzPreconditioner.for_dense_batch(A, [0, 0], [1, 1], 10.0, 1e-6)
+    minv = [np.zeros((n, n)) for n in sizes]
+    for j in range(max(sizes)):
+        e = torch.zeros(int(off[-1]), dtype=torch.float64, device="cuda")
+        for p in range(len(sizes)):
+            if j < sizes[p]:
+                e[int(off[p]) + j] = 1.0
+        y = sw.solve(e).cpu().numpy()
+        for p in range(len(sizes)):
+            if j < sizes[p]:
+                minv[p][:, j] = y[off[p]:off[p + 1]]
+    b = np.random.default_rng(32).standard_normal(off[-1])
+    x, stats = A.gmres(torch.tensor(b, device="cuda"), restart=restart, max_iters=max_iters,
+                       tol=1e-300, preconditioner=sw, graph=graph)
+    x = x.cpu().numpy()
+    for p, st in enumerate(stats):
+        a, e = off[p], off[p + 1]
+        K = oracle.dense_matvec(pos[p], pos[p], (1, 0.05), np.eye(sizes[p]))
+        bp = b[a:e]
+        xr = np.zeros(sizes[p])
+        left = max_iters
+        while left > 0:
+            xr += minv[p] @ _krylov_step(K @ minv[p], bp - K @ xr, min(restart, left))
+            left -= restart
+        want = np.linalg.norm(bp - K @ xr) / np.linalg.norm(bp)
+        got = np.linalg.norm(bp - K @ x[a:e]) / np.linalg.norm(bp)
+        print("preconditioned GMRES", restart, max_iters, p, "%.3e %.3e" % (got, want),
+              flush=True)
+        assert st["iterations"] == max_iters and not st["converged"], st
+        assert 1e-9 < want < 1.0 - 1e-6, want
+        assert abs(got - want) <= 1e-6 * want, (p, got, want)
 
 
 @pytest.mark.parametrize("dim", [1, 2, 3])
