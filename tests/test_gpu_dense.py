@@ -14,6 +14,7 @@ import torch
 
 import aboria_amd as ab
 import oracle
+from krylov_ref import gmres_iterate
 
 pytestmark = pytest.mark.gpu
 
@@ -66,22 +67,6 @@ def test_batched_gmres_residual(ctx, precision):
         assert res <= 1.01e-8, (p, res, st)
 
 
-def _krylov_step(A, r, k):
-    """argmin over d in span{r, A r, ..., A^(k-1) r} of ||r - A d||: an
-    orthonormal basis of the Krylov space (Gram-Schmidt twice per vector)
-    and a dense least-squares solve, sharing no Arnoldi recurrence or
-    Givens rotation with the solver under test."""
-    Q = np.zeros((len(r), k))
-    v = r / np.linalg.norm(r)
-    for j in range(k):
-        for _ in range(2):
-            v = v - Q[:, :j] @ (Q[:, :j].T @ v)
-        Q[:, j] = v / np.linalg.norm(v)
-        v = A @ Q[:, j]
-    y, *_ = np.linalg.lstsq(A @ Q, r, rcond=None)
-    return Q @ y
-
-
 @pytest.mark.parametrize("restart,max_iters,graph", [
     (50, 1, False), (50, 3, False), (50, 12, False), (4, 12, False), (7, 12, True)])
 def test_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters, graph):
@@ -90,8 +75,8 @@ def test_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters, graph):
     # space K_k(A, b - A x) that minimises ||b - A (x + d)||.  With a
     # tolerance no residual reaches (1e-300) every problem runs exactly
     # max_iters steps, cycles of `restart` steps and a shorter last one;
-    # the residual after each cycle is pinned here
-    # against a dense least-squares restatement (a dropped term, wrong sign
+    # the residual after each cycle is pinned here against the dense
+    # least-squares restatement of krylov_ref (a dropped term, wrong sign
     # or index in the Arnoldi or Givens updates leaves a larger residual).
     sizes = [150, 300, 600]
     pos, off = make_batch(sizes, 11)
@@ -105,11 +90,7 @@ def test_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters, graph):
         a, e = off[p], off[p + 1]
         K = oracle.dense_matvec(pos[p], pos[p], (1, 0.05), np.eye(sizes[p]))
         bp = b[a:e]
-        xr = np.zeros(sizes[p])
-        left = max_iters
-        while left > 0:
-            xr += _krylov_step(K, bp - K @ xr, min(restart, left))
-            left -= restart
+        xr = gmres_iterate(K, bp, restart, max_iters)
         want = np.linalg.norm(bp - K @ xr) / np.linalg.norm(bp)
         got = np.linalg.norm(bp - K @ x[a:e]) / np.linalg.norm(bp)
         assert st["iterations"] == max_iters and not st["converged"], st
@@ -149,11 +130,7 @@ def test_preconditioned_gmres_iterates_are_krylov_minimisers(ctx, restart, max_i
         a, e = off[p], off[p + 1]
         K = oracle.dense_matvec(pos[p], pos[p], (1, 0.05), np.eye(sizes[p]))
         bp = b[a:e]
-        xr = np.zeros(sizes[p])
-        left = max_iters
-        while left > 0:
-            xr += minv[p] @ _krylov_step(K @ minv[p], bp - K @ xr, min(restart, left))
-            left -= restart
+        xr = gmres_iterate(K, bp, restart, max_iters, minv=minv[p])
         want = np.linalg.norm(bp - K @ xr) / np.linalg.norm(bp)
         got = np.linalg.norm(bp - K @ x[a:e]) / np.linalg.norm(bp)
         print("preconditioned GMRES", restart, max_iters, p, "%.3e %.3e" % (got, want),

@@ -17,6 +17,7 @@ import aboria_amd as ab
 import oracle
 from aboria_amd import learned, workloads as wl
 from oracle import mlp as omlp
+from krylov_ref import gmres_iterate
 
 pytestmark = pytest.mark.gpu
 GAUSS = 1
@@ -169,6 +170,33 @@ def test_sparse_gmres_with_schwarz(ctx):
     r = op_matvec(oc, (GAUSS, sigma), radius, shift, x.cpu().numpy()) - b
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-8
     assert st["iterations"] < st0["iterations"]
+
+
+@pytest.mark.parametrize("restart,max_iters", [(50, 3), (3, 9)])
+def test_sparse_gmres_iterates_are_krylov_minimisers(ctx, restart, max_iters):
+    """The sparse operator's GMRES (its own operator and residual callbacks,
+    the diagonal shift, a nonzero initial guess) after a fixed number of
+    steps has the residual of the GMRES(m) iterate restated as Krylov
+    least-squares minimisers on the oracle's matrix (K + shift I), within
+    1e-7 relative."""
+    pos = wl.jittered_grid(24, 24, seed=40)
+    h = 1.0 / 24
+    sigma, radius, shift = 1.5 * h, 9 * h, 1e-5
+    cl, oc = build(ctx, pos, 16.0)
+    op = ab.SparseOperator(ctx, cl, cl.positions(), radius, ab.Kernel.gaussian(sigma))
+    n = cl.n_alive
+    rng = np.random.default_rng(41)
+    b, x0 = rng.normal(size=n), rng.normal(size=n)
+    x, st = op.gmres(torch.tensor(b, device="cuda"), x=torch.tensor(x0, device="cuda"),
+                     diag_shift=shift, restart=restart, max_iters=max_iters, tol=1e-300)
+    K = op_matvec(oc, (GAUSS, sigma), radius, shift, np.eye(n))
+    xr = gmres_iterate(K, b, restart, max_iters, x0=x0)
+    want = np.linalg.norm(b - K @ xr) / np.linalg.norm(b)
+    got = np.linalg.norm(b - K @ x.cpu().numpy()) / np.linalg.norm(b)
+    print("sparse GMRES", restart, max_iters, "%.3e %.3e" % (got, want), flush=True)
+    assert st["iterations"] == max_iters and not st["converged"], st
+    assert 1e-9 < want < 1.0 - 1e-6, want
+    assert abs(got - want) <= 1e-7 * want, (got, want)
 
 
 @pytest.mark.parametrize("dim,k,leaf", [(2, 32, 8.0), (3, 32, 20.0), (2, 1, 8.0), (2, 64, 200.0)])
